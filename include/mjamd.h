@@ -1,0 +1,175 @@
+/* mjamd.h — C ABI of the MI355X-native CG2D / GAD DST3 / exch2 halo library (libmjamd.so, gfx950).
+ *
+ * Drop-in boundary for the ocean time step's two data-parallel cores of mitgcm_jax (reference file:line citations
+ * below are into that repository). Every entry point takes plain device pointers, element strides and sizes; all
+ * device buffers belong to the caller; the library owns only the plans it creates (halo tables, DST3 pass tables).
+ * Every call is asynchronous on the given HIP stream (NULL = the null stream) and returns 0 (MJ_OK) or a negative
+ * code; mj_last_error() then describes the failure. Solver non-convergence is not an error: it is reported per
+ * member in mj_cg2d_diag (reference core/cg2d.py:267-268, diag numIters / lastResidual).
+ *
+ * Layout (reference mitgcm_jax/layout.py:1-54): FP64, C order, [tile, (k,) j, i] with OLx = OLy halos; batched
+ * (ensemble) arguments add a leading member axis addressed by an element stride. All arithmetic is FP64 in the
+ * reference's literal operation order with no fused multiply-add, except the explicitly mixed-precision solver.
+ *
+ * Sum orders of the CG2D global dot products (reference core/cg2d.py:141-169):
+ *   MJ_SUM_FORTRAN  per tile s = 0; DO j; DO i; s = s + a(i,j)  (strictly sequential), then tiles in order
+ *                   0 + tile1 + ... + tile13 (parallel/global_sum.py:20-26). Bitwise the reference "fortran" order.
+ *   MJ_SUM_TREE     per row of sNx points and per tile over its sNy row sums: fold onto 64 lanes
+ *                   (w[l] = v[l] + v[l+64] where l+64 < n), then butterfly w[l] += w[l^m], m = 32..1; lane 0 is
+ *                   the result. Tiles in the same fixed order. The reference's "tile" order is an XLA-chosen
+ *                   tree; this one is fixed and independent of the launch geometry, so results are bitwise
+ *                   reproducible for any block size and batch.
+ */
+#ifndef MJAMD_H
+#define MJAMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MJ_OK 0
+#define MJ_ERR_ARG (-1)   /* bad shape, null pointer, unsupported option */
+#define MJ_ERR_HIP (-2)   /* a HIP runtime call failed */
+#define MJ_ERR_SYNC (-3)  /* a device-side grid barrier timed out (the launch was abandoned safely) */
+
+#define MJ_SUM_FORTRAN 0
+#define MJ_SUM_TREE 1
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mj_version(void);
+
+/* Copies the last error message of the calling thread into buf (NUL-terminated); returns its full length. */
+int mj_last_error(char* buf, size_t n);
+
+/* ======================================================================================= exch2 halo exchange
+ * Replaces Exchanger.scalar / vector / exch_xy / exch_uv_xy (parallel/exchange.py:136-142, :170-186): for each
+ * written point d of the output component, out[d] = sign[d] * in_comp[d][src[d]], comp 1 = the u-like input,
+ * comp 2 = the v-like input; all other points keep their value. The tables are the written points of the probed
+ * exch2 maps (mitgcm_jax/data/exch_maps_13x90x90.npz, parallel/exchange.py:72-92), indices flat over
+ * [ntiles, ny, nx]. */
+typedef struct mj_halo_plan mj_halo_plan;
+
+/* nmaps = 1 (scalar kind: T, 3D, Z, SMs) or 2 (vector kind: u-map then v-map). For each map m: npts[m] written
+ * points with dst[m][p], src[m][p] (flat padded indices), comp[m][p] in {1,2}, sign[m][p] in {-1,+1}. */
+int mj_halo_plan_create(int nmaps, const int32_t* npts, const int32_t* const* dst, const int32_t* const* src,
+                        const int8_t* const* comp, const int8_t* const* sign, int ntiles, int ny, int nx,
+                        mj_halo_plan** out);
+int mj_halo_plan_destroy(mj_halo_plan* plan);
+/* 1 when no written point is the source of a written point (T, 3D, UVs, UVn, UV3s): in-place fills are exact. */
+int mj_halo_plan_inplace_ok(const mj_halo_plan* plan);
+
+/* Geometry of the arrays one halo call covers: element (outer o, tile t, inner n, j, i) lives at
+ * o*outer_stride + t*tile_stride + n*inner_stride + j*nx + i. [T, ny, nx]: nouter = ninner = 1;
+ * [T, K, ny, nx]: ninner = K, inner_stride = ny*nx, tile_stride = K*ny*nx; [B, T, K, ny, nx]: nouter = B. */
+typedef struct {
+    int64_t nouter, outer_stride, tile_stride, ninner, inner_stride;
+} mj_halo_geom;
+
+/* Scalar kind. out may equal in only when mj_halo_plan_inplace_ok(plan). out must already hold the values of the
+ * points the exchange does not write (the reference's "own" values). */
+int mj_halo_exchange(const mj_halo_plan* plan, const double* in, double* out, const mj_halo_geom* g, void* stream);
+/* Vector kind: (u_out, v_out) from (u_in, v_in). Same aliasing rule. */
+int mj_halo_exchange_uv(const mj_halo_plan* plan, const double* u_in, const double* v_in, double* u_out,
+                        double* v_out, const mj_halo_geom* g, void* stream);
+
+/* ======================================================================================= CG2D
+ * Replaces cg2d_fortran (core/cg2d.py:205-269) and the solve inside cg2d_solve (:351-375): normalise the rhs by
+ * cg2dNorm and 1/max|b|, exchange the first guess, PCG with the 5-point operator aW2d/aS2d/aC2d and the
+ * 5-point preconditioner pW/pS/pC in the Fortran update order, stop when err_sq < tol^2 or after max_iters
+ * iterations, un-normalise. The halos of x are left as cg2d.F leaves them: the exchanged normalised first guess
+ * where the T map writes, the first guess itself elsewhere. Runs all members of a batch in one persistent launch
+ * with per-member convergence masks. */
+typedef struct {
+    double firstResidual;  /* sqrt(err_sq) before the first iteration */
+    double lastResidual;   /* sqrt(err_sq) at exit */
+    double sumRHS;         /* global sum of the normalised rhs */
+    double rhsMax;         /* max |cg2d_b * cg2dNorm| */
+    int32_t numIters;      /* iterations done */
+    int32_t converged;     /* 1 if err_sq < tol^2 at exit */
+    int32_t outerIters;    /* refinement steps (mixed precision), 0 for the plain solve */
+    int32_t pad_;
+} mj_cg2d_diag;
+
+typedef struct {
+    const double *aW2d, *aS2d, *aC2d, *pW, *pS, *pC; /* [T, ny, nx] each (per member when member_stride != 0) */
+    int64_t member_stride;                           /* elements between members' operators; 0 = shared */
+} mj_cg2d_ops;
+
+typedef struct {
+    int batch;           /* members B >= 1 */
+    int ntiles, sNx, sNy, OL; /* 13, 90, 90, 4 (sNx, sNy in 64..128) */
+    int64_t member_stride;    /* elements between members of b, x0, x (>= ntiles*ny*nx) */
+    double tolerance;    /* cg2dTolerance (normalised target) */
+    int max_iters;       /* cg2dMaxIters */
+    int sum_order;       /* MJ_SUM_FORTRAN or MJ_SUM_TREE */
+    int block_threads;   /* 0 = library default; else a multiple of 64 in 64..1024 */
+    int grid_blocks;     /* 0 = as many co-resident blocks as useful */
+} mj_cg2d_config;
+
+/* Device workspace the solve needs (bytes), for the given configuration. */
+size_t mj_cg2d_workspace_bytes(const mj_cg2d_config* cfg);
+
+/* b, x0: [B][T, ny, nx] (member_stride apart); x: output, same shape (may not alias b or x0); cg2dNorm: device
+ * [B]; diag: device [B]; workspace: device, >= mj_cg2d_workspace_bytes(cfg), 256-byte aligned. plan_T: the T
+ * (scalar) halo plan. */
+int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* b, const double* x0, double* x,
+                  const double* cg2dNorm, const mj_cg2d_config* cfg, void* workspace, size_t workspace_bytes,
+                  mj_cg2d_diag* diag, void* stream);
+
+/* The linear operator of the implicit derivative (core/cg2d.py:292-298 _matvec): y = (A / cg2dNorm) v on the
+ * interior, 0 on halos, with v exchanged after zeroing its halos. Batched like mj_cg2d_solve. work: device
+ * scratch of B*member_stride doubles. */
+int mj_cg2d_matvec(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* v, double* y,
+                   const double* cg2dNorm, const mj_cg2d_config* cfg, double* work, void* stream);
+
+/* ======================================================================================= GAD DST3
+ * Replaces gad_advection / advect_tiles (pkgs/gad.py:460-560) with dst3_adv_x/y (:409-453): the 3-pass
+ * multi-dimensional DST3 on the LLC cubed sphere with per-facet sweep order, overlap/interior update regions and
+ * FILL_CS_CORNER_TR_RL / _UV_RS corner fills; gTracer = (localTij - tracer)/deltaTLev(k) on the whole tile array.
+ * With wFld given, a vertical DST3 flux (R) and its divergence are added (not in the reference, which runs the
+ * vertical implicitly). */
+typedef struct mj_gad_plan mj_gad_plan;
+
+/* Topology per tile (reference TileTopology, gad.py:75-106): face[t] in 1..6, edges[t*4 + {0,1,2,3}] = N, S,
+ * E, W facet-edge flags (0/1). */
+int mj_gad_plan_create(int ntiles, const int32_t* face, const uint8_t* edges, int sNx, int sNy, int OLx, int OLy,
+                       mj_gad_plan** out);
+int mj_gad_plan_destroy(mj_gad_plan* plan);
+/* Rasterises the plan's update regions: masks[((pass*2 + dir)*ntiles + t)*ny*nx + j*nx + i] = 1 where the
+ * X (dir 0) / Y (dir 1) update of pass 0..2 writes (host memory, 6*ntiles*ny*nx bytes); fills[(pass*4+f)*ntiles + t]
+ * = fill4dir (0 none) of f = {X before, X after, Y before, Y after}. For checking the plan against the tables. */
+int mj_gad_plan_describe(const mj_gad_plan* plan, uint8_t* masks, int32_t* fills);
+
+typedef struct {
+    /* 2-D [T, ny, nx], shared by all members */
+    const double *dyG, *dxG, *recip_dxC, *recip_dyC, *recip_rA, *rA, *maskInC;
+    /* 3-D [T, K, ny, nx] (member_stride3d apart when per-member) */
+    const double *maskW, *maskS, *maskC;
+    /* per level [K]: deltaTLev, drF, recip_drF, rhoFacC, recip_rhoFacC, deepFacC, recip_deepFacC, recip_deepFac2C,
+       recip_drC (interface k, used by the R flux only) */
+    const double *dTlev, *drF, *recip_drF, *rhoFacC, *recip_rhoFacC, *deepFacC, *recip_deepFacC, *recip_deepFac2C,
+        *recip_drC;
+    double oneSixth;
+} mj_gad_grid;
+
+typedef struct {
+    int batch;               /* members */
+    int nr;                  /* K levels per member array */
+    int64_t member_stride3d; /* elements between members' 3-D arrays (u, v, w, tracer, hFac*, gTracer) */
+    int64_t mask_member_stride; /* elements between members' maskW/maskS/maskC; 0 = shared */
+    const int32_t* nlev;     /* device [batch] active levels per member (levels >= nlev untouched), or NULL */
+} mj_gad_config;
+
+/* uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, gTracer: [B][T, K, ny, nx]; wFld: same or NULL (no R flux).
+ * work: device scratch of B*member_stride3d doubles when wFld != NULL (else may be NULL). */
+int mj_gad_dst3(const mj_gad_plan* plan, const mj_gad_grid* grid, const mj_gad_config* cfg, const double* uFld,
+                const double* vFld, const double* wFld, const double* tracer, const double* hFacW,
+                const double* hFacS, const double* recip_hFacC, double* gTracer, double* work, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MJAMD_H */

@@ -1,0 +1,119 @@
+"""ctypes binding of libmjamd.so, the HIP (gfx950) library behind every op of this package.
+
+The library is loaded from the package directory (built in-tree by `mit_jax_amd.build`). torch is imported first so
+that the library binds to the HIP runtime torch already loaded (both carry the soname libamdhip64.so.7). There is
+no fallback: if the library is missing or cannot be loaded, every op raises.
+"""
+
+import ctypes
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede the library load: one HIP runtime per process)
+
+LIB_PATH = Path(__file__).resolve().parent / "libmjamd.so"
+
+MJ_SUM_FORTRAN = 0
+MJ_SUM_TREE = 1
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int
+I64 = ctypes.c_int64
+F64 = ctypes.c_double
+
+
+class MjError(RuntimeError):
+    pass
+
+
+class HaloGeom(ctypes.Structure):
+    _fields_ = [("nouter", I64), ("outer_stride", I64), ("tile_stride", I64), ("ninner", I64),
+                ("inner_stride", I64)]
+
+
+class Cg2dDiag(ctypes.Structure):
+    _fields_ = [("firstResidual", F64), ("lastResidual", F64), ("sumRHS", F64), ("rhsMax", F64),
+                ("numIters", ctypes.c_int32), ("converged", ctypes.c_int32), ("outerIters", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
+class Cg2dOps(ctypes.Structure):
+    _fields_ = [("aW2d", P), ("aS2d", P), ("aC2d", P), ("pW", P), ("pS", P), ("pC", P), ("member_stride", I64)]
+
+
+class Cg2dConfig(ctypes.Structure):
+    _fields_ = [("batch", I32), ("ntiles", I32), ("sNx", I32), ("sNy", I32), ("OL", I32), ("member_stride", I64),
+                ("tolerance", F64), ("max_iters", I32), ("sum_order", I32), ("block_threads", I32),
+                ("grid_blocks", I32)]
+
+
+class GadGrid(ctypes.Structure):
+    _fields_ = [(k, P) for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "rA", "maskInC", "maskW",
+                                 "maskS", "maskC", "dTlev", "drF", "recip_drF", "rhoFacC", "recip_rhoFacC",
+                                 "deepFacC", "recip_deepFacC", "recip_deepFac2C", "recip_drC")] + [("oneSixth", F64)]
+
+
+class GadConfig(ctypes.Structure):
+    _fields_ = [("batch", I32), ("nr", I32), ("member_stride3d", I64), ("mask_member_stride", I64), ("nlev", P)]
+
+
+# name -> (restype, argtypes): every entry point declared in include/mjamd.h
+SIGNATURES = {
+    "mj_version": (I32, []),
+    "mj_last_error": (I32, [ctypes.c_char_p, ctypes.c_size_t]),
+    "mj_halo_plan_create": (I32, [I32, P, P, P, P, P, I32, I32, I32, ctypes.POINTER(P)]),
+    "mj_halo_plan_destroy": (I32, [P]),
+    "mj_halo_plan_inplace_ok": (I32, [P]),
+    "mj_halo_exchange": (I32, [P, P, P, ctypes.POINTER(HaloGeom), P]),
+    "mj_halo_exchange_uv": (I32, [P, P, P, P, P, ctypes.POINTER(HaloGeom), P]),
+    "mj_cg2d_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Cg2dConfig)]),
+    "mj_cg2d_solve": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, P, ctypes.POINTER(Cg2dConfig), P, ctypes.c_size_t,
+                            P, P]),
+    "mj_cg2d_matvec": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, ctypes.POINTER(Cg2dConfig), P, P]),
+    "mj_gad_plan_create": (I32, [I32, P, P, I32, I32, I32, I32, ctypes.POINTER(P)]),
+    "mj_gad_plan_destroy": (I32, [P]),
+    "mj_gad_plan_describe": (I32, [P, P, P]),
+    "mj_gad_dst3": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P, P]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises MjError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise MjError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                          "or `python -m mit_jax_amd.build`")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(2048)
+    lib().mj_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != 0:
+        raise MjError(f"{what} failed ({rc}): {last_error()}")
+
+
+def ptr(t):
+    """Device (or host) address of a tensor / ndarray, or None."""
+    if t is None:
+        return None
+    if isinstance(t, torch.Tensor):
+        return ctypes.c_void_p(t.data_ptr())
+    return t.ctypes.data_as(ctypes.c_void_p)
+
+
+def stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)

@@ -1,0 +1,98 @@
+// Shared helpers of the MI355X (gfx950) CG2D / DST3 / exch2 halo library.
+//
+// Everything here is plain FP64 arithmetic in literal order: the library is built with -ffp-contract=off so that
+// no multiply-add is fused and every result is the IEEE result of the written operation sequence. The CPU oracle
+// is built the same way; that is what makes GPU == oracle bitwise.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/mjamd.h"
+
+namespace mj {
+
+// ---------------------------------------------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+int fail_arg(const char* fmt, ...);
+
+#define MJ_HIP(call)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess) {                                                                          \
+            ::mj::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_));           \
+            return MJ_ERR_HIP;                                                                           \
+        }                                                                                                \
+    } while (0)
+
+#define MJ_REQUIRE(cond, ...)                                                                            \
+    do {                                                                                                 \
+        if (!(cond)) return ::mj::fail_arg(__VA_ARGS__);                                                 \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------------------------------- reductions
+// The fixed reduction tree of the "tile" sum order (documented in mjamd.h): n values (64 <= n <= 128) are folded
+// onto the 64 lanes of a wavefront, lane l holding v[l] + v[l+64] (v[l] alone when l+64 >= n), then a 6-step
+// butterfly w[l] = w[l] + w[l ^ m], m = 32, 16, ..., 1. Addition is commutative, so every lane ends with the same
+// bits; lane 0's value is the result. A row of 90 points and a tile of 90 row sums are both reduced this way.
+__device__ __forceinline__ double wave_tree(double lo, double hi, bool has_hi) {
+    double w = has_hi ? lo + hi : lo;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) w = w + __shfl_xor(w, m, 64);
+    return w;
+}
+
+// Sum of the per-tile partials in tile order starting from 0 (GLOBAL_SUM_TILE_RL, reference
+// mitgcm_jax/parallel/global_sum.py:20-26).
+__device__ __forceinline__ double tile_order_sum(const double* p, int ntiles) {
+    double s = 0.0;
+    for (int t = 0; t < ntiles; ++t) s = s + p[t];
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------------------- grid barrier
+// A monotonic arrival counter: barrier number k (1, 2, ...) completes when the counter reaches k * gridDim.x.
+// Only valid for a cooperative launch (all blocks co-resident). The wait is bounded by wall-clock time: a block
+// that waits longer than `limit_ticks` raises the shared error flag and every block leaves its loop at the next
+// check, so no wave can spin forever.
+struct GridSync {
+    unsigned int* counter;  // zeroed before the launch
+    int* error;             // zeroed before the launch; 1 = a barrier timed out
+    unsigned int nblocks;
+    unsigned int target;
+    unsigned long long limit_ticks;
+};
+
+__device__ __forceinline__ bool grid_sync(GridSync& g) {
+    __shared__ int s_ok;
+    __syncthreads();
+    g.target += g.nblocks;
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __threadfence();
+        atomicAdd(g.counter, 1u);
+        const unsigned long long t0 = wall_clock64();
+        while (__hip_atomic_load(g.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < g.target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (__hip_atomic_load(g.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > g.limit_ticks) {
+                atomicExch(g.error, 1);
+                ok = 0;
+                break;
+            }
+        }
+        __threadfence();
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+}  // namespace mj

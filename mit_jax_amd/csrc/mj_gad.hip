@@ -1,0 +1,464 @@
+// GAD DST3 tracer advection on gfx950.
+//
+// Horizontal part: the reference's GAD_ADVECTION with implicitAdvection (pkgs/gad.py:460-539, gad_advection.F):
+// three passes, each an X and a Y block whose action per tile comes from the facet pass table (which direction
+// sweeps, overlap-only / interior-only update regions, FILL_CS_CORNER_TR_RL corner fills before/after the flux).
+// Everything is tile-local, so one workgroup owns one (member, tile, level): the 98x98 FP64 tracer (localTij) and
+// the flux array live in LDS (2 x 76.8 KB, gfx950's 160 KB LDS), every input point is read from HBM once per sweep
+// that needs it, and only the tendency is written. The reference evaluates all six flux arrays on every tile and
+// masks; here each tile evaluates only the sweeps its table runs (2-3 of 6).
+//
+// Vertical part (R, only when wFld is given): a DST3 flux through the level interfaces applied after the
+// horizontal passes, with the same non-divergent update form. It is not in the reference (which runs the vertical
+// implicitly); its definition is written out at vert_kernel.
+#include <string.h>
+
+#include <algorithm>
+
+#include "mj_common.h"
+#include "mj_internal.h"
+
+namespace mj {
+
+constexpr int GNY = 98, GNX = 98, GPLANE = GNY * GNX;  // LLC90 tiles with OL = 4
+
+struct GadDev {
+    int B, T, K, OLx, OLy, sNx, sNy;
+    int64_t ms3, mms;
+    const MjTilePlan* tiles;
+    const double *dyG, *dxG, *recip_dxC, *recip_dyC, *recip_rA, *rA, *maskInC;
+    const double *maskW, *maskS, *maskC;
+    const double *dTlev, *drF, *recip_drF, *rhoFacC, *recip_rhoFacC, *deepFacC, *recip_deepFacC, *recip_deepFac2C,
+        *recip_drC;
+    double oneSixth;
+    const double *u, *v, *w, *tr, *hFacW, *hFacS, *recip_hFacC;
+    double* out;  // gTracer (no R flux) or localTij after the horizontal passes (R flux follows)
+    const int32_t* nlev;
+};
+
+// maskLocW / maskLocS: maskW / maskS with FILL_CS_CORNER_UV_RS(.FALSE.) applied (gad.py:263-281, :489-490).
+// (ia, ja) array indices; returns the corner-filled mask value at that point.
+__device__ __forceinline__ double mask_loc(const GadDev& a, const MjTilePlan& tp, const double* mW, const double* mS,
+                                           int ja, int ia, bool want_u) {
+    const int fi = ia - a.OLx + 1, fj = ja - a.OLy + 1;  // Fortran indices
+    const int sNx = a.sNx, sNy = a.sNy;
+    auto at = [&](const double* f, int i, int j) { return f[(j - 1 + a.OLy) * GNX + (i - 1 + a.OLx)]; };
+    if (fi <= 0 && fj <= 0 && tp.sw) {
+        const int i = 1 - fi, j = 1 - fj;
+        return want_u ? at(mS, 1 - j, 1 + i) : at(mW, 1 + j, 1 - i);
+    }
+    if (fi >= sNx + 1 && fj <= 0 && tp.se) {
+        const int i = fi - sNx, j = 1 - fj;
+        if (want_u) return i >= 2 ? at(mS, sNx + j, i) : at(mW, fi, fj);
+        return at(mW, sNx + 1 - j, 1 - i);
+    }
+    if (fi <= 0 && fj >= sNy + 1 && tp.nw) {
+        const int i = 1 - fi, j = fj - sNy;
+        if (want_u) return at(mS, 1 - j, sNy + 1 - i);
+        return j >= 2 ? at(mW, j, sNy + i) : at(mS, fi, fj);
+    }
+    if (fi >= sNx + 1 && fj >= sNy + 1 && tp.ne) {
+        const int i = fi - sNx, j = fj - sNy;
+        if (want_u) return i >= 2 ? at(mS, sNx + j, sNy + 2 - i) : at(mW, fi, fj);
+        return j >= 2 ? at(mW, sNx + 2 - j, sNy + i) : at(mS, fi, fj);
+    }
+    return want_u ? at(mW, fi, fj) : at(mS, fi, fj);
+}
+
+// FILL_CS_CORNER_TR_RL(fill4dir, withSigns=.FALSE.) on the LDS tile (gad.py:241-260): for each corner present and
+// i, j = 1..OL, dst <- src; no source is a destination, so all reads precede all writes within one call.
+__device__ __forceinline__ void corner_fill(double* loc, const GadDev& a, const MjTilePlan& tp, int dir, int tid) {
+    const int OL = a.OLx;  // OLx == OLy checked on the host
+    const int n = 4 * OL * OL;
+    double v = 0.0;
+    int dst = -1;
+    if (tid < n) {
+        const int c = tid / (OL * OL), r = tid - c * OL * OL;
+        const int j = r / OL + 1, i = r % OL + 1;
+        const int sNx = a.sNx, sNy = a.sNy;
+        const bool on = c == 0 ? tp.sw : c == 1 ? tp.se : c == 2 ? tp.nw : tp.ne;
+        if (on) {
+            int di, dj, si, sj;
+            if (dir == 1) {
+                if (c == 0) { di = 1 - i; dj = 1 - j; si = 1 - j; sj = i; }
+                else if (c == 1) { di = sNx + i; dj = 1 - j; si = sNx + j; sj = i; }
+                else if (c == 2) { di = 1 - i; dj = sNy + j; si = 1 - j; sj = sNy + 1 - i; }
+                else { di = sNx + i; dj = sNy + j; si = sNx + j; sj = sNy + 1 - i; }
+            } else {
+                if (c == 0) { di = 1 - i; dj = 1 - j; si = j; sj = 1 - i; }
+                else if (c == 1) { di = sNx + i; dj = 1 - j; si = sNx + 1 - j; sj = 1 - i; }
+                else if (c == 2) { di = 1 - i; dj = sNy + j; si = j; sj = sNy + i; }
+                else { di = sNx + i; dj = sNy + j; si = sNx + 1 - j; sj = sNy + i; }
+            }
+            dst = (dj - 1 + a.OLy) * GNX + (di - 1 + a.OLx);
+            v = loc[(sj - 1 + a.OLy) * GNX + (si - 1 + a.OLx)];
+        }
+    }
+    __syncthreads();
+    if (dst >= 0) loc[dst] = v;
+    __syncthreads();
+}
+
+// DST3 interface value (gad_dst3_adv_x.F:81-117 via gad.py:420-428): t_m2, t_m1 upstream side for positive
+// transport, t_0, t_p1 downstream; m_p1, m_0, m_m1 the face masks of Rjp, Rj, Rjm.
+__device__ __forceinline__ double dst3_flux(double tr_, double vel_cfl_abs, double oneSixth, double t_m2, double t_m1,
+                                            double t_0, double t_p1, double m_m1, double m_0, double m_p1) {
+    const double Rjp = (t_p1 - t_0) * m_p1;
+    const double Rj = (t_0 - t_m1) * m_0;
+    const double Rjm = (t_m1 - t_m2) * m_m1;
+    const double cfl = vel_cfl_abs;
+    const double d0 = (2. - cfl) * (1. - cfl) * oneSixth;
+    const double d1 = (1. - cfl * cfl) * oneSixth;
+    return 0.5 * (tr_ + fabs(tr_)) * (t_m1 + (d0 * Rj + d1 * Rjm)) + 0.5 * (tr_ - fabs(tr_)) * (t_0 - (d0 * Rj + d1 * Rjp));
+}
+
+__global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
+    __shared__ double loc[GPLANE];
+    __shared__ double af[GPLANE];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int k = blockIdx.x % a.K;
+    const int mt = blockIdx.x / a.K;
+    const int t = mt % a.T, m = mt / a.T;
+    if (a.nlev && k >= a.nlev[m]) return;
+    const MjTilePlan& tp = a.tiles[t];
+    const int64_t b3 = m * a.ms3 + ((int64_t)t * a.K + k) * GPLANE;
+    const int64_t bm = m * a.mms + ((int64_t)t * a.K + k) * GPLANE;
+    const int64_t b2 = (int64_t)t * GPLANE;
+    const double* tr = a.tr + b3;
+    const double* u = a.u + b3;
+    const double* v = a.v + b3;
+    const double* hW = a.hFacW + b3;
+    const double* hS = a.hFacS + b3;
+    const double* rhC = a.recip_hFacC + b3;
+    const double* mW = a.maskW + bm;
+    const double* mS = a.maskS + bm;
+    const double *dyG = a.dyG + b2, *dxG = a.dxG + b2, *rdxC = a.recip_dxC + b2, *rdyC = a.recip_dyC + b2;
+    const double *rrA = a.recip_rA + b2, *mInC = a.maskInC + b2;
+    const double dt = a.dTlev[k];
+    const double deepFacC = a.deepFacC[k], rdeepC = a.recip_deepFacC[k], rdeep2C = a.recip_deepFac2C[k];
+    const double rhoFacC = a.rhoFacC[k], rrhoFacC = a.recip_rhoFacC[k], drF = a.drF[k], rdrF = a.recip_drF[k];
+    const double oneSixth = a.oneSixth;
+
+    for (int q = tid; q < GPLANE; q += nthr) loc[q] = tr[q];
+    __syncthreads();
+
+    // uTrans = uFld * (dyG*deepFacC*drF*hFacW) * rhoFacC (gad.py:482-485), left to right
+    auto uTrans = [&](int q) { return u[q] * (dyG[q] * deepFacC * drF * hW[q]) * rhoFacC; };
+    auto vTrans = [&](int q) { return v[q] * (dxG[q] * deepFacC * drF * hS[q]) * rhoFacC; };
+    // update coefficient dt*recip_rhoFacC*recip_hFacC*recip_drF*recip_rA*recip_deepFac2C (gad.py:494)
+    auto coef = [&](int q) { return dt * rrhoFacC * rhC[q] * rdrF * rrA[q] * rdeep2C; };
+
+    for (int pass = 0; pass < 3; ++pass) {
+        for (int dir = 0; dir < 2; ++dir) {
+            const MjSweep& S = tp.sweep[pass][dir];
+            if (!S.run) continue;
+            if (S.fill_before) corner_fill(loc, a, tp, S.fill_before, tid);
+            if (S.nbox) {
+                for (int q = tid; q < GPLANE; q += nthr) {
+                    const int j = q / GNX, i = q - j * GNX;
+                    double f = 0.0;
+                    if (dir == 0) {
+                        if (i >= 2 && i <= GNX - 2) {  // i = 1-OLx+2 .. sNx+OLx-1 (gad_dst3_adv_x.F:80)
+                            const double cfl = fabs(u[q] * dt * rdxC[q] * rdeepC);
+                            f = dst3_flux(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1],
+                                          mask_loc(a, tp, mW, mS, j, i - 1, true), mask_loc(a, tp, mW, mS, j, i, true),
+                                          mask_loc(a, tp, mW, mS, j, i + 1, true));
+                        }
+                    } else {
+                        if (j >= 2 && j <= GNY - 2) {  // j = 1-OLy+2 .. sNy+OLy-1 (gad_dst3_adv_y.F:78)
+                            const double cfl = fabs(v[q] * dt * rdyC[q] * rdeepC);
+                            f = dst3_flux(vTrans(q), cfl, oneSixth, loc[q - 2 * GNX], loc[q - GNX], loc[q],
+                                          loc[q + GNX], mask_loc(a, tp, mW, mS, j - 1, i, false),
+                                          mask_loc(a, tp, mW, mS, j, i, false), mask_loc(a, tp, mW, mS, j + 1, i, false));
+                        }
+                    }
+                    af[q] = f;
+                }
+                __syncthreads();
+            }
+            if (S.fill_after) corner_fill(loc, a, tp, S.fill_after, tid);
+            for (int bx = 0; bx < S.nbox; ++bx) {
+                const MjBox B = S.box[bx];
+                const int bw = B.i1 - B.i0, n = bw * (B.j1 - B.j0);
+                for (int idx = tid; idx < n; idx += nthr) {
+                    const int q = (B.j0 + idx / bw) * GNX + B.i0 + idx % bw;
+                    // local - coef*(d(af) - tracer*d(trans))*maskInC (gad.py:511-514)
+                    double daf, dtr;
+                    if (dir == 0) {
+                        daf = af[q + 1] - af[q];
+                        dtr = uTrans(q + 1) - uTrans(q);
+                    } else {
+                        daf = af[q + GNX] - af[q];
+                        dtr = vTrans(q + GNX) - vTrans(q);
+                    }
+                    loc[q] = loc[q] - coef(q) * (daf - tr[q] * dtr) * mInC[q];
+                }
+            }
+            if (S.nbox) __syncthreads();
+        }
+    }
+    double* out = a.out + b3;
+    if (a.w) {
+        for (int q = tid; q < GPLANE; q += nthr) out[q] = loc[q];
+    } else {
+        for (int q = tid; q < GPLANE; q += nthr) out[q] = (loc[q] - tr[q]) / dt;  // gad.py:539
+    }
+}
+
+// Vertical DST3 (R) sweep after the horizontal passes, per column (member, tile, j, i), k = 0..nlev-1 downward.
+// Interface k (0..K) is the top of cell k; w(k) = wFld at interface k, positive upward; the transport is
+// rTrans(k) = w(k) * rA * mk(k), mk(k) = maskC(k-1) * maskC(k) for 1 <= k <= K-1, and 0 at the surface (k = 0)
+// and the bottom (k = K). The interface value is the DST3 formula with the upstream side below for rTrans > 0:
+//   Rj = (t(k-1) - t(k)) mk(k), Rjm = (t(k) - t(k+1)) mk(k+1), Rjp = (t(k-2) - t(k-1)) mk(k-1),
+//   c = |w(k) dt(k) recip_drC(k)|, d0 = (2-c)(1-c)/6, d1 = (1-c^2)/6,
+//   wT(k) = (rT+|rT|)/2 (t(k) + d0 Rj + d1 Rjm) + (rT-|rT|)/2 (t(k-1) - (d0 Rj + d1 Rjp)),
+// with t = localTij after the horizontal passes. Update (the horizontal form, net outflow wT(k) - wT(k+1)):
+//   local(k) <- local(k) - coef(k) ((wT(k) - wT(k+1)) - tracer(k) (rTrans(k) - rTrans(k+1))) maskC(k),
+// gTracer = (local - tracer)/dt(k) on the whole tile array.
+__global__ void __launch_bounds__(256) gad_vert_kernel(GadDev a, const double* loc_h) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mt = blockIdx.y;
+    if (q >= GPLANE) return;
+    const int t = mt % a.T, m = mt / a.T;
+    const int K = a.K;
+    const int kmax = a.nlev ? a.nlev[m] : K;
+    const int64_t b2 = (int64_t)t * GPLANE + q;
+    const double rA = a.rA[b2], rrA = a.recip_rA[b2];
+    auto off3 = [&](int k) { return m * a.ms3 + ((int64_t)t * K + k) * GPLANE + q; };
+    auto offm = [&](int k) { return m * a.mms + ((int64_t)t * K + k) * GPLANE + q; };
+    auto T_ = [&](int k) { return (k >= 0 && k < kmax) ? loc_h[off3(k)] : 0.0; };
+    auto mC = [&](int k) { return (k >= 0 && k < kmax) ? a.maskC[offm(k)] : 0.0; };
+    auto mk = [&](int k) { return (k >= 1 && k <= kmax - 1) ? mC(k - 1) * mC(k) : 0.0; };
+    auto rTr = [&](int k) { return (k >= 1 && k <= kmax - 1) ? a.w[off3(k)] * rA * mk(k) : 0.0; };
+    auto wT = [&](int k) {
+        if (k < 1 || k > kmax - 1) return 0.0;
+        const double cfl = fabs(a.w[off3(k)] * a.dTlev[k] * a.recip_drC[k]);
+        // upstream for positive (upward) transport: cell k, then k+1; downstream: k-1, then k-2
+        return dst3_flux(rTr(k), cfl, a.oneSixth, T_(k + 1), T_(k), T_(k - 1), T_(k - 2), mk(k + 1), mk(k), mk(k - 1));
+    };
+    double wTop = wT(0), rTop = rTr(0);
+    for (int k = 0; k < kmax; ++k) {
+        const double wBot = wT(k + 1), rBot = rTr(k + 1);
+        const int64_t o = off3(k);
+        const double tr = a.tr[o];
+        const double cf = a.dTlev[k] * a.recip_rhoFacC[k] * a.recip_hFacC[o] * a.recip_drF[k] * rrA * a.recip_deepFac2C[k];
+        const double loc = loc_h[o] - cf * ((wTop - wBot) - tr * (rTop - rBot)) * mC(k);
+        a.out[o] = (loc - tr) / a.dTlev[k];
+        wTop = wBot;
+        rTop = rBot;
+    }
+}
+
+}  // namespace mj
+
+using namespace mj;
+
+mj_gad_plan::~mj_gad_plan() {
+    if (d_tiles) (void)hipFree(d_tiles);
+}
+
+// ------------------------------------------------------------------------------------------------ plan (host)
+// Per-facet pass table of gad_advection.F:333-351 (reference gad.py:217-235) and the update regions / fill calls
+// of :376-774 (gad.py:339-375), as boxes. Array index of Fortran i is i - 1 + OLx.
+static void pass_table(int face, int pass, bool* overlapOnly, bool* interiorOnly, bool* cX, bool* cY) {
+    auto in = [&](int a, int b, int c) { return face == a || face == b || face == c; };
+    if (pass == 1) {
+        *overlapOnly = face % 3 == 0;
+        *interiorOnly = face % 3 != 0;
+        *cX = in(6, 1, 2);
+        *cY = in(3, 4, 5);
+    } else if (pass == 2) {
+        *overlapOnly = face % 3 == 2;
+        *interiorOnly = face % 3 == 1;
+        *cX = in(2, 3, 4);
+        *cY = in(5, 6, 1);
+    } else {
+        *overlapOnly = false;
+        *interiorOnly = true;
+        *cX = face == 5 || face == 6;
+        *cY = face == 2 || face == 3;
+    }
+}
+
+extern "C" int mj_gad_plan_create(int ntiles, const int32_t* face, const uint8_t* edges, int sNx, int sNy, int OLx,
+                                  int OLy, mj_gad_plan** out) {
+    MJ_REQUIRE(out && face && edges && ntiles > 0, "mj_gad_plan_create: bad argument");
+    MJ_REQUIRE(sNx + 2 * OLx == GNX && sNy + 2 * OLy == GNY && OLx == OLy,
+               "mj_gad_plan_create: the kernels are built for %dx%d tiles with equal halos (got sNx %d sNy %d OL %d,%d)",
+               GNY, GNX, sNx, sNy, OLx, OLy);
+    mj_gad_plan* P = new mj_gad_plan();
+    P->ntiles = ntiles;
+    P->sNx = sNx;
+    P->sNy = sNy;
+    P->OLx = OLx;
+    P->OLy = OLy;
+    P->tiles.resize(ntiles);
+    auto ix = [&](int i) { return (int16_t)(i - 1 + OLx); };
+    auto jx = [&](int j) { return (int16_t)(j - 1 + OLy); };
+    for (int t = 0; t < ntiles; ++t) {
+        const bool N = edges[4 * t] != 0, S = edges[4 * t + 1] != 0, E = edges[4 * t + 2] != 0,
+                   W = edges[4 * t + 3] != 0;
+        MjTilePlan& tp = P->tiles[t];
+        memset(&tp, 0, sizeof(tp));
+        tp.sw = W && S;  // fill_cs_corner_tr_rl.F:78-85
+        tp.se = E && S;
+        tp.nw = W && N;
+        tp.ne = E && N;
+        for (int pass = 1; pass <= 3; ++pass) {
+            bool ov, io, cX, cY;
+            pass_table(face[t], pass, &ov, &io, &cX, &cY);
+            MjSweep& X = tp.sweep[pass - 1][0];
+            MjSweep& Y = tp.sweep[pass - 1][1];
+            if (cX) {  // gad_advection.F:376-565
+                const bool doflux = !ov || N || S;
+                X.fill_before = (doflux && ov) ? 1 : 0;
+                X.fill_after = (doflux && ov && pass == 1) ? 2 : 0;
+                if (ov) {
+                    int iMin = 1 - OLx + 1, iMax = sNx + OLx - 1;
+                    if (W) iMin = 1;
+                    if (E) iMax = sNx;
+                    if (S) X.box[X.nbox++] = MjBox{ix(iMin), (int16_t)(ix(iMax) + 1), jx(1 - OLy), (int16_t)(jx(0) + 1)};
+                    if (N)
+                        X.box[X.nbox++] =
+                            MjBox{ix(iMin), (int16_t)(ix(iMax) + 1), jx(sNy + 1), (int16_t)(jx(sNy + OLy) + 1)};
+                } else {
+                    int jMin = 1 - OLy, jMax = sNy + OLy;
+                    if (io && S) jMin = 1;
+                    if (io && N) jMax = sNy;
+                    X.box[X.nbox++] =
+                        MjBox{ix(1 - OLx + 1), (int16_t)(ix(sNx + OLx - 1) + 1), jx(jMin), (int16_t)(jx(jMax) + 1)};
+                }
+                X.run = X.nbox > 0 || X.fill_before || X.fill_after;
+            }
+            if (cY) {  // gad_advection.F:585-774
+                const bool doflux = !ov || E || W;
+                Y.fill_before = (doflux && ov) ? 2 : 0;
+                Y.fill_after = (doflux && ov && pass == 1) ? 1 : 0;
+                if (ov) {
+                    int jMin = 1 - OLy + 1, jMax = sNy + OLy - 1;
+                    if (S) jMin = 1;
+                    if (N) jMax = sNy;
+                    if (W) Y.box[Y.nbox++] = MjBox{ix(1 - OLx), (int16_t)(ix(0) + 1), jx(jMin), (int16_t)(jx(jMax) + 1)};
+                    if (E)
+                        Y.box[Y.nbox++] =
+                            MjBox{ix(sNx + 1), (int16_t)(ix(sNx + OLx) + 1), jx(jMin), (int16_t)(jx(jMax) + 1)};
+                } else {
+                    int iMin = 1 - OLx, iMax = sNx + OLx;
+                    if (io && W) iMin = 1;
+                    if (io && E) iMax = sNx;
+                    Y.box[Y.nbox++] =
+                        MjBox{ix(iMin), (int16_t)(ix(iMax) + 1), jx(1 - OLy + 1), (int16_t)(jx(sNy + OLy - 1) + 1)};
+                }
+                Y.run = Y.nbox > 0 || Y.fill_before || Y.fill_after;
+            }
+        }
+    }
+    hipError_t e = hipMalloc((void**)&P->d_tiles, sizeof(MjTilePlan) * ntiles);
+    if (e == hipSuccess)
+        e = hipMemcpy(P->d_tiles, P->tiles.data(), sizeof(MjTilePlan) * ntiles, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete P;
+        set_error("mj_gad_plan_create: %s", hipGetErrorString(e));
+        return MJ_ERR_HIP;
+    }
+    *out = P;
+    return MJ_OK;
+}
+
+extern "C" int mj_gad_plan_destroy(mj_gad_plan* plan) {
+    delete plan;
+    return MJ_OK;
+}
+
+extern "C" int mj_gad_plan_describe(const mj_gad_plan* P, uint8_t* masks, int32_t* fills) {
+    MJ_REQUIRE(P, "mj_gad_plan_describe: null plan");
+    const int T = P->ntiles, plane = GPLANE;
+    if (masks) memset(masks, 0, (size_t)6 * T * plane);
+    for (int t = 0; t < T; ++t)
+        for (int pass = 0; pass < 3; ++pass)
+            for (int dir = 0; dir < 2; ++dir) {
+                const MjSweep& S = P->tiles[t].sweep[pass][dir];
+                if (masks)
+                    for (int b = 0; b < S.nbox; ++b)
+                        for (int j = S.box[b].j0; j < S.box[b].j1; ++j)
+                            for (int i = S.box[b].i0; i < S.box[b].i1; ++i)
+                                masks[((size_t)(pass * 2 + dir) * T + t) * plane + j * GNX + i] = 1;
+                if (fills) {
+                    fills[(pass * 4 + dir * 2 + 0) * T + t] = S.fill_before;
+                    fills[(pass * 4 + dir * 2 + 1) * T + t] = S.fill_after;
+                }
+            }
+    return MJ_OK;
+}
+
+extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c, const double* uFld,
+                           const double* vFld, const double* wFld, const double* tracer, const double* hFacW,
+                           const double* hFacS, const double* recip_hFacC, double* gTracer, double* work,
+                           void* stream) {
+    MJ_REQUIRE(P && g && c, "mj_gad_dst3: null plan/grid/config");
+    MJ_REQUIRE(uFld && vFld && tracer && hFacW && hFacS && recip_hFacC && gTracer, "mj_gad_dst3: null field");
+    MJ_REQUIRE(g->dyG && g->dxG && g->recip_dxC && g->recip_dyC && g->recip_rA && g->maskInC && g->maskW && g->maskS,
+               "mj_gad_dst3: null grid field");
+    MJ_REQUIRE(g->dTlev && g->drF && g->recip_drF && g->rhoFacC && g->recip_rhoFacC && g->deepFacC &&
+                   g->recip_deepFacC && g->recip_deepFac2C,
+               "mj_gad_dst3: null vertical grid field");
+    MJ_REQUIRE(c->batch >= 1 && c->nr >= 1, "mj_gad_dst3: batch and nr must be >= 1");
+    MJ_REQUIRE(c->member_stride3d >= (int64_t)P->ntiles * c->nr * GPLANE || c->batch == 1,
+               "mj_gad_dst3: member_stride3d too small");
+    MJ_REQUIRE((int64_t)c->batch * P->ntiles * c->nr < (1ll << 31), "mj_gad_dst3: too many (member, tile, level)s");
+    if (wFld) {
+        MJ_REQUIRE(work && g->maskC && g->rA && g->recip_drC, "mj_gad_dst3: the R flux needs work, maskC, rA, recip_drC");
+        MJ_REQUIRE(work != gTracer, "mj_gad_dst3: work must not alias gTracer");
+    }
+    GadDev a{};
+    a.B = c->batch;
+    a.T = P->ntiles;
+    a.K = c->nr;
+    a.OLx = P->OLx;
+    a.OLy = P->OLy;
+    a.sNx = P->sNx;
+    a.sNy = P->sNy;
+    a.ms3 = c->member_stride3d;
+    a.mms = c->mask_member_stride;
+    a.tiles = P->d_tiles;
+    a.dyG = g->dyG;
+    a.dxG = g->dxG;
+    a.recip_dxC = g->recip_dxC;
+    a.recip_dyC = g->recip_dyC;
+    a.recip_rA = g->recip_rA;
+    a.rA = g->rA;
+    a.maskInC = g->maskInC;
+    a.maskW = g->maskW;
+    a.maskS = g->maskS;
+    a.maskC = g->maskC;
+    a.dTlev = g->dTlev;
+    a.drF = g->drF;
+    a.recip_drF = g->recip_drF;
+    a.rhoFacC = g->rhoFacC;
+    a.recip_rhoFacC = g->recip_rhoFacC;
+    a.deepFacC = g->deepFacC;
+    a.recip_deepFacC = g->recip_deepFacC;
+    a.recip_deepFac2C = g->recip_deepFac2C;
+    a.recip_drC = g->recip_drC;
+    a.oneSixth = g->oneSixth;
+    a.u = uFld;
+    a.v = vFld;
+    a.w = wFld;
+    a.tr = tracer;
+    a.hFacW = hFacW;
+    a.hFacS = hFacS;
+    a.recip_hFacC = recip_hFacC;
+    a.out = wFld ? work : gTracer;
+    a.nlev = c->nlev;
+    hipStream_t st = as_stream(stream);
+    const unsigned nblk = (unsigned)(c->batch * P->ntiles * c->nr);
+    hipLaunchKernelGGL(gad_hz_kernel, dim3(nblk), dim3(1024), 0, st, a);
+    MJ_HIP(hipGetLastError());
+    if (wFld) {
+        a.out = gTracer;
+        hipLaunchKernelGGL(gad_vert_kernel, dim3((GPLANE + 255) / 256, (unsigned)(c->batch * P->ntiles)), dim3(256), 0,
+                           st, a, (const double*)work);
+        MJ_HIP(hipGetLastError());
+    }
+    return MJ_OK;
+}

@@ -1,0 +1,216 @@
+"""exch2 halo exchanges on the GPU, with the reference's Exchanger interface (parallel/exchange.py:145-222).
+
+`ExchangeMaps` holds the probed exch2 maps in compact form (the written points of each map: dst, src, comp, sign;
+`data/exch2_halo_13x90x90.npz`, made by `data/make_halo_tables.py` from the reference's map file).
+`Exchanger` runs them through libmjamd (mj_halo_exchange / mj_halo_exchange_uv) on CUDA float64 tensors
+[tile, ..., j, i] (any number of leading levels) and returns new tensors, like the reference's functional gathers.
+Kinds: T, 3D, Z, SMs (scalar) and UVs, UVn, UV3s, As, An, Bs, Bn, Ds (vector).
+"""
+
+import ctypes
+import threading
+from dataclasses import dataclass
+from functools import lru_cache
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _lib
+from .layout import Layout
+
+DATA = Path(__file__).resolve().parent / "data" / "exch2_halo_13x90x90.npz"
+SOURCE_SHA256 = "a31f0d49754259c3b1304c42f6734886605c0f4f1f72257b0654e8486a9d84fd"
+SCALAR_KINDS = ("T", "3D", "Z", "SMs")
+VECTOR_KINDS = ("UVs", "UVn", "As", "An", "Bs", "Bn", "Ds", "UV3s")
+
+
+@dataclass(frozen=True)
+class ExchangeMaps:
+    layout: Layout
+    compact: dict  # map name -> (dst int32, src int32, comp int8, sign int8), written points in dst order
+
+    @classmethod
+    def load(cls, path=DATA):
+        return _load_maps(str(path))
+
+    def full(self, name):
+        """The map over every padded point, as the reference stores it: (src int32, comp int8, sign int8);
+        unwritten points keep their own value (src = own index, comp 0, sign +1)."""
+        L = self.layout
+        n = L.nTiles * L.ny * L.nx
+        d, s, c, g = self.compact[name]
+        src = np.arange(n, dtype=np.int32)
+        comp = np.zeros(n, np.int8)
+        sign = np.ones(n, np.int8)
+        src[d], comp[d], sign[d] = s, c, g
+        return src, comp, sign
+
+    def names(self):
+        return sorted(self.compact)
+
+
+@lru_cache(maxsize=4)
+def _load_maps(path):
+    z = np.load(path, allow_pickle=False)
+    if str(z["source_sha256"]) != SOURCE_SHA256:
+        raise ValueError(f"{path}: not made from the known exch2 map file")
+    sNx, sNy, OLx, OLy, nT, Nr = (int(v) for v in z["layout"])
+    names = sorted({k.rsplit(".", 1)[0] for k in z.files if k not in ("layout", "source_sha256")})
+    comp = {n: tuple(np.ascontiguousarray(z[n + s]) for s in (".dst", ".src", ".comp", ".sign")) for n in names}
+    return ExchangeMaps(Layout(sNx, sNy, OLx, OLy, nT, Nr), comp)
+
+
+class HaloPlan:
+    """A device halo plan (mj_halo_plan) of one kind on one device."""
+
+    def __init__(self, maps: ExchangeMaps, kind, device):
+        L = maps.layout
+        names = [kind] if kind in SCALAR_KINDS else [kind + "_u", kind + "_v"]
+        arrs = [maps.compact[n] for n in names]
+        nm = len(arrs)
+        npts = (ctypes.c_int32 * nm)(*[len(a[0]) for a in arrs])
+        ptrs = [(ctypes.c_void_p * nm)(*[a[i].ctypes.data for a in arrs]) for i in range(4)]
+        self._keep = arrs
+        self.kind = kind
+        self.device = device
+        self.vector = nm == 2
+        h = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            _lib.check(_lib.lib().mj_halo_plan_create(nm, npts, *[ctypes.cast(p, ctypes.c_void_p) for p in ptrs],
+                                                      L.nTiles, L.ny, L.nx, ctypes.byref(h)), "mj_halo_plan_create")
+        self.handle = h
+        self.inplace_ok = bool(_lib.lib().mj_halo_plan_inplace_ok(h))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib._lib is not None:
+            _lib.lib().mj_halo_plan_destroy(h)
+
+
+_plans = {}
+_plans_lock = threading.Lock()
+
+
+def halo_plan(maps: ExchangeMaps, kind, device):
+    """Cached device plan of `kind` for `device` (a torch.device or index)."""
+    dev = torch.device(device)
+    idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    key = (id(maps), kind, idx)
+    with _plans_lock:
+        p = _plans.get(key)
+        if p is None:
+            p = _plans[key] = HaloPlan(maps, kind, idx)
+        return p
+
+
+def _geom(a, L):
+    """mj_halo_geom of a contiguous [T, ..., ny, nx] or [B, T, ..., ny, nx] tensor whose tile axis is `tile_axis`."""
+    plane = L.ny * L.nx
+    inner = int(np.prod(a.shape[1:-2])) if a.dim() > 3 else 1
+    return _lib.HaloGeom(1, a.numel(), inner * plane, inner, plane)
+
+
+def _require(a, L, what):
+    if not isinstance(a, torch.Tensor) or a.device.type != "cuda":
+        raise TypeError(f"{what}: expected a CUDA tensor (the exchange runs on the GPU only)")
+    if a.dtype != torch.float64:
+        raise TypeError(f"{what}: expected float64, got {a.dtype}")
+    if a.dim() < 3 or a.shape[0] != L.nTiles or tuple(a.shape[-2:]) != (L.ny, L.nx):
+        raise ValueError(f"{what}: expected [{L.nTiles}, ..., {L.ny}, {L.nx}], got {tuple(a.shape)}")
+    return a.contiguous()
+
+
+class Exchanger:
+    """Halo exchanges on [tile, ..., j, i] CUDA float64 tensors (reference parallel/exchange.py:145-217)."""
+
+    def __init__(self, maps: ExchangeMaps = None):
+        self.maps = maps or ExchangeMaps.load()
+        self.L = self.maps.layout
+
+    def plan(self, kind, device):
+        return halo_plan(self.maps, kind, device)
+
+    def scalar(self, a, kind="T", out=None):
+        a = _require(a, self.L, f"scalar({kind})")
+        p = self.plan(kind, a.device)
+        if p.vector:
+            raise ValueError(f"{kind} is a vector kind")
+        out = a.clone() if out is None else out
+        g = _geom(a, self.L)
+        _lib.check(_lib.lib().mj_halo_exchange(p.handle, _lib.ptr(a), _lib.ptr(out), ctypes.byref(g),
+                                               _lib.stream_ptr()), "mj_halo_exchange")
+        return out
+
+    def vector(self, u, v, kind="UVs"):
+        u = _require(u, self.L, f"vector({kind}) u")
+        v = _require(v, self.L, f"vector({kind}) v")
+        if u.shape != v.shape:
+            raise ValueError("u and v shapes differ")
+        p = self.plan(kind, u.device)
+        if not p.vector:
+            raise ValueError(f"{kind} is a scalar kind")
+        uo, vo = u.clone(), v.clone()
+        g = _geom(u, self.L)
+        _lib.check(_lib.lib().mj_halo_exchange_uv(p.handle, _lib.ptr(u), _lib.ptr(v), _lib.ptr(uo), _lib.ptr(vo),
+                                                  ctypes.byref(g), _lib.stream_ptr()), "mj_halo_exchange_uv")
+        return uo, vo
+
+    def fill_(self, a, kind="T"):
+        """In-place halo fill (only for kinds whose written points are never sources: T, 3D, UVs, UVn, UV3s)."""
+        a = _require(a, self.L, f"fill_({kind})")
+        p = self.plan(kind, a.device)
+        g = _geom(a, self.L)
+        _lib.check(_lib.lib().mj_halo_exchange(p.handle, _lib.ptr(a), _lib.ptr(a), ctypes.byref(g),
+                                               _lib.stream_ptr()), "mj_halo_exchange")
+        return a
+
+    # Fortran names (parallel/exchange.py:181-195)
+    def exch_xy(self, a):  # EXCH_XY_RL / EXCH_XYZ_RL / EXCH_3D_RL
+        return self.scalar(a, "T")
+
+    def exch_uv_xy(self, u, v, with_signs=True):  # EXCH_UV_XY_RL / EXCH_UV_XYZ_RL
+        return self.vector(u, v, "UVs" if with_signs else "UVn")
+
+    def exch_z(self, a):  # EXCH_Z_3D_RL
+        return self.scalar(a, "Z")
+
+    def exch_uv_agrid(self, u, v, with_signs=True):
+        return self.vector(u, v, "As" if with_signs else "An")
+
+    def exch_uv_bgrid(self, u, v, with_signs=True):
+        return self.vector(u, v, "Bs" if with_signs else "Bn")
+
+    # global reductions over tiles (parallel/exchange.py:199-209): one device holds every tile
+    def all_tiles(self, per_tile):
+        return per_tile
+
+    def global_sum_tile(self, phiTile):
+        """GLOBAL_SUM_TILE_RL: 0 + tile 1 + tile 2 + ... in tile order (parallel/global_sum.py:20-26)."""
+        s = torch.zeros(phiTile.shape[1:], dtype=phiTile.dtype, device=phiTile.device)
+        for t in range(phiTile.shape[0]):
+            s = s + phiTile[t]
+        return s
+
+    def global_max(self, a):
+        return torch.max(a)
+
+    def vary(self, x):
+        return x
+
+    def zero_padding(self, a):
+        return a
+
+
+_default = {}
+
+
+def default_exchanger(layout=None):
+    """The LLC90 13-tile exchanger (reference exchange.py:220-222)."""
+    key = layout
+    if key not in _default:
+        ex = Exchanger(ExchangeMaps.load())
+        if layout is not None and layout != ex.L:
+            raise ValueError(f"no exch2 maps for {layout}")
+        _default[key] = ex
+    return _default[key]

@@ -1,0 +1,233 @@
+"""GAD DST3 tracer advection on the GPU, with the reference's interface (mitgcm_jax pkgs/gad.py).
+
+`gad_advection(params, g, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC) -> gTracer` (reference
+gad.py:546-560) and `advect_tiles(params, tables, grid2d, vgrid, ...)` (:460-539) run GAD_ADVECTION's three-pass
+multi-dimensional DST3 on the LLC cubed sphere in one HIP launch (libmjamd mj_gad_dst3): one workgroup per (tile,
+level) keeps the tile in LDS through the per-facet X/Y sweeps, corner fills and update regions, and writes
+gTracer = (localTij - tracer)/deltaTLev(k) on the whole tile array. Fields are CUDA float64 [T, K, ny, nx], or
+[B, T, K, ny, nx] for an ensemble in the same launch (optional per-member level counts).
+
+`implicitAdvection=True` is the reference branch (vertical advection left to the implicit solver, wFld not read).
+`implicitAdvection=False` adds the vertical DST3 flux (R) of wFld after the horizontal passes, a formulation this
+library defines (the reference raises NotImplementedError for it, gad.py:164-166); see mj_gad.hip.
+`dst3_adv_x` / `dst3_adv_y` (gad.py:409-453) are not exposed separately: the fluxes exist only inside the fused
+kernel.
+"""
+
+import ctypes
+from dataclasses import dataclass, replace
+
+import numpy as np
+import torch
+
+from . import _lib
+from .layout import LLC90, Layout
+
+ENUM_DST3 = 30
+ONE_SIXTH = 1.0 / 6.0
+LLC90_FACETS = (90, 270, 90, 270, 90, 90, 270, 90, 270, 90)
+
+
+@dataclass(frozen=True)
+class TileTopology:
+    """exch2_myFace and facet-edge flags per tile (reference gad.py:75-106, w2_set_map_tiles.F:161-178,
+    w2_set_tile2tiles.F:88-110)."""
+    face: tuple
+    N_edge: tuple
+    S_edge: tuple
+    E_edge: tuple
+    W_edge: tuple
+
+    @classmethod
+    def from_facets(cls, dims_facets=LLC90_FACETS, layout=LLC90):
+        L = layout
+        face, N, S, E, W = [], [], [], [], []
+        for f in range(len(dims_facets) // 2):
+            fNx, fNy = dims_facets[2 * f], dims_facets[2 * f + 1]
+            for ty in range(fNy // L.sNy):
+                for tx in range(fNx // L.sNx):
+                    face.append(f + 1)
+                    N.append(not (ty + 1) * L.sNy < fNy)
+                    S.append(ty == 0)
+                    E.append(not (tx + 1) * L.sNx < fNx)
+                    W.append(tx == 0)
+        if len(face) != L.nTiles:
+            raise ValueError(f"facets give {len(face)} tiles, layout has {L.nTiles}")
+        return cls(tuple(face), tuple(N), tuple(S), tuple(E), tuple(W))
+
+
+@dataclass(frozen=True)
+class GADParams:
+    """One GAD_ADVECTION call (reference gad.py:124-187). Per-level tuples of Nr values."""
+    topo: TileTopology
+    layout: Layout = LLC90
+    dTtracerLev: tuple = (3600.0,) * 50
+    deepFacC: tuple = (1.0,) * 50
+    recip_deepFacC: tuple = (1.0,) * 50
+    recip_deepFac2C: tuple = (1.0,) * 50
+    recip_rhoFacC: tuple = (1.0,) * 50
+    oneSixth: float = ONE_SIXTH
+    advectionScheme: int = ENUM_DST3
+    implicitAdvection: bool = True
+    useCubedSphereExchange: bool = True
+    trIdentity: int = 1
+
+    def __post_init__(self):
+        if self.advectionScheme != ENUM_DST3:
+            raise NotImplementedError(f"advectionScheme={self.advectionScheme}: only ENUM_DST3=30 is implemented")
+        if not self.useCubedSphereExchange:
+            raise NotImplementedError("useCubedSphereExchange=F (npass=2 branch) is not implemented")
+
+    @classmethod
+    def llc90(cls, dt=3600.0, Nr=50, **kw):
+        one = (1.0,) * Nr
+        return cls(TileTopology.from_facets(), LLC90, (float(dt),) * Nr, one, one, one, one, **kw)
+
+
+class GadPlan:
+    """Device pass table of a topology (mj_gad_plan) on one device."""
+
+    def __init__(self, topo: TileTopology, L: Layout, device):
+        face = np.asarray(topo.face, np.int32)
+        edges = np.asarray([[n, s, e, w] for n, s, e, w in zip(topo.N_edge, topo.S_edge, topo.E_edge, topo.W_edge)],
+                           np.uint8)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            _lib.check(_lib.lib().mj_gad_plan_create(len(face), _lib.ptr(face), _lib.ptr(edges), L.sNx, L.sNy, L.OLx,
+                                                     L.OLy, ctypes.byref(h)), "mj_gad_plan_create")
+        self.handle = h
+        self.ntiles = len(face)
+        self.L = L
+
+    def describe(self):
+        """(masks uint8 [3, 2, T, ny*nx], fill_dirs int32 [3, 4, T]) rasterised from the plan."""
+        T, n = self.ntiles, self.L.ny * self.L.nx
+        masks = np.zeros((3, 2, T, n), np.uint8)
+        fills = np.zeros((3, 4, T), np.int32)
+        _lib.check(_lib.lib().mj_gad_plan_describe(self.handle, _lib.ptr(masks), _lib.ptr(fills)),
+                   "mj_gad_plan_describe")
+        return masks, fills
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib._lib is not None:
+            _lib.lib().mj_gad_plan_destroy(h)
+
+
+_plans = {}
+
+
+def gad_plan(topo, L, device):
+    dev = torch.device(device)
+    idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    key = (topo, L, idx)
+    if key not in _plans:
+        _plans[key] = GadPlan(topo, L, idx)
+    return _plans[key]
+
+
+def gad_tables(params):
+    """The device plan standing in for the reference's table dict (gad.py:386-389)."""
+    return gad_plan(params.topo, params.layout, torch.cuda.current_device())
+
+
+GRID2D = ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "maskInC", "maskW", "maskS")
+VGRID = ("drF", "recip_drF", "rhoFacC")
+
+
+def _dev(a, device, name):
+    if isinstance(a, torch.Tensor):
+        if a.device.type != "cuda":
+            raise TypeError(f"{name}: expected a CUDA tensor")
+        return a.to(dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+
+
+def advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, wFld=None,
+                 nlev=None, out=None):
+    """GAD_ADVECTION on all tiles (reference gad.py:460-539) -> gTracer, same shape as tracer.
+
+    tables: gad_tables(params) (or None). grid2d: dyG, dxG, recip_dxC, recip_dyC, recip_rA, maskInC [T, ny, nx]
+    and maskW, maskS [T, K, ny, nx] (or [B, T, K, ny, nx] per member); with wFld also rA [T, ny, nx] and maskC.
+    vgrid: drF, recip_drF, rhoFacC [K] (+ recip_drC with wFld). nlev: per-member level counts (ensemble)."""
+    L = params.layout
+    if not isinstance(tracer, torch.Tensor) or tracer.device.type != "cuda":
+        raise TypeError("tracer: expected a CUDA tensor (the advection runs on the GPU only)")
+    dev = tracer.device
+    batched = tracer.dim() == 5
+    tr = tracer.to(torch.float64).contiguous()
+    tr5 = tr if batched else tr.unsqueeze(0)
+    B, T, K, ny, nx = tr5.shape
+    if (T, ny, nx) != L.shape2d:
+        raise ValueError(f"tracer shape {tuple(tracer.shape)} does not match the layout")
+    fields = [_dev(a, dev, n) for a, n in ((uFld, "uFld"), (vFld, "vFld"), (hFacW, "hFacW"), (hFacS, "hFacS"),
+                                           (recip_hFacC, "recip_hFacC"))]
+    for f, n in zip(fields, ("uFld", "vFld", "hFacW", "hFacS", "recip_hFacC")):
+        if tuple(f.shape) != tuple(tr.shape):
+            raise ValueError(f"{n}: shape {tuple(f.shape)} != tracer {tuple(tr.shape)}")
+    w = None
+    if wFld is not None and not params.implicitAdvection:
+        w = _dev(wFld, dev, "wFld")
+        if tuple(w.shape) != tuple(tr.shape):
+            raise ValueError("wFld shape differs from tracer")
+    g = {k: _dev(grid2d[k], dev, k) for k in grid2d if grid2d[k] is not None}
+    for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "maskInC"):
+        if tuple(g[k].shape) != L.shape2d:
+            raise ValueError(f"{k}: expected {L.shape2d}")
+    mshape = tuple(g["maskW"].shape)
+    if mshape == (T, K, ny, nx):
+        mms = 0
+    elif mshape == (B, T, K, ny, nx) and batched:
+        mms = T * K * ny * nx
+    else:
+        raise ValueError(f"maskW: shape {mshape} is neither [T, K, ny, nx] nor per member")
+    vert = {
+        "dTlev": params.dTtracerLev, "deepFacC": params.deepFacC, "recip_deepFacC": params.recip_deepFacC,
+        "recip_deepFac2C": params.recip_deepFac2C, "recip_rhoFacC": params.recip_rhoFacC,
+        "drF": vgrid["drF"], "recip_drF": vgrid["recip_drF"], "rhoFacC": vgrid["rhoFacC"],
+        "recip_drC": vgrid.get("recip_drC"),
+    }
+    vt = {}
+    for k, v in vert.items():
+        if v is None:
+            continue
+        t = _dev(np.asarray(v, np.float64) if not isinstance(v, torch.Tensor) else v, dev, k).reshape(-1)
+        if t.numel() < K:
+            raise ValueError(f"{k}: {t.numel()} levels < {K}")
+        vt[k] = t
+    if w is not None:
+        for k in ("rA", "maskC"):
+            if k not in g:
+                raise ValueError(f"the vertical flux needs grid2d['{k}']")
+        if "recip_drC" not in vt:
+            raise ValueError("the vertical flux needs vgrid['recip_drC']")
+    P = lambda k, d=g: _lib.ptr(d.get(k))  # noqa: E731
+    grid_s = _lib.GadGrid(P("dyG"), P("dxG"), P("recip_dxC"), P("recip_dyC"), P("recip_rA"), P("rA"), P("maskInC"),
+                          P("maskW"), P("maskS"), P("maskC"), P("dTlev", vt), P("drF", vt), P("recip_drF", vt),
+                          P("rhoFacC", vt), P("recip_rhoFacC", vt), P("deepFacC", vt), P("recip_deepFacC", vt),
+                          P("recip_deepFac2C", vt), P("recip_drC", vt), float(params.oneSixth))
+    nl = None
+    if nlev is not None:
+        nl = torch.as_tensor(np.asarray(nlev, np.int32).reshape(-1), device=dev) if not isinstance(nlev, torch.Tensor) \
+            else nlev.to(device=dev, dtype=torch.int32).contiguous()
+        if nl.numel() != B or int(nl.max()) > K or int(nl.min()) < 0:
+            raise ValueError("nlev: one value in 0..K per member")
+    cfg = _lib.GadConfig(B, K, T * K * ny * nx, mms, _lib.ptr(nl))
+    plan = tables if isinstance(tables, GadPlan) else gad_plan(params.topo, L, dev)
+    gT = torch.empty_like(tr) if out is None else out
+    work = torch.empty_like(tr) if w is not None else None
+    u, v, hW, hS, rhC = fields
+    _lib.check(_lib.lib().mj_gad_dst3(plan.handle, ctypes.byref(grid_s), ctypes.byref(cfg), _lib.ptr(u), _lib.ptr(v),
+                                      _lib.ptr(w), _lib.ptr(tr), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(rhC),
+                                      _lib.ptr(gT), _lib.ptr(work), _lib.stream_ptr()), "mj_gad_dst3")
+    return gT
+
+
+def gad_advection(params, g, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC):
+    """GAD_ADVECTION(advectionScheme=ENUM_DST3) on all tiles -> gTracer (reference gad.py:546-560). g: an object
+    or dict with the GRID2D / VGRID fields (plus rA, maskC, recip_drC when implicitAdvection is False)."""
+    get = (lambda k: g.get(k)) if isinstance(g, dict) else (lambda k: getattr(g, k, None))
+    grid2d = {k: get(k) for k in GRID2D + ("rA", "maskC")}
+    vgrid = {k: get(k) for k in VGRID + ("recip_drC",)}
+    return advect_tiles(params, None, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC,
+                        wFld=None if params.implicitAdvection else wFld)

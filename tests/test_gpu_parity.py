@@ -1,0 +1,195 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on the same seeded inputs, bitwise.
+
+The library and the oracle are both compiled without fused multiply-adds and evaluate the reference's operation
+order, and the CG2D sums use the same order on both sides (the reference's sequential "fortran" order, or the
+library's fixed "tile" tree), so every comparison here is exact equality.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import dst3_case, maps, oracle_tables, to_dev
+
+pytestmark = pytest.mark.gpu
+
+KINDS_SCALAR = ("T", "3D", "Z", "SMs")
+KINDS_VECTOR = ("UVs", "UVn", "UV3s", "As", "An", "Bs", "Bn", "Ds")
+
+
+@pytest.fixture(scope="module")
+def ex():
+    from mit_jax_amd.exchange import default_exchanger
+    return default_exchanger()
+
+
+@pytest.mark.parametrize("kind", KINDS_SCALAR)
+@pytest.mark.parametrize("shape", [(13, 98, 98), (13, 7, 98, 98)])
+def test_halo_scalar_bitwise(dev, ex, kind, shape):
+    rng = np.random.default_rng(hash((kind, len(shape))) % 2**32)
+    a = rng.normal(size=shape)
+    got = ex.scalar(to_dev(a, dev), kind).cpu().numpy()
+    ref = oracle.scalar(maps().full(kind), a)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("kind", KINDS_VECTOR)
+@pytest.mark.parametrize("shape", [(13, 98, 98), (13, 5, 98, 98)])
+def test_halo_vector_bitwise(dev, ex, kind, shape):
+    rng = np.random.default_rng(hash((kind, len(shape), 1)) % 2**32)
+    u, v = rng.normal(size=shape), rng.normal(size=shape)
+    gu, gv = ex.vector(to_dev(u, dev), to_dev(v, dev), kind)
+    ru, rv = oracle.vector(maps().full(kind + "_u"), maps().full(kind + "_v"), u, v)
+    assert np.array_equal(gu.cpu().numpy(), ru)
+    assert np.array_equal(gv.cpu().numpy(), rv)
+
+
+def test_halo_inplace_fill_matches_functional(dev, ex):
+    rng = np.random.default_rng(5)
+    a = to_dev(rng.normal(size=(13, 50, 98, 98)), dev)
+    ref = ex.scalar(a, "T")
+    b = a.clone()
+    ex.fill_(b, "T")
+    assert bool((b == ref).all())
+
+
+def _cg_inputs(precond, seed=1):
+    from mit_jax_amd import synthetic as S
+    g, ops, norm = S.cg2d_case(0, precond)
+    return g, ops, norm, S.rhs(g, seed)
+
+
+@pytest.mark.parametrize("precond", ["diagonal", "5point"])
+@pytest.mark.parametrize("order", ["fortran", "tile"])
+def test_cg2d_bitwise(dev, ex, precond, order):
+    from mit_jax_amd import cg2d
+    g, ops, norm, b = _cg_inputs(precond)
+    rng = np.random.default_rng(9)
+    x0 = rng.normal(size=b.shape) * 1e-3  # a nonzero first guess: exercises the halo normalisation
+    print(f"cg2d {precond} {order}", flush=True)
+    x, d = cg2d.cg2d_fortran(ex, [to_dev(o, dev) for o in ops], to_dev(b, dev), to_dev(x0, dev), cg2dNorm=norm,
+                             tolerance=1e-12, max_iters=2000, sum_order=order)
+    d.check()
+    xr, dr = oracle.cg2d(maps().full("T"), ops, b, x0, cg2dNorm=norm, tolerance=1e-12, max_iters=2000,
+                         sum_order=order)
+    assert int(d["numIters"]) == dr["numIters"]
+    assert int(d["converged"]) == 1
+    for k in ("firstResidual", "lastResidual", "sumRHS", "rhsMax"):
+        assert float(d[k]) == dr[k], (k, float(d[k]), dr[k])
+    assert np.array_equal(x.cpu().numpy(), xr)
+
+
+def test_cg2d_batched_members_bitwise(dev, ex):
+    """Members with different rhs (one zero rhs: 0 iterations) and different max_iters outcomes in one launch;
+    each member equals its own single solve on the oracle."""
+    import torch
+    from mit_jax_amd import cg2d
+    g, ops, norm, _ = _cg_inputs("5point")
+    from mit_jax_amd import synthetic as S
+    bs = [S.rhs(g, s) for s in (11, 12)] + [np.zeros(g.L.shape2d), S.rhs(g, 13) * 1e-3]
+    B = len(bs)
+    b = np.stack(bs)
+    x0 = np.zeros_like(b)
+    x, d = cg2d.cg2d_fortran(ex, [to_dev(o, dev) for o in ops], to_dev(b, dev), to_dev(x0, dev), cg2dNorm=norm,
+                             tolerance=1e-11, max_iters=2000, sum_order="tile")
+    d.check()
+    xs = x.cpu().numpy()
+    for m in range(B):
+        xr, dr = oracle.cg2d(maps().full("T"), ops, bs[m], x0[m], cg2dNorm=norm, tolerance=1e-11, max_iters=2000,
+                             sum_order="tile")
+        assert int(d["numIters"][m]) == dr["numIters"], m
+        assert np.array_equal(xs[m], xr), m
+    assert int(d["numIters"][2]) == 0
+    # max_iters reached: not converged, stops at the cap
+    x, d = cg2d.cg2d_fortran(ex, [to_dev(o, dev) for o in ops], to_dev(b, dev), to_dev(x0, dev), cg2dNorm=norm,
+                             tolerance=1e-11, max_iters=7, sum_order="tile")
+    d.check()
+    assert d["numIters"].cpu().tolist() == [7, 7, 0, 7]
+    assert d["converged"].cpu().tolist() == [0, 0, 1, 0]
+    xr, dr = oracle.cg2d(maps().full("T"), ops, bs[1], x0[1], cg2dNorm=norm, tolerance=1e-11, max_iters=7,
+                         sum_order="tile")
+    assert np.array_equal(x[1].cpu().numpy(), xr)
+    torch.cuda.synchronize()
+
+
+def test_cg2d_block_size_independent(dev, ex):
+    """The reduction trees do not depend on the launch geometry: any block size gives the same bits."""
+    from mit_jax_amd import cg2d
+    g, ops, norm, b = _cg_inputs("5point")
+    x0 = np.zeros_like(b)
+    outs = []
+    for bt in (64, 256, 1024):
+        x, d = cg2d.cg2d_fortran(ex, [to_dev(o, dev) for o in ops], to_dev(b, dev), to_dev(x0, dev), cg2dNorm=norm,
+                                 tolerance=1e-12, max_iters=2000, sum_order="tile", block_threads=bt)
+        d.check()
+        outs.append(x.cpu().numpy())
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def _gad_dev(dev, g, u, v, w, tr):
+    from mit_jax_amd import synthetic as S
+    g2, vg = S.gad_inputs(g)
+    return {k: to_dev(a, dev) for k, a in g2.items()}, vg
+
+
+def test_dst3_bitwise(dev):
+    from mit_jax_amd import gad, synthetic as S
+    g, u, v, w, tr = dst3_case(0)
+    g2d, vg = _gad_dev(dev, g, u, v, w, tr)
+    params = gad.GADParams.llc90()
+    gT = gad.advect_tiles(params, None, g2d, vg, to_dev(u, dev), to_dev(v, dev), to_dev(tr, dev),
+                          to_dev(g.hFacW, dev), to_dev(g.hFacS, dev), to_dev(g.recip_hFacC, dev))
+    g2, _ = S.gad_inputs(g)
+    ref = oracle.advect_tiles(oracle_tables(), g2, vg, u, v, tr, g.hFacW, g.hFacS, g.recip_hFacC)
+    assert np.array_equal(gT.cpu().numpy(), ref)
+
+
+def test_dst3_with_vertical_bitwise(dev):
+    from mit_jax_amd import gad, synthetic as S
+    g, u, v, w, tr = dst3_case(0)
+    g2d, vg = _gad_dev(dev, g, u, v, w, tr)
+    params = gad.GADParams.llc90(implicitAdvection=False)
+    gT = gad.advect_tiles(params, None, g2d, vg, to_dev(u, dev), to_dev(v, dev), to_dev(tr, dev),
+                          to_dev(g.hFacW, dev), to_dev(g.hFacS, dev), to_dev(g.recip_hFacC, dev), wFld=to_dev(w, dev))
+    g2, _ = S.gad_inputs(g)
+    ref = oracle.gad_dst3(oracle_tables(), g2, vg, u, v, w, tr, g.hFacW, g.hFacS, g.recip_hFacC)
+    assert np.array_equal(gT.cpu().numpy(), ref)
+
+
+def test_dst3_batched_members(dev):
+    """Two members (perturbed tracer) in one launch, the second with 30 active levels: each equals its own
+    oracle run on the active levels; levels beyond a member's count are left untouched."""
+    import torch
+    from mit_jax_amd import gad, synthetic as S
+    g, u, v, w, tr = dst3_case(0)
+    g2d, vg = _gad_dev(dev, g, u, v, w, tr)
+    tr2 = tr + np.where(g.maskC > 0, np.random.default_rng(4).normal(0, 0.05, tr.shape), 0.0)
+    trb = np.stack([tr, tr2])
+    B = 2
+    params = gad.GADParams.llc90()
+    st = lambda a: to_dev(np.stack([a] * B), dev)  # noqa: E731
+    out = torch.full((B,) + tr.shape, 7.0, dtype=torch.float64, device=dev)
+    gT = gad.advect_tiles(params, None, g2d, vg, st(u), st(v), to_dev(trb, dev), st(g.hFacW), st(g.hFacS),
+                          st(g.recip_hFacC), nlev=[50, 30], out=out).cpu().numpy()
+    g2, _ = S.gad_inputs(g)
+    ref0 = oracle.advect_tiles(oracle_tables(), g2, vg, u, v, tr, g.hFacW, g.hFacS, g.recip_hFacC)
+    ref1 = oracle.advect_tiles(oracle_tables(), g2, vg, u, v, tr2, g.hFacW, g.hFacS, g.recip_hFacC)
+    assert np.array_equal(gT[0], ref0)
+    assert np.array_equal(gT[1][:, :30], ref1[:, :30])
+    assert np.all(gT[1][:, 30:] == 7.0)
+
+
+def test_runs_are_deterministic(dev, ex):
+    from mit_jax_amd import cg2d, gad
+    g, ops, norm, b = _cg_inputs("5point")
+    opsd = [to_dev(o, dev) for o in ops]
+    r = [cg2d.cg2d_fortran(ex, opsd, to_dev(b, dev), to_dev(np.zeros_like(b), dev), cg2dNorm=norm, tolerance=1e-12,
+                           max_iters=2000, sum_order="tile")[0].cpu().numpy() for _ in range(2)]
+    assert np.array_equal(r[0], r[1])
+    g, u, v, w, tr = dst3_case(0)
+    g2d, vg = _gad_dev(dev, g, u, v, w, tr)
+    params = gad.GADParams.llc90()
+    r = [gad.advect_tiles(params, None, g2d, vg, to_dev(u, dev), to_dev(v, dev), to_dev(tr, dev),
+                          to_dev(g.hFacW, dev), to_dev(g.hFacS, dev), to_dev(g.recip_hFacC, dev)).cpu().numpy()
+         for _ in range(2)]
+    assert np.array_equal(r[0], r[1])
