@@ -1,0 +1,232 @@
+"""Benchmark: ensemble member-steps/s of the CG2D + DST3 step on LLC90 (13 tiles x 90x90, 50 levels) on one MI355X.
+
+A member-step is one CG2D free-surface solve to the normalised target residual plus one GAD DST3 tracer step
+(X/Y flux passes, corner fills, flux divergence) of one ensemble member. One timed step runs the whole ensemble:
+one batched solve (mj_cg2d_solve, per-member operators and convergence) and one batched DST3 step (mj_gad_dst3,
+per-member level counts). Inputs are seeded synthetic fields resident in HBM before timing starts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--members B] [--precision fp64|mixed]
+
+Prints one JSON line. `roofline` is the dominant kernel (the CG2D solve) timed with HIP events on its stream:
+achieved = algorithmic bytes per launch / average launch time, with algorithmic bytes = sum over members of
+numIters * N2 * (6 coefficient + 6 vector) * 8 B (N2 = 105,300 interior points). `cpu_baseline` is the FP64 C
+oracle (OpenMP, all host cores) on a bounded sample of the same member-steps.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+N2 = 13 * 90 * 90
+LEVEL_CHOICES = (15, 30, 50)
+TOL = 1e-12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--members", type=int, default=128)
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--sum-order", choices=("tile", "fortran"), default="tile")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (the reference kernel for HBM counter scaling)")
+    return ap.parse_args()
+
+
+def member_levels(B, seed=4000):
+    return np.random.default_rng(seed).choice(LEVEL_CHOICES, size=B)
+
+
+def build_ensemble(B, dev):
+    """Per-member inputs on the device: level counts, operators, rhs, flow and tracer (config 4)."""
+    import torch
+    from mit_jax_amd import synthetic as S
+    from mit_jax_amd.exchange import default_exchanger
+
+    ex = default_exchanger()
+    nlev = member_levels(B)
+    cls = {}
+    for n in LEVEL_CHOICES:
+        g, ops, norm = S.cg2d_case(0, "5point", nlev=int(n))
+        u, v, w = S.flow(g, 2)
+        tr = S.tracer(g, 3)
+        cls[int(n)] = (g, ops, norm, u, v, tr)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
+    idx = torch.as_tensor([LEVEL_CHOICES.index(int(n)) for n in nlev], device=dev)
+
+    def per_member(get):
+        base = torch.stack([T(get(cls[n])) for n in LEVEL_CHOICES])
+        return base.index_select(0, idx).contiguous()
+
+    ops = [per_member(lambda c, k=k: c[1][k]) for k in range(6)]
+    norm = torch.as_tensor([cls[int(n)][2] for n in nlev], device=dev, dtype=torch.float64)
+    rhs = torch.stack([T(S.rhs(cls[int(n)][0], 100 + m)) for m, n in enumerate(nlev)])
+    x0 = torch.zeros_like(rhs)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4000)
+    u = per_member(lambda c: c[3])
+    v = per_member(lambda c: c[4])
+    tr = per_member(lambda c: c[5])
+    mW = per_member(lambda c: c[0].maskW)
+    mS = per_member(lambda c: c[0].maskS)
+    mC = per_member(lambda c: c[0].maskC)
+    hW = per_member(lambda c: c[0].hFacW)
+    hS = per_member(lambda c: c[0].hFacS)
+    rhC = per_member(lambda c: c[0].recip_hFacC)
+    # perturbations: tracer noise 0.05 degC on wet cells, velocity x (1 + 0.05 N(0,1)); halos re-exchanged
+    tr += torch.randn(tr.shape, generator=gen, device=dev, dtype=torch.float64) * 0.05 * mC
+    u *= 1.0 + 0.05 * torch.randn(u.shape, generator=gen, device=dev, dtype=torch.float64)
+    v *= 1.0 + 0.05 * torch.randn(v.shape, generator=gen, device=dev, dtype=torch.float64)
+    for m in range(B):
+        ex.fill_(tr[m], "T")
+        uu, vv = ex.vector(u[m], v[m], "UVs")
+        u[m].copy_(uu)
+        v[m].copy_(vv)
+    g0 = cls[50][0]
+    g2d = {k: T(g0.f[k]) for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "rA", "maskInC")}
+    g2d.update(maskW=mW, maskS=mS)
+    vg = {k: g0.v[k] for k in ("drF", "recip_drF", "rhoFacC", "recip_drC")}
+    return dict(ex=ex, nlev=nlev, nlev_t=torch.as_tensor(nlev.astype(np.int32), device=dev), ops=ops, norm=norm,
+                rhs=rhs, x0=x0, u=u, v=v, tr=tr, hW=hW, hS=hS, rhC=rhC, g2d=g2d, vg=vg, cls=cls)
+
+
+def cpu_baseline(E, seconds):
+    """FP64 oracle member-steps on the host cores: CG2D solve (fortran order) + DST3 step per member, cycling
+    through the three level classes until `seconds` have passed (at least one member-step)."""
+    import oracle
+    from oracle import tables as OT
+    from mit_jax_amd import synthetic as S
+
+    tab = OT.tables(OT.Topology.from_facets(OT.LLC90_FACETS))
+    Tm = E["ex"].maps.full("T")
+    t0 = time.perf_counter()
+    n = 0
+    done = []
+    while True:
+        lv = LEVEL_CHOICES[n % 3]
+        g, ops, norm, u, v, tr = E["cls"][lv]
+        b = S.rhs(g, 100 + n)
+        ta = time.perf_counter()
+        oracle.cg2d(Tm, ops, b, np.zeros_like(b), cg2dNorm=norm, tolerance=TOL, max_iters=2000,
+                    sum_order="fortran")
+        g2, vg = S.gad_inputs(g)
+        oracle.advect_tiles(tab, g2, vg, u, v, tr, g.hFacW, g.hFacS, g.recip_hFacC)
+        done.append(time.perf_counter() - ta)
+        n += 1
+        if time.perf_counter() - t0 > seconds and n >= 1:
+            break
+    el = sum(done)
+    return dict(value=n / el, unit="member-steps/s", cores=oracle.threads(), kind="port",
+                sample=f"{n} member-steps (levels cycling 15/30/50; FP64 CG2D to 1e-12 + one DST3 step each), "
+                       f"{el:.1f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the ensemble shards over GPUs with no communication, so --gpus {args.gpus}"
+              " is not run here (one process per GPU would each run this same single-GPU step)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from mit_jax_amd import cg2d, gad
+
+    dev = torch.device("cuda:0")
+    B = args.members
+    print(f"bench: building {B} members", file=sys.stderr, flush=True)
+    E = build_ensemble(B, dev)
+    params = gad.GADParams.llc90()
+    gT = torch.empty_like(E["tr"])
+    x = torch.empty_like(E["rhs"])
+    ev = []
+
+    def step(record):
+        if record:
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record()
+        if args.precision == "fp64":
+            _, d = cg2d.cg2d_fortran(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
+                                     max_iters=2000, sum_order=args.sum_order, out=x)
+        else:
+            from mit_jax_amd import mixed
+            _, d = mixed.cg2d_mixed(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
+                                    max_iters=2000, out=x)
+        if record:
+            e1.record()
+        gad.advect_tiles(params, None, E["g2d"], E["vg"], E["u"], E["v"], E["tr"], E["hW"], E["hS"], E["rhC"],
+                         nlev=E["nlev_t"], out=gT)
+        if record:
+            e2.record()
+            ev.append((e0, e1, e2))
+        return d
+
+    for _ in range(args.warmup):
+        d = step(False)
+    torch.cuda.synchronize()
+    if args.copy_probe:
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    print("bench: timing", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d = step(True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    d.check()
+    iters = d["numIters"].cpu().numpy().astype(np.int64)
+    conv = d["converged"].cpu().numpy()
+    cg_ms = np.mean([a.elapsed_time(b) for a, b, _ in ev])
+    gad_ms = np.mean([b.elapsed_time(c) for _, b, c in ev])
+    s_v = 8 if args.precision == "fp64" else 4
+    cg_bytes = float(iters.sum()) * N2 * (6 * 8 + 6 * s_v)
+    gad_bytes = float(sum(80 * 13 * 98 * 98 * int(n) + 6 * 13 * 98 * 98 * 8 for n in E["nlev"]))
+    achieved = cg_bytes / (cg_ms * 1e-3) / 1e9
+    ms = el / args.steps * 1e3
+    out = {
+        "metric": "member-steps/s, cg2d+DST3 at LLC90; mixed-refined vs FP64 at equal residual",
+        "value": B * args.steps / el,
+        "unit": "member-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"LLC90 ensemble: {B} members x (CG2D to 1e-12 normalised residual + one DST3 "
+                               "step), 13x90x90 tiles, 50 levels, member level counts 15/30/50",
+                   "members": B, "precision": args.precision, "sum_order": args.sum_order,
+                   "cg2d_iterations_mean": float(iters.mean()), "cg2d_iterations_max": int(iters.max()),
+                   "all_converged": bool((conv == 1).all())},
+        "roofline": {"bound": "hbm", "kernel": "cg2d_fused_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launch_ms": cg_ms, "algorithmic_bytes": cg_bytes},
+        "dst3": {"launch_ms": gad_ms, "achieved_GBs": gad_bytes / (gad_ms * 1e-3) / 1e9,
+                 "algorithmic_bytes": gad_bytes},
+    }
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(E, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
