@@ -142,6 +142,10 @@ int mj_gad_plan_destroy(mj_gad_plan* plan);
  * X (dir 0) / Y (dir 1) update of pass 0..2 writes (host memory, 6*ntiles*ny*nx bytes); fills[(pass*4+f)*ntiles + t]
  * = fill4dir (0 none) of f = {X before, X after, Y before, Y after}. For checking the plan against the tables. */
 int mj_gad_plan_describe(const mj_gad_plan* plan, uint8_t* masks, int32_t* fills);
+/* The same rasterisation straight from a topology, on the host only (no device, no plan object): the per-facet pass
+ * table of gad_advection.F:333-351 and the update regions / fill calls of :376-774 (reference gad.py:217-383). */
+int mj_gad_tables_host(int ntiles, const int32_t* face, const uint8_t* edges, int sNx, int sNy, int OLx, int OLy,
+                       uint8_t* masks, int32_t* fills);
 
 typedef struct {
     /* 2-D [T, ny, nx], shared by all members */

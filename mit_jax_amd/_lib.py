@@ -72,6 +72,7 @@ SIGNATURES = {
     "mj_gad_plan_create": (I32, [I32, P, P, I32, I32, I32, I32, ctypes.POINTER(P)]),
     "mj_gad_plan_destroy": (I32, [P]),
     "mj_gad_plan_describe": (I32, [P, P, P]),
+    "mj_gad_tables_host": (I32, [I32, P, P, I32, I32, I32, I32, P, P]),
     "mj_gad_dst3": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P, P]),
 }
 

@@ -280,13 +280,9 @@ static void pass_table(int face, int pass, bool* overlapOnly, bool* interiorOnly
     }
 }
 
-extern "C" int mj_gad_plan_create(int ntiles, const int32_t* face, const uint8_t* edges, int sNx, int sNy, int OLx,
-                                  int OLy, mj_gad_plan** out) {
-    MJ_REQUIRE(out && face && edges && ntiles > 0, "mj_gad_plan_create: bad argument");
-    MJ_REQUIRE(sNx + 2 * OLx == GNX && sNy + 2 * OLy == GNY && OLx == OLy,
-               "mj_gad_plan_create: the kernels are built for %dx%d tiles with equal halos (got sNx %d sNy %d OL %d,%d)",
-               GNY, GNX, sNx, sNy, OLx, OLy);
-    mj_gad_plan* P = new mj_gad_plan();
+// Fills P's host tables from the topology (no device work).
+static int build_plan_host(mj_gad_plan* P, int ntiles, const int32_t* face, const uint8_t* edges, int sNx, int sNy,
+                           int OLx, int OLy) {
     P->ntiles = ntiles;
     P->sNx = sNx;
     P->sNy = sNy;
@@ -353,6 +349,17 @@ extern "C" int mj_gad_plan_create(int ntiles, const int32_t* face, const uint8_t
             }
         }
     }
+    return MJ_OK;
+}
+
+extern "C" int mj_gad_plan_create(int ntiles, const int32_t* face, const uint8_t* edges, int sNx, int sNy, int OLx,
+                                  int OLy, mj_gad_plan** out) {
+    MJ_REQUIRE(out && face && edges && ntiles > 0, "mj_gad_plan_create: bad argument");
+    MJ_REQUIRE(sNx + 2 * OLx == GNX && sNy + 2 * OLy == GNY && OLx == OLy,
+               "mj_gad_plan_create: the kernels are built for %dx%d tiles with equal halos (got sNx %d sNy %d OL %d,%d)",
+               GNY, GNX, sNx, sNy, OLx, OLy);
+    mj_gad_plan* P = new mj_gad_plan();
+    build_plan_host(P, ntiles, face, edges, sNx, sNy, OLx, OLy);
     hipError_t e = hipMalloc((void**)&P->d_tiles, sizeof(MjTilePlan) * ntiles);
     if (e == hipSuccess)
         e = hipMemcpy(P->d_tiles, P->tiles.data(), sizeof(MjTilePlan) * ntiles, hipMemcpyHostToDevice);
@@ -370,8 +377,28 @@ extern "C" int mj_gad_plan_destroy(mj_gad_plan* plan) {
     return MJ_OK;
 }
 
+static int describe(const mj_gad_plan* P, uint8_t* masks, int32_t* fills);
+
 extern "C" int mj_gad_plan_describe(const mj_gad_plan* P, uint8_t* masks, int32_t* fills) {
     MJ_REQUIRE(P, "mj_gad_plan_describe: null plan");
+    return describe(P, masks, fills);
+}
+
+extern "C" int mj_gad_tables_host(int ntiles, const int32_t* face, const uint8_t* edges, int sNx, int sNy, int OLx,
+                                  int OLy, uint8_t* masks, int32_t* fills) {
+    MJ_REQUIRE(face && edges && ntiles > 0, "mj_gad_tables_host: bad argument");
+    MJ_REQUIRE(sNx + 2 * OLx == GNX && sNy + 2 * OLy == GNY && OLx == OLy, "mj_gad_tables_host: unsupported layout");
+    mj_gad_plan P;
+    P.ntiles = ntiles;
+    P.sNx = sNx;
+    P.sNy = sNy;
+    P.OLx = OLx;
+    P.OLy = OLy;
+    build_plan_host(&P, ntiles, face, edges, sNx, sNy, OLx, OLy);
+    return describe(&P, masks, fills);
+}
+
+static int describe(const mj_gad_plan* P, uint8_t* masks, int32_t* fills) {
     const int T = P->ntiles, plane = GPLANE;
     if (masks) memset(masks, 0, (size_t)6 * T * plane);
     for (int t = 0; t < T; ++t)

@@ -114,6 +114,20 @@ class GadPlan:
             _lib.lib().mj_gad_plan_destroy(h)
 
 
+def plan_tables_host(topo: TileTopology, L: Layout = LLC90):
+    """The library's update regions and fill calls for `topo`, rasterised on the host (no GPU needed):
+    (masks uint8 [3, 2, T, ny*nx], fill_dirs int32 [3, 4, T])."""
+    face = np.asarray(topo.face, np.int32)
+    edges = np.asarray([[n, s, e, w] for n, s, e, w in zip(topo.N_edge, topo.S_edge, topo.E_edge, topo.W_edge)],
+                       np.uint8)
+    T, n = len(face), L.ny * L.nx
+    masks = np.zeros((3, 2, T, n), np.uint8)
+    fills = np.zeros((3, 4, T), np.int32)
+    _lib.check(_lib.lib().mj_gad_tables_host(T, _lib.ptr(face), _lib.ptr(edges), L.sNx, L.sNy, L.OLx, L.OLy,
+                                             _lib.ptr(masks), _lib.ptr(fills)), "mj_gad_tables_host")
+    return masks, fills
+
+
 _plans = {}
 
 

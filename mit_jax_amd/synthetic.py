@@ -8,8 +8,8 @@ Recipes (reference files the steps follow):
   grid        rA ~ U(0.5e10, 1.5e10) m^2 on tile interiors; face lengths dxG = dyG = dxC = dyC from sqrt(rA)
               averaged to faces; halos by the T / UVn exchanges as INI_CURVILINEAR_GRID does
               (grid/load.py:396-416).
-  depth       a smooth random field per LLC facet (facet shapes of io/llc.py:16), 0-6000 m, thresholded so that
-              ~30 % of the columns are land (depth <= 0).
+  depth       a smooth random field per LLC facet (facet shapes of io/llc.py:16), 0-6000 m, thresholded per
+              facet so that ~30 % of the columns of every facet are land (depth <= 0).
   hFac, masks ini_masks_etc (grid/load.py:485-594): hFacMin = 0.2, hFacMinDr = 5 m, hFacW/S = min of the
               neighbours + UVn exchange, walls where dyG/dxG = 0, masks = hFac != 0.
   operator    UPDATE_CG2D (core/solve_for_pressure.py:29-65) with implicSurfPress = implicDiv2DFlow =
@@ -153,11 +153,12 @@ def make_grid(seed=0, nlev=None, L=LLC90, land_frac=0.30, hFacMin=0.2, hFacMinDr
     f.update(rA=rA, recip_rA=_recip(rA), dyG=dyG, dxG=dxG, dxC=dxC, dyC=dyC, recip_dxC=_recip(dxC),
              recip_dyC=_recip(dyC))
     # depth: smooth per facet, land fraction by threshold
+    # thresholded per facet, so that every facet (the Arctic cap included) has ~land_frac land
     fac = {k: _smooth_field(rng, s) for k, s in FACET_SHAPES.items()}
-    allv = np.concatenate([a.ravel() for a in fac.values()])
-    thr = np.quantile(allv, land_frac)
-    top = allv.max()
-    dep = {k: np.where(a > thr, 6000.0 * (a - thr) / (top - thr), 0.0) for k, a in fac.items()}
+    dep = {}
+    for k, a in fac.items():
+        thr, top = np.quantile(a, land_frac), a.max()
+        dep[k] = np.where(a > thr, 6000.0 * (a - thr) / (top - thr), 0.0)
     depth = np.zeros(L.shape2d)
     depth[:, J, I] = facets_to_tiles(dep, L)
     depth = _exch_np(maps, "T", depth)
