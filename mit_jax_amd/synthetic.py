@@ -158,7 +158,7 @@ def make_grid(seed=0, nlev=None, L=LLC90, land_frac=0.30, hFacMin=0.2, hFacMinDr
     dep = {}
     for k, a in fac.items():
         thr, top = np.quantile(a, land_frac), a.max()
-        dep[k] = np.where(a > thr, 6000.0 * (a - thr) / (top - thr), 0.0)
+        dep[k] = np.where(a > thr, np.minimum(6000.0 * (a - thr) / (top - thr), 6000.0), 0.0)
     depth = np.zeros(L.shape2d)
     depth[:, J, I] = facets_to_tiles(dep, L)
     depth = _exch_np(maps, "T", depth)
