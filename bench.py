@@ -71,7 +71,9 @@ def build_ensemble(B, dev):
         base = torch.stack([T(get(cls[n])) for n in LEVEL_CHOICES])
         return base.index_select(0, idx).contiguous()
 
-    ops = [per_member(lambda c, k=k: c[1][k]) for k in range(6)]
+    # members with the same level count share one operator: 3 slots, op_index picks each member's
+    ops = [torch.stack([T(cls[n][1][k]) for n in LEVEL_CHOICES]) for k in range(6)]
+    op_index = idx.to(torch.int32)
     norm = torch.as_tensor([cls[int(n)][2] for n in nlev], device=dev, dtype=torch.float64)
     rhs = torch.stack([T(S.rhs(cls[int(n)][0], 100 + m)) for m, n in enumerate(nlev)])
     x0 = torch.zeros_like(rhs)
@@ -99,7 +101,8 @@ def build_ensemble(B, dev):
     g2d = {k: T(g0.f[k]) for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "rA", "maskInC")}
     g2d.update(maskW=mW, maskS=mS)
     vg = {k: g0.v[k] for k in ("drF", "recip_drF", "rhoFacC", "recip_drC")}
-    return dict(ex=ex, nlev=nlev, nlev_t=torch.as_tensor(nlev.astype(np.int32), device=dev), ops=ops, norm=norm,
+    return dict(ex=ex, nlev=nlev, nlev_t=torch.as_tensor(nlev.astype(np.int32), device=dev), ops=ops,
+                op_index=op_index, norm=norm,
                 rhs=rhs, x0=x0, u=u, v=v, tr=tr, hW=hW, hS=hS, rhC=rhC, g2d=g2d, vg=vg, cls=cls)
 
 
@@ -158,11 +161,11 @@ def main():
             e0.record()
         if args.precision == "fp64":
             _, d = cg2d.cg2d_fortran(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
-                                     max_iters=2000, sum_order=args.sum_order, out=x)
+                                     max_iters=2000, sum_order=args.sum_order, out=x, op_index=E["op_index"])
         else:
             from mit_jax_amd import mixed
             _, d = mixed.cg2d_mixed(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
-                                    max_iters=2000, out=x)
+                                    max_iters=2000, out=x, op_index=E["op_index"])
         if record:
             e1.record()
         gad.advect_tiles(params, None, E["g2d"], E["vg"], E["u"], E["v"], E["tr"], E["hW"], E["hS"], E["rhC"],

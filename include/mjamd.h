@@ -94,18 +94,19 @@ typedef struct {
 } mj_cg2d_diag;
 
 typedef struct {
-    const double *aW2d, *aS2d, *aC2d, *pW, *pS, *pC; /* [T, ny, nx] each (per member when member_stride != 0) */
-    int64_t member_stride;                           /* elements between members' operators; 0 = shared */
+    const double *aW2d, *aS2d, *aC2d, *pW, *pS, *pC; /* [T, ny, nx] each (one per operator slot) */
+    int64_t member_stride;                           /* elements between operator slots; 0 = one shared slot */
+    const int32_t* op_index;                         /* device [B]: slot of each member, or NULL (slot = member) */
 } mj_cg2d_ops;
 
 typedef struct {
     int batch;           /* members B >= 1 */
-    int ntiles, sNx, sNy, OL; /* 13, 90, 90, 4 (sNx, sNy in 64..128) */
+    int ntiles, sNx, sNy, OL; /* 13, 90, 90, 4 (sNx, sNy in 64..90) */
     int64_t member_stride;    /* elements between members of b, x0, x (>= ntiles*ny*nx) */
     double tolerance;    /* cg2dTolerance (normalised target) */
     int max_iters;       /* cg2dMaxIters */
     int sum_order;       /* MJ_SUM_FORTRAN or MJ_SUM_TREE */
-    int block_threads;   /* 0 = library default; else a multiple of 64 in 64..1024 */
+    int block_threads;   /* 0 = library default (512); else a multiple of 64 in 64..512 */
     int grid_blocks;     /* 0 = as many co-resident blocks as useful */
 } mj_cg2d_config;
 

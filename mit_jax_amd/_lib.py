@@ -37,7 +37,8 @@ class Cg2dDiag(ctypes.Structure):
 
 
 class Cg2dOps(ctypes.Structure):
-    _fields_ = [("aW2d", P), ("aS2d", P), ("aC2d", P), ("pW", P), ("pS", P), ("pC", P), ("member_stride", I64)]
+    _fields_ = [("aW2d", P), ("aS2d", P), ("aC2d", P), ("pW", P), ("pS", P), ("pC", P), ("member_stride", I64),
+                ("op_index", P)]
 
 
 class Cg2dConfig(ctypes.Structure):

@@ -75,21 +75,37 @@ def _as_batch(a, L, name):
     return a.contiguous()
 
 
-def _ops_struct(ops, B, L, device):
+def _ops_struct(ops, B, L, device, op_index=None):
+    """The six operator arrays as one slot [T, ny, nx] shared by every member, one slot per member
+    [B, T, ny, nx], or C slots [C, T, ny, nx] with op_index (B slot numbers: members sharing an operator, e.g.
+    the same level count, read the same coefficients)."""
     ops = [o if isinstance(o, torch.Tensor) else torch.as_tensor(np.asarray(o), device=device) for o in ops]
     ops = [o.to(device=device, dtype=torch.float64).contiguous() for o in ops]
     shp = {tuple(o.shape) for o in ops}
     if len(shp) != 1:
         raise ValueError("the six operator arrays must have the same shape")
     shp = shp.pop()
+    idx = None
     if shp == L.shape2d:
         ms = 0
-    elif shp == (B,) + L.shape2d:
+        if op_index is not None:
+            raise ValueError("op_index given with a single shared operator")
+    elif len(shp) == 4 and shp[1:] == L.shape2d:
         ms = L.npad
+        C = shp[0]
+        if op_index is None:
+            if C != B:
+                raise ValueError(f"{C} operator slots for {B} members need op_index")
+        else:
+            idx = torch.as_tensor(np.asarray(op_index, np.int32).reshape(-1), device=device) \
+                if not isinstance(op_index, torch.Tensor) else op_index.to(device=device, dtype=torch.int32)
+            idx = idx.contiguous()
+            if idx.numel() != B or int(idx.min()) < 0 or int(idx.max()) >= C:
+                raise ValueError(f"op_index: {B} slot numbers in 0..{C - 1} expected")
     else:
-        raise ValueError(f"operator shape {shp} is neither {L.shape2d} nor [B, ...] with B = {B}")
-    s = _lib.Cg2dOps(*[o.data_ptr() for o in ops], ms)
-    return s, ops
+        raise ValueError(f"operator shape {shp} is neither {L.shape2d} nor [C, {L.nTiles}, {L.ny}, {L.nx}]")
+    s = _lib.Cg2dOps(*[o.data_ptr() for o in ops], ms, idx.data_ptr() if idx is not None else None)
+    return s, ops + ([idx] if idx is not None else [])
 
 
 def _norm_tensor(cg2dNorm, B, device):
@@ -112,8 +128,9 @@ def config(L, B, tolerance, max_iters, sum_order, block_threads=0, grid_blocks=0
 
 
 def cg2d_fortran(ex, ops, cg2d_b, cg2d_x, *, cg2dNorm, tolerance, max_iters, sum_order="fortran", nIterMin=-1,
-                 sum_unroll=1, block_threads=0, grid_blocks=0, out=None):
-    """CG2D (reference core/cg2d.py:205-269). ops = (aW2d, aS2d, aC2d, pW, pS, pC). Returns (x, diag)."""
+                 sum_unroll=1, block_threads=0, grid_blocks=0, out=None, op_index=None):
+    """CG2D (reference core/cg2d.py:205-269). ops = (aW2d, aS2d, aC2d, pW, pS, pC). Returns (x, diag).
+    Batched: cg2d_b, cg2d_x [B, T, ny, nx]; ops shared, per member, or per slot with op_index."""
     if nIterMin >= 0:
         raise NotImplementedError("nIterMin >= 0 (min-residual solution) is not ported")
     L = ex.L
@@ -126,7 +143,7 @@ def cg2d_fortran(ex, ops, cg2d_b, cg2d_x, *, cg2dNorm, tolerance, max_iters, sum
     if x0.shape[0] != B:
         raise ValueError("cg2d_b and cg2d_x batch sizes differ")
     dev = b.device
-    ops_s, ops_keep = _ops_struct(ops, B, L, dev)
+    ops_s, ops_keep = _ops_struct(ops, B, L, dev, op_index)
     norm = _norm_tensor(cg2dNorm, B, dev)
     cfg = config(L, B, tolerance, max_iters, sum_order, block_threads, grid_blocks)
     lib = _lib.lib()
@@ -144,7 +161,7 @@ def cg2d_fortran(ex, ops, cg2d_b, cg2d_x, *, cg2dNorm, tolerance, max_iters, sum
     return (x if batched else x[0]), Cg2dDiag(diag, batched)
 
 
-def matvec(ex, A, v, cg2dNorm):
+def matvec(ex, A, v, cg2dNorm, op_index=None):
     """(A / cg2dNorm) v on the interior, 0 on halos, v's halos zeroed then exchanged (reference _matvec,
     core/cg2d.py:292-298). A = (aW2d, aS2d, aC2d). v [T, ny, nx] or [B, T, ny, nx]."""
     L = ex.L
@@ -153,7 +170,7 @@ def matvec(ex, A, v, cg2dNorm):
     B = vb.shape[0]
     dev = vb.device
     A = tuple(A)
-    ops_s, keep = _ops_struct(A + A, B, L, dev)  # the preconditioner slots are not read
+    ops_s, keep = _ops_struct(A + A, B, L, dev, op_index)  # the preconditioner slots are not read
     norm = _norm_tensor(cg2dNorm, B, dev)
     cfg = config(L, B, 0.0, 0, "tile")
     y = torch.empty_like(vb)

@@ -43,9 +43,9 @@ struct CgDev {
     const int8_t* ring_sign;
     double *r0, *r1, *s0, *s1;
     double *pmax, *psum, *perr, *peta, *psig;  // partials: [B*T], [B*T], [2][B*T] x3
-    double* prod;  // MJ_SUM_FORTRAN: per block [2][sNy*sNx] staging of the per-point values
-    int32_t* niter;
+    const int32_t* op_index;                   // operator slot per member (NULL: slot = member)
     int* nactive;
+    int32_t* mdone;  // [B] iterations at which a member finished, -1 while active
     mj_cg2d_diag* diag;
 };
 
@@ -61,44 +61,91 @@ __device__ __forceinline__ double op_M(const double* pW, const double* pS, const
     return pC[p] * r[p] + pW[p] * r[p - 1] + pW[p + 1] * r[p + 1] + pS[p] * r[p - nx] + pS[p + nx] * r[p + nx];
 }
 
-// ------------------------------------------------------------------------------------------------ row reductions
-// rowbuf[k][128] in LDS (tree order row sums); prod[k][sNy*sNx] in the block's global staging area (fortran
-// order per-point values, summed sequentially by one thread).
-template <int MODE>
-struct Red {
-    double* rowbuf;  // [2][128]
-    double* prod;    // [2][sNy*sNx] (MODE == MJ_SUM_FORTRAN only), global memory
-    int sNx, sNy;
+// ------------------------------------------------------------------------------------------------ LDS tile staging
+// A unit's (member, tile) interior plus its width-1 ring, (sNy+2) x (sNx+2) points, is staged in LDS for the
+// stencils. LDS index of padded (j, i): (j - OL + 1) * RW + (i - OL + 1), RW = sNx + 2.
+constexpr int kRingMax = 92;                    // sNx, sNy <= 90 (LLC90)
+constexpr int kTileLds = kRingMax * kRingMax;   // 8464 doubles = 67.7 KB per staged tile
 
-    __device__ __forceinline__ void put(int k, int jl, int lane, double v0, double v1, bool has1) {
-        if (MODE == MJ_SUM_TREE) {
-            const double w = wave_tree(v0, v1, has1);
-            if (lane == 0) rowbuf[k * 128 + jl] = w;
-        } else {
-            double* pr = prod + (size_t)k * sNy * sNx + jl * sNx;
-            pr[lane] = v0;
-            if (has1) pr[lane + 64] = v1;
+// Reduction of a per-point array over the tile interior: P(jl, il) read through `get`, the tree or sequential
+// order of MODE. Entry: P complete (after a __syncthreads). Returns the tile partial in thread 0. Exit with a
+// __syncthreads (P and rowbuf reusable).
+template <int MODE, class Get>
+__device__ __forceinline__ double tile_reduce(Get get, double* rowbuf, int sNx, int sNy, int tid) {
+    const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    double res = 0.0;
+    if (MODE == MJ_SUM_TREE) {
+        const bool hx = lane + 64 < sNx;
+        for (int jl = wave; jl < sNy; jl += nwaves) {
+            const double w = wave_tree(get(jl, lane), hx ? get(jl, lane + 64) : 0.0, hx);
+            if (lane == 0) rowbuf[jl] = w;
         }
+        __syncthreads();
+        if (wave == 0) {
+            const bool hy = lane + 64 < sNy;
+            res = wave_tree(rowbuf[lane], hy ? rowbuf[lane + 64] : 0.0, hy);
+        }
+    } else {
+        if (tid == 0)
+            for (int jl = 0; jl < sNy; ++jl)
+                for (int il = 0; il < sNx; ++il) res = res + get(jl, il);
     }
-    // After __syncthreads: the tile partial of slot k, valid in thread 0 (called by wave 0 only).
-    __device__ __forceinline__ double tile(int k, int lane) {
-        if (MODE == MJ_SUM_TREE) {
-            const double* rb = rowbuf + k * 128;
-            const bool has1 = lane + 64 < sNy;
-            return wave_tree(rb[lane], has1 ? rb[lane + 64] : 0.0, has1);
-        } else {
-            double s = 0.0;
-            if (lane == 0) {
-                const double* pr = prod + (size_t)k * sNy * sNx;
-                const int n = sNy * sNx;
-                for (int q = 0; q < n; ++q) s = s + pr[q];
+    __syncthreads();
+    return res;
+}
+
+// Stage the ring region of a padded member array into LDS, 8 loads per thread in flight.
+__device__ __forceinline__ void stage_tile(double* L, const double* __restrict__ g, int tbase, int nx, int OL, int RW,
+                                           int RH, int tid) {
+    const int n = RW * RH, nt = blockDim.x;
+    for (int e0 = 0; e0 < n; e0 += 8 * nt) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int e = e0 + k * nt + tid;
+            if (e < n) {
+                const int jr = e / RW, ir = e - jr * RW;
+                v[k] = g[tbase + (jr + OL - 1) * nx + (ir + OL - 1)];
             }
-            return s;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int e = e0 + k * nt + tid;
+            if (e < n) L[e] = v[k];
         }
     }
-};
+}
 
-// Per row helper: wave w of the block processes rows jl = w, w + nwaves, ...; lane l holds i = l, l + 64.
+// The thread index through an empty asm the compiler cannot see through: values derived from it inside a
+// phase are computed there, not hoisted out of the persistent iteration loop (where they would all stay live
+// at once and spill).
+__device__ __forceinline__ int opaque_tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+// Stencils on an LDS ring tile at LDS index e (row stride RW), coefficients at padded index p.
+__device__ __forceinline__ double lds_A(const double* __restrict__ aW, const double* __restrict__ aS,
+                                        const double* __restrict__ aC, const double* L, int p, int nx, int e, int RW) {
+    return aW[p] * L[e - 1] + aW[p + 1] * L[e + 1] + aS[p] * L[e - RW] + aS[p + nx] * L[e + RW] + aC[p] * L[e];
+}
+__device__ __forceinline__ double lds_M(const double* __restrict__ pW, const double* __restrict__ pS,
+                                        const double* __restrict__ pC, const double* L, int p, int nx, int e, int RW) {
+    return pC[p] * L[e] + pW[p] * L[e - 1] + pW[p + 1] * L[e + 1] + pS[p] * L[e - RW] + pS[p + nx] * L[e + RW];
+}
+
+// Interior point loop: e over sNy*sNx; (jl, il), padded index p, LDS ring-tile index q.
+#define MJ_INTERIOR(...)                                                                                  \
+    for (int e_ = tid; e_ < a.sNy * a.sNx; e_ += blockDim.x) {                                            \
+        const int jl = e_ / a.sNx, il = e_ - jl * a.sNx;                                                  \
+        const int p = tbase + (a.OL + jl) * a.nx + a.OL + il;                                             \
+        const int q = (jl + 1) * RW + il + 1;                                                             \
+        __VA_ARGS__                                                                                       \
+    }
+
+// Global-memory row loop of the once-per-solve phases: wave w takes rows jl = w, w + nwaves, ...; lane l holds
+// i = l and i = l + 64.
 #define MJ_ROWS(...)                                                                                      \
     for (int jl = wave; jl < a.sNy; jl += nwaves) {                                                       \
         const int prow = tbase + (a.OL + jl) * a.nx + a.OL;                                               \
@@ -108,27 +155,203 @@ struct Red {
         __VA_ARGS__                                                                                       \
     }
 
-__device__ __forceinline__ double member_rhs_norm(const CgDev& a, int m) {
-    double mx = 0.0;
-    for (int t = 0; t < a.T; ++t) mx = fmax(mx, a.pmax[m * a.T + t]);
-    return mx != 0.0 ? 1.0 / mx : 1.0;
-}
 __device__ __forceinline__ double member_rhs_max(const CgDev& a, int m) {
     double mx = 0.0;
     for (int t = 0; t < a.T; ++t) mx = fmax(mx, a.pmax[m * a.T + t]);
     return mx;
 }
+__device__ __forceinline__ double member_rhs_norm(const CgDev& a, int m) {
+    const double mx = member_rhs_max(a, m);
+    return mx != 0.0 ? 1.0 / mx : 1.0;
+}
+
+#ifndef MJ_CG_THREADS
+#define MJ_CG_THREADS 1024
+#endif  // block size of the fused solve (one block per CU: its LDS tiles take 136 KB)
+#ifndef MJ_BATCH
+#define MJ_BATCH 2
+#endif  // interior points per thread whose loads are issued together
+constexpr int kMaxMembers = 2048;  // members of one batched solve (LDS member lists)
+constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+// The LDS tiles of the fused kernel, at file scope so the per-unit phase functions below address them directly.
+__shared__ double g_R[kTileLds];
+__shared__ double g_S[kTileLds];
+__shared__ double g_rowbuf[128];
+
+// Interior loop of the phase functions: MJ_BATCH points per thread, all loads first (registers c0..c6[k]), then
+// the arithmetic.
+#define MJ_UNIT_INTERIOR(LOADS, COMPUTE)                                                                  \
+    for (int base_ = 0; base_ < sNy * sNx; base_ += MJ_BATCH * nt) {                                      \
+        double c0[MJ_BATCH], c1[MJ_BATCH], c2[MJ_BATCH], c3[MJ_BATCH], c4[MJ_BATCH], c5[MJ_BATCH],         \
+            c6[MJ_BATCH];                                                                                 \
+        int pk[MJ_BATCH], qk[MJ_BATCH], ek[MJ_BATCH];                                                     \
+        _Pragma("unroll") for (int k = 0; k < MJ_BATCH; ++k) {                                            \
+            const int e_ = base_ + k * nt + tid;                                                          \
+            ek[k] = e_;                                                                                   \
+            if (e_ < sNy * sNx) {                                                                         \
+                const int jl = e_ / sNx, il = e_ - jl * sNx;                                              \
+                const int p = tbase + (OL + jl) * nx + OL + il;                                           \
+                pk[k] = p;                                                                                \
+                qk[k] = (jl + 1) * RW + il + 1;                                                           \
+                LOADS                                                                                     \
+            }                                                                                             \
+        }                                                                                                 \
+        _Pragma("unroll") for (int k = 0; k < MJ_BATCH; ++k) {                                            \
+            const int e_ = ek[k];                                                                         \
+            if (e_ < sNy * sNx) {                                                                         \
+                const int p = pk[k], q = qk[k];                                                           \
+                (void)p;                                                                                  \
+                COMPUTE                                                                                   \
+            }                                                                                             \
+        }                                                                                                 \
+    }
+
+// Phase B(n) of one unit: s_new = M r + beta s_old on interior and ring (ring from its interior sources), then
+// sigma = sum s_new (A s_new). Returns the tile partial of sigma (thread 0).
+template <int MODE>
+__device__ __noinline__ double unit_phase_B(const double* __restrict__ rin, const double* __restrict__ sold,
+                                            double* __restrict__ snew, const double* __restrict__ pW,
+                                            const double* __restrict__ pS, const double* __restrict__ pC,
+                                            const double* __restrict__ aW, const double* __restrict__ aS,
+                                            const double* __restrict__ aC, const int32_t* __restrict__ ring_dst,
+                                            const int32_t* __restrict__ ring_src, const int8_t* __restrict__ ring_sign,
+                                            int rk0, int rk1, double beta, int tbase, int nx, int OL, int sNx,
+                                            int sNy) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int RW = sNx + 2, RH = sNy + 2;
+    double* R = g_R;
+    double* S = g_S;
+    stage_tile(R, rin, tbase, nx, OL, RW, RH, tid);
+    for (int e = tid; e < RW * RH; e += nt) S[e] = 0.0;
+    __syncthreads();
+    MJ_UNIT_INTERIOR(
+        {
+            c0[k] = pC[p];
+            c1[k] = pW[p];
+            c2[k] = pW[p + 1];
+            c3[k] = pS[p];
+            c4[k] = pS[p + nx];
+            c5[k] = sold[p];
+        },
+        {
+            const double v = c0[k] * R[q] + c1[k] * R[q - 1] + c2[k] * R[q + 1] + c3[k] * R[q - RW] +
+                             c4[k] * R[q + RW] + beta * c5[k];
+            snew[p] = v;
+            S[q] = v;
+        })
+    for (int k2 = rk0 + tid; k2 < rk1; k2 += nt) {
+        const int s = ring_src[k2];
+        double v = op_M(pW, pS, pC, rin, s, nx) + beta * sold[s];
+        v = ring_sign[k2] < 0 ? -v : v;
+        const int d = ring_dst[k2];
+        snew[d] = v;
+        const int dq = d - tbase, dj = dq / nx, di = dq - dj * nx;
+        S[(dj - OL + 1) * RW + (di - OL + 1)] = v;
+    }
+    __syncthreads();
+    MJ_UNIT_INTERIOR(
+        {
+            c0[k] = aW[p];
+            c1[k] = aW[p + 1];
+            c2[k] = aS[p];
+            c3[k] = aS[p + nx];
+            c4[k] = aC[p];
+        },
+        {
+            R[e_] = S[q] * (c0[k] * S[q - 1] + c1[k] * S[q + 1] + c2[k] * S[q - RW] + c3[k] * S[q + RW] +
+                            c4[k] * S[q]);
+        })
+    __syncthreads();
+    return tile_reduce<MODE>([&](int jl, int il) { return R[jl * sNx + il]; }, g_rowbuf, sNx, sNy, tid);
+}
+
+// Phase C(n) of one unit: x += alpha s_new; r_new = r - alpha A s_new on interior and ring; then the tile
+// partials of err = sum r_new^2 (returned in .x) and eta(n+1) = sum (M r_new) r_new (in .y), thread 0.
+template <int MODE>
+__device__ __noinline__ double2 unit_phase_C(const double* __restrict__ snew, const double* __restrict__ rin,
+                                             double* __restrict__ rout, double* __restrict__ x,
+                                             const double* __restrict__ pW, const double* __restrict__ pS,
+                                             const double* __restrict__ pC, const double* __restrict__ aW,
+                                             const double* __restrict__ aS, const double* __restrict__ aC,
+                                             const int32_t* __restrict__ ring_dst,
+                                             const int32_t* __restrict__ ring_src,
+                                             const int8_t* __restrict__ ring_sign, int rk0, int rk1, double alpha,
+                                             int tbase, int nx, int OL, int sNx, int sNy) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int RW = sNx + 2, RH = sNy + 2;
+    double* R = g_R;
+    double* S = g_S;
+    stage_tile(S, snew, tbase, nx, OL, RW, RH, tid);
+    for (int e = tid; e < RW * RH; e += nt) R[e] = 0.0;
+    __syncthreads();
+    MJ_UNIT_INTERIOR(
+        {
+            c0[k] = aW[p];
+            c1[k] = aW[p + 1];
+            c2[k] = aS[p];
+            c3[k] = aS[p + nx];
+            c4[k] = aC[p];
+            c5[k] = x[p];
+            c6[k] = rin[p];
+        },
+        {
+            x[p] = c5[k] + alpha * S[q];
+            const double v = c6[k] - alpha * (c0[k] * S[q - 1] + c1[k] * S[q + 1] + c2[k] * S[q - RW] +
+                                              c3[k] * S[q + RW] + c4[k] * S[q]);
+            rout[p] = v;
+            R[q] = v;
+        })
+    for (int k2 = rk0 + tid; k2 < rk1; k2 += nt) {
+        const int s = ring_src[k2];
+        double v = rin[s] - alpha * op_A(aW, aS, aC, snew, s, nx);
+        v = ring_sign[k2] < 0 ? -v : v;
+        const int d = ring_dst[k2];
+        rout[d] = v;
+        const int dq = d - tbase, dj = dq / nx, di = dq - dj * nx;
+        R[(dj - OL + 1) * RW + (di - OL + 1)] = v;
+    }
+    __syncthreads();
+    MJ_UNIT_INTERIOR(
+        {
+            c0[k] = pC[p];
+            c1[k] = pW[p];
+            c2[k] = pW[p + 1];
+            c3[k] = pS[p];
+            c4[k] = pS[p + nx];
+        },
+        {
+            S[e_] = (c0[k] * R[q] + c1[k] * R[q - 1] + c2[k] * R[q + 1] + c3[k] * R[q - RW] + c4[k] * R[q + RW]) *
+                    R[q];
+        })
+    __syncthreads();
+    double2 out;
+    out.x = tile_reduce<MODE>(
+        [&](int jl, int il) {
+            const double v = R[(jl + 1) * RW + il + 1];
+            return v * v;
+        },
+        g_rowbuf, sNx, sNy, tid);
+    out.y = tile_reduce<MODE>([&](int jl, int il) { return S[jl * sNx + il]; }, g_rowbuf, sNx, sNy, tid);
+    return out;
+}
 
 template <int MODE>
-__global__ void __launch_bounds__(1024) cg2d_fused_kernel(CgDev a, GridSync gs) {
-    __shared__ double rowbuf[2 * 128];
+__global__ void __launch_bounds__(MJ_CG_THREADS) cg2d_fused_kernel(CgDev a, GridSync gs) {
+    double* R = g_R;
+    double* S = g_S;
+    double* rowbuf = g_rowbuf;
     __shared__ double redbuf[16];
-    __shared__ int s_flag;
-    Red<MODE> red{rowbuf, MODE == MJ_SUM_FORTRAN ? a.prod + (size_t)blockIdx.x * 2 * a.sNx * a.sNy : nullptr, a.sNx,
-                  a.sNy};
+    __shared__ int wcnt[32];
+    __shared__ int s_nact, s_nfin;
+    __shared__ short alist[kMaxMembers], flist[kMaxMembers];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int nunits = a.B * a.T;
     const int plane = a.plane;
+    const int RW = a.sNx + 2, RH = a.sNy + 2;
+    const int sNx = a.sNx, sNy = a.sNy;
+    auto opbase = [&](int m) { return (int64_t)(a.op_index ? a.op_index[m] : m) * a.oms; };
+    for (int m = blockIdx.x * blockDim.x + tid; m < a.B; m += gridDim.x * blockDim.x) a.mdone[m] = -1;
 
     // ---------------------------------------------------------------- I0: per-tile max |b * cg2dNorm|
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
@@ -161,10 +384,11 @@ __global__ void __launch_bounds__(1024) cg2d_fused_kernel(CgDev a, GridSync gs) 
         const double* b = a.b + m * a.ms;
         double* x = a.x + m * a.ms;
         double *r0 = a.r0 + m * a.ms, *r1 = a.r1 + m * a.ms, *s0 = a.s0 + m * a.ms, *s1 = a.s1 + m * a.ms;
+        const int64_t ob = opbase(m);
         // halo points: the first guess itself; the work vectors: 0 (s0 entirely)
-        for (int q = tid; q < plane; q += blockDim.x) {
-            const int j = q / a.nx, i = q - j * a.nx, p = tbase + q;
-            const bool inner = j >= a.OL && j < a.OL + a.sNy && i >= a.OL && i < a.OL + a.sNx;
+        for (int qq = tid; qq < plane; qq += blockDim.x) {
+            const int j = qq / a.nx, i = qq - j * a.nx, p = tbase + qq;
+            const bool inner = j >= a.OL && j < a.OL + sNy && i >= a.OL && i < a.OL + sNx;
             s0[p] = 0.0;
             if (!inner) {
                 x[p] = x0[p];
@@ -180,34 +404,26 @@ __global__ void __launch_bounds__(1024) cg2d_fused_kernel(CgDev a, GridSync gs) 
         for (int k = a.full_off[t] + tid; k < a.full_off[t + 1]; k += blockDim.x) {
             const int s = a.full_src[k];
             const int sq = s % plane, sj = sq / a.nx, si = sq - sj * a.nx;
-            const bool inner = sj >= a.OL && sj < a.OL + a.sNy && si >= a.OL && si < a.OL + a.sNx;
+            const bool inner = sj >= a.OL && sj < a.OL + sNy && si >= a.OL && si < a.OL + sNx;
             const double v = inner ? x0[s] * rhsNorm : x0[s];
             x[a.full_dst[k]] = a.full_sign[k] < 0 ? -v : v;
         }
         __syncthreads();
-        MJ_ROWS({
-            const double bn0 = b[p0] * nm * rhsNorm;
-            const double r_0 = bn0 - op_A(a.aW + m * a.oms, a.aS + m * a.oms, a.aC + m * a.oms, x, p0, a.nx);
-            r0[p0] = r_0;
-            double bn1 = 0.0, r_1 = 0.0;
-            if (has1) {
-                bn1 = b[p1] * nm * rhsNorm;
-                r_1 = bn1 - op_A(a.aW + m * a.oms, a.aS + m * a.oms, a.aC + m * a.oms, x, p1, a.nx);
-                r0[p1] = r_1;
-            }
-            red.put(0, jl, lane, r_0 * r_0, r_1 * r_1, has1);
-            red.put(1, jl, lane, bn0, bn1, has1);
+        // r = b - A x on the interior; per-point r^2 into R, b into S (as sNy x sNx arrays)
+        MJ_INTERIOR({
+            const double bn = b[p] * nm * rhsNorm;
+            const double rr = bn - op_A(a.aW + ob, a.aS + ob, a.aC + ob, x, p, a.nx);
+            r0[p] = rr;
+            R[e_] = rr * rr;
+            S[e_] = bn;
         })
         __syncthreads();
-        if (wave == 0) {
-            const double e = red.tile(0, lane);
-            const double sr = red.tile(1, lane);
-            if (lane == 0) {
-                a.perr[u] = e;  // slot 0: err(0)
-                a.psum[u] = sr;
-            }
+        const double e0 = tile_reduce<MODE>([&](int jl, int il) { return R[jl * sNx + il]; }, rowbuf, sNx, sNy, tid);
+        const double sr = tile_reduce<MODE>([&](int jl, int il) { return S[jl * sNx + il]; }, rowbuf, sNx, sNy, tid);
+        if (tid == 0) {
+            a.perr[u] = e0;  // slot 0: err(0)
+            a.psum[u] = sr;
         }
-        __syncthreads();
     }
     if (!grid_sync(gs)) goto done;
 
@@ -215,22 +431,18 @@ __global__ void __launch_bounds__(1024) cg2d_fused_kernel(CgDev a, GridSync gs) 
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int m = u / a.T, t = u - m * a.T, tbase = t * plane;
         double* r0 = a.r0 + m * a.ms;
-        const double *pW = a.pW + m * a.oms, *pS = a.pS + m * a.oms, *pC = a.pC + m * a.oms;
+        const int64_t ob = opbase(m);
         for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
             const double v = r0[a.ring_src[k]];
             r0[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
         }
         __syncthreads();
-        MJ_ROWS({
-            const double q0 = op_M(pW, pS, pC, r0, p0, a.nx);
-            const double q1 = has1 ? op_M(pW, pS, pC, r0, p1, a.nx) : 0.0;
-            red.put(0, jl, lane, q0 * r0[p0], has1 ? q1 * r0[p1] : 0.0, has1);
-        })
+        stage_tile(R, r0, tbase, a.nx, a.OL, RW, RH, tid);
         __syncthreads();
-        if (wave == 0) {
-            const double e = red.tile(0, lane);
-            if (lane == 0) a.peta[a.B * a.T + u] = e;  // slot 1: eta(1)
-        }
+        MJ_INTERIOR({ S[e_] = lds_M(a.pW + ob, a.pS + ob, a.pC + ob, R, p, a.nx, q, RW) * R[q]; })
+        __syncthreads();
+        const double e1 = tile_reduce<MODE>([&](int jl, int il) { return S[jl * sNx + il]; }, rowbuf, sNx, sNy, tid);
+        if (tid == 0) a.peta[a.B * a.T + u] = e1;  // slot 1: eta(1)
         if (t == 0 && tid == 0) {
             const double err0 = tile_order_sum(a.perr + m * a.T, a.T);
             a.diag[m].firstResidual = sqrt(err0);
@@ -238,123 +450,117 @@ __global__ void __launch_bounds__(1024) cg2d_fused_kernel(CgDev a, GridSync gs) 
             a.diag[m].rhsMax = member_rhs_max(a, m);
             a.diag[m].outerIters = 0;
             a.diag[m].pad_ = 0;
-            a.niter[m] = -1;
         }
-        __syncthreads();
     }
     if (!grid_sync(gs)) goto done;
 
     // ---------------------------------------------------------------- iterations
+    // Every block derives the same member status from the same partials at the start of B(n): finished earlier
+    // (mdone[m] < n-1), finishing now (stopping rule on err(n-1)), or active. Active members are compacted into
+    // alist and their units are spread evenly over the grid, tile-major and in contiguous chunks per XCD (block b
+    // runs on XCD b % kXcds), so the members of one tile share its coefficients in that XCD's L2.
     for (int n = 1;; ++n) {
         const int BT = a.B * a.T;
         const double* peta_n = a.peta + (n & 1) * BT;
         const double* peta_m = a.peta + ((n - 1) & 1) * BT;
         const double* perr_m = a.perr + ((n - 1) & 1) * BT;
         double* psig_n = a.psig + (n & 1) * BT;
-        // ------------------------------------------------------------ phase B(n)
-        for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
-            const int m = u / a.T, t = u - m * a.T, tbase = t * plane;
-            const int ni = a.niter[m];
-            if (ni >= 0 && ni < n - 1) continue;  // finished in an earlier iteration
-            const double errp = tile_order_sum(perr_m + m * a.T, a.T);
-            if (!(n - 1 < a.max_iters) || errp < a.tol_sq) {
-                // member done after n-1 iterations (cg2d.F:213/:346 stopping rule)
-                const double rhsNorm = member_rhs_norm(a, m);
-                double* x = a.x + m * a.ms;
-                MJ_ROWS({
-                    x[p0] = x[p0] / rhsNorm;
-                    if (has1) x[p1] = x[p1] / rhsNorm;
-                })
-                if (t == 0 && tid == 0) {
-                    a.niter[m] = n - 1;
-                    a.diag[m].lastResidual = sqrt(errp);
-                    a.diag[m].numIters = n - 1;
-                    a.diag[m].converged = errp < a.tol_sq ? 1 : 0;
-                    atomicSub(a.nactive, 1);
+        // ------------------------------------------------------------ member status and compaction
+        for (int c0 = 0; c0 < a.B; c0 += blockDim.x) {
+            const int m = c0 + tid;
+            int code = 0;  // 0 finished earlier, 1 finishing now, 2 active
+            if (m < a.B) {
+                const int st = a.mdone[m];
+                if (!(st >= 0 && st < n - 1)) {
+                    const double errp = tile_order_sum(perr_m + m * a.T, a.T);
+                    code = (!(n - 1 < a.max_iters) || errp < a.tol_sq) ? 1 : 2;
                 }
-                __syncthreads();
-                continue;
             }
+            const unsigned long long ma = __ballot(code == 2), mf = __ballot(code == 1);
+            const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+            if (lane == 0) {
+                wcnt[wave] = __popcll(ma);
+                wcnt[16 + wave] = __popcll(mf);
+            }
+            __syncthreads();
+            int oa = c0 == 0 ? 0 : s_nact, of = c0 == 0 ? 0 : s_nfin;
+            for (int w = 0; w < wave; ++w) {
+                oa += wcnt[w];
+                of += wcnt[16 + w];
+            }
+            if (code == 2) alist[oa + __popcll(ma & below)] = (short)m;
+            if (code == 1) flist[of + __popcll(mf & below)] = (short)m;
+            __syncthreads();
+            if (tid == 0) {
+                int ta = c0 == 0 ? 0 : s_nact, tf = c0 == 0 ? 0 : s_nfin;
+                for (int w = 0; w < nwaves; ++w) {
+                    ta += wcnt[w];
+                    tf += wcnt[16 + w];
+                }
+                s_nact = ta;
+                s_nfin = tf;
+            }
+            __syncthreads();
+        }
+        const int nact = s_nact, nfin = s_nfin;
+        // ------------------------------------------------------------ finishing members (cg2d.F:380-395)
+        for (int it = blockIdx.x; it < nfin * a.T; it += gridDim.x) {
+            const int tid = opaque_tid();
+            const int m = flist[it / a.T], t = it % a.T, tbase = t * plane;
+            const double rhsNorm = member_rhs_norm(a, m);
+            double* x = a.x + m * a.ms;
+            MJ_INTERIOR({ x[p] = x[p] / rhsNorm; })
+            if (t == 0 && tid == 0) {
+                const double errp = tile_order_sum(perr_m + m * a.T, a.T);
+                a.diag[m].lastResidual = sqrt(errp);
+                a.diag[m].numIters = n - 1;
+                a.diag[m].converged = errp < a.tol_sq ? 1 : 0;
+            }
+        }
+        if (blockIdx.x == 0)
+            for (int k = tid; k < nfin; k += blockDim.x) a.mdone[flist[k]] = n - 1;
+        if (nact == 0) break;
+        // XCD-contiguous chunk of the nact*T active units this block works on
+        const int nitems = nact * a.T;
+        const int xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
+        const int gx = (gridDim.x - xcd + kXcds - 1) / kXcds;
+        const int i0 = (int)((int64_t)nitems * xcd / kXcds), i1 = (int)((int64_t)nitems * (xcd + 1) / kXcds);
+        // ------------------------------------------------------------ phase B(n)
+        for (int it = i0 + lb; it < i1; it += gx) {
+            const int t = it / nact, m = alist[it - t * nact], u = m * a.T + t, tbase = t * plane;
             const double eta = tile_order_sum(peta_n + m * a.T, a.T);
             const double etaNM1 = n == 1 ? 1.0 : tile_order_sum(peta_m + m * a.T, a.T);
             const double beta = eta / etaNM1;
             const double* rin = ((n - 1) & 1 ? a.r1 : a.r0) + m * a.ms;
             const double* sold = ((n - 1) & 1 ? a.s1 : a.s0) + m * a.ms;
             double* snew = (n & 1 ? a.s1 : a.s0) + m * a.ms;
-            const double *pW = a.pW + m * a.oms, *pS = a.pS + m * a.oms, *pC = a.pC + m * a.oms;
-            const double *aW = a.aW + m * a.oms, *aS = a.aS + m * a.oms, *aC = a.aC + m * a.oms;
-            MJ_ROWS({
-                snew[p0] = op_M(pW, pS, pC, rin, p0, a.nx) + beta * sold[p0];
-                if (has1) snew[p1] = op_M(pW, pS, pC, rin, p1, a.nx) + beta * sold[p1];
-            })
-            for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-                const int s = a.ring_src[k];
-                const double v = op_M(pW, pS, pC, rin, s, a.nx) + beta * sold[s];
-                snew[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
-            }
-            __syncthreads();
-            MJ_ROWS({
-                const double v0 = snew[p0] * op_A(aW, aS, aC, snew, p0, a.nx);
-                const double v1 = has1 ? snew[p1] * op_A(aW, aS, aC, snew, p1, a.nx) : 0.0;
-                red.put(0, jl, lane, v0, v1, has1);
-            })
-            __syncthreads();
-            if (wave == 0) {
-                const double sg = red.tile(0, lane);
-                if (lane == 0) psig_n[u] = sg;
-            }
-            __syncthreads();
+            const int64_t ob = opbase(m);
+            const double sg = unit_phase_B<MODE>(rin, sold, snew, a.pW + ob, a.pS + ob, a.pC + ob, a.aW + ob,
+                                                 a.aS + ob, a.aC + ob, a.ring_dst, a.ring_src, a.ring_sign,
+                                                 a.ring_off[t], a.ring_off[t + 1], beta, tbase, a.nx, a.OL, sNx, sNy);
+            if (tid == 0) psig_n[u] = sg;
         }
         if (!grid_sync(gs)) goto done;
-        if (tid == 0) s_flag = __hip_atomic_load(a.nactive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        if (s_flag == 0) break;
         // ------------------------------------------------------------ phase C(n)
         double* perr_n = a.perr + (n & 1) * BT;
         double* peta_p = a.peta + ((n + 1) & 1) * BT;
-        for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
-            const int m = u / a.T, t = u - m * a.T, tbase = t * plane;
-            if (a.niter[m] >= 0) continue;
+        for (int it = i0 + lb; it < i1; it += gx) {
+            const int t = it / nact, m = alist[it - t * nact], u = m * a.T + t, tbase = t * plane;
             const double eta = tile_order_sum(peta_n + m * a.T, a.T);
             const double alpha = eta / tile_order_sum(psig_n + m * a.T, a.T);
             const double* snew = (n & 1 ? a.s1 : a.s0) + m * a.ms;
             const double* rin = ((n - 1) & 1 ? a.r1 : a.r0) + m * a.ms;
             double* rout = (n & 1 ? a.r1 : a.r0) + m * a.ms;
-            double* x = a.x + m * a.ms;
-            const double *pW = a.pW + m * a.oms, *pS = a.pS + m * a.oms, *pC = a.pC + m * a.oms;
-            const double *aW = a.aW + m * a.oms, *aS = a.aS + m * a.oms, *aC = a.aC + m * a.oms;
-            MJ_ROWS({
-                x[p0] = x[p0] + alpha * snew[p0];
-                rout[p0] = rin[p0] - alpha * op_A(aW, aS, aC, snew, p0, a.nx);
-                if (has1) {
-                    x[p1] = x[p1] + alpha * snew[p1];
-                    rout[p1] = rin[p1] - alpha * op_A(aW, aS, aC, snew, p1, a.nx);
-                }
-            })
-            for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-                const int s = a.ring_src[k];
-                const double v = rin[s] - alpha * op_A(aW, aS, aC, snew, s, a.nx);
-                rout[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
+            const int64_t ob = opbase(m);
+            const double2 eq = unit_phase_C<MODE>(snew, rin, rout, a.x + m * a.ms, a.pW + ob, a.pS + ob, a.pC + ob,
+                                                  a.aW + ob, a.aS + ob, a.aC + ob, a.ring_dst, a.ring_src,
+                                                  a.ring_sign, a.ring_off[t], a.ring_off[t + 1], alpha, tbase, a.nx,
+                                                  a.OL, sNx, sNy);
+            const double e = eq.x, qv = eq.y;
+            if (tid == 0) {
+                perr_n[u] = e;
+                peta_p[u] = qv;
             }
-            __syncthreads();
-            MJ_ROWS({
-                const double e0 = rout[p0] * rout[p0];
-                const double e1 = has1 ? rout[p1] * rout[p1] : 0.0;
-                red.put(0, jl, lane, e0, e1, has1);
-                const double q0 = op_M(pW, pS, pC, rout, p0, a.nx) * rout[p0];
-                const double q1 = has1 ? op_M(pW, pS, pC, rout, p1, a.nx) * rout[p1] : 0.0;
-                red.put(1, jl, lane, q0, q1, has1);
-            })
-            __syncthreads();
-            if (wave == 0) {
-                const double e = red.tile(0, lane);
-                const double q = red.tile(1, lane);
-                if (lane == 0) {
-                    perr_n[u] = e;
-                    peta_p[u] = q;
-                }
-            }
-            __syncthreads();
         }
         if (!grid_sync(gs)) goto done;
     }
@@ -376,8 +582,8 @@ __global__ void cg2d_init_kernel(unsigned int* counter, int* error, int* nactive
 
 // (A / cg2dNorm) v on the interior, 0 on halos, v already exchanged (core/cg2d.py:292-298)
 __global__ void cg2d_matvec_kernel(const double* aW, const double* aS, const double* aC, int64_t oms,
-                                   const double* v, double* y, const double* norm, int64_t ms, int T, int ny, int nx,
-                                   int OL, int sNx, int sNy) {
+                                   const int32_t* op_index, const double* v, double* y, const double* norm, int64_t ms,
+                                   int T, int ny, int nx, int OL, int sNx, int sNy) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int m = blockIdx.y;
     const int plane = ny * nx;
@@ -385,7 +591,8 @@ __global__ void cg2d_matvec_kernel(const double* aW, const double* aS, const dou
     const int r = (int)(q % plane), j = r / nx, i = r - j * nx;
     const bool inner = j >= OL && j < OL + sNy && i >= OL && i < OL + sNx;
     const int p = (int)q;
-    y[m * ms + p] = inner ? op_A(aW + m * oms, aS + m * oms, aC + m * oms, v + m * ms, p, nx) / norm[m] : 0.0;
+    const int64_t ob = (int64_t)(op_index ? op_index[m] : m) * oms;
+    y[m * ms + p] = inner ? op_A(aW + ob, aS + ob, aC + ob, v + m * ms, p, nx) / norm[m] : 0.0;
 }
 
 __global__ void zero_halo_copy_kernel(const double* v, double* w, int64_t ms, int T, int ny, int nx, int OL, int sNx,
@@ -406,10 +613,8 @@ using namespace mj;
 // ------------------------------------------------------------------------------------------------ host side
 namespace {
 
-constexpr int kFortranMaxBlocks = 1024;  // grid cap of the fortran sum order (bounds its staging area)
-
 struct WsLayout {
-    size_t r0, r1, s0, s1, pmax, psum, perr, peta, psig, niter, misc, prod, total;
+    size_t r0, r1, s0, s1, pmax, psum, perr, peta, psig, mdone, misc, total;
 };
 
 WsLayout ws_layout(const mj_cg2d_config* c) {
@@ -427,27 +632,26 @@ WsLayout ws_layout(const mj_cg2d_config* c) {
     L.perr = o; o += 2 * bt;
     L.peta = o; o += 2 * bt;
     L.psig = o; o += 2 * bt;
-    L.niter = o; o += al((size_t)c->batch * 4);
+    L.mdone = o; o += al((size_t)c->batch * 4);
     L.misc = o; o += 256;  // barrier counter, error flag, nactive
-    L.prod = o;
-    if (c->sum_order == MJ_SUM_FORTRAN) o += al((size_t)kFortranMaxBlocks * 2 * c->sNx * c->sNy * 8);
     L.total = o;
     return L;
 }
 
 int check_cfg(const mj_cg2d_config* c) {
     MJ_REQUIRE(c, "cg2d: null config");
-    MJ_REQUIRE(c->batch >= 1 && c->batch <= 65535, "cg2d: batch %d out of range", c->batch);
+    MJ_REQUIRE(c->batch >= 1 && c->batch <= kMaxMembers, "cg2d: batch %d out of range 1..%d", c->batch,
+               kMaxMembers);
     MJ_REQUIRE(c->ntiles >= 1 && c->OL >= 1, "cg2d: bad layout");
-    MJ_REQUIRE(c->sNx >= 64 && c->sNx <= 128 && c->sNy >= 64 && c->sNy <= 128,
-               "cg2d: sNx, sNy must be in 64..128 (got %d, %d)", c->sNx, c->sNy);
+    MJ_REQUIRE(c->sNx >= 64 && c->sNx <= kRingMax - 2 && c->sNy >= 64 && c->sNy <= kRingMax - 2,
+               "cg2d: sNx, sNy must be in 64..%d (got %d, %d)", kRingMax - 2, c->sNx, c->sNy);
     const int64_t plane = (int64_t)(c->sNx + 2 * c->OL) * (c->sNy + 2 * c->OL);
     MJ_REQUIRE(c->member_stride >= c->ntiles * plane, "cg2d: member_stride too small");
     MJ_REQUIRE(c->sum_order == MJ_SUM_FORTRAN || c->sum_order == MJ_SUM_TREE, "cg2d: bad sum_order %d", c->sum_order);
     MJ_REQUIRE(c->max_iters >= 0, "cg2d: max_iters < 0");
     MJ_REQUIRE(c->block_threads == 0 || (c->block_threads % 64 == 0 && c->block_threads >= 64 &&
-                                         c->block_threads <= 1024),
-               "cg2d: block_threads must be a multiple of 64 in 64..1024");
+                                         c->block_threads <= MJ_CG_THREADS),
+               "cg2d: block_threads must be a multiple of 64 in 64..%d", MJ_CG_THREADS);
     return MJ_OK;
 }
 
@@ -572,8 +776,8 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     a.perr = (double*)(w + L.perr);
     a.peta = (double*)(w + L.peta);
     a.psig = (double*)(w + L.psig);
-    a.niter = (int32_t*)(w + L.niter);
-    a.prod = (double*)(w + L.prod);
+    a.op_index = ops->op_index;
+    a.mdone = (int32_t*)(w + L.mdone);
     unsigned int* counter = (unsigned int*)(w + L.misc);
     int* error = (int*)(w + L.misc + 64);
     a.nactive = (int*)(w + L.misc + 128);
@@ -588,7 +792,7 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     MJ_REQUIRE(coop, "mj_cg2d_solve: device does not support cooperative launches");
     const int nunits = a.B * a.T;
     int threads = cfg->block_threads;
-    if (threads == 0) threads = nunits >= 2 * ncu ? 256 : 1024;
+    if (threads == 0) threads = MJ_CG_THREADS;
     const size_t shmem = 0;
     const void* kfn = cfg->sum_order == MJ_SUM_FORTRAN ? (const void*)cg2d_fused_kernel<MJ_SUM_FORTRAN>
                                                        : (const void*)cg2d_fused_kernel<MJ_SUM_TREE>;
@@ -597,8 +801,7 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     MJ_REQUIRE(per_cu >= 1, "mj_cg2d_solve: kernel cannot be resident with %d threads", threads);
     int grid = std::min(nunits, per_cu * ncu);
     if (cfg->grid_blocks > 0) grid = std::min(grid, cfg->grid_blocks);
-    if (cfg->sum_order == MJ_SUM_FORTRAN) grid = std::min(grid, kFortranMaxBlocks);
-
+    MJ_REQUIRE(grid >= kXcds || grid == nunits, "mj_cg2d_solve: grid of %d blocks is too small", grid);
     GridSync gs{counter, error, (unsigned)grid, 0u,
                 (unsigned long long)(clk_khz > 0 ? clk_khz : 100000) * 1000ull * 4ull /* 4 s */};
     hipStream_t st = as_stream(stream);
@@ -625,7 +828,7 @@ extern "C" int mj_cg2d_matvec(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops
     mj_halo_geom g{cfg->batch, cfg->member_stride, (int64_t)nx * ny, 1, 0};
     if ((rc = mj_halo_exchange(plan_T, work, work, &g, stream))) return rc;
     hipLaunchKernelGGL(cg2d_matvec_kernel, grid, dim3(256), 0, st, ops->aW2d, ops->aS2d, ops->aC2d,
-                       ops->member_stride, (const double*)work, y, cg2dNorm, cfg->member_stride, cfg->ntiles, ny, nx,
+                       ops->member_stride, ops->op_index, (const double*)work, y, cg2dNorm, cfg->member_stride, cfg->ntiles, ny, nx,
                        cfg->OL, cfg->sNx, cfg->sNy);
     MJ_HIP(hipGetLastError());
     return MJ_OK;

@@ -112,13 +112,35 @@ def test_cg2d_batched_members_bitwise(dev, ex):
     torch.cuda.synchronize()
 
 
+def test_cg2d_operator_slots_bitwise(dev, ex):
+    """Members reading different operator slots (5-point and diagonal preconditioners) in one launch through
+    op_index: each equals its own single solve on the oracle, in both sum orders."""
+    from mit_jax_amd import cg2d, synthetic as S
+    g, ops5, norm = S.cg2d_case(0, "5point")
+    _, opsd, _ = S.cg2d_case(0, "diagonal")
+    slots = [to_dev(np.stack([a5, ad]), dev) for a5, ad in zip(ops5, opsd)]
+    op_index = [1, 0, 0, 1, 1]
+    bs = [S.rhs(g, 20 + m) for m in range(len(op_index))]
+    b = np.stack(bs)
+    for order in ("tile", "fortran"):
+        x, d = cg2d.cg2d_fortran(ex, slots, to_dev(b, dev), to_dev(np.zeros_like(b), dev), cg2dNorm=norm,
+                                 tolerance=1e-12, max_iters=2000, sum_order=order, op_index=op_index)
+        d.check()
+        xs = x.cpu().numpy()
+        for m, k in enumerate(op_index):
+            xr, dr = oracle.cg2d(maps().full("T"), (ops5, opsd)[k], bs[m], np.zeros_like(bs[m]), cg2dNorm=norm,
+                                 tolerance=1e-12, max_iters=2000, sum_order=order)
+            assert int(d["numIters"][m]) == dr["numIters"], (order, m)
+            assert np.array_equal(xs[m], xr), (order, m)
+
+
 def test_cg2d_block_size_independent(dev, ex):
     """The reduction trees do not depend on the launch geometry: any block size gives the same bits."""
     from mit_jax_amd import cg2d
     g, ops, norm, b = _cg_inputs("5point")
     x0 = np.zeros_like(b)
     outs = []
-    for bt in (64, 256, 1024):
+    for bt in (128, 256, 512):
         x, d = cg2d.cg2d_fortran(ex, [to_dev(o, dev) for o in ops], to_dev(b, dev), to_dev(x0, dev), cg2dNorm=norm,
                                  tolerance=1e-12, max_iters=2000, sum_order="tile", block_threads=bt)
         d.check()
