@@ -3,7 +3,8 @@
  * Drop-in boundary for the ocean time step's two data-parallel cores of mitgcm_jax (reference file:line citations
  * below are into that repository). Every entry point takes plain device pointers, element strides and sizes; all
  * device buffers belong to the caller; the library owns only the plans it creates (halo tables, DST3 pass tables).
- * Every call is asynchronous on the given HIP stream (NULL = the null stream) and returns 0 (MJ_OK) or a negative
+ * Every call is asynchronous on the given HIP stream (NULL = the null stream; the one exception is mj_cg2d_solve in
+ * MJ_CG_LAUNCH_PHASES mode, which waits for the stream between chunks) and returns 0 (MJ_OK) or a negative
  * code; mj_last_error() then describes the failure. Solver non-convergence is not an error: it is reported per
  * member in mj_cg2d_diag (reference core/cg2d.py:267-268, diag numIters / lastResidual).
  *
@@ -37,6 +38,19 @@ extern "C" {
 
 #define MJ_SUM_FORTRAN 0
 #define MJ_SUM_TREE 1
+
+/* Launch shapes of mj_cg2d_solve (same arithmetic, same bits):
+ *   PERSISTENT    one cooperative launch with grid barriers; asynchronous. Best for small batches.
+ *   PHASES        one launch per CG phase, one block per (member, tile); the host enqueues poll_iters iterations
+ *                 at a time and reads the number of active members between chunks (the call returns once all
+ *                 members have finished). Best for large batches.
+ *   PHASES_FIXED  as PHASES, but enqueues exactly max_iters iterations (finished members' blocks exit at once)
+ *                 and never reads back: asynchronous and capturable in a HIP graph.
+ *   AUTO          PERSISTENT below 256 (member, tile) units, PHASES from there. */
+#define MJ_CG_LAUNCH_AUTO 0
+#define MJ_CG_LAUNCH_PERSISTENT 1
+#define MJ_CG_LAUNCH_PHASES 2
+#define MJ_CG_LAUNCH_PHASES_FIXED 3
 
 /* Library version (major*10000 + minor*100 + patch). */
 int mj_version(void);
@@ -101,13 +115,15 @@ typedef struct {
 
 typedef struct {
     int batch;           /* members B >= 1 */
-    int ntiles, sNx, sNy, OL; /* 13, 90, 90, 4 (sNx, sNy in 64..90) */
+    int ntiles, sNx, sNy, OL; /* 13, 90, 90, 4 (sNx, sNy in 64..128) */
     int64_t member_stride;    /* elements between members of b, x0, x (>= ntiles*ny*nx) */
     double tolerance;    /* cg2dTolerance (normalised target) */
     int max_iters;       /* cg2dMaxIters */
     int sum_order;       /* MJ_SUM_FORTRAN or MJ_SUM_TREE */
-    int block_threads;   /* 0 = library default (512); else a multiple of 64 in 64..512 */
-    int grid_blocks;     /* 0 = as many co-resident blocks as useful */
+    int block_threads;   /* 0 = launch-mode default (persistent 1024, phases 256); else a multiple of 64 up to it */
+    int grid_blocks;     /* persistent launch: 0 = as many co-resident blocks as useful */
+    int launch_mode;     /* MJ_CG_LAUNCH_*; results are bitwise the same in every mode */
+    int poll_iters;      /* MJ_CG_LAUNCH_PHASES: iterations enqueued between reads of the active count (0 = 32) */
 } mj_cg2d_config;
 
 /* Device workspace the solve needs (bytes), for the given configuration. */

@@ -44,7 +44,7 @@ class Cg2dOps(ctypes.Structure):
 class Cg2dConfig(ctypes.Structure):
     _fields_ = [("batch", I32), ("ntiles", I32), ("sNx", I32), ("sNy", I32), ("OL", I32), ("member_stride", I64),
                 ("tolerance", F64), ("max_iters", I32), ("sum_order", I32), ("block_threads", I32),
-                ("grid_blocks", I32)]
+                ("grid_blocks", I32), ("launch_mode", I32), ("poll_iters", I32)]
 
 
 class GadGrid(ctypes.Structure):

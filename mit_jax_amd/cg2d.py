@@ -2,7 +2,7 @@
 
 `cg2d_fortran(ex, ops, cg2d_b, cg2d_x, *, cg2dNorm, tolerance, max_iters, sum_order, nIterMin=-1,
 sum_unroll=1) -> (x, diag)` (reference core/cg2d.py:205-269) runs the literal preconditioned CG of cg2d.F through
-one persistent HIP launch (libmjamd mj_cg2d_solve): same normalisation, preconditioner, update order and stopping
+libmjamd's batched HIP solve (mj_cg2d_solve): same normalisation, preconditioner, update order and stopping
 rule `err_sq < tolerance**2`; x leaves with the Fortran halos (exchanged normalised first guess). Arrays are CUDA
 float64 [T, ny, nx], or [B, T, ny, nx] for a batch of members solved in the same launch (per-member convergence;
 ops shared [T, ny, nx] or per member [B, T, ny, nx]; cg2dNorm a scalar or B values).
@@ -120,17 +120,24 @@ def _norm_tensor(cg2dNorm, B, device):
     return n.contiguous()
 
 
-def config(L, B, tolerance, max_iters, sum_order, block_threads=0, grid_blocks=0):
+LAUNCH_MODES = {"auto": 0, "persistent": 1, "phases": 2, "phases_fixed": 3}
+
+
+def config(L, B, tolerance, max_iters, sum_order, block_threads=0, grid_blocks=0, launch="auto", poll_iters=0):
     if sum_order not in SUM_ORDERS:
         raise ValueError(sum_order)
+    if launch not in LAUNCH_MODES:
+        raise ValueError(f"launch must be one of {sorted(LAUNCH_MODES)}")
     return _lib.Cg2dConfig(B, L.nTiles, L.sNx, L.sNy, L.OLx, L.npad, float(tolerance), int(max_iters),
-                           SUM_ORDERS[sum_order], int(block_threads), int(grid_blocks))
+                           SUM_ORDERS[sum_order], int(block_threads), int(grid_blocks), LAUNCH_MODES[launch],
+                           int(poll_iters))
 
 
 def cg2d_fortran(ex, ops, cg2d_b, cg2d_x, *, cg2dNorm, tolerance, max_iters, sum_order="fortran", nIterMin=-1,
-                 sum_unroll=1, block_threads=0, grid_blocks=0, out=None, op_index=None):
+                 sum_unroll=1, block_threads=0, grid_blocks=0, out=None, op_index=None, launch="auto", poll_iters=0):
     """CG2D (reference core/cg2d.py:205-269). ops = (aW2d, aS2d, aC2d, pW, pS, pC). Returns (x, diag).
-    Batched: cg2d_b, cg2d_x [B, T, ny, nx]; ops shared, per member, or per slot with op_index."""
+    Batched: cg2d_b, cg2d_x [B, T, ny, nx]; ops shared, per member, or per slot with op_index. launch picks the
+    launch shape ("auto", "persistent", "phases", "phases_fixed"); every shape gives the same bits."""
     if nIterMin >= 0:
         raise NotImplementedError("nIterMin >= 0 (min-residual solution) is not ported")
     L = ex.L
@@ -145,7 +152,7 @@ def cg2d_fortran(ex, ops, cg2d_b, cg2d_x, *, cg2dNorm, tolerance, max_iters, sum
     dev = b.device
     ops_s, ops_keep = _ops_struct(ops, B, L, dev, op_index)
     norm = _norm_tensor(cg2dNorm, B, dev)
-    cfg = config(L, B, tolerance, max_iters, sum_order, block_threads, grid_blocks)
+    cfg = config(L, B, tolerance, max_iters, sum_order, block_threads, grid_blocks, launch, poll_iters)
     lib = _lib.lib()
     wsb = lib.mj_cg2d_workspace_bytes(ctypes.byref(cfg))
     if wsb == 0:

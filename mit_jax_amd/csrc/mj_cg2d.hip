@@ -1,4 +1,4 @@
-// CG2D on gfx950: one persistent cooperative launch per (batched) solve, two grid barriers per CG iteration.
+// CG2D on gfx950: the batched preconditioned conjugate-gradient solve of the free-surface equation.
 //
 // Algorithm: the literal cg2d_fortran of the reference (core/cg2d.py:205-269, cg2d.F:110-395). Per member:
 //   b <- cg2d_b * cg2dNorm; rhsMax = max|b|; rhsNorm = 1/rhsMax (1 if 0); b <- b * rhsNorm; x <- x0 * rhsNorm on
@@ -6,21 +6,30 @@
 //   !(err < tol^2): q = M r; eta = sum q r; beta = eta/eta_prev; s = q + beta s; exch s; q = A s;
 //   alpha = eta / sum s q; x += alpha s; r -= alpha q; err = sum r^2; exch r. x /= rhsNorm on the interior.
 //
-// Work decomposition: a unit is one (member, tile); a block owns whole units, its waves own rows (one 90-point
-// row per wave: lane l holds i = l and i = l + 64), so the fixed row/tile reduction trees of MJ_SUM_TREE (or the
-// sequential MJ_SUM_FORTRAN sum, staged per block) are computed inside one block. Global sums are the fixed tile-order
-// sum of the per-tile partials, recomputed identically by every block that needs them: no floating-point atomics,
-// results independent of the launch geometry.
+// Work decomposition: a unit is one (member, tile), processed by one block whose waves own rows (one 90-point row
+// per wave: lane l holds i = l and i = l + 64), so the fixed row/tile reduction trees of MJ_SUM_TREE (or the
+// sequential MJ_SUM_FORTRAN sum, staged per unit) are computed inside one block. Global sums are the fixed
+// tile-order sum of the per-tile partials, recomputed identically by every block that needs them: no
+// floating-point atomics, results independent of the launch geometry.
 //
 // Exchanges: the stencils read only the width-1 ring (i = 0, sNx+1 for j = 1..sNy; j = 0, sNy+1 for i = 1..sNx),
 // whose written points are all sourced from interior points of other tiles. Instead of exchanging after each
-// update (a third barrier), a block computes the ring values of its own tile directly from the sources with the
-// same arithmetic the source's owner uses, which gives the same bits as the exchanged copy. s and r are ping-pong
-// buffers so a source is never overwritten while another block reads it. Per iteration:
-//   phase B(n): beta from eta(n), eta(n-1); s_n = M r_{n-1} + beta s_{n-1} on interior + ring; sigma = sum s_n A s_n
+// update, a block computes the ring values of its own tile directly from the sources with the same arithmetic the
+// source's owner uses, which gives the same bits as the exchanged copy. s and r are ping-pong buffers so a source
+// is never overwritten while another block reads it. Per iteration two phases:
+//   phase B(n): stopping rule on err(n-1) (finished members un-normalise x and leave); beta = eta(n)/eta(n-1);
+//               s_n = M r_{n-1} + beta s_{n-1} on interior + ring; sigma = sum s_n A s_n
 //   phase C(n): alpha = eta(n)/sigma; x += alpha s_n; r_n = r_{n-1} - alpha A s_n on interior + ring;
 //               err(n) = sum r_n^2; eta(n+1) = sum (M r_n) r_n
 // which is the reference recurrence with eta(n+1) computed one phase early (the value is the same).
+//
+// Two launch shapes run the same unit functions, so their results are bitwise identical:
+//   persistent: one cooperative launch, phases separated by grid barriers (small batches: no launch gaps, no host
+//               round trips, fully asynchronous);
+//   phases:     one ordinary launch per phase with one block per unit (large batches: high occupancy, the block
+//               scheduler balances the units of active members; units of finished members exit at once). The host
+//               enqueues iterations in chunks and reads the count of active members between chunks, or enqueues
+//               exactly max_iters iterations (no host read, capturable in a graph).
 #include <algorithm>
 
 #include "mj_common.h"
@@ -42,12 +51,24 @@ struct CgDev {
     const int32_t *ring_off, *ring_dst, *ring_src;
     const int8_t* ring_sign;
     double *r0, *r1, *s0, *s1;
-    double *pmax, *psum, *perr, *peta, *psig;  // partials: [B*T], [B*T], [2][B*T] x3
+    double *pmax, *psum, *perr, *peta, *psig;  // partials [m * T + t]: pmax, psum [B*T]; perr, peta, psig [2][B*T]
     const int32_t* op_index;                   // operator slot per member (NULL: slot = member)
-    int* nactive;
-    int32_t* mdone;  // [B] iterations at which a member finished, -1 while active
+    double* prod;                              // MJ_SUM_FORTRAN staging: [B*T][2][sNy*sNx]
+    int* nactive;                              // members not yet finished
+    int32_t* mdone;                            // [B] iterations at which a member finished, -1 while active
     mj_cg2d_diag* diag;
 };
+
+#ifndef MJ_CG_THREADS
+#define MJ_CG_THREADS 1024
+#endif  // largest block of the persistent launch
+#ifndef MJ_PH_THREADS
+#define MJ_PH_THREADS 256
+#endif  // largest block of the phase launches
+#ifndef MJ_PH_WAVES_PER_EU
+#define MJ_PH_WAVES_PER_EU 8
+#endif  // occupancy target of the phase launches (waves per SIMD): bounds their register use
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
 // ------------------------------------------------------------------------------------------------ stencils
 // apply_operator (core/cg2d.py:181-188): aW(i)x(i-1) + aW(i+1)x(i+1) + aS(j)x(j-1) + aS(j+1)x(j+1) + aC x
@@ -61,100 +82,92 @@ __device__ __forceinline__ double op_M(const double* pW, const double* pS, const
     return pC[p] * r[p] + pW[p] * r[p - 1] + pW[p + 1] * r[p + 1] + pS[p] * r[p - nx] + pS[p + nx] * r[p + nx];
 }
 
-// ------------------------------------------------------------------------------------------------ LDS tile staging
-// A unit's (member, tile) interior plus its width-1 ring, (sNy+2) x (sNx+2) points, is staged in LDS for the
-// stencils. LDS index of padded (j, i): (j - OL + 1) * RW + (i - OL + 1), RW = sNx + 2.
-constexpr int kRingMax = 92;                    // sNx, sNy <= 90 (LLC90)
-constexpr int kTileLds = kRingMax * kRingMax;   // 8464 doubles = 67.7 KB per staged tile
+// ------------------------------------------------------------------------------------------------ reductions
+// Reduction of per-point values over a tile interior, MJ_SUM_TREE: each wave reduces whole rows with wave_tree
+// (lane l holds i = l and i = l + 64) into a row-sum array in LDS, then one wave reduces the row sums.
+// MJ_SUM_FORTRAN: per-point values are staged in global memory and one lane adds them in j/i order.
 
-// Reduction of a per-point array over the tile interior: P(jl, il) read through `get`, the tree or sequential
-// order of MODE. Entry: P complete (after a __syncthreads). Returns the tile partial in thread 0. Exit with a
-// __syncthreads (P and rowbuf reusable).
-template <int MODE, class Get>
-__device__ __forceinline__ double tile_reduce(Get get, double* rowbuf, int sNx, int sNy, int tid) {
-    const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-    double res = 0.0;
-    if (MODE == MJ_SUM_TREE) {
-        const bool hx = lane + 64 < sNx;
-        for (int jl = wave; jl < sNy; jl += nwaves) {
-            const double w = wave_tree(get(jl, lane), hx ? get(jl, lane + 64) : 0.0, hx);
-            if (lane == 0) rowbuf[jl] = w;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            const bool hy = lane + 64 < sNy;
-            res = wave_tree(rowbuf[lane], hy ? rowbuf[lane + 64] : 0.0, hy);
-        }
-    } else {
-        if (tid == 0)
-            for (int jl = 0; jl < sNy; ++jl)
-                for (int il = 0; il < sNx; ++il) res = res + get(jl, il);
-    }
-    __syncthreads();
-    return res;
-}
-
-// Stage the ring region of a padded member array into LDS, 8 loads per thread in flight.
-__device__ __forceinline__ void stage_tile(double* L, const double* __restrict__ g, int tbase, int nx, int OL, int RW,
-                                           int RH, int tid) {
-    const int n = RW * RH, nt = blockDim.x;
-    for (int e0 = 0; e0 < n; e0 += 8 * nt) {
-        double v[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int e = e0 + k * nt + tid;
-            if (e < n) {
-                const int jr = e / RW, ir = e - jr * RW;
-                v[k] = g[tbase + (jr + OL - 1) * nx + (ir + OL - 1)];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int e = e0 + k * nt + tid;
-            if (e < n) L[e] = v[k];
-        }
-    }
-}
-
-// The thread index through an empty asm the compiler cannot see through: values derived from it inside a
-// phase are computed there, not hoisted out of the persistent iteration loop (where they would all stay live
-// at once and spill).
-__device__ __forceinline__ int opaque_tid() {
-    int t = threadIdx.x;
-    asm volatile("" : "+v"(t));
-    return t;
-}
-
-// Stencils on an LDS ring tile at LDS index e (row stride RW), coefficients at padded index p.
-__device__ __forceinline__ double lds_A(const double* __restrict__ aW, const double* __restrict__ aS,
-                                        const double* __restrict__ aC, const double* L, int p, int nx, int e, int RW) {
-    return aW[p] * L[e - 1] + aW[p + 1] * L[e + 1] + aS[p] * L[e - RW] + aS[p + nx] * L[e + RW] + aC[p] * L[e];
-}
-__device__ __forceinline__ double lds_M(const double* __restrict__ pW, const double* __restrict__ pS,
-                                        const double* __restrict__ pC, const double* L, int p, int nx, int e, int RW) {
-    return pC[p] * L[e] + pW[p] * L[e - 1] + pW[p + 1] * L[e + 1] + pS[p] * L[e - RW] + pS[p + nx] * L[e + RW];
-}
-
-// Interior point loop: e over sNy*sNx; (jl, il), padded index p, LDS ring-tile index q.
-#define MJ_INTERIOR(...)                                                                                  \
-    for (int e_ = tid; e_ < a.sNy * a.sNx; e_ += blockDim.x) {                                            \
-        const int jl = e_ / a.sNx, il = e_ - jl * a.sNx;                                                  \
-        const int p = tbase + (a.OL + jl) * a.nx + a.OL + il;                                             \
-        const int q = (jl + 1) * RW + il + 1;                                                             \
-        __VA_ARGS__                                                                                       \
-    }
-
-// Global-memory row loop of the once-per-solve phases: wave w takes rows jl = w, w + nwaves, ...; lane l holds
-// i = l and i = l + 64.
+// Row loop: wave w of the block takes rows jl = w, w + nwaves, ...; p0 = (jl, il = lane), p1 = (jl, lane + 64).
+// Unrolled so that a wave has the loads of several rows in flight at once.
+#ifndef MJ_ROW_UNROLL
+#define MJ_ROW_UNROLL 2
+#endif
+#define MJ_PRAGMA(x) _Pragma(#x)
+#define MJ_UNROLL(n) MJ_PRAGMA(unroll n)
 #define MJ_ROWS(...)                                                                                      \
-    for (int jl = wave; jl < a.sNy; jl += nwaves) {                                                       \
-        const int prow = tbase + (a.OL + jl) * a.nx + a.OL;                                               \
+    MJ_UNROLL(MJ_ROW_UNROLL)                                                                              \
+    for (int jl = wave; jl < sNy; jl += nwaves) {                                                         \
+        const int prow = tbase + (OL + jl) * nx + OL;                                                     \
         const int p0 = prow + lane;                                                                       \
-        const bool has1 = lane + 64 < a.sNx;                                                              \
+        const bool has1 = lane + 64 < sNx;                                                                \
         const int p1 = prow + lane + 64;                                                                  \
         __VA_ARGS__                                                                                       \
     }
 
+#define MJ_UNIT_IDS                                                                                       \
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;              \
+    const int sNx = a.sNx, sNy = a.sNy, OL = a.OL, nx = a.nx, tbase = t * a.plane;                        \
+    (void)tid;                                                                                            \
+    (void)sNx;                                                                                            \
+    (void)OL;                                                                                             \
+    (void)nx;                                                                                             \
+    (void)tbase;
+
+// Per-block LDS: row sums of two reductions, and values broadcast from thread 0.
+__shared__ double g_rows[2 * 128];
+__shared__ double g_bcast[2];
+__shared__ int g_ibcast;
+
+// Store one row's per-point values of reduction r: tree order -> wave_tree into the row sums; fortran order -> the
+// unit's global staging area (stage[r][jl][il]).
+template <int MODE>
+__device__ __forceinline__ void put_row(int r, int jl, int lane, double v0, double v1, bool has1, double* stage,
+                                        int sNx, int sNy) {
+    if (MODE == MJ_SUM_TREE) {
+        const double w = wave_tree(v0, v1, has1);
+        if (lane == 0) g_rows[r * 128 + jl] = w;
+    } else {
+        double* pr = stage + ((size_t)r * sNy + jl) * sNx;
+        pr[lane] = v0;
+        if (has1) pr[lane + 64] = v1;
+    }
+}
+
+// After a __syncthreads: tile partial of reduction r (valid in lane 0 of the calling wave).
+template <int MODE>
+__device__ __forceinline__ double tile_of(int r, int lane, const double* stage, int sNx, int sNy) {
+    if (MODE == MJ_SUM_TREE) {
+        const double* rb = g_rows + r * 128;
+        const bool h = lane + 64 < sNy;
+        return wave_tree(rb[lane], h ? rb[lane + 64] : 0.0, h);
+    }
+    double s = 0.0;
+    if (lane == 0) {
+        const double* pr = stage + (size_t)r * sNy * sNx;
+        for (int q = 0; q < sNy * sNx; ++q) s = s + pr[q];
+    }
+    return s;
+}
+
+// Reductions 0 and 1 of the unit written to dst0[idx] and dst1[idx] (dst1 NULL: only reduction 0).
+template <int MODE>
+__device__ __forceinline__ void store_tile_sums(double* dst0, double* dst1, int idx, int lane, int wave, int nwaves,
+                                                const double* stage, int sNx, int sNy) {
+    __syncthreads();
+    const int nr = dst1 ? 2 : 1;
+    for (int r = wave; r < nr; r += nwaves) {
+        const double v = tile_of<MODE>(r, lane, stage, sNx, sNy);
+        if (lane == 0) (r == 0 ? dst0 : dst1)[idx] = v;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int64_t op_base(const CgDev& a, int m) {
+    return (int64_t)(a.op_index ? a.op_index[m] : m) * a.oms;
+}
+__device__ __forceinline__ double* stage_of(const CgDev& a, int m, int t) {
+    return a.prod + ((size_t)m * a.T + t) * 2 * a.sNx * a.sNy;
+}
 __device__ __forceinline__ double member_rhs_max(const CgDev& a, int m) {
     double mx = 0.0;
     for (int t = 0; t < a.T; ++t) mx = fmax(mx, a.pmax[m * a.T + t]);
@@ -164,412 +177,450 @@ __device__ __forceinline__ double member_rhs_norm(const CgDev& a, int m) {
     const double mx = member_rhs_max(a, m);
     return mx != 0.0 ? 1.0 / mx : 1.0;
 }
+__device__ __forceinline__ double* R(const CgDev& a, int n) { return n & 1 ? a.r1 : a.r0; }
+__device__ __forceinline__ double* S(const CgDev& a, int n) { return n & 1 ? a.s1 : a.s0; }
 
-#ifndef MJ_CG_THREADS
-#define MJ_CG_THREADS 1024
-#endif  // block size of the fused solve (one block per CU: its LDS tiles take 136 KB)
-#ifndef MJ_BATCH
-#define MJ_BATCH 2
-#endif  // interior points per thread whose loads are issued together
-constexpr int kMaxMembers = 2048;  // members of one batched solve (LDS member lists)
-constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin over them
-
-// The LDS tiles of the fused kernel, at file scope so the per-unit phase functions below address them directly.
-__shared__ double g_R[kTileLds];
-__shared__ double g_S[kTileLds];
-__shared__ double g_rowbuf[128];
-
-// Interior loop of the phase functions: MJ_BATCH points per thread, all loads first (registers c0..c6[k]), then
-// the arithmetic.
-#define MJ_UNIT_INTERIOR(LOADS, COMPUTE)                                                                  \
-    for (int base_ = 0; base_ < sNy * sNx; base_ += MJ_BATCH * nt) {                                      \
-        double c0[MJ_BATCH], c1[MJ_BATCH], c2[MJ_BATCH], c3[MJ_BATCH], c4[MJ_BATCH], c5[MJ_BATCH],         \
-            c6[MJ_BATCH];                                                                                 \
-        int pk[MJ_BATCH], qk[MJ_BATCH], ek[MJ_BATCH];                                                     \
-        _Pragma("unroll") for (int k = 0; k < MJ_BATCH; ++k) {                                            \
-            const int e_ = base_ + k * nt + tid;                                                          \
-            ek[k] = e_;                                                                                   \
-            if (e_ < sNy * sNx) {                                                                         \
-                const int jl = e_ / sNx, il = e_ - jl * sNx;                                              \
-                const int p = tbase + (OL + jl) * nx + OL + il;                                           \
-                pk[k] = p;                                                                                \
-                qk[k] = (jl + 1) * RW + il + 1;                                                           \
-                LOADS                                                                                     \
-            }                                                                                             \
-        }                                                                                                 \
-        _Pragma("unroll") for (int k = 0; k < MJ_BATCH; ++k) {                                            \
-            const int e_ = ek[k];                                                                         \
-            if (e_ < sNy * sNx) {                                                                         \
-                const int p = pk[k], q = qk[k];                                                           \
-                (void)p;                                                                                  \
-                COMPUTE                                                                                   \
-            }                                                                                             \
-        }                                                                                                 \
-    }
-
-// Phase B(n) of one unit: s_new = M r + beta s_old on interior and ring (ring from its interior sources), then
-// sigma = sum s_new (A s_new). Returns the tile partial of sigma (thread 0).
-template <int MODE>
-__device__ __noinline__ double unit_phase_B(const double* __restrict__ rin, const double* __restrict__ sold,
-                                            double* __restrict__ snew, const double* __restrict__ pW,
-                                            const double* __restrict__ pS, const double* __restrict__ pC,
-                                            const double* __restrict__ aW, const double* __restrict__ aS,
-                                            const double* __restrict__ aC, const int32_t* __restrict__ ring_dst,
-                                            const int32_t* __restrict__ ring_src, const int8_t* __restrict__ ring_sign,
-                                            int rk0, int rk1, double beta, int tbase, int nx, int OL, int sNx,
-                                            int sNy) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int RW = sNx + 2, RH = sNy + 2;
-    double* R = g_R;
-    double* S = g_S;
-    stage_tile(R, rin, tbase, nx, OL, RW, RH, tid);
-    for (int e = tid; e < RW * RH; e += nt) S[e] = 0.0;
+// ------------------------------------------------------------------------------------------------ unit functions
+// I0: per-tile max |b * cg2dNorm| (cg2d.F:150-160; the maximum is order free).
+__device__ __forceinline__ void unit_I0(const CgDev& a, int m, int t) {
+    MJ_UNIT_IDS
+    const double* b = a.b + m * a.ms;
+    const double nm = a.norm[m];
+    double mx = 0.0;
+    MJ_ROWS({
+        mx = fmax(mx, fabs(b[p0] * nm));
+        if (has1) mx = fmax(mx, fabs(b[p1] * nm));
+    })
+    for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) g_rows[wave] = mx;
     __syncthreads();
-    MJ_UNIT_INTERIOR(
-        {
-            c0[k] = pC[p];
-            c1[k] = pW[p];
-            c2[k] = pW[p + 1];
-            c3[k] = pS[p];
-            c4[k] = pS[p + nx];
-            c5[k] = sold[p];
-        },
-        {
-            const double v = c0[k] * R[q] + c1[k] * R[q - 1] + c2[k] * R[q + 1] + c3[k] * R[q - RW] +
-                             c4[k] * R[q + RW] + beta * c5[k];
-            snew[p] = v;
-            S[q] = v;
-        })
-    for (int k2 = rk0 + tid; k2 < rk1; k2 += nt) {
-        const int s = ring_src[k2];
-        double v = op_M(pW, pS, pC, rin, s, nx) + beta * sold[s];
-        v = ring_sign[k2] < 0 ? -v : v;
-        const int d = ring_dst[k2];
-        snew[d] = v;
-        const int dq = d - tbase, dj = dq / nx, di = dq - dj * nx;
-        S[(dj - OL + 1) * RW + (di - OL + 1)] = v;
+    if (tid == 0) {
+        double v = 0.0;
+        for (int w = 0; w < nwaves; ++w) v = fmax(v, g_rows[w]);
+        a.pmax[m * a.T + t] = v;
     }
     __syncthreads();
-    MJ_UNIT_INTERIOR(
-        {
-            c0[k] = aW[p];
-            c1[k] = aW[p + 1];
-            c2[k] = aS[p];
-            c3[k] = aS[p + nx];
-            c4[k] = aC[p];
-        },
-        {
-            R[e_] = S[q] * (c0[k] * S[q - 1] + c1[k] * S[q + 1] + c2[k] * S[q - RW] + c3[k] * S[q + RW] +
-                            c4[k] * S[q]);
-        })
-    __syncthreads();
-    return tile_reduce<MODE>([&](int jl, int il) { return R[jl * sNx + il]; }, g_rowbuf, sNx, sNy, tid);
 }
 
-// Phase C(n) of one unit: x += alpha s_new; r_new = r - alpha A s_new on interior and ring; then the tile
-// partials of err = sum r_new^2 (returned in .x) and eta(n+1) = sum (M r_new) r_new (in .y), thread 0.
+// I1: x = normalised first guess (halo points of the tile: the exchanged value), r = b - A x on the interior,
+// tile partials of err(0) = sum r^2 and sumRHS = sum b. Work vectors start at 0 (s0 entirely).
 template <int MODE>
-__device__ __noinline__ double2 unit_phase_C(const double* __restrict__ snew, const double* __restrict__ rin,
-                                             double* __restrict__ rout, double* __restrict__ x,
-                                             const double* __restrict__ pW, const double* __restrict__ pS,
-                                             const double* __restrict__ pC, const double* __restrict__ aW,
-                                             const double* __restrict__ aS, const double* __restrict__ aC,
-                                             const int32_t* __restrict__ ring_dst,
-                                             const int32_t* __restrict__ ring_src,
-                                             const int8_t* __restrict__ ring_sign, int rk0, int rk1, double alpha,
-                                             int tbase, int nx, int OL, int sNx, int sNy) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int RW = sNx + 2, RH = sNy + 2;
-    double* R = g_R;
-    double* S = g_S;
-    stage_tile(S, snew, tbase, nx, OL, RW, RH, tid);
-    for (int e = tid; e < RW * RH; e += nt) R[e] = 0.0;
-    __syncthreads();
-    MJ_UNIT_INTERIOR(
-        {
-            c0[k] = aW[p];
-            c1[k] = aW[p + 1];
-            c2[k] = aS[p];
-            c3[k] = aS[p + nx];
-            c4[k] = aC[p];
-            c5[k] = x[p];
-            c6[k] = rin[p];
-        },
-        {
-            x[p] = c5[k] + alpha * S[q];
-            const double v = c6[k] - alpha * (c0[k] * S[q - 1] + c1[k] * S[q + 1] + c2[k] * S[q - RW] +
-                                              c3[k] * S[q + RW] + c4[k] * S[q]);
-            rout[p] = v;
-            R[q] = v;
-        })
-    for (int k2 = rk0 + tid; k2 < rk1; k2 += nt) {
-        const int s = ring_src[k2];
-        double v = rin[s] - alpha * op_A(aW, aS, aC, snew, s, nx);
-        v = ring_sign[k2] < 0 ? -v : v;
-        const int d = ring_dst[k2];
-        rout[d] = v;
-        const int dq = d - tbase, dj = dq / nx, di = dq - dj * nx;
-        R[(dj - OL + 1) * RW + (di - OL + 1)] = v;
-    }
-    __syncthreads();
-    MJ_UNIT_INTERIOR(
-        {
-            c0[k] = pC[p];
-            c1[k] = pW[p];
-            c2[k] = pW[p + 1];
-            c3[k] = pS[p];
-            c4[k] = pS[p + nx];
-        },
-        {
-            S[e_] = (c0[k] * R[q] + c1[k] * R[q - 1] + c2[k] * R[q + 1] + c3[k] * R[q - RW] + c4[k] * R[q + RW]) *
-                    R[q];
-        })
-    __syncthreads();
-    double2 out;
-    out.x = tile_reduce<MODE>(
-        [&](int jl, int il) {
-            const double v = R[(jl + 1) * RW + il + 1];
-            return v * v;
-        },
-        g_rowbuf, sNx, sNy, tid);
-    out.y = tile_reduce<MODE>([&](int jl, int il) { return S[jl * sNx + il]; }, g_rowbuf, sNx, sNy, tid);
-    return out;
-}
-
-template <int MODE>
-__global__ void __launch_bounds__(MJ_CG_THREADS) cg2d_fused_kernel(CgDev a, GridSync gs) {
-    double* R = g_R;
-    double* S = g_S;
-    double* rowbuf = g_rowbuf;
-    __shared__ double redbuf[16];
-    __shared__ int wcnt[32];
-    __shared__ int s_nact, s_nfin;
-    __shared__ short alist[kMaxMembers], flist[kMaxMembers];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-    const int nunits = a.B * a.T;
+__device__ __forceinline__ void unit_I1(const CgDev& a, int m, int t) {
+    MJ_UNIT_IDS
     const int plane = a.plane;
-    const int RW = a.sNx + 2, RH = a.sNy + 2;
-    const int sNx = a.sNx, sNy = a.sNy;
-    auto opbase = [&](int m) { return (int64_t)(a.op_index ? a.op_index[m] : m) * a.oms; };
-    for (int m = blockIdx.x * blockDim.x + tid; m < a.B; m += gridDim.x * blockDim.x) a.mdone[m] = -1;
-
-    // ---------------------------------------------------------------- I0: per-tile max |b * cg2dNorm|
-    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
-        const int m = u / a.T, t = u - m * a.T, tbase = t * plane;
-        const double* b = a.b + m * a.ms;
-        const double nm = a.norm[m];
-        double mx = 0.0;
-        MJ_ROWS({
-            mx = fmax(mx, fabs(b[p0] * nm));
-            if (has1) mx = fmax(mx, fabs(b[p1] * nm));
-        })
-        for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-        if (lane == 0) redbuf[wave] = mx;
-        __syncthreads();
-        if (tid == 0) {
-            double v = 0.0;
-            for (int w = 0; w < nwaves; ++w) v = fmax(v, redbuf[w]);
-            a.pmax[u] = v;
+    const double rhsNorm = member_rhs_norm(a, m);
+    const double nm = a.norm[m];
+    const double* x0 = a.x0 + m * a.ms;
+    const double* b = a.b + m * a.ms;
+    double* x = a.x + m * a.ms;
+    double *r0 = a.r0 + m * a.ms, *r1 = a.r1 + m * a.ms, *s0 = a.s0 + m * a.ms, *s1 = a.s1 + m * a.ms;
+    double* stage = stage_of(a, m, t);
+    const int64_t ob = op_base(a, m);
+    for (int qq = tid; qq < plane; qq += blockDim.x) {
+        const int j = qq / nx, i = qq - j * nx, p = tbase + qq;
+        const bool inner = j >= OL && j < OL + sNy && i >= OL && i < OL + sNx;
+        s0[p] = 0.0;
+        if (!inner) {
+            x[p] = x0[p];
+            r0[p] = 0.0;
+            r1[p] = 0.0;
+            s1[p] = 0.0;
+        } else {
+            x[p] = x0[p] * rhsNorm;
         }
-        __syncthreads();
     }
-    if (!grid_sync(gs)) goto done;
+    __syncthreads();
+    // exchanged points: the pre-exchange value at the source (normalised where the source is interior)
+    for (int k = a.full_off[t] + tid; k < a.full_off[t + 1]; k += blockDim.x) {
+        const int s = a.full_src[k];
+        const int sq = s % plane, sj = sq / nx, si = sq - sj * nx;
+        const bool inner = sj >= OL && sj < OL + sNy && si >= OL && si < OL + sNx;
+        const double v = inner ? x0[s] * rhsNorm : x0[s];
+        x[a.full_dst[k]] = a.full_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
+    MJ_ROWS({
+        const double bn0 = b[p0] * nm * rhsNorm;
+        const double r_0 = bn0 - op_A(a.aW + ob, a.aS + ob, a.aC + ob, x, p0, nx);
+        r0[p0] = r_0;
+        double bn1 = 0.0, r_1 = 0.0;
+        if (has1) {
+            bn1 = b[p1] * nm * rhsNorm;
+            r_1 = bn1 - op_A(a.aW + ob, a.aS + ob, a.aC + ob, x, p1, nx);
+            r0[p1] = r_1;
+        }
+        put_row<MODE>(0, jl, lane, r_0 * r_0, r_1 * r_1, has1, stage, sNx, sNy);
+        put_row<MODE>(1, jl, lane, bn0, bn1, has1, stage, sNx, sNy);
+    })
+    store_tile_sums<MODE>(a.perr, a.psum, m * a.T + t, lane, wave, nwaves, stage, sNx, sNy);  // perr slot 0: err(0)
+}
 
-    // ---------------------------------------------------------------- I1: x (normalised, exchanged), r = b - A x
-    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
-        const int m = u / a.T, t = u - m * a.T, tbase = t * plane;
-        const double rhsNorm = member_rhs_norm(a, m);
-        const double nm = a.norm[m];
-        const double* x0 = a.x0 + m * a.ms;
-        const double* b = a.b + m * a.ms;
-        double* x = a.x + m * a.ms;
-        double *r0 = a.r0 + m * a.ms, *r1 = a.r1 + m * a.ms, *s0 = a.s0 + m * a.ms, *s1 = a.s1 + m * a.ms;
-        const int64_t ob = opbase(m);
-        // halo points: the first guess itself; the work vectors: 0 (s0 entirely)
-        for (int qq = tid; qq < plane; qq += blockDim.x) {
-            const int j = qq / a.nx, i = qq - j * a.nx, p = tbase + qq;
-            const bool inner = j >= a.OL && j < a.OL + sNy && i >= a.OL && i < a.OL + sNx;
-            s0[p] = 0.0;
-            if (!inner) {
-                x[p] = x0[p];
-                r0[p] = 0.0;
-                r1[p] = 0.0;
-                s1[p] = 0.0;
-            } else {
-                x[p] = x0[p] * rhsNorm;
+// I2: exchange r (ring), tile partial of eta(1) = sum (M r) r; tile 0 initialises the member's diag and status.
+template <int MODE>
+__device__ __forceinline__ void unit_I2(const CgDev& a, int m, int t) {
+    MJ_UNIT_IDS
+    double* r0 = a.r0 + m * a.ms;
+    double* stage = stage_of(a, m, t);
+    const int64_t ob = op_base(a, m);
+    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const double v = r0[a.ring_src[k]];
+        r0[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
+    MJ_ROWS({
+        const double q0 = op_M(a.pW + ob, a.pS + ob, a.pC + ob, r0, p0, nx) * r0[p0];
+        const double q1 = has1 ? op_M(a.pW + ob, a.pS + ob, a.pC + ob, r0, p1, nx) * r0[p1] : 0.0;
+        put_row<MODE>(0, jl, lane, q0, q1, has1, stage, sNx, sNy);
+    })
+    store_tile_sums<MODE>(a.peta + a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, stage, sNx, sNy);  // eta(1)
+    if (t == 0 && tid == 0) {
+        const double err0 = tile_order_sum(a.perr + m * a.T, a.T);
+        a.diag[m].firstResidual = sqrt(err0);
+        a.diag[m].sumRHS = tile_order_sum(a.psum + m * a.T, a.T);
+        a.diag[m].rhsMax = member_rhs_max(a, m);
+        a.diag[m].outerIters = 0;
+        a.diag[m].pad_ = 0;
+        a.mdone[m] = -1;
+    }
+}
+
+// Status of member m at the start of B(n), the same in every block: 0 finished earlier, 1 finishing now
+// (stopping rule on err(n-1), cg2d.F:245-250), 2 active. Thread 0 computes it; the block reads g_ibcast, and for
+// status 1 g_bcast[0] = err(n-1).
+__device__ __forceinline__ int status_B(const CgDev& a, int m, int n) {
+    if (threadIdx.x == 0) {
+        int code = 0;
+        const int st = a.mdone[m];
+        if (!(st >= 0 && st < n - 1)) {
+            const double errp = tile_order_sum(a.perr + ((n - 1) & 1) * a.B * a.T + m * a.T, a.T);
+            code = (!(n - 1 < a.max_iters) || errp < a.tol_sq) ? 1 : 2;
+            g_bcast[0] = errp;
+            if (code == 2) {
+                const double* pe = a.peta + m * a.T;
+                const double eta = tile_order_sum(pe + (n & 1) * a.B * a.T, a.T);
+                const double etaNM1 = n == 1 ? 1.0 : tile_order_sum(pe + ((n - 1) & 1) * a.B * a.T, a.T);
+                g_bcast[1] = eta / etaNM1;  // beta (cg2d.F:268-271)
             }
         }
-        __syncthreads();
-        // exchanged points: the pre-exchange value at the source (normalised where the source is interior)
-        for (int k = a.full_off[t] + tid; k < a.full_off[t + 1]; k += blockDim.x) {
-            const int s = a.full_src[k];
-            const int sq = s % plane, sj = sq / a.nx, si = sq - sj * a.nx;
-            const bool inner = sj >= a.OL && sj < a.OL + sNy && si >= a.OL && si < a.OL + sNx;
-            const double v = inner ? x0[s] * rhsNorm : x0[s];
-            x[a.full_dst[k]] = a.full_sign[k] < 0 ? -v : v;
-        }
-        __syncthreads();
-        // r = b - A x on the interior; per-point r^2 into R, b into S (as sNy x sNx arrays)
-        MJ_INTERIOR({
-            const double bn = b[p] * nm * rhsNorm;
-            const double rr = bn - op_A(a.aW + ob, a.aS + ob, a.aC + ob, x, p, a.nx);
-            r0[p] = rr;
-            R[e_] = rr * rr;
-            S[e_] = bn;
-        })
-        __syncthreads();
-        const double e0 = tile_reduce<MODE>([&](int jl, int il) { return R[jl * sNx + il]; }, rowbuf, sNx, sNy, tid);
-        const double sr = tile_reduce<MODE>([&](int jl, int il) { return S[jl * sNx + il]; }, rowbuf, sNx, sNy, tid);
-        if (tid == 0) {
-            a.perr[u] = e0;  // slot 0: err(0)
-            a.psum[u] = sr;
+        g_ibcast = code;
+    }
+    __syncthreads();
+    const int code = g_ibcast;
+    return code;
+}
+
+// Finishing member at B(n): un-normalise x on the tile (cg2d.F:380-395); tile 0 completes the diag.
+__device__ __forceinline__ void unit_finish(const CgDev& a, int m, int t, int n, double errp) {
+    MJ_UNIT_IDS
+    const double rhsNorm = member_rhs_norm(a, m);
+    double* x = a.x + m * a.ms;
+    MJ_ROWS({
+        x[p0] = x[p0] / rhsNorm;
+        if (has1) x[p1] = x[p1] / rhsNorm;
+    })
+    if (t == 0 && tid == 0) {
+        a.diag[m].lastResidual = sqrt(errp);
+        a.diag[m].numIters = n - 1;
+        a.diag[m].converged = errp < a.tol_sq ? 1 : 0;
+        a.mdone[m] = n - 1;
+        atomicSub(a.nactive, 1);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ row sweeps
+// The iteration phases sweep rows: each wave owns a strip of consecutive rows and lane l owns the column pair
+// (2l, 2l+1), loaded and stored as one 16-byte access. Down the strip the three rows a 5-point stencil reads are
+// kept in registers (one new row loaded per step); the i-1 / i+1 neighbours come from the adjacent lane, and the
+// halo columns -1 and sNx from one extra load by the first / last lane. Every value is computed with the same
+// expression, in the same term order, as op_A / op_M.
+struct Lanes {
+    int lane, c, last;  // c: this lane's first column (clamped in range for idle lanes), last: lane of column sNx-1
+    bool act;           // lane owns columns
+};
+__device__ __forceinline__ Lanes lanes_of(int lane, int sNx) {
+    Lanes L;
+    L.lane = lane;
+    L.act = 2 * lane < sNx;
+    L.c = L.act ? 2 * lane : sNx - 2;
+    L.last = sNx / 2 - 1;
+    return L;
+}
+__device__ __forceinline__ double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
+__device__ __forceinline__ void st2(double* p, double2 v) { *reinterpret_cast<double2*>(p) = v; }
+// Value at the halo column on this lane's open side (lane 0: column -1, last lane: column sNx), row base prow.
+__device__ __forceinline__ double halo_of(const double* v, int prow, const Lanes& L, int sNx) {
+    return (L.lane == 0 || L.lane == L.last) ? v[prow + (L.lane == 0 ? -1 : sNx)] : 0.0;
+}
+// Left neighbour of column c and right neighbour of column c+1.
+__device__ __forceinline__ double left_of(double2 v, double h, const Lanes& L) {
+    const double w = __shfl_up(v.y, 1, 64);
+    return L.lane == 0 ? h : w;
+}
+__device__ __forceinline__ double right_of(double2 v, double h, const Lanes& L) {
+    const double w = __shfl_down(v.x, 1, 64);
+    return L.lane == L.last ? h : w;
+}
+// op_A / op_M at columns c, c+1 of one row: Xm, Xc, Xp rows j-1, j, j+1; xl, xr the outer neighbours.
+__device__ __forceinline__ double2 pair_A(double2 AW, double awr, double2 AS0, double2 AS1, double2 AC, double2 Xm,
+                                          double2 Xc, double2 Xp, double xl, double xr) {
+    double2 q;
+    q.x = AW.x * xl + AW.y * Xc.y + AS0.x * Xm.x + AS1.x * Xp.x + AC.x * Xc.x;
+    q.y = AW.y * Xc.x + awr * xr + AS0.y * Xm.y + AS1.y * Xp.y + AC.y * Xc.y;
+    return q;
+}
+__device__ __forceinline__ double2 pair_M(double2 PC, double2 PW, double pwr, double2 PS0, double2 PS1, double2 Rm,
+                                          double2 Rc, double2 Rp, double rl, double rr) {
+    double2 q;
+    q.x = PC.x * Rc.x + PW.x * rl + PW.y * Rc.y + PS0.x * Rm.x + PS1.x * Rp.x;
+    q.y = PC.y * Rc.y + PW.y * Rc.x + pwr * rr + PS0.y * Rm.y + PS1.y * Rp.y;
+    return q;
+}
+// One row's per-point values of reduction r held as column pairs: tree order regroups them onto the canonical
+// lanes (lane l: v[l] and v[l + 64]) and reduces with wave_tree into the row sums; fortran order stages them.
+template <int MODE>
+__device__ __forceinline__ void put_row2(int r, int jl, const Lanes& L, double2 v, double* stage, int sNx, int sNy) {
+    if (MODE == MJ_SUM_TREE) {
+        const int l = L.lane, s0 = l >> 1, s1 = 32 + (l >> 1);
+        const double a0 = __shfl(v.x, s0, 64), a1 = __shfl(v.y, s0, 64);
+        const double b0 = __shfl(v.x, s1, 64), b1 = __shfl(v.y, s1, 64);
+        const double lo = (l & 1) ? a1 : a0, hi = (l & 1) ? b1 : b0;
+        const double w = wave_tree(lo, hi, l + 64 < sNx);
+        if (l == 0) g_rows[r * 128 + jl] = w;
+    } else {
+        if (L.act) st2(stage + ((size_t)r * sNy + jl) * sNx + L.c, v);
+    }
+}
+
+// Strip of rows of this wave: [j0, j1).
+#define MJ_STRIP                                                                                         \
+    const int rows_ = (sNy + nwaves - 1) / nwaves, j0 = wave * rows_, j1 = min(j0 + rows_, sNy);        \
+    const Lanes L = lanes_of(lane, sNx);
+#define MJ_ROW(j) (tbase + (OL + (j)) * nx + OL)
+
+// Phase B(n) of an active member: s_n = M r_{n-1} + beta s_{n-1} on interior and ring (ring from its interior
+// sources), then the tile partial of sigma = sum s_n (A s_n).
+template <int MODE>
+__device__ __forceinline__ void unit_B(const CgDev& a, int m, int t, int n, double beta) {
+    MJ_UNIT_IDS
+    MJ_STRIP
+    const int64_t ob = op_base(a, m);
+    const double *__restrict__ pW = a.pW + ob, *__restrict__ pS = a.pS + ob, *__restrict__ pC = a.pC + ob;
+    const double *__restrict__ aW = a.aW + ob, *__restrict__ aS = a.aS + ob, *__restrict__ aC = a.aC + ob;
+    const double* __restrict__ r = R(a, n - 1) + m * a.ms;
+    const double* __restrict__ so = S(a, n - 1) + m * a.ms;
+    double* sn = S(a, n) + m * a.ms;
+    double* stage = stage_of(a, m, t);
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        double2 Rm = ld2(r + p - nx), Rc = ld2(r + p), PS0 = ld2(pS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const double2 Rp = ld2(r + p + nx), PS1 = ld2(pS + p + nx);
+            const double2 PC = ld2(pC + p), PW = ld2(pW + p), SO = ld2(so + p);
+            const double hr = halo_of(r, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
+            const double2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
+                                     right_of(Rc, hr, L));
+            double2 v;
+            v.x = q.x + beta * SO.x;
+            v.y = q.y + beta * SO.y;
+            if (L.act) st2(sn + p, v);
+            Rm = Rc;
+            Rc = Rp;
+            PS0 = PS1;
         }
     }
-    if (!grid_sync(gs)) goto done;
-
-    // ---------------------------------------------------------------- I2: exchange r, eta(1) = sum (M r) r
-    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
-        const int m = u / a.T, t = u - m * a.T, tbase = t * plane;
-        double* r0 = a.r0 + m * a.ms;
-        const int64_t ob = opbase(m);
-        for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-            const double v = r0[a.ring_src[k]];
-            r0[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
-        }
-        __syncthreads();
-        stage_tile(R, r0, tbase, a.nx, a.OL, RW, RH, tid);
-        __syncthreads();
-        MJ_INTERIOR({ S[e_] = lds_M(a.pW + ob, a.pS + ob, a.pC + ob, R, p, a.nx, q, RW) * R[q]; })
-        __syncthreads();
-        const double e1 = tile_reduce<MODE>([&](int jl, int il) { return S[jl * sNx + il]; }, rowbuf, sNx, sNy, tid);
-        if (tid == 0) a.peta[a.B * a.T + u] = e1;  // slot 1: eta(1)
-        if (t == 0 && tid == 0) {
-            const double err0 = tile_order_sum(a.perr + m * a.T, a.T);
-            a.diag[m].firstResidual = sqrt(err0);
-            a.diag[m].sumRHS = tile_order_sum(a.psum + m * a.T, a.T);
-            a.diag[m].rhsMax = member_rhs_max(a, m);
-            a.diag[m].outerIters = 0;
-            a.diag[m].pad_ = 0;
+    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const int s = a.ring_src[k];
+        const double v = op_M(pW, pS, pC, r, s, nx) + beta * so[s];
+        sn[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        double2 Xm = ld2(sn + p - nx), Xc = ld2(sn + p), AS0 = ld2(aS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const double2 Xp = ld2(sn + p + nx), AS1 = ld2(aS + p + nx);
+            const double2 AW = ld2(aW + p), AC = ld2(aC + p);
+            const double hx = halo_of(sn, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
+            const double2 q = pair_A(AW, right_of(AW, hw, L), AS0, AS1, AC, Xm, Xc, Xp, left_of(Xc, hx, L),
+                                     right_of(Xc, hx, L));
+            double2 v;
+            v.x = Xc.x * q.x;
+            v.y = Xc.y * q.y;
+            if (!L.act) v = make_double2(0.0, 0.0);
+            put_row2<MODE>(0, j, L, v, stage, sNx, sNy);
+            Xm = Xc;
+            Xc = Xp;
+            AS0 = AS1;
         }
     }
-    if (!grid_sync(gs)) goto done;
+    store_tile_sums<MODE>(a.psig + (n & 1) * a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, stage, sNx, sNy);
+}
 
-    // ---------------------------------------------------------------- iterations
-    // Every block derives the same member status from the same partials at the start of B(n): finished earlier
-    // (mdone[m] < n-1), finishing now (stopping rule on err(n-1)), or active. Active members are compacted into
-    // alist and their units are spread evenly over the grid, tile-major and in contiguous chunks per XCD (block b
-    // runs on XCD b % kXcds), so the members of one tile share its coefficients in that XCD's L2.
+// Phase C(n) of an active member: x += alpha s_n; r_n = r_{n-1} - alpha A s_n on interior and ring; then the tile
+// partials of err(n) = sum r_n^2 and eta(n+1) = sum (M r_n) r_n.
+template <int MODE>
+__device__ __forceinline__ void unit_C(const CgDev& a, int m, int t, int n, double alpha) {
+    MJ_UNIT_IDS
+    MJ_STRIP
+    const int64_t ob = op_base(a, m);
+    const double *__restrict__ pW = a.pW + ob, *__restrict__ pS = a.pS + ob, *__restrict__ pC = a.pC + ob;
+    const double *__restrict__ aW = a.aW + ob, *__restrict__ aS = a.aS + ob, *__restrict__ aC = a.aC + ob;
+    const double* __restrict__ sv = S(a, n) + m * a.ms;
+    const double* __restrict__ rin = R(a, n - 1) + m * a.ms;
+    double* rout = R(a, n) + m * a.ms;
+    double* x = a.x + m * a.ms;
+    double* stage = stage_of(a, m, t);
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        double2 Xm = ld2(sv + p - nx), Xc = ld2(sv + p), AS0 = ld2(aS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const double2 Xp = ld2(sv + p + nx), AS1 = ld2(aS + p + nx);
+            const double2 AW = ld2(aW + p), AC = ld2(aC + p), RI = ld2(rin + p), XV = ld2(x + p);
+            const double hx = halo_of(sv, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
+            const double2 q = pair_A(AW, right_of(AW, hw, L), AS0, AS1, AC, Xm, Xc, Xp, left_of(Xc, hx, L),
+                                     right_of(Xc, hx, L));
+            double2 xn, rn;
+            xn.x = XV.x + alpha * Xc.x;
+            xn.y = XV.y + alpha * Xc.y;
+            rn.x = RI.x - alpha * q.x;
+            rn.y = RI.y - alpha * q.y;
+            if (L.act) {
+                st2(x + p, xn);
+                st2(rout + p, rn);
+            }
+            Xm = Xc;
+            Xc = Xp;
+            AS0 = AS1;
+        }
+    }
+    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const int s = a.ring_src[k];
+        const double v = rin[s] - alpha * op_A(aW, aS, aC, sv, s, nx);
+        rout[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        double2 Rm = ld2(rout + p - nx), Rc = ld2(rout + p), PS0 = ld2(pS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const double2 Rp = ld2(rout + p + nx), PS1 = ld2(pS + p + nx);
+            const double2 PC = ld2(pC + p), PW = ld2(pW + p);
+            const double hr = halo_of(rout, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
+            const double2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
+                                     right_of(Rc, hr, L));
+            double2 e, h;
+            e.x = Rc.x * Rc.x;
+            e.y = Rc.y * Rc.y;
+            h.x = q.x * Rc.x;
+            h.y = q.y * Rc.y;
+            if (!L.act) {
+                e = make_double2(0.0, 0.0);
+                h = e;
+            }
+            put_row2<MODE>(0, j, L, e, stage, sNx, sNy);
+            put_row2<MODE>(1, j, L, h, stage, sNx, sNy);
+            Rm = Rc;
+            Rc = Rp;
+            PS0 = PS1;
+        }
+    }
+    const int BT = a.B * a.T;
+    store_tile_sums<MODE>(a.perr + (n & 1) * BT, a.peta + ((n + 1) & 1) * BT, m * a.T + t, lane, wave, nwaves, stage,
+                          sNx, sNy);
+}
+
+// Whole phases of one unit (shared by both launch shapes).
+template <int MODE>
+__device__ __forceinline__ void unit_phase_B(const CgDev& a, int m, int t, int n) {
+    const int code = status_B(a, m, n);
+    if (code == 1) unit_finish(a, m, t, n, g_bcast[0]);
+    if (code == 2) unit_B<MODE>(a, m, t, n, g_bcast[1]);
+    __syncthreads();
+}
+template <int MODE>
+__device__ __forceinline__ void unit_phase_C(const CgDev& a, int m, int t, int n) {
+    if (threadIdx.x == 0) {
+        int code = 0;
+        if (a.mdone[m] < 0) {
+            const double* pe = a.peta + (n & 1) * a.B * a.T + m * a.T;
+            g_bcast[0] = tile_order_sum(pe, a.T) / tile_order_sum(a.psig + (n & 1) * a.B * a.T + m * a.T, a.T);
+            code = 1;
+        }
+        g_ibcast = code;
+    }
+    __syncthreads();
+    if (g_ibcast) unit_C<MODE>(a, m, t, n, g_bcast[0]);
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------ persistent
+// Units u = blockIdx.x, blockIdx.x + gridDim.x, ...; tile-major (t = u / B), so concurrent blocks share a tile's
+// coefficients.
+template <int MODE>
+__global__ void __launch_bounds__(MJ_CG_THREADS) cg2d_persistent_kernel(CgDev a, GridSync gs) {
+    const int nunits = a.B * a.T;
+#define MJ_FOR_UNITS(...)                                                   \
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {                  \
+        const int t = u / a.B, m = u - t * a.B;                             \
+        __VA_ARGS__                                                         \
+    }
+    MJ_FOR_UNITS(unit_I0(a, m, t);)
+    if (!grid_sync(gs)) goto done;
+    MJ_FOR_UNITS(unit_I1<MODE>(a, m, t);)
+    if (!grid_sync(gs)) goto done;
+    MJ_FOR_UNITS(unit_I2<MODE>(a, m, t); __syncthreads();)
+    if (!grid_sync(gs)) goto done;
     for (int n = 1;; ++n) {
-        const int BT = a.B * a.T;
-        const double* peta_n = a.peta + (n & 1) * BT;
-        const double* peta_m = a.peta + ((n - 1) & 1) * BT;
-        const double* perr_m = a.perr + ((n - 1) & 1) * BT;
-        double* psig_n = a.psig + (n & 1) * BT;
-        // ------------------------------------------------------------ member status and compaction
-        for (int c0 = 0; c0 < a.B; c0 += blockDim.x) {
-            const int m = c0 + tid;
-            int code = 0;  // 0 finished earlier, 1 finishing now, 2 active
-            if (m < a.B) {
-                const int st = a.mdone[m];
-                if (!(st >= 0 && st < n - 1)) {
-                    const double errp = tile_order_sum(perr_m + m * a.T, a.T);
-                    code = (!(n - 1 < a.max_iters) || errp < a.tol_sq) ? 1 : 2;
-                }
-            }
-            const unsigned long long ma = __ballot(code == 2), mf = __ballot(code == 1);
-            const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-            if (lane == 0) {
-                wcnt[wave] = __popcll(ma);
-                wcnt[16 + wave] = __popcll(mf);
-            }
-            __syncthreads();
-            int oa = c0 == 0 ? 0 : s_nact, of = c0 == 0 ? 0 : s_nfin;
-            for (int w = 0; w < wave; ++w) {
-                oa += wcnt[w];
-                of += wcnt[16 + w];
-            }
-            if (code == 2) alist[oa + __popcll(ma & below)] = (short)m;
-            if (code == 1) flist[of + __popcll(mf & below)] = (short)m;
-            __syncthreads();
-            if (tid == 0) {
-                int ta = c0 == 0 ? 0 : s_nact, tf = c0 == 0 ? 0 : s_nfin;
-                for (int w = 0; w < nwaves; ++w) {
-                    ta += wcnt[w];
-                    tf += wcnt[16 + w];
-                }
-                s_nact = ta;
-                s_nfin = tf;
-            }
-            __syncthreads();
-        }
-        const int nact = s_nact, nfin = s_nfin;
-        // ------------------------------------------------------------ finishing members (cg2d.F:380-395)
-        for (int it = blockIdx.x; it < nfin * a.T; it += gridDim.x) {
-            const int tid = opaque_tid();
-            const int m = flist[it / a.T], t = it % a.T, tbase = t * plane;
-            const double rhsNorm = member_rhs_norm(a, m);
-            double* x = a.x + m * a.ms;
-            MJ_INTERIOR({ x[p] = x[p] / rhsNorm; })
-            if (t == 0 && tid == 0) {
-                const double errp = tile_order_sum(perr_m + m * a.T, a.T);
-                a.diag[m].lastResidual = sqrt(errp);
-                a.diag[m].numIters = n - 1;
-                a.diag[m].converged = errp < a.tol_sq ? 1 : 0;
-            }
-        }
-        if (blockIdx.x == 0)
-            for (int k = tid; k < nfin; k += blockDim.x) a.mdone[flist[k]] = n - 1;
-        if (nact == 0) break;
-        // XCD-contiguous chunk of the nact*T active units this block works on
-        const int nitems = nact * a.T;
-        const int xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
-        const int gx = (gridDim.x - xcd + kXcds - 1) / kXcds;
-        const int i0 = (int)((int64_t)nitems * xcd / kXcds), i1 = (int)((int64_t)nitems * (xcd + 1) / kXcds);
-        // ------------------------------------------------------------ phase B(n)
-        for (int it = i0 + lb; it < i1; it += gx) {
-            const int t = it / nact, m = alist[it - t * nact], u = m * a.T + t, tbase = t * plane;
-            const double eta = tile_order_sum(peta_n + m * a.T, a.T);
-            const double etaNM1 = n == 1 ? 1.0 : tile_order_sum(peta_m + m * a.T, a.T);
-            const double beta = eta / etaNM1;
-            const double* rin = ((n - 1) & 1 ? a.r1 : a.r0) + m * a.ms;
-            const double* sold = ((n - 1) & 1 ? a.s1 : a.s0) + m * a.ms;
-            double* snew = (n & 1 ? a.s1 : a.s0) + m * a.ms;
-            const int64_t ob = opbase(m);
-            const double sg = unit_phase_B<MODE>(rin, sold, snew, a.pW + ob, a.pS + ob, a.pC + ob, a.aW + ob,
-                                                 a.aS + ob, a.aC + ob, a.ring_dst, a.ring_src, a.ring_sign,
-                                                 a.ring_off[t], a.ring_off[t + 1], beta, tbase, a.nx, a.OL, sNx, sNy);
-            if (tid == 0) psig_n[u] = sg;
-        }
+        MJ_FOR_UNITS(unit_phase_B<MODE>(a, m, t, n);)
         if (!grid_sync(gs)) goto done;
-        // ------------------------------------------------------------ phase C(n)
-        double* perr_n = a.perr + (n & 1) * BT;
-        double* peta_p = a.peta + ((n + 1) & 1) * BT;
-        for (int it = i0 + lb; it < i1; it += gx) {
-            const int t = it / nact, m = alist[it - t * nact], u = m * a.T + t, tbase = t * plane;
-            const double eta = tile_order_sum(peta_n + m * a.T, a.T);
-            const double alpha = eta / tile_order_sum(psig_n + m * a.T, a.T);
-            const double* snew = (n & 1 ? a.s1 : a.s0) + m * a.ms;
-            const double* rin = ((n - 1) & 1 ? a.r1 : a.r0) + m * a.ms;
-            double* rout = (n & 1 ? a.r1 : a.r0) + m * a.ms;
-            const int64_t ob = opbase(m);
-            const double2 eq = unit_phase_C<MODE>(snew, rin, rout, a.x + m * a.ms, a.pW + ob, a.pS + ob, a.pC + ob,
-                                                  a.aW + ob, a.aS + ob, a.aC + ob, a.ring_dst, a.ring_src,
-                                                  a.ring_sign, a.ring_off[t], a.ring_off[t + 1], alpha, tbase, a.nx,
-                                                  a.OL, sNx, sNy);
-            const double e = eq.x, qv = eq.y;
-            if (tid == 0) {
-                perr_n[u] = e;
-                peta_p[u] = qv;
-            }
-        }
+        // nactive only changes in B phases: every block reads the same value here
+        if (__hip_atomic_load(a.nactive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+        MJ_FOR_UNITS(unit_phase_C<MODE>(a, m, t, n);)
         if (!grid_sync(gs)) goto done;
     }
+#undef MJ_FOR_UNITS
 done:
     // a timed-out barrier: mark every member's diag (converged = -2) so the host sees the failure
-    if (__hip_atomic_load(gs.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 && tid == 0) {
+    if (__hip_atomic_load(gs.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 && threadIdx.x == 0) {
         for (int u = blockIdx.x; u < nunits; u += gridDim.x)
-            if (u % a.T == 0) a.diag[u / a.T].converged = -2;
+            if (u < a.B) a.diag[u].converged = -2;
     }
+}
+
+// ------------------------------------------------------------------------------------------------ phases
+// One block per unit. gridDim.x is a multiple of kXcds; block b runs on XCD b % kXcds and takes unit
+// (b % kXcds) * (gridDim.x / kXcds) + b / kXcds, so each XCD works through one contiguous, tile-major range of
+// units and the coefficients of its tiles stay in its L2.
+__device__ __forceinline__ bool phase_unit(const CgDev& a, int& m, int& t) {
+    const int chunk = gridDim.x / kXcds;
+    const int u = (blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
+    if (u >= a.B * a.T) return false;
+    t = u / a.B;
+    m = u - t * a.B;
+    return true;
+}
+
+enum { PH_I0 = 0, PH_I1 = 1, PH_I2 = 2, PH_B = 3, PH_C = 4 };
+
+template <int MODE, int PH>
+__global__ void __launch_bounds__(MJ_PH_THREADS) __attribute__((amdgpu_waves_per_eu(MJ_PH_WAVES_PER_EU)))
+cg2d_phase_kernel(CgDev a, int n) {
+    int m, t;
+    if (!phase_unit(a, m, t)) return;
+    if (PH == PH_B || PH == PH_C) {
+        // units of members that finished earlier leave before touching anything else
+        const int st = a.mdone[m];
+        if (st >= 0 && (PH == PH_C || st < n - 1)) return;
+    }
+    if (PH == PH_I0) unit_I0(a, m, t);
+    if (PH == PH_I1) unit_I1<MODE>(a, m, t);
+    if (PH == PH_I2) unit_I2<MODE>(a, m, t);
+    if (PH == PH_B) unit_phase_B<MODE>(a, m, t, n);
+    if (PH == PH_C) unit_phase_C<MODE>(a, m, t, n);
 }
 
 __global__ void cg2d_init_kernel(unsigned int* counter, int* error, int* nactive, int B) {
@@ -614,7 +665,7 @@ using namespace mj;
 namespace {
 
 struct WsLayout {
-    size_t r0, r1, s0, s1, pmax, psum, perr, peta, psig, mdone, misc, total;
+    size_t r0, r1, s0, s1, pmax, psum, perr, peta, psig, mdone, misc, prod, total;
 };
 
 WsLayout ws_layout(const mj_cg2d_config* c) {
@@ -634,28 +685,51 @@ WsLayout ws_layout(const mj_cg2d_config* c) {
     L.psig = o; o += 2 * bt;
     L.mdone = o; o += al((size_t)c->batch * 4);
     L.misc = o; o += 256;  // barrier counter, error flag, nactive
+    L.prod = o;
+    if (c->sum_order == MJ_SUM_FORTRAN) o += al((size_t)c->batch * c->ntiles * 2 * c->sNx * c->sNy * 8);
     L.total = o;
     return L;
 }
 
+int launch_mode_of(const mj_cg2d_config* c) {
+    if (c->launch_mode != MJ_CG_LAUNCH_AUTO) return c->launch_mode;
+    // one-tile-per-block phases fill the chip once the batch has a few hundred units; below that the persistent
+    // launch (no launch gaps, no host round trips) is faster
+    return c->batch * c->ntiles >= 256 ? MJ_CG_LAUNCH_PHASES : MJ_CG_LAUNCH_PERSISTENT;
+}
+
+int block_threads_of(const mj_cg2d_config* c, int mode) {
+    const int limit = mode == MJ_CG_LAUNCH_PERSISTENT ? MJ_CG_THREADS : MJ_PH_THREADS;
+    if (c->block_threads == 0) return limit;
+    MJ_REQUIRE(c->block_threads % 64 == 0 && c->block_threads >= 64 && c->block_threads <= limit,
+               "cg2d: block_threads must be a multiple of 64 in 64..%d for this launch mode", limit);
+    return c->block_threads;
+}
+
 int check_cfg(const mj_cg2d_config* c) {
     MJ_REQUIRE(c, "cg2d: null config");
-    MJ_REQUIRE(c->batch >= 1 && c->batch <= kMaxMembers, "cg2d: batch %d out of range 1..%d", c->batch,
-               kMaxMembers);
+    MJ_REQUIRE(c->batch >= 1 && c->batch <= (1 << 20), "cg2d: batch %d out of range", c->batch);
     MJ_REQUIRE(c->ntiles >= 1 && c->OL >= 1, "cg2d: bad layout");
-    MJ_REQUIRE(c->sNx >= 64 && c->sNx <= kRingMax - 2 && c->sNy >= 64 && c->sNy <= kRingMax - 2,
-               "cg2d: sNx, sNy must be in 64..%d (got %d, %d)", kRingMax - 2, c->sNx, c->sNy);
+    MJ_REQUIRE(c->sNx >= 64 && c->sNx <= 128 && c->sNy >= 64 && c->sNy <= 128,
+               "cg2d: sNx, sNy must be in 64..128 (got %d, %d)", c->sNx, c->sNy);
+    // the row sweeps move column pairs as 16-byte accesses: every row start must be an even element offset
+    MJ_REQUIRE(c->sNx % 2 == 0 && c->OL % 2 == 0 && c->member_stride % 2 == 0,
+               "cg2d: sNx, OL and member_stride must be even (got %d, %d, %lld)", c->sNx, c->OL,
+               (long long)c->member_stride);
     const int64_t plane = (int64_t)(c->sNx + 2 * c->OL) * (c->sNy + 2 * c->OL);
     MJ_REQUIRE(c->member_stride >= c->ntiles * plane, "cg2d: member_stride too small");
+    MJ_REQUIRE((int64_t)c->batch * c->member_stride < ((int64_t)1 << 31), "cg2d: batch * member_stride exceeds int32");
     MJ_REQUIRE(c->sum_order == MJ_SUM_FORTRAN || c->sum_order == MJ_SUM_TREE, "cg2d: bad sum_order %d", c->sum_order);
     MJ_REQUIRE(c->max_iters >= 0, "cg2d: max_iters < 0");
-    MJ_REQUIRE(c->block_threads == 0 || (c->block_threads % 64 == 0 && c->block_threads >= 64 &&
-                                         c->block_threads <= MJ_CG_THREADS),
-               "cg2d: block_threads must be a multiple of 64 in 64..%d", MJ_CG_THREADS);
+    MJ_REQUIRE(c->launch_mode >= MJ_CG_LAUNCH_AUTO && c->launch_mode <= MJ_CG_LAUNCH_PHASES_FIXED,
+               "cg2d: bad launch_mode %d", c->launch_mode);
+    MJ_REQUIRE(c->poll_iters >= 0, "cg2d: poll_iters < 0");
+    const int bt = block_threads_of(c, launch_mode_of(c));
+    if (bt < 0) return bt;
     return MJ_OK;
 }
 
-// Per-tile tables of the T plan for the fused kernel (see mj_halo_plan::cg_*).
+// Per-tile tables of the T plan for the solve (see mj_halo_plan::cg_*).
 int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
     std::lock_guard<std::mutex> lk(P->cg_mu);
     if (P->cg_full_off && P->cg_sNx == sNx && P->cg_sNy == sNy && P->cg_OL == OL) return MJ_OK;
@@ -680,7 +754,7 @@ int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
             const bool ring = ((j == OL - 1 || j == OL + sNy) && i >= OL && i < OL + sNx) ||
                               ((i == OL - 1 || i == OL + sNx) && j >= OL && j < OL + sNy);
             if (ring) {
-                // the fused kernel computes ring values from their sources: they must be interior points
+                // the solve computes ring values from their sources: they must be interior points
                 MJ_REQUIRE(interior(P->h_src[0][k]), "cg2d: ring point %d has a non-interior source %d", d,
                            P->h_src[0][k]);
                 rdst.push_back(d);
@@ -713,6 +787,42 @@ int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
     return MJ_OK;
 }
 
+template <int MODE>
+int launch_phases(const CgDev& a, const mj_cg2d_config* cfg, int threads, int mode, hipStream_t st) {
+    const int nunits = a.B * a.T;
+    const dim3 grid((unsigned)((nunits + kXcds - 1) / kXcds * kXcds)), block((unsigned)threads);
+    auto go = [&](int ph, int n) -> int {
+        switch (ph) {
+            case PH_I0: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I0>), grid, block, 0, st, a, n); break;
+            case PH_I1: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I1>), grid, block, 0, st, a, n); break;
+            case PH_I2: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I2>), grid, block, 0, st, a, n); break;
+            case PH_B: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_B>), grid, block, 0, st, a, n); break;
+            default: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_C>), grid, block, 0, st, a, n); break;
+        }
+        MJ_HIP(hipGetLastError());
+        return MJ_OK;
+    };
+    int rc;
+    if ((rc = go(PH_I0, 0)) || (rc = go(PH_I1, 0)) || (rc = go(PH_I2, 0))) return rc;
+    // B(n) for n = 1 .. max_iters + 1 (the last one only finishes members), C(n) for n <= max_iters
+    const int nlast = cfg->max_iters + 1;
+    const int chunk = cfg->poll_iters > 0 ? cfg->poll_iters : 32;
+    int n = 1;
+    while (n <= nlast) {
+        const int nend = mode == MJ_CG_LAUNCH_PHASES_FIXED ? nlast + 1 : std::min(nlast + 1, n + chunk);
+        for (; n < nend; ++n) {
+            if ((rc = go(PH_B, n))) return rc;
+            if (n <= cfg->max_iters && (rc = go(PH_C, n))) return rc;
+        }
+        if (n > nlast) break;
+        int h = 0;
+        MJ_HIP(hipMemcpyAsync(&h, a.nactive, sizeof(int), hipMemcpyDeviceToHost, st));
+        MJ_HIP(hipStreamSynchronize(st));
+        if (h == 0) break;
+    }
+    return MJ_OK;
+}
+
 }  // namespace
 
 extern "C" size_t mj_cg2d_workspace_bytes(const mj_cg2d_config* cfg) {
@@ -728,12 +838,19 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     MJ_REQUIRE(plan_T && ops && b && x0 && x && cg2dNorm && workspace && diag, "mj_cg2d_solve: null argument");
     MJ_REQUIRE(ops->aW2d && ops->aS2d && ops->aC2d && ops->pW && ops->pS && ops->pC, "mj_cg2d_solve: null operator");
     MJ_REQUIRE(x != b && x != x0, "mj_cg2d_solve: x must not alias b or x0");
+    MJ_REQUIRE(ops->member_stride % 2 == 0, "mj_cg2d_solve: operator member_stride must be even");
+    for (const void* q : {(const void*)x, (const void*)ops->aW2d, (const void*)ops->aS2d, (const void*)ops->aC2d,
+                          (const void*)ops->pW, (const void*)ops->pS, (const void*)ops->pC})
+        MJ_REQUIRE(((uintptr_t)q & 15) == 0, "mj_cg2d_solve: x and the operator arrays must be 16-byte aligned");
     MJ_REQUIRE(((uintptr_t)workspace & 255) == 0, "mj_cg2d_solve: workspace must be 256-byte aligned");
     const WsLayout L = ws_layout(cfg);
     MJ_REQUIRE(workspace_bytes >= L.total, "mj_cg2d_solve: workspace %zu < %zu bytes", workspace_bytes, L.total);
     mj_halo_plan* P = const_cast<mj_halo_plan*>(plan_T);
     MJ_REQUIRE(P->ntiles == cfg->ntiles, "mj_cg2d_solve: plan has %d tiles, config %d", P->ntiles, cfg->ntiles);
     if ((rc = ensure_cg_tables(P, cfg->sNx, cfg->sNy, cfg->OL))) return rc;
+    const int mode = launch_mode_of(cfg);
+    const int threads = block_threads_of(cfg, mode);
+    if (threads < 0) return threads;
 
     char* w = (char*)workspace;
     CgDev a{};
@@ -778,10 +895,18 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     a.psig = (double*)(w + L.psig);
     a.op_index = ops->op_index;
     a.mdone = (int32_t*)(w + L.mdone);
+    a.prod = (double*)(w + L.prod);
     unsigned int* counter = (unsigned int*)(w + L.misc);
     int* error = (int*)(w + L.misc + 64);
     a.nactive = (int*)(w + L.misc + 128);
     a.diag = diag;
+
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(cg2d_init_kernel, dim3(1), dim3(64), 0, st, counter, error, a.nactive, a.B);
+    MJ_HIP(hipGetLastError());
+    if (mode != MJ_CG_LAUNCH_PERSISTENT)
+        return cfg->sum_order == MJ_SUM_FORTRAN ? launch_phases<MJ_SUM_FORTRAN>(a, cfg, threads, mode, st)
+                                                : launch_phases<MJ_SUM_TREE>(a, cfg, threads, mode, st);
 
     int dev = 0;
     MJ_HIP(hipGetDevice(&dev));
@@ -790,25 +915,17 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     MJ_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
     MJ_HIP(hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, dev));
     MJ_REQUIRE(coop, "mj_cg2d_solve: device does not support cooperative launches");
-    const int nunits = a.B * a.T;
-    int threads = cfg->block_threads;
-    if (threads == 0) threads = MJ_CG_THREADS;
-    const size_t shmem = 0;
-    const void* kfn = cfg->sum_order == MJ_SUM_FORTRAN ? (const void*)cg2d_fused_kernel<MJ_SUM_FORTRAN>
-                                                       : (const void*)cg2d_fused_kernel<MJ_SUM_TREE>;
+    const void* kfn = cfg->sum_order == MJ_SUM_FORTRAN ? (const void*)cg2d_persistent_kernel<MJ_SUM_FORTRAN>
+                                                       : (const void*)cg2d_persistent_kernel<MJ_SUM_TREE>;
     int per_cu = 0;
-    MJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, shmem));
+    MJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, 0));
     MJ_REQUIRE(per_cu >= 1, "mj_cg2d_solve: kernel cannot be resident with %d threads", threads);
-    int grid = std::min(nunits, per_cu * ncu);
+    int grid = std::min(a.B * a.T, per_cu * ncu);
     if (cfg->grid_blocks > 0) grid = std::min(grid, cfg->grid_blocks);
-    MJ_REQUIRE(grid >= kXcds || grid == nunits, "mj_cg2d_solve: grid of %d blocks is too small", grid);
     GridSync gs{counter, error, (unsigned)grid, 0u,
                 (unsigned long long)(clk_khz > 0 ? clk_khz : 100000) * 1000ull * 4ull /* 4 s */};
-    hipStream_t st = as_stream(stream);
-    hipLaunchKernelGGL(cg2d_init_kernel, dim3(1), dim3(64), 0, st, counter, error, a.nactive, a.B);
-    MJ_HIP(hipGetLastError());
     void* args[] = {(void*)&a, (void*)&gs};
-    MJ_HIP(hipLaunchCooperativeKernel(kfn, dim3(grid), dim3(threads), args, (unsigned)shmem, st));
+    MJ_HIP(hipLaunchCooperativeKernel(kfn, dim3(grid), dim3(threads), args, 0u, st));
     return MJ_OK;
 }
 
@@ -828,8 +945,8 @@ extern "C" int mj_cg2d_matvec(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops
     mj_halo_geom g{cfg->batch, cfg->member_stride, (int64_t)nx * ny, 1, 0};
     if ((rc = mj_halo_exchange(plan_T, work, work, &g, stream))) return rc;
     hipLaunchKernelGGL(cg2d_matvec_kernel, grid, dim3(256), 0, st, ops->aW2d, ops->aS2d, ops->aC2d,
-                       ops->member_stride, ops->op_index, (const double*)work, y, cg2dNorm, cfg->member_stride, cfg->ntiles, ny, nx,
-                       cfg->OL, cfg->sNx, cfg->sNy);
+                       ops->member_stride, ops->op_index, (const double*)work, y, cg2dNorm, cfg->member_stride,
+                       cfg->ntiles, ny, nx, cfg->OL, cfg->sNx, cfg->sNy);
     MJ_HIP(hipGetLastError());
     return MJ_OK;
 }

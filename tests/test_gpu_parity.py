@@ -79,9 +79,13 @@ def test_cg2d_bitwise(dev, ex, precond, order):
     assert np.array_equal(x.cpu().numpy(), xr)
 
 
-def test_cg2d_batched_members_bitwise(dev, ex):
+LAUNCHES = ["persistent", "phases", "phases_fixed"]
+
+
+@pytest.mark.parametrize("launch", LAUNCHES)
+def test_cg2d_batched_members_bitwise(dev, ex, launch):
     """Members with different rhs (one zero rhs: 0 iterations) and different max_iters outcomes in one launch;
-    each member equals its own single solve on the oracle."""
+    each member equals its own single solve on the oracle, in every launch shape."""
     import torch
     from mit_jax_amd import cg2d
     g, ops, norm, _ = _cg_inputs("5point")
@@ -90,8 +94,9 @@ def test_cg2d_batched_members_bitwise(dev, ex):
     B = len(bs)
     b = np.stack(bs)
     x0 = np.zeros_like(b)
+    print(f"cg2d batched {launch}", flush=True)
     x, d = cg2d.cg2d_fortran(ex, [to_dev(o, dev) for o in ops], to_dev(b, dev), to_dev(x0, dev), cg2dNorm=norm,
-                             tolerance=1e-11, max_iters=2000, sum_order="tile")
+                             tolerance=1e-11, max_iters=2000, sum_order="tile", launch=launch, poll_iters=5)
     d.check()
     xs = x.cpu().numpy()
     for m in range(B):
@@ -102,7 +107,7 @@ def test_cg2d_batched_members_bitwise(dev, ex):
     assert int(d["numIters"][2]) == 0
     # max_iters reached: not converged, stops at the cap
     x, d = cg2d.cg2d_fortran(ex, [to_dev(o, dev) for o in ops], to_dev(b, dev), to_dev(x0, dev), cg2dNorm=norm,
-                             tolerance=1e-11, max_iters=7, sum_order="tile")
+                             tolerance=1e-11, max_iters=7, sum_order="tile", launch=launch, poll_iters=3)
     d.check()
     assert d["numIters"].cpu().tolist() == [7, 7, 0, 7]
     assert d["converged"].cpu().tolist() == [0, 0, 1, 0]
@@ -112,7 +117,8 @@ def test_cg2d_batched_members_bitwise(dev, ex):
     torch.cuda.synchronize()
 
 
-def test_cg2d_operator_slots_bitwise(dev, ex):
+@pytest.mark.parametrize("launch", ["persistent", "phases"])
+def test_cg2d_operator_slots_bitwise(dev, ex, launch):
     """Members reading different operator slots (5-point and diagonal preconditioners) in one launch through
     op_index: each equals its own single solve on the oracle, in both sum orders."""
     from mit_jax_amd import cg2d, synthetic as S
@@ -123,8 +129,9 @@ def test_cg2d_operator_slots_bitwise(dev, ex):
     bs = [S.rhs(g, 20 + m) for m in range(len(op_index))]
     b = np.stack(bs)
     for order in ("tile", "fortran"):
+        print(f"cg2d slots {launch} {order}", flush=True)
         x, d = cg2d.cg2d_fortran(ex, slots, to_dev(b, dev), to_dev(np.zeros_like(b), dev), cg2dNorm=norm,
-                                 tolerance=1e-12, max_iters=2000, sum_order=order, op_index=op_index)
+                                 tolerance=1e-12, max_iters=2000, sum_order=order, op_index=op_index, launch=launch)
         d.check()
         xs = x.cpu().numpy()
         for m, k in enumerate(op_index):
@@ -135,14 +142,17 @@ def test_cg2d_operator_slots_bitwise(dev, ex):
 
 
 def test_cg2d_block_size_independent(dev, ex):
-    """The reduction trees do not depend on the launch geometry: any block size gives the same bits."""
+    """The reduction trees do not depend on the launch geometry: any launch shape and block size gives the same
+    bits."""
     from mit_jax_amd import cg2d
     g, ops, norm, b = _cg_inputs("5point")
     x0 = np.zeros_like(b)
     outs = []
-    for bt in (128, 256, 512):
+    for launch, bt in (("persistent", 128), ("persistent", 512), ("persistent", 1024), ("phases", 64),
+                       ("phases", 256)):
+        print(f"cg2d geometry {launch} {bt}", flush=True)
         x, d = cg2d.cg2d_fortran(ex, [to_dev(o, dev) for o in ops], to_dev(b, dev), to_dev(x0, dev), cg2dNorm=norm,
-                                 tolerance=1e-12, max_iters=2000, sum_order="tile", block_threads=bt)
+                                 tolerance=1e-12, max_iters=2000, sum_order="tile", block_threads=bt, launch=launch)
         d.check()
         outs.append(x.cpu().numpy())
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
