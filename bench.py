@@ -37,7 +37,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--members", type=int, default=128)
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
+    ap.add_argument("--no-fp64-reference", action="store_true",
+                    help="with --precision mixed: skip timing the FP64 solve (the ratio and the error vs FP64)")
     ap.add_argument("--sum-order", choices=("tile", "fortran"), default="tile")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -144,7 +146,7 @@ def main():
               " is not run here (one process per GPU would each run this same single-GPU step)", file=sys.stderr)
         sys.exit(2)
     import torch
-    from mit_jax_amd import cg2d, gad
+    from mit_jax_amd import cg2d, gad, mixed
 
     dev = torch.device("cuda:0")
     B = args.members
@@ -153,57 +155,73 @@ def main():
     params = gad.GADParams.llc90()
     gT = torch.empty_like(E["tr"])
     x = torch.empty_like(E["rhs"])
-    ev = []
 
-    def step(record):
-        if record:
+    def solve(precision, out):
+        if precision == "fp64":
+            return cg2d.cg2d_fortran(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
+                                     max_iters=2000, sum_order=args.sum_order, out=out, op_index=E["op_index"])
+        return mixed.cg2d_mixed(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
+                                max_iters=4000, out=out, op_index=E["op_index"])
+
+    def step(precision, ev=None):
+        if ev is not None:
             e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
             e0.record()
-        if args.precision == "fp64":
-            _, d = cg2d.cg2d_fortran(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
-                                     max_iters=2000, sum_order=args.sum_order, out=x, op_index=E["op_index"])
-        else:
-            from mit_jax_amd import mixed
-            _, d = mixed.cg2d_mixed(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
-                                    max_iters=2000, out=x, op_index=E["op_index"])
-        if record:
+        _, d = solve(precision, x)
+        if ev is not None:
             e1.record()
         gad.advect_tiles(params, None, E["g2d"], E["vg"], E["u"], E["v"], E["tr"], E["hW"], E["hS"], E["rhC"],
                          nlev=E["nlev_t"], out=gT)
-        if record:
+        if ev is not None:
             e2.record()
             ev.append((e0, e1, e2))
         return d
 
-    for _ in range(args.warmup):
-        d = step(False)
-    torch.cuda.synchronize()
+    def timed(precision):
+        for _ in range(args.warmup):
+            step(precision)
+        torch.cuda.synchronize()
+        ev = []
+        print(f"bench: timing {precision}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            d = step(precision, ev)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        d.check()
+        return dict(el=el, d=d, cg_ms=float(np.mean([a.elapsed_time(b) for a, b, _ in ev])),
+                    gad_ms=float(np.mean([b.elapsed_time(c) for _, b, c in ev])))
+
     if args.copy_probe:
         a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    print("bench: timing", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        d = step(True)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    d.check()
+    run = timed(args.precision)
+    d = run["d"]
     iters = d["numIters"].cpu().numpy().astype(np.int64)
+    outer = d["outerIters"].cpu().numpy().astype(np.int64)
     conv = d["converged"].cpu().numpy()
-    cg_ms = np.mean([a.elapsed_time(b) for a, b, _ in ev])
-    gad_ms = np.mean([b.elapsed_time(c) for _, b, c in ev])
-    s_v = 8 if args.precision == "fp64" else 4
-    cg_bytes = float(iters.sum()) * N2 * (6 * 8 + 6 * s_v)
+    if args.precision == "fp64":
+        cg_bytes = float(iters.sum()) * N2 * (6 * 8 + 6 * 8)
+        kernel = "cg2d_phase_kernel"
+    else:
+        # inner iterations: FP32 coefficients and vectors; refinements: FP64 true residual (5 coefficient/vector
+        # reads: aW, aS, aC, x, b) and r written, plus the restart's FP32 conversion (r read, r32 written, d zeroed)
+        cg_bytes = float(iters.sum()) * N2 * (6 * 4 + 6 * 4) + float(outer.sum()) * N2 * (5 * 8 + 3 * 8)
+        kernel = "mx_step_kernel"
     gad_bytes = float(sum(80 * 13 * 98 * 98 * int(n) + 6 * 13 * 98 * 98 * 8 for n in E["nlev"]))
-    achieved = cg_bytes / (cg_ms * 1e-3) / 1e9
-    ms = el / args.steps * 1e3
+    achieved = cg_bytes / (run["cg_ms"] * 1e-3) / 1e9
+    ms = run["el"] / args.steps * 1e3
+    cfg = {"workload": f"LLC90 ensemble: {B} members x (CG2D to 1e-12 normalised residual + one DST3 "
+                       "step), 13x90x90 tiles, 50 levels, member level counts 15/30/50",
+           "members": B, "precision": args.precision, "sum_order": args.sum_order if args.precision == "fp64" else
+           "tile", "cg2d_iterations_mean": float(iters.mean()), "cg2d_iterations_max": int(iters.max()),
+           "refinements_mean": float(outer.mean()), "all_converged": bool((conv == 1).all())}
     out = {
         "metric": "member-steps/s, cg2d+DST3 at LLC90; mixed-refined vs FP64 at equal residual",
-        "value": B * args.steps / el,
+        "value": B * args.steps / run["el"],
         "unit": "member-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -214,17 +232,29 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"LLC90 ensemble: {B} members x (CG2D to 1e-12 normalised residual + one DST3 "
-                               "step), 13x90x90 tiles, 50 levels, member level counts 15/30/50",
-                   "members": B, "precision": args.precision, "sum_order": args.sum_order,
-                   "cg2d_iterations_mean": float(iters.mean()), "cg2d_iterations_max": int(iters.max()),
-                   "all_converged": bool((conv == 1).all())},
-        "roofline": {"bound": "hbm", "kernel": "cg2d_fused_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "config": cfg,
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "launch_ms": cg_ms, "algorithmic_bytes": cg_bytes},
-        "dst3": {"launch_ms": gad_ms, "achieved_GBs": gad_bytes / (gad_ms * 1e-3) / 1e9,
+                     "solve_ms": run["cg_ms"], "algorithmic_bytes": cg_bytes},
+        "dst3": {"launch_ms": run["gad_ms"], "achieved_GBs": gad_bytes / (run["gad_ms"] * 1e-3) / 1e9,
                  "algorithmic_bytes": gad_bytes},
     }
+    if args.precision == "mixed" and not args.no_fp64_reference:
+        ref = timed("fp64")
+        xr = torch.empty_like(x)
+        solve("fp64", xr)
+        solve("mixed", x)
+        torch.cuda.synchronize()
+        inner = cg2d.interior(E["ex"].L, dev)
+        diff = torch.where(inner, (x - xr).abs(), 0.0).amax(dim=(1, 2, 3))
+        scale = torch.where(inner, xr.abs(), 0.0).amax(dim=(1, 2, 3))
+        rel = (diff / scale).cpu().numpy()
+        fp64_value = B * args.steps / ref["el"]
+        out["fp64"] = {"value": fp64_value, "ms_per_step": ref["el"] / args.steps * 1e3, "solve_ms": ref["cg_ms"],
+                       "cg2d_iterations_mean": float(ref["d"]["numIters"].float().mean())}
+        out["mixed_over_fp64"] = out["value"] / fp64_value
+        out["error_vs_fp64"] = {"max_rel": float(rel.max()), "mean_rel": float(rel.mean()),
+                                "measure": "per member max|x_mixed - x_fp64| / max|x_fp64| over the interior"}
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(E, args.cpu_seconds)

@@ -88,6 +88,13 @@ int mj_halo_exchange(const mj_halo_plan* plan, const double* in, double* out, co
 /* Vector kind: (u_out, v_out) from (u_in, v_in). Same aliasing rule. */
 int mj_halo_exchange_uv(const mj_halo_plan* plan, const double* u_in, const double* v_in, double* u_out,
                         double* v_out, const mj_halo_geom* g, void* stream);
+/* Transpose of mj_halo_exchange / _uv (reverse mode; the reference checks this adjoint identity in
+ * test_exchange.py:106-118): from the cotangents of the exchanged arrays, the cotangents of the inputs. A point's
+ * cotangent is its own where no map writes it, plus sign * the cotangent of every written point that reads it
+ * (summed in a fixed order: deterministic). Scalar plans: vbar_out, vbar_in NULL. Every point of the inputs'
+ * cotangent arrays covered by the geometry is written; outputs must not alias inputs. */
+int mj_halo_exchange_adjoint(const mj_halo_plan* plan, const double* ubar_out, const double* vbar_out,
+                             double* ubar_in, double* vbar_in, const mj_halo_geom* g, void* stream);
 
 /* ======================================================================================= CG2D
  * Replaces cg2d_fortran (core/cg2d.py:205-269) and the solve inside cg2d_solve (:351-375): normalise the rhs by
@@ -142,6 +149,37 @@ int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const doub
 int mj_cg2d_matvec(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* v, double* y,
                    const double* cg2dNorm, const mj_cg2d_config* cfg, double* work, void* stream);
 
+/* Cotangent of the operator coefficients in the implicit derivative of the solve (core/cg2d.py:326-348: the
+ * transpose of dA -> -_matvec(dA, x), applied to lambda = A^-T w): gaW[q] = -(lam[q] xe[q-1] + lam[q-1] xe[q]) /
+ * cg2dNorm, gaS likewise with q-nx, gaC[q] = -lam[q] xe[q] / cg2dNorm, where xe = exch(where(inner, x, 0)) and
+ * lam is zero off the interior. x, lam, gaW, gaS, gaC: [B][T, ny, nx] (cfg->member_stride apart); cg2dNorm: device
+ * [B]; work: device scratch of B*member_stride doubles. Only cfg's batch/layout fields are read. */
+int mj_cg2d_coeff_grad(const mj_halo_plan* plan_T, const double* x, const double* lam, const double* cg2dNorm,
+                       double* gaW, double* gaS, double* gaC, const mj_cg2d_config* cfg, double* work, void* stream);
+
+/* ---- mixed-precision CG2D (no single counterpart in the reference: it solves the system of cg2d_fortran,
+ * core/cg2d.py:205-269, to the same normalised stopping rule err < tolerance^2, by FP32 inner PCG with FP64
+ * iterative refinement). True residuals, dot products (MJ_SUM_TREE order), alpha/beta and the solution update are
+ * FP64; the inner operator/preconditioner copies, inner vectors and stencils are FP32. diag: numIters = total inner
+ * iterations, outerIters = refinements, lastResidual = the last FP64 true residual (normalised). */
+typedef struct {
+    int op_slots;            /* number of operator slots when ops.op_index is set (else ignored) */
+    double inner_tolerance;  /* inner solve stops when its residual norm falls below this fraction (e.g. 1e-5) */
+    int inner_max_iters;     /* inner iterations per refinement */
+    int max_outer;           /* refinements */
+} mj_cg2d_mixed_config;
+
+/* Workspace of mj_cg2d_solve_mixed; ops_member_stride as in mj_cg2d_ops (0 = one shared operator). */
+size_t mj_cg2d_mixed_workspace_bytes(const mj_cg2d_config* cfg, const mj_cg2d_mixed_config* mcfg,
+                                     int64_t ops_member_stride);
+
+/* Arguments as mj_cg2d_solve. cfg: sum_order must be MJ_SUM_TREE; launch_mode AUTO (= PHASES), PHASES or
+ * PHASES_FIXED; max_iters caps the total inner iterations. */
+int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* b, const double* x0,
+                        double* x, const double* cg2dNorm, const mj_cg2d_config* cfg,
+                        const mj_cg2d_mixed_config* mcfg, void* workspace, size_t workspace_bytes,
+                        mj_cg2d_diag* diag, void* stream);
+
 /* ======================================================================================= GAD DST3
  * Replaces gad_advection / advect_tiles (pkgs/gad.py:460-560) with dst3_adv_x/y (:409-453): the 3-pass
  * multi-dimensional DST3 on the LLC cubed sphere with per-facet sweep order, overlap/interior update regions and
@@ -189,6 +227,17 @@ typedef struct {
 int mj_gad_dst3(const mj_gad_plan* plan, const mj_gad_grid* grid, const mj_gad_config* cfg, const double* uFld,
                 const double* vFld, const double* wFld, const double* tracer, const double* hFacW,
                 const double* hFacS, const double* recip_hFacC, double* gTracer, double* work, void* stream);
+
+/* Reverse mode of mj_gad_dst3 without the R flux (wFld NULL; the reference's advect_tiles, pkgs/gad.py:460-539,
+ * differentiated as test_gad.py:297-346 does): for the cotangent gTracer_bar of gTracer, writes the cotangents of
+ * tracer, uFld and vFld (every padded point of [B][T, nr, ny, nx]; levels at or beyond a member's nlev get 0).
+ * The grid, hFac and mask inputs are constants. work: device, >= mj_gad_dst3_adjoint_workspace_bytes(). */
+size_t mj_gad_dst3_adjoint_workspace_bytes(void);
+int mj_gad_dst3_adjoint(const mj_gad_plan* plan, const mj_gad_grid* grid, const mj_gad_config* cfg,
+                        const double* uFld, const double* vFld, const double* tracer, const double* hFacW,
+                        const double* hFacS, const double* recip_hFacC, const double* gTracer_bar,
+                        double* tracer_bar, double* uFld_bar, double* vFld_bar, double* work, size_t work_bytes,
+                        void* stream);
 
 #ifdef __cplusplus
 }

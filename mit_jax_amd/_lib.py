@@ -47,6 +47,10 @@ class Cg2dConfig(ctypes.Structure):
                 ("grid_blocks", I32), ("launch_mode", I32), ("poll_iters", I32)]
 
 
+class Cg2dMixedConfig(ctypes.Structure):
+    _fields_ = [("op_slots", I32), ("inner_tolerance", F64), ("inner_max_iters", I32), ("max_outer", I32)]
+
+
 class GadGrid(ctypes.Structure):
     _fields_ = [(k, P) for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "rA", "maskInC", "maskW",
                                  "maskS", "maskC", "dTlev", "drF", "recip_drF", "rhoFacC", "recip_rhoFacC",
@@ -66,14 +70,23 @@ SIGNATURES = {
     "mj_halo_plan_inplace_ok": (I32, [P]),
     "mj_halo_exchange": (I32, [P, P, P, ctypes.POINTER(HaloGeom), P]),
     "mj_halo_exchange_uv": (I32, [P, P, P, P, P, ctypes.POINTER(HaloGeom), P]),
+    "mj_halo_exchange_adjoint": (I32, [P, P, P, P, P, ctypes.POINTER(HaloGeom), P]),
     "mj_cg2d_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Cg2dConfig)]),
     "mj_cg2d_solve": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, P, ctypes.POINTER(Cg2dConfig), P, ctypes.c_size_t,
                             P, P]),
     "mj_cg2d_matvec": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, ctypes.POINTER(Cg2dConfig), P, P]),
+    "mj_cg2d_coeff_grad": (I32, [P, P, P, P, P, P, P, ctypes.POINTER(Cg2dConfig), P, P]),
+    "mj_cg2d_mixed_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Cg2dConfig), ctypes.POINTER(Cg2dMixedConfig),
+                                                        I64]),
+    "mj_cg2d_solve_mixed": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, P, ctypes.POINTER(Cg2dConfig),
+                                  ctypes.POINTER(Cg2dMixedConfig), P, ctypes.c_size_t, P, P]),
     "mj_gad_plan_create": (I32, [I32, P, P, I32, I32, I32, I32, ctypes.POINTER(P)]),
     "mj_gad_plan_destroy": (I32, [P]),
     "mj_gad_plan_describe": (I32, [P, P, P]),
     "mj_gad_tables_host": (I32, [I32, P, P, I32, I32, I32, I32, P, P]),
+    "mj_gad_dst3_adjoint_workspace_bytes": (ctypes.c_size_t, []),
+    "mj_gad_dst3_adjoint": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P,
+                                  P, P, ctypes.c_size_t, P]),
     "mj_gad_dst3": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P, P]),
 }
 

@@ -32,6 +32,7 @@
 //               exactly max_iters iterations (no host read, capturable in a graph).
 #include <algorithm>
 
+#include "mj_cg_sweep.h"
 #include "mj_common.h"
 #include "mj_internal.h"
 
@@ -68,140 +69,16 @@ struct CgDev {
 #ifndef MJ_PH_WAVES_PER_EU
 #define MJ_PH_WAVES_PER_EU 8
 #endif  // occupancy target of the phase launches (waves per SIMD): bounds their register use
-constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
-
-// ------------------------------------------------------------------------------------------------ stencils
-// apply_operator (core/cg2d.py:181-188): aW(i)x(i-1) + aW(i+1)x(i+1) + aS(j)x(j-1) + aS(j+1)x(j+1) + aC x
-__device__ __forceinline__ double op_A(const double* aW, const double* aS, const double* aC, const double* x, int p,
-                                       int nx) {
-    return aW[p] * x[p - 1] + aW[p + 1] * x[p + 1] + aS[p] * x[p - nx] + aS[p + nx] * x[p + nx] + aC[p] * x[p];
-}
-// apply_preconditioner (core/cg2d.py:191-198): pC r + pW(i)r(i-1) + pW(i+1)r(i+1) + pS(j)r(j-1) + pS(j+1)r(j+1)
-__device__ __forceinline__ double op_M(const double* pW, const double* pS, const double* pC, const double* r, int p,
-                                       int nx) {
-    return pC[p] * r[p] + pW[p] * r[p - 1] + pW[p + 1] * r[p + 1] + pS[p] * r[p - nx] + pS[p + nx] * r[p + nx];
-}
-
-// ------------------------------------------------------------------------------------------------ reductions
-// Reduction of per-point values over a tile interior, MJ_SUM_TREE: each wave reduces whole rows with wave_tree
-// (lane l holds i = l and i = l + 64) into a row-sum array in LDS, then one wave reduces the row sums.
-// MJ_SUM_FORTRAN: per-point values are staged in global memory and one lane adds them in j/i order.
-
-// Row loop: wave w of the block takes rows jl = w, w + nwaves, ...; p0 = (jl, il = lane), p1 = (jl, lane + 64).
-// Unrolled so that a wave has the loads of several rows in flight at once.
-#ifndef MJ_ROW_UNROLL
-#define MJ_ROW_UNROLL 2
-#endif
-#define MJ_PRAGMA(x) _Pragma(#x)
-#define MJ_UNROLL(n) MJ_PRAGMA(unroll n)
-#define MJ_ROWS(...)                                                                                      \
-    MJ_UNROLL(MJ_ROW_UNROLL)                                                                              \
-    for (int jl = wave; jl < sNy; jl += nwaves) {                                                         \
-        const int prow = tbase + (OL + jl) * nx + OL;                                                     \
-        const int p0 = prow + lane;                                                                       \
-        const bool has1 = lane + 64 < sNx;                                                                \
-        const int p1 = prow + lane + 64;                                                                  \
-        __VA_ARGS__                                                                                       \
-    }
-
-#define MJ_UNIT_IDS                                                                                       \
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;              \
-    const int sNx = a.sNx, sNy = a.sNy, OL = a.OL, nx = a.nx, tbase = t * a.plane;                        \
-    (void)tid;                                                                                            \
-    (void)sNx;                                                                                            \
-    (void)OL;                                                                                             \
-    (void)nx;                                                                                             \
-    (void)tbase;
-
-// Per-block LDS: row sums of two reductions, and values broadcast from thread 0.
-__shared__ double g_rows[2 * 128];
-__shared__ double g_bcast[2];
-__shared__ int g_ibcast;
-
-// Store one row's per-point values of reduction r: tree order -> wave_tree into the row sums; fortran order -> the
-// unit's global staging area (stage[r][jl][il]).
-template <int MODE>
-__device__ __forceinline__ void put_row(int r, int jl, int lane, double v0, double v1, bool has1, double* stage,
-                                        int sNx, int sNy) {
-    if (MODE == MJ_SUM_TREE) {
-        const double w = wave_tree(v0, v1, has1);
-        if (lane == 0) g_rows[r * 128 + jl] = w;
-    } else {
-        double* pr = stage + ((size_t)r * sNy + jl) * sNx;
-        pr[lane] = v0;
-        if (has1) pr[lane + 64] = v1;
-    }
-}
-
-// After a __syncthreads: tile partial of reduction r (valid in lane 0 of the calling wave).
-template <int MODE>
-__device__ __forceinline__ double tile_of(int r, int lane, const double* stage, int sNx, int sNy) {
-    if (MODE == MJ_SUM_TREE) {
-        const double* rb = g_rows + r * 128;
-        const bool h = lane + 64 < sNy;
-        return wave_tree(rb[lane], h ? rb[lane + 64] : 0.0, h);
-    }
-    double s = 0.0;
-    if (lane == 0) {
-        const double* pr = stage + (size_t)r * sNy * sNx;
-        for (int q = 0; q < sNy * sNx; ++q) s = s + pr[q];
-    }
-    return s;
-}
-
-// Reductions 0 and 1 of the unit written to dst0[idx] and dst1[idx] (dst1 NULL: only reduction 0).
-template <int MODE>
-__device__ __forceinline__ void store_tile_sums(double* dst0, double* dst1, int idx, int lane, int wave, int nwaves,
-                                                const double* stage, int sNx, int sNy) {
-    __syncthreads();
-    const int nr = dst1 ? 2 : 1;
-    for (int r = wave; r < nr; r += nwaves) {
-        const double v = tile_of<MODE>(r, lane, stage, sNx, sNy);
-        if (lane == 0) (r == 0 ? dst0 : dst1)[idx] = v;
-    }
-    __syncthreads();
-}
-
 __device__ __forceinline__ int64_t op_base(const CgDev& a, int m) {
     return (int64_t)(a.op_index ? a.op_index[m] : m) * a.oms;
 }
 __device__ __forceinline__ double* stage_of(const CgDev& a, int m, int t) {
     return a.prod + ((size_t)m * a.T + t) * 2 * a.sNx * a.sNy;
 }
-__device__ __forceinline__ double member_rhs_max(const CgDev& a, int m) {
-    double mx = 0.0;
-    for (int t = 0; t < a.T; ++t) mx = fmax(mx, a.pmax[m * a.T + t]);
-    return mx;
-}
-__device__ __forceinline__ double member_rhs_norm(const CgDev& a, int m) {
-    const double mx = member_rhs_max(a, m);
-    return mx != 0.0 ? 1.0 / mx : 1.0;
-}
 __device__ __forceinline__ double* R(const CgDev& a, int n) { return n & 1 ? a.r1 : a.r0; }
 __device__ __forceinline__ double* S(const CgDev& a, int n) { return n & 1 ? a.s1 : a.s0; }
 
 // ------------------------------------------------------------------------------------------------ unit functions
-// I0: per-tile max |b * cg2dNorm| (cg2d.F:150-160; the maximum is order free).
-__device__ __forceinline__ void unit_I0(const CgDev& a, int m, int t) {
-    MJ_UNIT_IDS
-    const double* b = a.b + m * a.ms;
-    const double nm = a.norm[m];
-    double mx = 0.0;
-    MJ_ROWS({
-        mx = fmax(mx, fabs(b[p0] * nm));
-        if (has1) mx = fmax(mx, fabs(b[p1] * nm));
-    })
-    for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-    if (lane == 0) g_rows[wave] = mx;
-    __syncthreads();
-    if (tid == 0) {
-        double v = 0.0;
-        for (int w = 0; w < nwaves; ++w) v = fmax(v, g_rows[w]);
-        a.pmax[m * a.T + t] = v;
-    }
-    __syncthreads();
-}
-
 // I1: x = normalised first guess (halo points of the tile: the exchanged value), r = b - A x on the interior,
 // tile partials of err(0) = sum r^2 and sumRHS = sum b. Work vectors start at 0 (s0 entirely).
 template <int MODE>
@@ -210,7 +87,6 @@ __device__ __forceinline__ void unit_I1(const CgDev& a, int m, int t) {
     const int plane = a.plane;
     const double rhsNorm = member_rhs_norm(a, m);
     const double nm = a.norm[m];
-    const double* x0 = a.x0 + m * a.ms;
     const double* b = a.b + m * a.ms;
     double* x = a.x + m * a.ms;
     double *r0 = a.r0 + m * a.ms, *r1 = a.r1 + m * a.ms, *s0 = a.s0 + m * a.ms, *s1 = a.s1 + m * a.ms;
@@ -221,24 +97,12 @@ __device__ __forceinline__ void unit_I1(const CgDev& a, int m, int t) {
         const bool inner = j >= OL && j < OL + sNy && i >= OL && i < OL + sNx;
         s0[p] = 0.0;
         if (!inner) {
-            x[p] = x0[p];
             r0[p] = 0.0;
             r1[p] = 0.0;
             s1[p] = 0.0;
-        } else {
-            x[p] = x0[p] * rhsNorm;
         }
     }
-    __syncthreads();
-    // exchanged points: the pre-exchange value at the source (normalised where the source is interior)
-    for (int k = a.full_off[t] + tid; k < a.full_off[t + 1]; k += blockDim.x) {
-        const int s = a.full_src[k];
-        const int sq = s % plane, sj = sq / nx, si = sq - sj * nx;
-        const bool inner = sj >= OL && sj < OL + sNy && si >= OL && si < OL + sNx;
-        const double v = inner ? x0[s] * rhsNorm : x0[s];
-        x[a.full_dst[k]] = a.full_sign[k] < 0 ? -v : v;
-    }
-    __syncthreads();
+    first_guess(a, m, t, rhsNorm);
     MJ_ROWS({
         const double bn0 = b[p0] * nm * rhsNorm;
         const double r_0 = bn0 - op_A(a.aW + ob, a.aS + ob, a.aC + ob, x, p0, nx);
@@ -326,76 +190,6 @@ __device__ __forceinline__ void unit_finish(const CgDev& a, int m, int t, int n,
         atomicSub(a.nactive, 1);
     }
 }
-
-// ------------------------------------------------------------------------------------------------ row sweeps
-// The iteration phases sweep rows: each wave owns a strip of consecutive rows and lane l owns the column pair
-// (2l, 2l+1), loaded and stored as one 16-byte access. Down the strip the three rows a 5-point stencil reads are
-// kept in registers (one new row loaded per step); the i-1 / i+1 neighbours come from the adjacent lane, and the
-// halo columns -1 and sNx from one extra load by the first / last lane. Every value is computed with the same
-// expression, in the same term order, as op_A / op_M.
-struct Lanes {
-    int lane, c, last;  // c: this lane's first column (clamped in range for idle lanes), last: lane of column sNx-1
-    bool act;           // lane owns columns
-};
-__device__ __forceinline__ Lanes lanes_of(int lane, int sNx) {
-    Lanes L;
-    L.lane = lane;
-    L.act = 2 * lane < sNx;
-    L.c = L.act ? 2 * lane : sNx - 2;
-    L.last = sNx / 2 - 1;
-    return L;
-}
-__device__ __forceinline__ double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
-__device__ __forceinline__ void st2(double* p, double2 v) { *reinterpret_cast<double2*>(p) = v; }
-// Value at the halo column on this lane's open side (lane 0: column -1, last lane: column sNx), row base prow.
-__device__ __forceinline__ double halo_of(const double* v, int prow, const Lanes& L, int sNx) {
-    return (L.lane == 0 || L.lane == L.last) ? v[prow + (L.lane == 0 ? -1 : sNx)] : 0.0;
-}
-// Left neighbour of column c and right neighbour of column c+1.
-__device__ __forceinline__ double left_of(double2 v, double h, const Lanes& L) {
-    const double w = __shfl_up(v.y, 1, 64);
-    return L.lane == 0 ? h : w;
-}
-__device__ __forceinline__ double right_of(double2 v, double h, const Lanes& L) {
-    const double w = __shfl_down(v.x, 1, 64);
-    return L.lane == L.last ? h : w;
-}
-// op_A / op_M at columns c, c+1 of one row: Xm, Xc, Xp rows j-1, j, j+1; xl, xr the outer neighbours.
-__device__ __forceinline__ double2 pair_A(double2 AW, double awr, double2 AS0, double2 AS1, double2 AC, double2 Xm,
-                                          double2 Xc, double2 Xp, double xl, double xr) {
-    double2 q;
-    q.x = AW.x * xl + AW.y * Xc.y + AS0.x * Xm.x + AS1.x * Xp.x + AC.x * Xc.x;
-    q.y = AW.y * Xc.x + awr * xr + AS0.y * Xm.y + AS1.y * Xp.y + AC.y * Xc.y;
-    return q;
-}
-__device__ __forceinline__ double2 pair_M(double2 PC, double2 PW, double pwr, double2 PS0, double2 PS1, double2 Rm,
-                                          double2 Rc, double2 Rp, double rl, double rr) {
-    double2 q;
-    q.x = PC.x * Rc.x + PW.x * rl + PW.y * Rc.y + PS0.x * Rm.x + PS1.x * Rp.x;
-    q.y = PC.y * Rc.y + PW.y * Rc.x + pwr * rr + PS0.y * Rm.y + PS1.y * Rp.y;
-    return q;
-}
-// One row's per-point values of reduction r held as column pairs: tree order regroups them onto the canonical
-// lanes (lane l: v[l] and v[l + 64]) and reduces with wave_tree into the row sums; fortran order stages them.
-template <int MODE>
-__device__ __forceinline__ void put_row2(int r, int jl, const Lanes& L, double2 v, double* stage, int sNx, int sNy) {
-    if (MODE == MJ_SUM_TREE) {
-        const int l = L.lane, s0 = l >> 1, s1 = 32 + (l >> 1);
-        const double a0 = __shfl(v.x, s0, 64), a1 = __shfl(v.y, s0, 64);
-        const double b0 = __shfl(v.x, s1, 64), b1 = __shfl(v.y, s1, 64);
-        const double lo = (l & 1) ? a1 : a0, hi = (l & 1) ? b1 : b0;
-        const double w = wave_tree(lo, hi, l + 64 < sNx);
-        if (l == 0) g_rows[r * 128 + jl] = w;
-    } else {
-        if (L.act) st2(stage + ((size_t)r * sNy + jl) * sNx + L.c, v);
-    }
-}
-
-// Strip of rows of this wave: [j0, j1).
-#define MJ_STRIP                                                                                         \
-    const int rows_ = (sNy + nwaves - 1) / nwaves, j0 = wave * rows_, j1 = min(j0 + rows_, sNy);        \
-    const Lanes L = lanes_of(lane, sNx);
-#define MJ_ROW(j) (tbase + (OL + (j)) * nx + OL)
 
 // Phase B(n) of an active member: s_n = M r_{n-1} + beta s_{n-1} on interior and ring (ring from its interior
 // sources), then the tile partial of sigma = sum s_n (A s_n).
@@ -592,25 +386,13 @@ done:
 }
 
 // ------------------------------------------------------------------------------------------------ phases
-// One block per unit. gridDim.x is a multiple of kXcds; block b runs on XCD b % kXcds and takes unit
-// (b % kXcds) * (gridDim.x / kXcds) + b / kXcds, so each XCD works through one contiguous, tile-major range of
-// units and the coefficients of its tiles stay in its L2.
-__device__ __forceinline__ bool phase_unit(const CgDev& a, int& m, int& t) {
-    const int chunk = gridDim.x / kXcds;
-    const int u = (blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
-    if (u >= a.B * a.T) return false;
-    t = u / a.B;
-    m = u - t * a.B;
-    return true;
-}
-
 enum { PH_I0 = 0, PH_I1 = 1, PH_I2 = 2, PH_B = 3, PH_C = 4 };
 
 template <int MODE, int PH>
 __global__ void __launch_bounds__(MJ_PH_THREADS) __attribute__((amdgpu_waves_per_eu(MJ_PH_WAVES_PER_EU)))
 cg2d_phase_kernel(CgDev a, int n) {
     int m, t;
-    if (!phase_unit(a, m, t)) return;
+    if (!phase_unit(a.B, a.T, m, t)) return;
     if (PH == PH_B || PH == PH_C) {
         // units of members that finished earlier leave before touching anything else
         const int st = a.mdone[m];
@@ -655,6 +437,64 @@ __global__ void zero_halo_copy_kernel(const double* v, double* w, int64_t ms, in
     const int r = (int)(q % plane), j = r / nx, i = r - j * nx;
     const bool inner = j >= OL && j < OL + sNy && i >= OL && i < OL + sNx;
     w[m * ms + q] = inner ? v[m * ms + q] : 0.0;
+}
+
+// Per-tile tables of the T plan for the solve (see mj_halo_plan::cg_*).
+int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
+    std::lock_guard<std::mutex> lk(P->cg_mu);
+    if (P->cg_full_off && P->cg_sNx == sNx && P->cg_sNy == sNy && P->cg_OL == OL) return MJ_OK;
+    MJ_REQUIRE(!P->cg_full_off, "cg2d: the T plan was already used with another layout");
+    MJ_REQUIRE(P->nmaps == 1, "cg2d: plan_T must be a scalar (T) plan");
+    const int nx = P->nx, ny = P->ny, T = P->ntiles, plane = nx * ny;
+    MJ_REQUIRE(nx == sNx + 2 * OL && ny == sNy + 2 * OL, "cg2d: plan layout %dx%d does not match sNx/OL", ny, nx);
+    std::vector<int32_t> foff(T + 1, 0), roff(T + 1, 0), fdst, fsrc, rdst, rsrc;
+    std::vector<int8_t> fsg, rsg;
+    auto interior = [&](int q) {
+        const int r = q % plane, j = r / nx, i = r % nx;
+        return j >= OL && j < OL + sNy && i >= OL && i < OL + sNx;
+    };
+    for (int t = 0; t < T; ++t) {
+        for (int k = 0; k < P->npts[0]; ++k) {
+            const int d = P->h_dst[0][k];
+            if (d / plane != t) continue;
+            fdst.push_back(d);
+            fsrc.push_back(P->h_src[0][k]);
+            fsg.push_back(P->h_sign[0][k]);
+            const int r = d % plane, j = r / nx, i = r % nx;
+            const bool ring = ((j == OL - 1 || j == OL + sNy) && i >= OL && i < OL + sNx) ||
+                              ((i == OL - 1 || i == OL + sNx) && j >= OL && j < OL + sNy);
+            if (ring) {
+                // the solve computes ring values from their sources: they must be interior points
+                MJ_REQUIRE(interior(P->h_src[0][k]), "cg2d: ring point %d has a non-interior source %d", d,
+                           P->h_src[0][k]);
+                rdst.push_back(d);
+                rsrc.push_back(P->h_src[0][k]);
+                rsg.push_back(P->h_sign[0][k]);
+            }
+        }
+        foff[t + 1] = (int32_t)fdst.size();
+        roff[t + 1] = (int32_t)rdst.size();
+        P->cg_ring_max = std::max(P->cg_ring_max, roff[t + 1] - roff[t]);
+    }
+    auto up = [](const void* h, size_t n, void** d) -> int {
+        MJ_HIP(hipMalloc(d, n ? n : 4));
+        if (n) MJ_HIP(hipMemcpy(*d, h, n, hipMemcpyHostToDevice));
+        return MJ_OK;
+    };
+    int rc;
+    if ((rc = up(foff.data(), foff.size() * 4, (void**)&P->cg_full_off)) ||
+        (rc = up(fdst.data(), fdst.size() * 4, (void**)&P->cg_full_dst)) ||
+        (rc = up(fsrc.data(), fsrc.size() * 4, (void**)&P->cg_full_src)) ||
+        (rc = up(fsg.data(), fsg.size(), (void**)&P->cg_full_sign)) ||
+        (rc = up(roff.data(), roff.size() * 4, (void**)&P->cg_ring_off)) ||
+        (rc = up(rdst.data(), rdst.size() * 4, (void**)&P->cg_ring_dst)) ||
+        (rc = up(rsrc.data(), rsrc.size() * 4, (void**)&P->cg_ring_src)) ||
+        (rc = up(rsg.data(), rsg.size(), (void**)&P->cg_ring_sign)))
+        return rc;
+    P->cg_sNx = sNx;
+    P->cg_sNy = sNy;
+    P->cg_OL = OL;
+    return MJ_OK;
 }
 
 }  // namespace mj
@@ -726,64 +566,6 @@ int check_cfg(const mj_cg2d_config* c) {
     MJ_REQUIRE(c->poll_iters >= 0, "cg2d: poll_iters < 0");
     const int bt = block_threads_of(c, launch_mode_of(c));
     if (bt < 0) return bt;
-    return MJ_OK;
-}
-
-// Per-tile tables of the T plan for the solve (see mj_halo_plan::cg_*).
-int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
-    std::lock_guard<std::mutex> lk(P->cg_mu);
-    if (P->cg_full_off && P->cg_sNx == sNx && P->cg_sNy == sNy && P->cg_OL == OL) return MJ_OK;
-    MJ_REQUIRE(!P->cg_full_off, "cg2d: the T plan was already used with another layout");
-    MJ_REQUIRE(P->nmaps == 1, "cg2d: plan_T must be a scalar (T) plan");
-    const int nx = P->nx, ny = P->ny, T = P->ntiles, plane = nx * ny;
-    MJ_REQUIRE(nx == sNx + 2 * OL && ny == sNy + 2 * OL, "cg2d: plan layout %dx%d does not match sNx/OL", ny, nx);
-    std::vector<int32_t> foff(T + 1, 0), roff(T + 1, 0), fdst, fsrc, rdst, rsrc;
-    std::vector<int8_t> fsg, rsg;
-    auto interior = [&](int q) {
-        const int r = q % plane, j = r / nx, i = r % nx;
-        return j >= OL && j < OL + sNy && i >= OL && i < OL + sNx;
-    };
-    for (int t = 0; t < T; ++t) {
-        for (int k = 0; k < P->npts[0]; ++k) {
-            const int d = P->h_dst[0][k];
-            if (d / plane != t) continue;
-            fdst.push_back(d);
-            fsrc.push_back(P->h_src[0][k]);
-            fsg.push_back(P->h_sign[0][k]);
-            const int r = d % plane, j = r / nx, i = r % nx;
-            const bool ring = ((j == OL - 1 || j == OL + sNy) && i >= OL && i < OL + sNx) ||
-                              ((i == OL - 1 || i == OL + sNx) && j >= OL && j < OL + sNy);
-            if (ring) {
-                // the solve computes ring values from their sources: they must be interior points
-                MJ_REQUIRE(interior(P->h_src[0][k]), "cg2d: ring point %d has a non-interior source %d", d,
-                           P->h_src[0][k]);
-                rdst.push_back(d);
-                rsrc.push_back(P->h_src[0][k]);
-                rsg.push_back(P->h_sign[0][k]);
-            }
-        }
-        foff[t + 1] = (int32_t)fdst.size();
-        roff[t + 1] = (int32_t)rdst.size();
-        P->cg_ring_max = std::max(P->cg_ring_max, roff[t + 1] - roff[t]);
-    }
-    auto up = [](const void* h, size_t n, void** d) -> int {
-        MJ_HIP(hipMalloc(d, n ? n : 4));
-        if (n) MJ_HIP(hipMemcpy(*d, h, n, hipMemcpyHostToDevice));
-        return MJ_OK;
-    };
-    int rc;
-    if ((rc = up(foff.data(), foff.size() * 4, (void**)&P->cg_full_off)) ||
-        (rc = up(fdst.data(), fdst.size() * 4, (void**)&P->cg_full_dst)) ||
-        (rc = up(fsrc.data(), fsrc.size() * 4, (void**)&P->cg_full_src)) ||
-        (rc = up(fsg.data(), fsg.size(), (void**)&P->cg_full_sign)) ||
-        (rc = up(roff.data(), roff.size() * 4, (void**)&P->cg_ring_off)) ||
-        (rc = up(rdst.data(), rdst.size() * 4, (void**)&P->cg_ring_dst)) ||
-        (rc = up(rsrc.data(), rsrc.size() * 4, (void**)&P->cg_ring_src)) ||
-        (rc = up(rsg.data(), rsg.size(), (void**)&P->cg_ring_sign)))
-        return rc;
-    P->cg_sNx = sNx;
-    P->cg_sNy = sNy;
-    P->cg_OL = OL;
     return MJ_OK;
 }
 

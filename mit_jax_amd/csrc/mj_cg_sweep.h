@@ -1,0 +1,266 @@
+// Device building blocks of the CG2D solves (FP64 and mixed precision): the 5-point stencils in the reference's
+// term order, the fixed row/tile reduction trees, and the row sweeps over column pairs.
+#pragma once
+
+#include "mj_common.h"
+
+namespace mj {
+
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+// ------------------------------------------------------------------------------------------------ stencils
+// apply_operator (core/cg2d.py:181-188): aW(i)x(i-1) + aW(i+1)x(i+1) + aS(j)x(j-1) + aS(j+1)x(j+1) + aC x
+template <typename T>
+__device__ __forceinline__ T op_A(const T* aW, const T* aS, const T* aC, const T* x, int p, int nx) {
+    return aW[p] * x[p - 1] + aW[p + 1] * x[p + 1] + aS[p] * x[p - nx] + aS[p + nx] * x[p + nx] + aC[p] * x[p];
+}
+// apply_preconditioner (core/cg2d.py:191-198): pC r + pW(i)r(i-1) + pW(i+1)r(i+1) + pS(j)r(j-1) + pS(j+1)r(j+1)
+template <typename T>
+__device__ __forceinline__ T op_M(const T* pW, const T* pS, const T* pC, const T* r, int p, int nx) {
+    return pC[p] * r[p] + pW[p] * r[p - 1] + pW[p + 1] * r[p + 1] + pS[p] * r[p - nx] + pS[p + nx] * r[p + nx];
+}
+
+// ------------------------------------------------------------------------------------------------ reductions
+// Reduction of per-point values over a tile interior, MJ_SUM_TREE: each wave reduces whole rows with wave_tree
+// (lane l holds i = l and i = l + 64) into a row-sum array in LDS, then one wave reduces the row sums.
+// MJ_SUM_FORTRAN: per-point values are staged in global memory and one lane adds them in j/i order.
+
+// Row loop: wave w of the block takes rows jl = w, w + nwaves, ...; p0 = (jl, il = lane), p1 = (jl, lane + 64).
+// Unrolled so that a wave has the loads of several rows in flight at once.
+#ifndef MJ_ROW_UNROLL
+#define MJ_ROW_UNROLL 2
+#endif
+#define MJ_PRAGMA(x) _Pragma(#x)
+#define MJ_UNROLL(n) MJ_PRAGMA(unroll n)
+#define MJ_ROWS(...)                                                                                      \
+    MJ_UNROLL(MJ_ROW_UNROLL)                                                                              \
+    for (int jl = wave; jl < sNy; jl += nwaves) {                                                         \
+        const int prow = tbase + (OL + jl) * nx + OL;                                                     \
+        const int p0 = prow + lane;                                                                       \
+        const bool has1 = lane + 64 < sNx;                                                                \
+        const int p1 = prow + lane + 64;                                                                  \
+        __VA_ARGS__                                                                                       \
+    }
+
+#define MJ_UNIT_IDS                                                                                       \
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;              \
+    const int sNx = a.sNx, sNy = a.sNy, OL = a.OL, nx = a.nx, tbase = t * a.plane;                        \
+    (void)tid;                                                                                            \
+    (void)sNx;                                                                                            \
+    (void)OL;                                                                                             \
+    (void)nx;                                                                                             \
+    (void)tbase;
+
+// Per-block LDS: row sums of two reductions, and values broadcast from thread 0.
+static __shared__ double g_rows[2 * 128];
+static __shared__ double g_bcast[4];
+static __shared__ int g_ibcast;
+
+// Store one row's per-point values of reduction r: tree order -> wave_tree into the row sums; fortran order -> the
+// unit's global staging area (stage[r][jl][il]).
+template <int MODE>
+__device__ __forceinline__ void put_row(int r, int jl, int lane, double v0, double v1, bool has1, double* stage,
+                                        int sNx, int sNy) {
+    if (MODE == MJ_SUM_TREE) {
+        const double w = wave_tree(v0, v1, has1);
+        if (lane == 0) g_rows[r * 128 + jl] = w;
+    } else {
+        double* pr = stage + ((size_t)r * sNy + jl) * sNx;
+        pr[lane] = v0;
+        if (has1) pr[lane + 64] = v1;
+    }
+}
+
+// After a __syncthreads: tile partial of reduction r (valid in lane 0 of the calling wave).
+template <int MODE>
+__device__ __forceinline__ double tile_of(int r, int lane, const double* stage, int sNx, int sNy) {
+    if (MODE == MJ_SUM_TREE) {
+        const double* rb = g_rows + r * 128;
+        const bool h = lane + 64 < sNy;
+        return wave_tree(rb[lane], h ? rb[lane + 64] : 0.0, h);
+    }
+    double s = 0.0;
+    if (lane == 0) {
+        const double* pr = stage + (size_t)r * sNy * sNx;
+        for (int q = 0; q < sNy * sNx; ++q) s = s + pr[q];
+    }
+    return s;
+}
+
+// Reductions 0 and 1 of the unit written to dst0[idx] and dst1[idx] (dst1 NULL: only reduction 0).
+template <int MODE>
+__device__ __forceinline__ void store_tile_sums(double* dst0, double* dst1, int idx, int lane, int wave, int nwaves,
+                                                const double* stage, int sNx, int sNy) {
+    __syncthreads();
+    const int nr = dst1 ? 2 : 1;
+    for (int r = wave; r < nr; r += nwaves) {
+        const double v = tile_of<MODE>(r, lane, stage, sNx, sNy);
+        if (lane == 0) (r == 0 ? dst0 : dst1)[idx] = v;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------ row sweeps
+// The iteration phases sweep rows: each wave owns a strip of consecutive rows and lane l owns the column pair
+// (2l, 2l+1), loaded and stored as one 16-byte access. Down the strip the three rows a 5-point stencil reads are
+// kept in registers (one new row loaded per step); the i-1 / i+1 neighbours come from the adjacent lane, and the
+// halo columns -1 and sNx from one extra load by the first / last lane. Every value is computed with the same
+// expression, in the same term order, as op_A / op_M.
+struct Lanes {
+    int lane, c, last;  // c: this lane's first column (clamped in range for idle lanes), last: lane of column sNx-1
+    bool act;           // lane owns columns
+};
+__device__ __forceinline__ Lanes lanes_of(int lane, int sNx) {
+    Lanes L;
+    L.lane = lane;
+    L.act = 2 * lane < sNx;
+    L.c = L.act ? 2 * lane : sNx - 2;
+    L.last = sNx / 2 - 1;
+    return L;
+}
+template <typename T> struct pair_type;
+template <> struct pair_type<double> { using type = double2; };
+template <> struct pair_type<float> { using type = float2; };
+template <typename T> using P2 = typename pair_type<T>::type;
+
+template <typename T>
+__device__ __forceinline__ P2<T> ld2(const T* p) { return *reinterpret_cast<const P2<T>*>(p); }
+template <typename T>
+__device__ __forceinline__ void st2(T* p, P2<T> v) { *reinterpret_cast<P2<T>*>(p) = v; }
+// Value at the halo column on this lane's open side (lane 0: column -1, last lane: column sNx), row base prow.
+template <typename T>
+__device__ __forceinline__ T halo_of(const T* v, int prow, const Lanes& L, int sNx) {
+    return (L.lane == 0 || L.lane == L.last) ? v[prow + (L.lane == 0 ? -1 : sNx)] : T(0);
+}
+// Left neighbour of column c and right neighbour of column c+1.
+template <typename T>
+__device__ __forceinline__ T left_of(P2<T> v, T h, const Lanes& L) {
+    const T w = __shfl_up(v.y, 1, 64);
+    return L.lane == 0 ? h : w;
+}
+template <typename T>
+__device__ __forceinline__ T right_of(P2<T> v, T h, const Lanes& L) {
+    const T w = __shfl_down(v.x, 1, 64);
+    return L.lane == L.last ? h : w;
+}
+// op_A / op_M at columns c, c+1 of one row: Xm, Xc, Xp rows j-1, j, j+1; xl, xr the outer neighbours.
+template <typename T>
+__device__ __forceinline__ P2<T> pair_A(P2<T> AW, T awr, P2<T> AS0, P2<T> AS1, P2<T> AC, P2<T> Xm, P2<T> Xc,
+                                        P2<T> Xp, T xl, T xr) {
+    P2<T> q;
+    q.x = AW.x * xl + AW.y * Xc.y + AS0.x * Xm.x + AS1.x * Xp.x + AC.x * Xc.x;
+    q.y = AW.y * Xc.x + awr * xr + AS0.y * Xm.y + AS1.y * Xp.y + AC.y * Xc.y;
+    return q;
+}
+template <typename T>
+__device__ __forceinline__ P2<T> pair_M(P2<T> PC, P2<T> PW, T pwr, P2<T> PS0, P2<T> PS1, P2<T> Rm, P2<T> Rc,
+                                        P2<T> Rp, T rl, T rr) {
+    P2<T> q;
+    q.x = PC.x * Rc.x + PW.x * rl + PW.y * Rc.y + PS0.x * Rm.x + PS1.x * Rp.x;
+    q.y = PC.y * Rc.y + PW.y * Rc.x + pwr * rr + PS0.y * Rm.y + PS1.y * Rp.y;
+    return q;
+}
+// One row's per-point values of reduction r held as column pairs: tree order regroups them onto the canonical
+// lanes (lane l: v[l] and v[l + 64]) and reduces with wave_tree into the row sums; fortran order stages them.
+template <int MODE>
+__device__ __forceinline__ void put_row2(int r, int jl, const Lanes& L, double2 v, double* stage, int sNx, int sNy) {
+    if (MODE == MJ_SUM_TREE) {
+        const int l = L.lane, s0 = l >> 1, s1 = 32 + (l >> 1);
+        const double a0 = __shfl(v.x, s0, 64), a1 = __shfl(v.y, s0, 64);
+        const double b0 = __shfl(v.x, s1, 64), b1 = __shfl(v.y, s1, 64);
+        const double lo = (l & 1) ? a1 : a0, hi = (l & 1) ? b1 : b0;
+        const double w = wave_tree(lo, hi, l + 64 < sNx);
+        if (l == 0) g_rows[r * 128 + jl] = w;
+    } else {
+        if (L.act) st2(stage + ((size_t)r * sNy + jl) * sNx + L.c, v);
+    }
+}
+
+// Strip of rows of this wave: [j0, j1).
+#define MJ_STRIP                                                                                         \
+    const int rows_ = (sNy + nwaves - 1) / nwaves, j0 = wave * rows_, j1 = min(j0 + rows_, sNy);        \
+    const Lanes L = lanes_of(lane, sNx);
+#define MJ_ROW(j) (tbase + (OL + (j)) * nx + OL)
+
+// ------------------------------------------------------------------------------------------------ set-up units
+// These take any device-argument struct D with the fields b, x0, x, norm, pmax, ms, T, plane, sNx, sNy, OL, nx and
+// the full T-map tables full_off/full_dst/full_src/full_sign.
+template <class D>
+__device__ __forceinline__ double member_rhs_max(const D& a, int m) {
+    double mx = 0.0;
+    for (int t = 0; t < a.T; ++t) mx = fmax(mx, a.pmax[m * a.T + t]);
+    return mx;
+}
+template <class D>
+__device__ __forceinline__ double member_rhs_norm(const D& a, int m) {
+    const double mx = member_rhs_max(a, m);
+    return mx != 0.0 ? 1.0 / mx : 1.0;
+}
+
+// I0: per-tile max |b * cg2dNorm| (cg2d.F:150-160; the maximum is order free).
+template <class D>
+__device__ __forceinline__ void unit_I0(const D& a, int m, int t) {
+    MJ_UNIT_IDS
+    const double* b = a.b + m * a.ms;
+    const double nm = a.norm[m];
+    double mx = 0.0;
+    MJ_ROWS({
+        mx = fmax(mx, fabs(b[p0] * nm));
+        if (has1) mx = fmax(mx, fabs(b[p1] * nm));
+    })
+    for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) g_rows[wave] = mx;
+    __syncthreads();
+    if (tid == 0) {
+        double v = 0.0;
+        for (int w = 0; w < nwaves; ++w) v = fmax(v, g_rows[w]);
+        a.pmax[m * a.T + t] = v;
+    }
+    __syncthreads();
+}
+
+// Halo points of x written by the exchange of the normalised first guess (cg2d.F:165-172): the pre-exchange value
+// at the source, normalised where the source is interior.
+template <class D>
+__device__ __forceinline__ void exchanged_first_guess(const D& a, int m, int t, double rhsNorm) {
+    const int plane = a.plane, nx = a.nx, OL = a.OL, sNx = a.sNx, sNy = a.sNy;
+    const double* x0 = a.x0 + m * a.ms;
+    double* x = a.x + m * a.ms;
+    for (int k = a.full_off[t] + threadIdx.x; k < a.full_off[t + 1]; k += blockDim.x) {
+        const int s = a.full_src[k];
+        const int sq = s % plane, sj = sq / nx, si = sq - sj * nx;
+        const bool inner = sj >= OL && sj < OL + sNy && si >= OL && si < OL + sNx;
+        const double v = inner ? x0[s] * rhsNorm : x0[s];
+        x[a.full_dst[k]] = a.full_sign[k] < 0 ? -v : v;
+    }
+}
+
+// x = x0 * rhsNorm on the interior, x0 on the halo, then the exchanged halo points; ends with a block barrier.
+template <class D>
+__device__ __forceinline__ void first_guess(const D& a, int m, int t, double rhsNorm) {
+    const int plane = a.plane, nx = a.nx, OL = a.OL, sNx = a.sNx, sNy = a.sNy, tbase = t * plane;
+    const double* x0 = a.x0 + m * a.ms;
+    double* x = a.x + m * a.ms;
+    for (int qq = threadIdx.x; qq < plane; qq += blockDim.x) {
+        const int j = qq / nx, i = qq - j * nx, p = tbase + qq;
+        const bool inner = j >= OL && j < OL + sNy && i >= OL && i < OL + sNx;
+        x[p] = inner ? x0[p] * rhsNorm : x0[p];
+    }
+    __syncthreads();
+    exchanged_first_guess(a, m, t, rhsNorm);
+    __syncthreads();
+}
+
+// Unit of a phase launch: gridDim.x is a multiple of kXcds; block b runs on XCD b % kXcds and takes unit
+// (b % kXcds) * (gridDim.x / kXcds) + b / kXcds, so each XCD works through one contiguous, tile-major range of
+// units and the coefficients of its tiles stay in its L2. Returns false for the padding blocks.
+__device__ __forceinline__ bool phase_unit(int B, int T, int& m, int& t) {
+    const int chunk = gridDim.x / kXcds;
+    const int u = (blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
+    if (u >= B * T) return false;
+    t = u / B;
+    m = u - t * B;
+    return true;
+}
+
+}  // namespace mj

@@ -65,37 +65,57 @@ __device__ __forceinline__ double mask_loc(const GadDev& a, const MjTilePlan& tp
     return want_u ? at(mW, fi, fj) : at(mS, fi, fj);
 }
 
-// FILL_CS_CORNER_TR_RL(fill4dir, withSigns=.FALSE.) on the LDS tile (gad.py:241-260): for each corner present and
-// i, j = 1..OL, dst <- src; no source is a destination, so all reads precede all writes within one call.
-__device__ __forceinline__ void corner_fill(double* loc, const GadDev& a, const MjTilePlan& tp, int dir, int tid) {
+// FILL_CS_CORNER_TR_RL(fill4dir, withSigns=.FALSE.) (gad.py:241-260): the (dst, src) pair of thread tid, for each
+// corner present and i, j = 1..OL (false: no pair). Destinations are distinct, sources are distinct, and no
+// source is a destination.
+__device__ __forceinline__ bool corner_pair(const GadDev& a, const MjTilePlan& tp, int dir, int tid, int& dst,
+                                            int& src) {
     const int OL = a.OLx;  // OLx == OLy checked on the host
-    const int n = 4 * OL * OL;
-    double v = 0.0;
-    int dst = -1;
-    if (tid < n) {
-        const int c = tid / (OL * OL), r = tid - c * OL * OL;
-        const int j = r / OL + 1, i = r % OL + 1;
-        const int sNx = a.sNx, sNy = a.sNy;
-        const bool on = c == 0 ? tp.sw : c == 1 ? tp.se : c == 2 ? tp.nw : tp.ne;
-        if (on) {
-            int di, dj, si, sj;
-            if (dir == 1) {
-                if (c == 0) { di = 1 - i; dj = 1 - j; si = 1 - j; sj = i; }
-                else if (c == 1) { di = sNx + i; dj = 1 - j; si = sNx + j; sj = i; }
-                else if (c == 2) { di = 1 - i; dj = sNy + j; si = 1 - j; sj = sNy + 1 - i; }
-                else { di = sNx + i; dj = sNy + j; si = sNx + j; sj = sNy + 1 - i; }
-            } else {
-                if (c == 0) { di = 1 - i; dj = 1 - j; si = j; sj = 1 - i; }
-                else if (c == 1) { di = sNx + i; dj = 1 - j; si = sNx + 1 - j; sj = 1 - i; }
-                else if (c == 2) { di = 1 - i; dj = sNy + j; si = j; sj = sNy + i; }
-                else { di = sNx + i; dj = sNy + j; si = sNx + 1 - j; sj = sNy + i; }
-            }
-            dst = (dj - 1 + a.OLy) * GNX + (di - 1 + a.OLx);
-            v = loc[(sj - 1 + a.OLy) * GNX + (si - 1 + a.OLx)];
-        }
+    if (tid >= 4 * OL * OL) return false;
+    const int c = tid / (OL * OL), r = tid - c * OL * OL;
+    const int j = r / OL + 1, i = r % OL + 1;
+    const int sNx = a.sNx, sNy = a.sNy;
+    const bool on = c == 0 ? tp.sw : c == 1 ? tp.se : c == 2 ? tp.nw : tp.ne;
+    if (!on) return false;
+    int di, dj, si, sj;
+    if (dir == 1) {
+        if (c == 0) { di = 1 - i; dj = 1 - j; si = 1 - j; sj = i; }
+        else if (c == 1) { di = sNx + i; dj = 1 - j; si = sNx + j; sj = i; }
+        else if (c == 2) { di = 1 - i; dj = sNy + j; si = 1 - j; sj = sNy + 1 - i; }
+        else { di = sNx + i; dj = sNy + j; si = sNx + j; sj = sNy + 1 - i; }
+    } else {
+        if (c == 0) { di = 1 - i; dj = 1 - j; si = j; sj = 1 - i; }
+        else if (c == 1) { di = sNx + i; dj = 1 - j; si = sNx + 1 - j; sj = 1 - i; }
+        else if (c == 2) { di = 1 - i; dj = sNy + j; si = j; sj = sNy + i; }
+        else { di = sNx + i; dj = sNy + j; si = sNx + 1 - j; sj = sNy + i; }
     }
+    dst = (dj - 1 + a.OLy) * GNX + (di - 1 + a.OLx);
+    src = (sj - 1 + a.OLy) * GNX + (si - 1 + a.OLx);
+    return true;
+}
+
+// The fill on a tile (LDS or the block's own global plane): all reads precede all writes.
+__device__ __forceinline__ void corner_fill(double* loc, const GadDev& a, const MjTilePlan& tp, int dir, int tid) {
+    int dst = -1, src = -1;
+    double v = 0.0;
+    const bool on = corner_pair(a, tp, dir, tid, dst, src);
+    if (on) v = loc[src];
     __syncthreads();
-    if (dst >= 0) loc[dst] = v;
+    if (on) loc[dst] = v;
+    __syncthreads();
+}
+
+// Transpose of the fill on a cotangent plane: the destination's cotangent moves to its source.
+__device__ __forceinline__ void corner_fill_T(double* bar, const GadDev& a, const MjTilePlan& tp, int dir, int tid) {
+    int dst = -1, src = -1;
+    double v = 0.0;
+    const bool on = corner_pair(a, tp, dir, tid, dst, src);
+    if (on) v = bar[dst];
+    __syncthreads();
+    if (on) {
+        bar[src] = bar[src] + v;
+        bar[dst] = 0.0;
+    }
     __syncthreads();
 }
 
@@ -203,6 +223,240 @@ __global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
     } else {
         for (int q = tid; q < GPLANE; q += nthr) out[q] = (loc[q] - tr[q]) / dt;  // gad.py:539
     }
+}
+
+// ------------------------------------------------------------------------------------------------ reverse mode
+constexpr int kAdjChunk = 1024;  // blocks per reverse launch (each uses 7 planes of the workspace)
+// Partial derivatives of one DST3 interface value f = A (t_m1 + d0 Rj + d1 Rjm) + Bm (t_0 - d0 Rj - d1 Rjp)
+// (A = (tr + |tr|)/2, Bm = (tr - |tr|)/2, d0 = (2-c)(1-c)/6, d1 = (1-c^2)/6): w[0..3] = df/dt_m2, df/dt_m1,
+// df/dt_0, df/dt_p1 (independent of the tracer), dtr = df/dtr, dc = df/dcfl. The derivative of |x| at 0 is taken
+// as 0 (sign(0) = 0), which is what a central difference across x = 0 gives.
+struct FluxPartials {
+    double w[4], dtr, dc;
+};
+__device__ __forceinline__ FluxPartials dst3_partials(double tr_, double cfl, double oneSixth, double t_m2,
+                                                      double t_m1, double t_0, double t_p1, double m_m1, double m_0,
+                                                      double m_p1) {
+    const double Rjp = (t_p1 - t_0) * m_p1;
+    const double Rj = (t_0 - t_m1) * m_0;
+    const double Rjm = (t_m1 - t_m2) * m_m1;
+    const double d0 = (2. - cfl) * (1. - cfl) * oneSixth;
+    const double d1 = (1. - cfl * cfl) * oneSixth;
+    const double A = 0.5 * (tr_ + fabs(tr_)), Bm = 0.5 * (tr_ - fabs(tr_));
+    const double sg = tr_ > 0.0 ? 1.0 : (tr_ < 0.0 ? -1.0 : 0.0);
+    FluxPartials P;
+    P.w[0] = -A * d1 * m_m1;
+    P.w[1] = A * (1.0 - d0 * m_0 + d1 * m_m1) + Bm * (d0 * m_0);
+    P.w[2] = A * (d0 * m_0) + Bm * (1.0 - d0 * m_0 + d1 * m_p1);
+    P.w[3] = -Bm * d1 * m_p1;
+    const double Pv = t_m1 + (d0 * Rj + d1 * Rjm), Nv = t_0 - (d0 * Rj + d1 * Rjp);
+    P.dtr = 0.5 * (1.0 + sg) * Pv + 0.5 * (1.0 - sg) * Nv;
+    const double d0c = (2.0 * cfl - 3.0) * oneSixth, d1c = -2.0 * cfl * oneSixth;
+    P.dc = A * (d0c * Rj + d1c * Rjm) - Bm * (d0c * Rj + d1c * Rjp);
+    return P;
+}
+
+// Reverse mode of gad_hz_kernel (horizontal DST3 without R flux) for one (member, tile, level): given the
+// cotangent gbar of gTracer, writes the cotangents of the tracer, uFld and vFld planes (every padded point; the
+// grid, hFac and mask inputs are treated as constants). The forward is replayed with a snapshot of the tile
+// before each sweep (work: 7 planes per block, the first holds the running cotangent of localTij); the sweeps
+// are then reversed: the box update (af, tracer and transport terms), the corner fill after the flux, the flux
+// (weights of the four tracer values, transport and CFL terms), the corner fill before it.
+__global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const double* gbar, double* trb, double* ub,
+                                                              double* vb, double* work, int blk0) {
+    __shared__ double loc[GPLANE];
+    __shared__ double fb[GPLANE];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int blk = blk0 + blockIdx.x;
+    const int k = blk % a.K;
+    const int mt = blk / a.K;
+    const int t = mt % a.T, m = mt / a.T;
+    const int64_t b3 = m * a.ms3 + ((int64_t)t * a.K + k) * GPLANE;
+    double* TB = trb + b3;
+    double* UB = ub + b3;
+    double* VB = vb + b3;
+    if (a.nlev && k >= a.nlev[m]) {  // no dependence: zero cotangents
+        for (int q = tid; q < GPLANE; q += nthr) {
+            TB[q] = 0.0;
+            UB[q] = 0.0;
+            VB[q] = 0.0;
+        }
+        return;
+    }
+    const MjTilePlan& tp = a.tiles[t];
+    const int64_t bm = m * a.mms + ((int64_t)t * a.K + k) * GPLANE;
+    const int64_t b2 = (int64_t)t * GPLANE;
+    const double* tr = a.tr + b3;
+    const double* u = a.u + b3;
+    const double* v = a.v + b3;
+    const double* hW = a.hFacW + b3;
+    const double* hS = a.hFacS + b3;
+    const double* rhC = a.recip_hFacC + b3;
+    const double* mW = a.maskW + bm;
+    const double* mS = a.maskS + bm;
+    const double *dyG = a.dyG + b2, *dxG = a.dxG + b2, *rdxC = a.recip_dxC + b2, *rdyC = a.recip_dyC + b2;
+    const double *rrA = a.recip_rA + b2, *mInC = a.maskInC + b2;
+    const double dt = a.dTlev[k];
+    const double deepFacC = a.deepFacC[k], rdeepC = a.recip_deepFacC[k], rdeep2C = a.recip_deepFac2C[k];
+    const double rhoFacC = a.rhoFacC[k], rrhoFacC = a.recip_rhoFacC[k], drF = a.drF[k], rdrF = a.recip_drF[k];
+    const double oneSixth = a.oneSixth;
+    double* Lb = work + (size_t)blockIdx.x * 7 * GPLANE;
+    double* snap = Lb + GPLANE;
+
+    auto uTrans = [&](int q) { return u[q] * (dyG[q] * deepFacC * drF * hW[q]) * rhoFacC; };
+    auto vTrans = [&](int q) { return v[q] * (dxG[q] * deepFacC * drF * hS[q]) * rhoFacC; };
+    auto coef = [&](int q) { return dt * rrhoFacC * rhC[q] * rdrF * rrA[q] * rdeep2C; };
+    auto in_box = [&](const MjSweep& S, int q) {
+        const int j = q / GNX, i = q - j * GNX;
+        for (int bx = 0; bx < S.nbox; ++bx)
+            if (i >= S.box[bx].i0 && i < S.box[bx].i1 && j >= S.box[bx].j0 && j < S.box[bx].j1) return true;
+        return false;
+    };
+    // flux partials at q in direction dir from the tile state in loc; false outside the flux domain
+    auto partials = [&](int dir, int q, FluxPartials& P) {
+        const int j = q / GNX, i = q - j * GNX;
+        if (dir == 0) {
+            if (i < 2 || i > GNX - 2) return false;
+            const double cfl = fabs(u[q] * dt * rdxC[q] * rdeepC);
+            P = dst3_partials(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1],
+                              mask_loc(a, tp, mW, mS, j, i - 1, true), mask_loc(a, tp, mW, mS, j, i, true),
+                              mask_loc(a, tp, mW, mS, j, i + 1, true));
+        } else {
+            if (j < 2 || j > GNY - 2) return false;
+            const double cfl = fabs(v[q] * dt * rdyC[q] * rdeepC);
+            P = dst3_partials(vTrans(q), cfl, oneSixth, loc[q - 2 * GNX], loc[q - GNX], loc[q], loc[q + GNX],
+                              mask_loc(a, tp, mW, mS, j - 1, i, false), mask_loc(a, tp, mW, mS, j, i, false),
+                              mask_loc(a, tp, mW, mS, j + 1, i, false));
+        }
+        return true;
+    };
+
+    // ---- forward replay, snapshot before each sweep
+    int order[6], ns = 0;
+    for (int q = tid; q < GPLANE; q += nthr) loc[q] = tr[q];
+    __syncthreads();
+    for (int pass = 0; pass < 3; ++pass) {
+        for (int dir = 0; dir < 2; ++dir) {
+            const MjSweep& S = tp.sweep[pass][dir];
+            if (!S.run) continue;
+            double* sn = snap + (size_t)ns * GPLANE;
+            for (int q = tid; q < GPLANE; q += nthr) sn[q] = loc[q];
+            order[ns++] = pass * 2 + dir;
+            if (S.fill_before) corner_fill(loc, a, tp, S.fill_before, tid);
+            if (S.nbox) {
+                for (int q = tid; q < GPLANE; q += nthr) {
+                    const int j = q / GNX, i = q - j * GNX;
+                    double f = 0.0;
+                    if (dir == 0) {
+                        if (i >= 2 && i <= GNX - 2) {
+                            const double cfl = fabs(u[q] * dt * rdxC[q] * rdeepC);
+                            f = dst3_flux(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1],
+                                          mask_loc(a, tp, mW, mS, j, i - 1, true), mask_loc(a, tp, mW, mS, j, i, true),
+                                          mask_loc(a, tp, mW, mS, j, i + 1, true));
+                        }
+                    } else {
+                        if (j >= 2 && j <= GNY - 2) {
+                            const double cfl = fabs(v[q] * dt * rdyC[q] * rdeepC);
+                            f = dst3_flux(vTrans(q), cfl, oneSixth, loc[q - 2 * GNX], loc[q - GNX], loc[q],
+                                          loc[q + GNX], mask_loc(a, tp, mW, mS, j - 1, i, false),
+                                          mask_loc(a, tp, mW, mS, j, i, false), mask_loc(a, tp, mW, mS, j + 1, i, false));
+                        }
+                    }
+                    fb[q] = f;
+                }
+                __syncthreads();
+            }
+            if (S.fill_after) corner_fill(loc, a, tp, S.fill_after, tid);
+            for (int bx = 0; bx < S.nbox; ++bx) {
+                const MjBox Bx = S.box[bx];
+                const int bw = Bx.i1 - Bx.i0, n = bw * (Bx.j1 - Bx.j0);
+                for (int idx = tid; idx < n; idx += nthr) {
+                    const int q = (Bx.j0 + idx / bw) * GNX + Bx.i0 + idx % bw;
+                    double daf, dtr;
+                    if (dir == 0) {
+                        daf = fb[q + 1] - fb[q];
+                        dtr = uTrans(q + 1) - uTrans(q);
+                    } else {
+                        daf = fb[q + GNX] - fb[q];
+                        dtr = vTrans(q + GNX) - vTrans(q);
+                    }
+                    loc[q] = loc[q] - coef(q) * (daf - tr[q] * dtr) * mInC[q];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- cotangent of the output (localTij - tracer) / dt
+    for (int q = tid; q < GPLANE; q += nthr) {
+        const double g = gbar[b3 + q];
+        Lb[q] = g / dt;
+        TB[q] = -(g / dt);
+        UB[q] = 0.0;
+        VB[q] = 0.0;
+    }
+    __syncthreads();
+
+    // ---- sweeps in reverse
+    for (int s = ns - 1; s >= 0; --s) {
+        const int pass = order[s] >> 1, dir = order[s] & 1;
+        const MjSweep& S = tp.sweep[pass][dir];
+        const int d = dir == 0 ? 1 : GNX;
+        double* VBd = dir == 0 ? UB : VB;
+        const double* vel = dir == 0 ? u : v;
+        const double* rdC = dir == 0 ? rdxC : rdyC;
+        const double* len = dir == 0 ? dyG : dxG;
+        const double* hF = dir == 0 ? hW : hS;
+        const double* sn = snap + (size_t)s * GPLANE;
+        for (int q = tid; q < GPLANE; q += nthr) loc[q] = sn[q];
+        __syncthreads();
+        if (S.fill_before) corner_fill(loc, a, tp, S.fill_before, tid);
+        auto trans = [&](int q) { return dir == 0 ? uTrans(q) : vTrans(q); };
+        // box update: loc[q] -= c_q (af[q+d] - af[q] - tr[q] (T[q+d] - T[q])), c_q = coef(q) maskInC(q)
+        for (int p = tid; p < GPLANE; p += nthr) {
+            double f = 0.0, tb = 0.0;
+            if (S.nbox) {
+                const bool inp = in_box(S, p), inm = p - d >= 0 && in_box(S, p - d);
+                if (inp) {
+                    const double cg = coef(p) * mInC[p] * Lb[p];
+                    f = f + cg;
+                    tb = tb - cg * tr[p];
+                    TB[p] = TB[p] + cg * (trans(p + d) - trans(p));
+                }
+                if (inm) {
+                    const double cg = coef(p - d) * mInC[p - d] * Lb[p - d];
+                    f = f - cg;
+                    tb = tb + cg * tr[p - d];
+                }
+            }
+            fb[p] = f;
+            // transport cotangent -> velocity: T = vel (len deepFacC drF hFac) rhoFacC
+            if (tb != 0.0) VBd[p] = VBd[p] + tb * ((len[p] * deepFacC * drF * hF[p]) * rhoFacC);
+        }
+        __syncthreads();
+        if (S.fill_after) corner_fill_T(Lb, a, tp, S.fill_after, tid);
+        // flux: Lb[p] += sum_k w_k(q) fb[q] over the fluxes q = p + 2d, p + d, p, p - d that read p
+        for (int p = tid; p < GPLANE; p += nthr) {
+            double acc = 0.0;
+            FluxPartials P;
+            for (int kk = 0; kk < 4; ++kk) {
+                const int q = p + (2 - kk) * d;  // loc index p is t_m2 (kk=0) .. t_p1 (kk=3) of flux q
+                if (q < 0 || q >= GPLANE || fb[q] == 0.0) continue;
+                if (partials(dir, q, P)) acc = acc + P.w[kk] * fb[q];
+            }
+            Lb[p] = Lb[p] + acc;
+            if (fb[p] != 0.0 && partials(dir, p, P)) {
+                const double g = fb[p];
+                const double dT = (len[p] * deepFacC * drF * hF[p]) * rhoFacC;
+                const double arg = vel[p] * dt * rdC[p] * rdeepC;
+                const double sgn = arg > 0.0 ? 1.0 : (arg < 0.0 ? -1.0 : 0.0);
+                VBd[p] = VBd[p] + (P.dtr * g) * dT + (P.dc * g) * (sgn * dt * rdC[p] * rdeepC);
+            }
+        }
+        __syncthreads();
+        if (S.fill_before) corner_fill_T(Lb, a, tp, S.fill_before, tid);
+    }
+    for (int q = tid; q < GPLANE; q += nthr) TB[q] = TB[q] + Lb[q];
 }
 
 // Vertical DST3 (R) sweep after the horizontal passes, per column (member, tile, j, i), k = 0..nlev-1 downward.
@@ -418,25 +672,11 @@ static int describe(const mj_gad_plan* P, uint8_t* masks, int32_t* fills) {
     return MJ_OK;
 }
 
-extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c, const double* uFld,
-                           const double* vFld, const double* wFld, const double* tracer, const double* hFacW,
-                           const double* hFacS, const double* recip_hFacC, double* gTracer, double* work,
-                           void* stream) {
-    MJ_REQUIRE(P && g && c, "mj_gad_dst3: null plan/grid/config");
-    MJ_REQUIRE(uFld && vFld && tracer && hFacW && hFacS && recip_hFacC && gTracer, "mj_gad_dst3: null field");
-    MJ_REQUIRE(g->dyG && g->dxG && g->recip_dxC && g->recip_dyC && g->recip_rA && g->maskInC && g->maskW && g->maskS,
-               "mj_gad_dst3: null grid field");
-    MJ_REQUIRE(g->dTlev && g->drF && g->recip_drF && g->rhoFacC && g->recip_rhoFacC && g->deepFacC &&
-                   g->recip_deepFacC && g->recip_deepFac2C,
-               "mj_gad_dst3: null vertical grid field");
-    MJ_REQUIRE(c->batch >= 1 && c->nr >= 1, "mj_gad_dst3: batch and nr must be >= 1");
-    MJ_REQUIRE(c->member_stride3d >= (int64_t)P->ntiles * c->nr * GPLANE || c->batch == 1,
-               "mj_gad_dst3: member_stride3d too small");
-    MJ_REQUIRE((int64_t)c->batch * P->ntiles * c->nr < (1ll << 31), "mj_gad_dst3: too many (member, tile, level)s");
-    if (wFld) {
-        MJ_REQUIRE(work && g->maskC && g->rA && g->recip_drC, "mj_gad_dst3: the R flux needs work, maskC, rA, recip_drC");
-        MJ_REQUIRE(work != gTracer, "mj_gad_dst3: work must not alias gTracer");
-    }
+namespace {
+// Device arguments of the DST3 kernels shared by the forward and reverse entry points.
+GadDev gad_dev(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c, const double* uFld,
+               const double* vFld, const double* wFld, const double* tracer, const double* hFacW, const double* hFacS,
+               const double* recip_hFacC) {
     GadDev a{};
     a.B = c->batch;
     a.T = P->ntiles;
@@ -475,8 +715,32 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
     a.hFacW = hFacW;
     a.hFacS = hFacS;
     a.recip_hFacC = recip_hFacC;
-    a.out = wFld ? work : gTracer;
     a.nlev = c->nlev;
+    return a;
+}
+}  // namespace
+
+extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c, const double* uFld,
+                           const double* vFld, const double* wFld, const double* tracer, const double* hFacW,
+                           const double* hFacS, const double* recip_hFacC, double* gTracer, double* work,
+                           void* stream) {
+    MJ_REQUIRE(P && g && c, "mj_gad_dst3: null plan/grid/config");
+    MJ_REQUIRE(uFld && vFld && tracer && hFacW && hFacS && recip_hFacC && gTracer, "mj_gad_dst3: null field");
+    MJ_REQUIRE(g->dyG && g->dxG && g->recip_dxC && g->recip_dyC && g->recip_rA && g->maskInC && g->maskW && g->maskS,
+               "mj_gad_dst3: null grid field");
+    MJ_REQUIRE(g->dTlev && g->drF && g->recip_drF && g->rhoFacC && g->recip_rhoFacC && g->deepFacC &&
+                   g->recip_deepFacC && g->recip_deepFac2C,
+               "mj_gad_dst3: null vertical grid field");
+    MJ_REQUIRE(c->batch >= 1 && c->nr >= 1, "mj_gad_dst3: batch and nr must be >= 1");
+    MJ_REQUIRE(c->member_stride3d >= (int64_t)P->ntiles * c->nr * GPLANE || c->batch == 1,
+               "mj_gad_dst3: member_stride3d too small");
+    MJ_REQUIRE((int64_t)c->batch * P->ntiles * c->nr < (1ll << 31), "mj_gad_dst3: too many (member, tile, level)s");
+    if (wFld) {
+        MJ_REQUIRE(work && g->maskC && g->rA && g->recip_drC, "mj_gad_dst3: the R flux needs work, maskC, rA, recip_drC");
+        MJ_REQUIRE(work != gTracer, "mj_gad_dst3: work must not alias gTracer");
+    }
+    GadDev a = gad_dev(P, g, c, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC);
+    a.out = wFld ? work : gTracer;
     hipStream_t st = as_stream(stream);
     const unsigned nblk = (unsigned)(c->batch * P->ntiles * c->nr);
     hipLaunchKernelGGL(gad_hz_kernel, dim3(nblk), dim3(1024), 0, st, a);
@@ -485,6 +749,37 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
         a.out = gTracer;
         hipLaunchKernelGGL(gad_vert_kernel, dim3((GPLANE + 255) / 256, (unsigned)(c->batch * P->ntiles)), dim3(256), 0,
                            st, a, (const double*)work);
+        MJ_HIP(hipGetLastError());
+    }
+    return MJ_OK;
+}
+
+extern "C" size_t mj_gad_dst3_adjoint_workspace_bytes(void) { return (size_t)kAdjChunk * 7 * GPLANE * sizeof(double); }
+
+extern "C" int mj_gad_dst3_adjoint(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c,
+                                   const double* uFld, const double* vFld, const double* tracer, const double* hFacW,
+                                   const double* hFacS, const double* recip_hFacC, const double* gTracer_bar,
+                                   double* tracer_bar, double* uFld_bar, double* vFld_bar, double* work,
+                                   size_t work_bytes, void* stream) {
+    MJ_REQUIRE(P && g && c, "mj_gad_dst3_adjoint: null plan/grid/config");
+    MJ_REQUIRE(uFld && vFld && tracer && hFacW && hFacS && recip_hFacC && gTracer_bar && tracer_bar && uFld_bar &&
+                   vFld_bar && work,
+               "mj_gad_dst3_adjoint: null field");
+    MJ_REQUIRE(g->dyG && g->dxG && g->recip_dxC && g->recip_dyC && g->recip_rA && g->maskInC && g->maskW && g->maskS,
+               "mj_gad_dst3_adjoint: null grid field");
+    MJ_REQUIRE(c->batch >= 1 && c->nr >= 1, "mj_gad_dst3_adjoint: batch and nr must be >= 1");
+    MJ_REQUIRE(c->member_stride3d >= (int64_t)P->ntiles * c->nr * GPLANE || c->batch == 1,
+               "mj_gad_dst3_adjoint: member_stride3d too small");
+    MJ_REQUIRE((int64_t)c->batch * P->ntiles * c->nr < (1ll << 31), "mj_gad_dst3_adjoint: too many blocks");
+    MJ_REQUIRE(work_bytes >= mj_gad_dst3_adjoint_workspace_bytes(), "mj_gad_dst3_adjoint: work %zu < %zu bytes",
+               work_bytes, mj_gad_dst3_adjoint_workspace_bytes());
+    GadDev a = gad_dev(P, g, c, uFld, vFld, nullptr, tracer, hFacW, hFacS, recip_hFacC);
+    hipStream_t st = as_stream(stream);
+    const int nblk = c->batch * P->ntiles * c->nr;
+    for (int b0 = 0; b0 < nblk; b0 += kAdjChunk) {
+        const int n = std::min(kAdjChunk, nblk - b0);
+        hipLaunchKernelGGL(gad_hz_adjoint_kernel, dim3((unsigned)n), dim3(1024), 0, st, a, gTracer_bar, tracer_bar,
+                           uFld_bar, vFld_bar, work, b0);
         MJ_HIP(hipGetLastError());
     }
     return MJ_OK;

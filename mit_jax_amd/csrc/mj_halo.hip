@@ -58,6 +58,33 @@ __global__ void __launch_bounds__(256) halo_gather_kernel(const int32_t* __restr
     out[base + dt * tile_stride + dr] = sign[p] < 0 ? -v : v;
 }
 
+// Transpose of the exchange (reverse mode): the cotangent of input component c at point q is its own cotangent
+// where no map writes q, plus sign * (cotangent of each written point that reads q), summed in table order.
+__global__ void __launch_bounds__(256) halo_transpose_kernel(const int32_t* __restrict__ off,
+                                                             const int32_t* __restrict__ tdst,
+                                                             const int8_t* __restrict__ tmap,
+                                                             const int8_t* __restrict__ tsign,
+                                                             const int8_t* __restrict__ written, const double* own,
+                                                             const double* bar0, const double* bar1, double* out,
+                                                             int total, int plane,
+                                                             int64_t tile_stride, int64_t ninner, int64_t inner_stride,
+                                                             int64_t outer_stride) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    const int64_t lead = blockIdx.y;
+    const int64_t n = lead % ninner;
+    const int64_t o = lead / ninner;
+    const int64_t base = o * outer_stride + n * inner_stride;
+    auto at = [&](int p) { return base + (int64_t)(p / plane) * tile_stride + p % plane; };
+    double acc = written[q] ? 0.0 : own[at(q)];
+    for (int k = off[q]; k < off[q + 1]; ++k) {
+        const double* from = tmap[k] == 1 ? bar1 : bar0;  // cotangent of the output of the map the entry is in
+        const double v = from[at(tdst[k])];
+        acc = acc + (tsign[k] < 0 ? -v : v);
+    }
+    out[at(q)] = acc;
+}
+
 }  // namespace mj
 
 using namespace mj;
@@ -153,6 +180,45 @@ mj_halo_plan::~mj_halo_plan() {
     for (void* q : {(void*)cg_full_off, (void*)cg_full_dst, (void*)cg_full_src, (void*)cg_full_sign,
                     (void*)cg_ring_off, (void*)cg_ring_dst, (void*)cg_ring_src, (void*)cg_ring_sign})
         if (q) (void)hipFree(q);
+    for (int c = 0; c < 2; ++c)
+        for (void* q : {(void*)tr_off[c], (void*)tr_dst[c], (void*)tr_map[c], (void*)tr_sign[c], (void*)tr_written[c]})
+            if (q) (void)hipFree(q);
+}
+
+static int ensure_transpose(mj_halo_plan* P) {
+    std::lock_guard<std::mutex> lk(P->tr_mu);
+    if (P->tr_off[0]) return MJ_OK;
+    const int total = P->ntiles * P->ny * P->nx;
+    for (int c = 0; c < P->nmaps; ++c) {
+        std::vector<int32_t> cnt(total + 1, 0), dst;
+        std::vector<int8_t> map, sg;
+        for (int m = 0; m < P->nmaps; ++m)
+            for (int k = 0; k < P->npts[m]; ++k)
+                if (P->h_comp[m][k] - 1 == c) ++cnt[P->h_src[m][k] + 1];
+        for (int q = 0; q < total; ++q) cnt[q + 1] += cnt[q];
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        dst.assign(cnt[total], 0);
+        map.assign(cnt[total], 0);
+        sg.assign(cnt[total], 0);
+        for (int m = 0; m < P->nmaps; ++m)
+            for (int k = 0; k < P->npts[m]; ++k)
+                if (P->h_comp[m][k] - 1 == c) {
+                    const int at = fill[P->h_src[m][k]]++;
+                    dst[at] = P->h_dst[m][k];
+                    map[at] = (int8_t)m;
+                    sg[at] = P->h_sign[m][k];
+                }
+        std::vector<int8_t> wr(total, 0);
+        for (int k = 0; k < P->npts[c]; ++k) wr[P->h_dst[c][k]] = 1;
+        int rc;
+        if ((rc = upload(cnt.data(), cnt.size() * 4, (void**)&P->tr_off[c])) ||
+            (rc = upload(dst.data(), dst.size() * 4, (void**)&P->tr_dst[c])) ||
+            (rc = upload(map.data(), map.size(), (void**)&P->tr_map[c])) ||
+            (rc = upload(sg.data(), sg.size(), (void**)&P->tr_sign[c])) ||
+            (rc = upload(wr.data(), wr.size(), (void**)&P->tr_written[c])))
+            return rc;
+    }
+    return MJ_OK;
 }
 
 extern "C" int mj_halo_plan_destroy(mj_halo_plan* plan) {
@@ -205,4 +271,29 @@ extern "C" int mj_halo_exchange_uv(const mj_halo_plan* plan, const double* u_in,
     hipStream_t st = as_stream(stream);
     if ((rc = launch_map(plan, 0, u_in, v_in, u_out, g, st))) return rc;
     return launch_map(plan, 1, u_in, v_in, v_out, g, st);
+}
+
+extern "C" int mj_halo_exchange_adjoint(const mj_halo_plan* plan, const double* ubar_out, const double* vbar_out,
+                                        double* ubar_in, double* vbar_in, const mj_halo_geom* g, void* stream) {
+    int rc = check_geom(plan, g);
+    if (rc) return rc;
+    const bool uv = plan->nmaps == 2;
+    MJ_REQUIRE(ubar_out && ubar_in && (!uv || (vbar_out && vbar_in)), "mj_halo_exchange_adjoint: null array");
+    MJ_REQUIRE(ubar_in != ubar_out && (!uv || (vbar_in != vbar_out && vbar_in != ubar_out && ubar_in != vbar_out)),
+               "mj_halo_exchange_adjoint: outputs must not alias inputs");
+    mj_halo_plan* P = const_cast<mj_halo_plan*>(plan);
+    if ((rc = ensure_transpose(P))) return rc;
+    const int total = P->ntiles * P->ny * P->nx;
+    hipStream_t st = as_stream(stream);
+    dim3 grid((total + 255) / 256, (unsigned)(g->nouter * g->ninner));
+    // map m writes output component m (its own cotangent is that output's); entries of map m read its cotangent
+    const double* bar1 = uv ? vbar_out : ubar_out;
+    for (int c = 0; c < P->nmaps; ++c) {
+        hipLaunchKernelGGL(halo_transpose_kernel, grid, dim3(256), 0, st, P->tr_off[c], P->tr_dst[c], P->tr_map[c],
+                           P->tr_sign[c], P->tr_written[c], c == 0 ? ubar_out : vbar_out, ubar_out, bar1,
+                           c == 0 ? ubar_in : vbar_in, total, P->ny * P->nx, g->tile_stride, g->ninner,
+                           g->inner_stride, g->outer_stride);
+        MJ_HIP(hipGetLastError());
+    }
+    return MJ_OK;
 }

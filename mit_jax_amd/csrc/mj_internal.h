@@ -28,6 +28,14 @@ struct mj_halo_plan {
     int8_t* cg_ring_sign = nullptr;
     int cg_ring_max = 0;  // largest per-tile ring count
 
+    // Transpose tables (built once, on first use): per input component c (0 = u-like, 1 = v-like) and padded
+    // point q, the written points that read q (tr_off[c][q] .. tr_off[c][q+1] into tr_dst/tr_map/tr_sign, sorted by
+    // map then destination), and per map whether q is written (tr_written[m][q]).
+    std::mutex tr_mu;
+    int32_t *tr_off[2] = {nullptr, nullptr}, *tr_dst[2] = {nullptr, nullptr};
+    int8_t *tr_map[2] = {nullptr, nullptr}, *tr_sign[2] = {nullptr, nullptr};
+    int8_t* tr_written[2] = {nullptr, nullptr};
+
     ~mj_halo_plan();
 };
 
@@ -49,6 +57,12 @@ struct MjTilePlan {
     MjSweep sweep[3][2];  // [pass][dir: 0 = X, 1 = Y]
     int8_t sw, se, nw, ne;
 };
+
+namespace mj {
+// Builds (once per plan, for one (sNx, sNy, OL)) the per-tile full and ring tables of a scalar T plan that the CG2D
+// solves use; the ring points must all have interior sources.
+int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL);
+}  // namespace mj
 
 struct mj_gad_plan {
     int ntiles = 0, sNx = 0, sNy = 0, OLx = 0, OLy = 0;

@@ -1,0 +1,647 @@
+// Mixed-precision CG2D on gfx950: FP32 inner PCG with FP64 iterative refinement, batched over members.
+//
+// The system is the one cg2d_fortran solves (reference core/cg2d.py:205-269): the same normalisation (b * cg2dNorm,
+// rhsNorm = 1/max|b|, normalised first guess with exchanged halos), the same 5-point operator and preconditioner,
+// the same stopping rule err < tol^2 on the normalised residual. Per member:
+//   r = bn - A x                                     (FP64: true residual, FP64 operator)
+//   err = sum r^2                                    (FP64); stop when err < tol^2
+//   sigma = sqrt(err); r32 = r / sigma               (inner right-hand side of unit norm)
+//   PCG on A32 d = r32 from d = 0 in FP32 with FP32 operator/preconditioner copies; every dot product is formed
+//   and summed in FP64 (the fixed MJ_SUM_TREE order) and alpha/beta are FP64 ratios; the inner solve stops when
+//   its recursive residual falls below inner_tolerance (or after inner_max_iters)
+//   x = x + sigma * d                                (FP64 solution update)
+// and repeat. max_iters caps the total inner iterations and max_outer the refinements; a member that reaches a
+// cap finishes with converged = 0. Un-normalisation and the halos of x are as cg2d_fortran leaves them.
+//
+// Work decomposition as the FP64 solve: one block per (member, tile), row sweeps over column pairs, rings computed
+// from their interior sources, per-tile partials summed in tile order. Members move through their own state
+// machine, one state per launch ("step"): every block of a member derives the member's action for the step from
+// the member's record of the previous step and the partials, so all tiles of a member act alike; tile 0 writes
+// the record of the step (records are double-buffered by step parity). States:
+//   RST   err from the true-residual partials; finish, or restart the inner solve (r32, d = 0, eta(1))
+//   IB    inner phase B(n): inner stop -> x += sigma d (next: TRUE); else s = z + beta s, sigma_n = s.A32 s
+//   IC    inner phase C(n): d += alpha s; r32 -= alpha A32 s; err(n), eta(n+1)
+//   TRUE  x's ring from its sources; r = bn - A x; err partials
+#include <algorithm>
+
+#include "mj_cg_sweep.h"
+#include "mj_common.h"
+#include "mj_internal.h"
+
+namespace mj {
+
+struct MxRec {
+    int mode, n, iters, outer;  // state, inner iteration, inner iterations done, refinements started
+    double sigma;               // norm of the residual the current inner solve started from
+};
+enum { MX_RST = 0, MX_IB = 1, MX_IC = 2, MX_TRUE = 3 };
+enum { AX_FINISH = 1, AX_RESTART, AX_UPD, AX_IB, AX_IC, AX_TRUE };
+
+struct MxDev {
+    int B, T, sNx, sNy, OL, nx, ny, plane;
+    int64_t ms, oms, cms;  // member stride; FP64 operator slot stride; FP32 coefficient slot stride
+    const double *aW, *aS, *aC;
+    const float *aW32, *aS32, *aC32, *pW32, *pS32, *pC32;
+    const double *b, *x0;
+    double* x;
+    const double* norm;
+    double tol_sq, itol_sq;
+    int max_iters, inner_max, max_outer;
+    const int32_t *full_off, *full_dst, *full_src;
+    const int8_t* full_sign;
+    const int32_t *ring_off, *ring_dst, *ring_src;
+    const int8_t* ring_sign;
+    double* r64;
+    float *r0, *r1, *s0, *s1, *d;
+    double *pmax, *psum, *perr64, *perr, *peta, *psig;  // [B*T] x3, [2][B*T] x3
+    const int32_t* op_index;
+    MxRec* rec;  // [2][B]
+    int* nactive;
+    int32_t* mdone;  // [B] step at which a member finished, -1 while active
+    mj_cg2d_diag* diag;
+};
+
+#ifndef MJ_MX_THREADS
+#define MJ_MX_THREADS 256
+#endif
+
+__device__ __forceinline__ int slot_of(const MxDev& a, int m) { return a.op_index ? a.op_index[m] : m; }
+__device__ __forceinline__ float* R32(const MxDev& a, int n) { return n & 1 ? a.r1 : a.r0; }
+__device__ __forceinline__ float* S32(const MxDev& a, int n) { return n & 1 ? a.s1 : a.s0; }
+
+// I1: x = normalised first guess with exchanged halos; r = bn - A x (FP64); partials of err and sumRHS; the
+// member's record starts in RST.
+__device__ __forceinline__ void mx_I1(const MxDev& a, int m, int t) {
+    MJ_UNIT_IDS
+    const double rhsNorm = member_rhs_norm(a, m);
+    const double nm = a.norm[m];
+    const double* b = a.b + m * a.ms;
+    double* x = a.x + m * a.ms;
+    double* r = a.r64 + m * a.ms;
+    const int64_t ob = (int64_t)slot_of(a, m) * a.oms;
+    first_guess(a, m, t, rhsNorm);
+    MJ_ROWS({
+        const double bn0 = b[p0] * nm * rhsNorm;
+        const double r_0 = bn0 - op_A(a.aW + ob, a.aS + ob, a.aC + ob, x, p0, nx);
+        r[p0] = r_0;
+        double bn1 = 0.0, r_1 = 0.0;
+        if (has1) {
+            bn1 = b[p1] * nm * rhsNorm;
+            r_1 = bn1 - op_A(a.aW + ob, a.aS + ob, a.aC + ob, x, p1, nx);
+            r[p1] = r_1;
+        }
+        put_row<MJ_SUM_TREE>(0, jl, lane, r_0 * r_0, r_1 * r_1, has1, nullptr, sNx, sNy);
+        put_row<MJ_SUM_TREE>(1, jl, lane, bn0, bn1, has1, nullptr, sNx, sNy);
+    })
+    store_tile_sums<MJ_SUM_TREE>(a.perr64, a.psum, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
+    if (t == 0 && tid == 0) {
+        a.rec[m] = MxRec{MX_RST, 0, 0, 0, 0.0};
+        a.mdone[m] = -1;
+    }
+}
+
+// Finish: x /= rhsNorm on the interior; the halo points the true-residual steps used for x's ring get back the
+// exchanged normalised first guess.
+__device__ __forceinline__ void mx_finish(const MxDev& a, int m, int t) {
+    MJ_UNIT_IDS
+    const double rhsNorm = member_rhs_norm(a, m);
+    double* x = a.x + m * a.ms;
+    MJ_ROWS({
+        x[p0] = x[p0] / rhsNorm;
+        if (has1) x[p1] = x[p1] / rhsNorm;
+    })
+    exchanged_first_guess(a, m, t, rhsNorm);
+}
+
+// Restart: r32 = r / sigma on interior and ring (ring from the sources' r), d = 0, partial of eta(1).
+__device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double sigma) {
+    MJ_UNIT_IDS
+    MJ_STRIP
+    const double* r = a.r64 + m * a.ms;
+    float* r32 = R32(a, 0) + m * a.ms;
+    float* d = a.d + m * a.ms;
+    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
+    const float *__restrict__ pW = a.pW32 + oc, *__restrict__ pS = a.pS32 + oc, *__restrict__ pC = a.pC32 + oc;
+    MJ_ROWS({
+        r32[p0] = (float)(r[p0] / sigma);
+        d[p0] = 0.0f;
+        if (has1) {
+            r32[p1] = (float)(r[p1] / sigma);
+            d[p1] = 0.0f;
+        }
+    })
+    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const float v = (float)(r[a.ring_src[k]] / sigma);
+        r32[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        float2 Rm = ld2(r32 + p - nx), Rc = ld2(r32 + p), PS0 = ld2(pS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const float2 Rp = ld2(r32 + p + nx), PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
+            const float hr = halo_of(r32, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
+            const float2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
+                                    right_of(Rc, hr, L));
+            double2 h = make_double2((double)q.x * (double)Rc.x, (double)q.y * (double)Rc.y);
+            if (!L.act) h = make_double2(0.0, 0.0);
+            put_row2<MJ_SUM_TREE>(0, j, L, h, nullptr, sNx, sNy);
+            Rm = Rc;
+            Rc = Rp;
+            PS0 = PS1;
+        }
+    }
+    store_tile_sums<MJ_SUM_TREE>(a.peta + a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
+}
+
+// Solution update of a finished inner solve: x = x + sigma * d on the interior (FP64).
+__device__ __forceinline__ void mx_update(const MxDev& a, int m, int t, double sigma) {
+    MJ_UNIT_IDS
+    MJ_STRIP
+    double* x = a.x + m * a.ms;
+    const float* d = a.d + m * a.ms;
+    for (int j = j0; j < j1; ++j) {
+        const int p = MJ_ROW(j) + L.c;
+        const double2 X = ld2(x + p);
+        const float2 D = ld2(d + p);
+        if (L.act) st2(x + p, make_double2(X.x + sigma * (double)D.x, X.y + sigma * (double)D.y));
+    }
+}
+
+// True residual: x's ring from its sources (the updated interior of other tiles), r = bn - A x (FP64), partials
+// of err.
+__device__ __forceinline__ void mx_true(const MxDev& a, int m, int t) {
+    MJ_UNIT_IDS
+    MJ_STRIP
+    const double rhsNorm = member_rhs_norm(a, m);
+    const double nm = a.norm[m];
+    const double* b = a.b + m * a.ms;
+    double* x = a.x + m * a.ms;
+    double* r = a.r64 + m * a.ms;
+    const int64_t ob = (int64_t)slot_of(a, m) * a.oms;
+    const double *__restrict__ aW = a.aW + ob, *__restrict__ aS = a.aS + ob, *__restrict__ aC = a.aC + ob;
+    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const double v = x[a.ring_src[k]];
+        x[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        double2 Xm = ld2(x + p - nx), Xc = ld2(x + p), AS0 = ld2(aS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const double2 Xp = ld2(x + p + nx), AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
+            const double2 Bv = ld2(b + p);
+            const double hx = halo_of(x, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
+            const double2 q = pair_A(AW, right_of(AW, hw, L), AS0, AS1, AC, Xm, Xc, Xp, left_of(Xc, hx, L),
+                                     right_of(Xc, hx, L));
+            double2 rv;
+            rv.x = Bv.x * nm * rhsNorm - q.x;
+            rv.y = Bv.y * nm * rhsNorm - q.y;
+            if (L.act) st2(r + p, rv);
+            double2 e = make_double2(rv.x * rv.x, rv.y * rv.y);
+            if (!L.act) e = make_double2(0.0, 0.0);
+            put_row2<MJ_SUM_TREE>(0, j, L, e, nullptr, sNx, sNy);
+            Xm = Xc;
+            Xc = Xp;
+            AS0 = AS1;
+        }
+    }
+    store_tile_sums<MJ_SUM_TREE>(a.perr64, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
+}
+
+// Inner phase B(n): s_n = z + beta s_{n-1} (z = M32 r32_{n-1}; s_1 = z) on interior and ring, sigma_n partial.
+__device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64) {
+    MJ_UNIT_IDS
+    MJ_STRIP
+    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
+    const float *__restrict__ pW = a.pW32 + oc, *__restrict__ pS = a.pS32 + oc, *__restrict__ pC = a.pC32 + oc;
+    const float *__restrict__ aW = a.aW32 + oc, *__restrict__ aS = a.aS32 + oc, *__restrict__ aC = a.aC32 + oc;
+    const float* __restrict__ r = R32(a, n - 1) + m * a.ms;
+    const float* __restrict__ so = S32(a, n - 1) + m * a.ms;
+    float* sn = S32(a, n) + m * a.ms;
+    const float beta = (float)beta64;
+    const bool first = n == 1;
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        float2 Rm = ld2(r + p - nx), Rc = ld2(r + p), PS0 = ld2(pS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const float2 Rp = ld2(r + p + nx), PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
+            const float hr = halo_of(r, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
+            const float2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
+                                    right_of(Rc, hr, L));
+            float2 v = q;
+            if (!first) {
+                const float2 SO = ld2(so + p);
+                v.x = q.x + beta * SO.x;
+                v.y = q.y + beta * SO.y;
+            }
+            if (L.act) st2(sn + p, v);
+            Rm = Rc;
+            Rc = Rp;
+            PS0 = PS1;
+        }
+    }
+    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const int s = a.ring_src[k];
+        const float q = op_M(pW, pS, pC, r, s, nx);
+        const float v = first ? q : q + beta * so[s];
+        sn[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        float2 Xm = ld2(sn + p - nx), Xc = ld2(sn + p), AS0 = ld2(aS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const float2 Xp = ld2(sn + p + nx), AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
+            const float hx = halo_of(sn, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
+            const float2 q = pair_A(AW, right_of(AW, hw, L), AS0, AS1, AC, Xm, Xc, Xp, left_of(Xc, hx, L),
+                                    right_of(Xc, hx, L));
+            double2 v = make_double2((double)Xc.x * (double)q.x, (double)Xc.y * (double)q.y);
+            if (!L.act) v = make_double2(0.0, 0.0);
+            put_row2<MJ_SUM_TREE>(0, j, L, v, nullptr, sNx, sNy);
+            Xm = Xc;
+            Xc = Xp;
+            AS0 = AS1;
+        }
+    }
+    store_tile_sums<MJ_SUM_TREE>(a.psig + (n & 1) * a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx,
+                                 sNy);
+}
+
+// Inner phase C(n): d += alpha s_n; r32_n = r32_{n-1} - alpha A32 s_n on interior and ring; err(n), eta(n+1).
+__device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, double alpha64) {
+    MJ_UNIT_IDS
+    MJ_STRIP
+    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
+    const float *__restrict__ pW = a.pW32 + oc, *__restrict__ pS = a.pS32 + oc, *__restrict__ pC = a.pC32 + oc;
+    const float *__restrict__ aW = a.aW32 + oc, *__restrict__ aS = a.aS32 + oc, *__restrict__ aC = a.aC32 + oc;
+    const float* __restrict__ sv = S32(a, n) + m * a.ms;
+    const float* __restrict__ rin = R32(a, n - 1) + m * a.ms;
+    float* rout = R32(a, n) + m * a.ms;
+    float* d = a.d + m * a.ms;
+    const float alpha = (float)alpha64;
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        float2 Xm = ld2(sv + p - nx), Xc = ld2(sv + p), AS0 = ld2(aS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const float2 Xp = ld2(sv + p + nx), AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
+            const float2 RI = ld2(rin + p), DV = ld2(d + p);
+            const float hx = halo_of(sv, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
+            const float2 q = pair_A(AW, right_of(AW, hw, L), AS0, AS1, AC, Xm, Xc, Xp, left_of(Xc, hx, L),
+                                    right_of(Xc, hx, L));
+            float2 dn, rn;
+            dn.x = DV.x + alpha * Xc.x;
+            dn.y = DV.y + alpha * Xc.y;
+            rn.x = RI.x - alpha * q.x;
+            rn.y = RI.y - alpha * q.y;
+            if (L.act) {
+                st2(d + p, dn);
+                st2(rout + p, rn);
+            }
+            Xm = Xc;
+            Xc = Xp;
+            AS0 = AS1;
+        }
+    }
+    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const int s = a.ring_src[k];
+        const float v = rin[s] - alpha * op_A(aW, aS, aC, sv, s, nx);
+        rout[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        float2 Rm = ld2(rout + p - nx), Rc = ld2(rout + p), PS0 = ld2(pS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const float2 Rp = ld2(rout + p + nx), PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
+            const float hr = halo_of(rout, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
+            const float2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
+                                    right_of(Rc, hr, L));
+            double2 e = make_double2((double)Rc.x * (double)Rc.x, (double)Rc.y * (double)Rc.y);
+            double2 h = make_double2((double)q.x * (double)Rc.x, (double)q.y * (double)Rc.y);
+            if (!L.act) {
+                e = make_double2(0.0, 0.0);
+                h = e;
+            }
+            put_row2<MJ_SUM_TREE>(0, j, L, e, nullptr, sNx, sNy);
+            put_row2<MJ_SUM_TREE>(1, j, L, h, nullptr, sNx, sNy);
+            Rm = Rc;
+            Rc = Rp;
+            PS0 = PS1;
+        }
+    }
+    const int BT = a.B * a.T;
+    store_tile_sums<MJ_SUM_TREE>(a.perr + (n & 1) * BT, a.peta + ((n + 1) & 1) * BT, m * a.T + t, lane, wave, nwaves,
+                                 nullptr, sNx, sNy);
+}
+
+__global__ void __launch_bounds__(MJ_MX_THREADS) mx_init_kernel(MxDev a, int ph) {
+    int m, t;
+    if (!phase_unit(a.B, a.T, m, t)) return;
+    if (ph == 0)
+        unit_I0(a, m, t);
+    else
+        mx_I1(a, m, t);
+}
+
+// One step of every member's state machine.
+__global__ void __launch_bounds__(MJ_MX_THREADS) mx_step_kernel(MxDev a, int k) {
+    int m, t;
+    if (!phase_unit(a.B, a.T, m, t)) return;
+    const int st = a.mdone[m];
+    if (st >= 0 && st < k) return;  // finished in an earlier step
+    const int BT = a.B * a.T;
+    if (threadIdx.x == 0) {
+        const MxRec r = a.rec[((k - 1) & 1) * a.B + m];
+        MxRec nr = r;
+        int act = 0;
+        double c = 0.0;
+        const double* pe = a.peta + m * a.T;
+        switch (r.mode) {
+            case MX_RST: {
+                const double e = tile_order_sum(a.perr64 + m * a.T, a.T);
+                c = e;
+                if (t == 0 && r.outer == 0) {
+                    a.diag[m].firstResidual = sqrt(e);
+                    a.diag[m].sumRHS = tile_order_sum(a.psum + m * a.T, a.T);
+                    a.diag[m].rhsMax = member_rhs_max(a, m);
+                    a.diag[m].pad_ = 0;
+                }
+                if (e < a.tol_sq || r.iters >= a.max_iters || r.outer >= a.max_outer) {
+                    act = AX_FINISH;
+                } else {
+                    act = AX_RESTART;
+                    nr = MxRec{MX_IB, 1, r.iters, r.outer + 1, sqrt(e)};
+                }
+                break;
+            }
+            case MX_IB: {
+                bool stop = r.iters >= a.max_iters || r.n - 1 >= a.inner_max;
+                if (!stop && r.n > 1) stop = tile_order_sum(a.perr + ((r.n - 1) & 1) * BT + m * a.T, a.T) < a.itol_sq;
+                if (stop) {
+                    act = AX_UPD;
+                    nr.mode = MX_TRUE;
+                } else {
+                    act = AX_IB;
+                    c = r.n == 1 ? 0.0
+                                 : tile_order_sum(pe + (r.n & 1) * BT, a.T) / tile_order_sum(pe + ((r.n - 1) & 1) * BT, a.T);
+                    nr.mode = MX_IC;
+                }
+                break;
+            }
+            case MX_IC:
+                act = AX_IC;
+                c = tile_order_sum(pe + (r.n & 1) * BT, a.T) / tile_order_sum(a.psig + (r.n & 1) * BT + m * a.T, a.T);
+                nr = MxRec{MX_IB, r.n + 1, r.iters + 1, r.outer, r.sigma};
+                break;
+            default:
+                act = AX_TRUE;
+                nr.mode = MX_RST;
+                break;
+        }
+        g_ibcast = act;
+        g_bcast[0] = c;
+        g_bcast[1] = act == AX_RESTART ? nr.sigma : r.sigma;
+        g_bcast[2] = (double)r.n;
+        if (t == 0) {
+            a.rec[(k & 1) * a.B + m] = nr;
+            if (act == AX_FINISH) {
+                a.diag[m].lastResidual = sqrt(c);
+                a.diag[m].numIters = r.iters;
+                a.diag[m].outerIters = r.outer;
+                a.diag[m].converged = c < a.tol_sq ? 1 : 0;
+                a.mdone[m] = k;
+                atomicSub(a.nactive, 1);
+            }
+        }
+    }
+    __syncthreads();
+    const int act = g_ibcast, n = (int)g_bcast[2];
+    const double c = g_bcast[0], sigma = g_bcast[1];
+    switch (act) {
+        case AX_FINISH: mx_finish(a, m, t); break;
+        case AX_RESTART: mx_restart(a, m, t, sigma); break;
+        case AX_UPD: mx_update(a, m, t, sigma); break;
+        case AX_IB: mx_IB(a, m, t, n, c); break;
+        case AX_IC: mx_IC(a, m, t, n, c); break;
+        default: mx_true(a, m, t); break;
+    }
+}
+
+// FP32 copies of the six coefficient arrays of every slot: dst[k][slot][q].
+__global__ void mx_convert_kernel(const double* aW, const double* aS, const double* aC, const double* pW,
+                                  const double* pS, const double* pC, int64_t oms, int nslots, int64_t tp,
+                                  float* dst) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (q >= tp) return;
+    const double* src[6] = {aW, aS, aC, pW, pS, pC};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst[((int64_t)k * nslots + s) * tp + q] = (float)src[k][s * oms + q];
+}
+
+__global__ void mx_reset_kernel(int* nactive, int B) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *nactive = B;
+}
+
+}  // namespace mj
+
+using namespace mj;
+
+namespace {
+
+struct MxLayout {
+    size_t r64, r0, r1, s0, s1, d, coef, pmax, psum, perr64, perr, peta, psig, rec, mdone, misc, total;
+};
+
+int nslots_of(const mj_cg2d_config* c, const mj_cg2d_ops* ops, const mj_cg2d_mixed_config* mc) {
+    if (ops && ops->member_stride == 0) return 1;
+    return mc->op_slots > 0 ? mc->op_slots : c->batch;
+}
+
+MxLayout mx_layout(const mj_cg2d_config* c, int nslots) {
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    MxLayout L{};
+    const size_t n = (size_t)c->batch * c->member_stride;
+    const size_t tp = (size_t)c->ntiles * (c->sNx + 2 * c->OL) * (c->sNy + 2 * c->OL);
+    const size_t bt = al((size_t)c->batch * c->ntiles * 8);
+    size_t o = 0;
+    L.r64 = o; o += al(n * 8);
+    L.r0 = o; o += al(n * 4);
+    L.r1 = o; o += al(n * 4);
+    L.s0 = o; o += al(n * 4);
+    L.s1 = o; o += al(n * 4);
+    L.d = o; o += al(n * 4);
+    L.coef = o; o += al(6 * (size_t)nslots * tp * 4);
+    L.pmax = o; o += bt;
+    L.psum = o; o += bt;
+    L.perr64 = o; o += bt;
+    L.perr = o; o += 2 * bt;
+    L.peta = o; o += 2 * bt;
+    L.psig = o; o += 2 * bt;
+    L.rec = o; o += al(2 * (size_t)c->batch * sizeof(MxRec));
+    L.mdone = o; o += al((size_t)c->batch * 4);
+    L.misc = o; o += 256;
+    L.total = o;
+    return L;
+}
+
+int check_mixed(const mj_cg2d_config* c, const mj_cg2d_mixed_config* mc) {
+    MJ_REQUIRE(c && mc, "cg2d mixed: null config");
+    MJ_REQUIRE(c->batch >= 1 && c->ntiles >= 1, "cg2d mixed: bad batch/layout");
+    MJ_REQUIRE(c->sNx >= 64 && c->sNx <= 128 && c->sNy >= 64 && c->sNy <= 128 && c->sNx % 2 == 0 && c->OL >= 2 &&
+                   c->OL % 2 == 0,
+               "cg2d mixed: sNx, sNy must be in 64..128, sNx and OL even");
+    const int64_t plane = (int64_t)(c->sNx + 2 * c->OL) * (c->sNy + 2 * c->OL);
+    MJ_REQUIRE(c->member_stride >= c->ntiles * plane && c->member_stride % 2 == 0,
+               "cg2d mixed: member_stride must be even and >= ntiles * ny * nx");
+    MJ_REQUIRE((int64_t)c->batch * c->member_stride < ((int64_t)1 << 31), "cg2d mixed: batch * member_stride too large");
+    MJ_REQUIRE(c->sum_order == MJ_SUM_TREE, "cg2d mixed: only MJ_SUM_TREE dot products");
+    MJ_REQUIRE(c->max_iters >= 0, "cg2d mixed: max_iters < 0");
+    MJ_REQUIRE(c->tolerance > 0.0, "cg2d mixed: tolerance must be > 0");
+    MJ_REQUIRE(c->launch_mode == MJ_CG_LAUNCH_AUTO || c->launch_mode == MJ_CG_LAUNCH_PHASES ||
+                   c->launch_mode == MJ_CG_LAUNCH_PHASES_FIXED,
+               "cg2d mixed: launch_mode must be AUTO, PHASES or PHASES_FIXED");
+    MJ_REQUIRE(c->block_threads == 0 || (c->block_threads % 64 == 0 && c->block_threads >= 64 &&
+                                         c->block_threads <= MJ_MX_THREADS),
+               "cg2d mixed: block_threads must be a multiple of 64 in 64..%d", MJ_MX_THREADS);
+    MJ_REQUIRE(c->poll_iters >= 0, "cg2d mixed: poll_iters < 0");
+    MJ_REQUIRE(mc->inner_tolerance > 0.0 && mc->inner_tolerance < 1.0, "cg2d mixed: inner_tolerance must be in (0, 1)");
+    MJ_REQUIRE(mc->inner_max_iters >= 1 && mc->max_outer >= 1, "cg2d mixed: inner_max_iters, max_outer must be >= 1");
+    MJ_REQUIRE(mc->op_slots >= 0, "cg2d mixed: op_slots < 0");
+    return MJ_OK;
+}
+
+}  // namespace
+
+extern "C" size_t mj_cg2d_mixed_workspace_bytes(const mj_cg2d_config* cfg, const mj_cg2d_mixed_config* mcfg,
+                                                int64_t ops_member_stride) {
+    if (check_mixed(cfg, mcfg)) return 0;
+    mj_cg2d_ops o{};
+    o.member_stride = ops_member_stride;
+    return mx_layout(cfg, nslots_of(cfg, &o, mcfg)).total;
+}
+
+extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* b,
+                                   const double* x0, double* x, const double* cg2dNorm, const mj_cg2d_config* cfg,
+                                   const mj_cg2d_mixed_config* mcfg, void* workspace, size_t workspace_bytes,
+                                   mj_cg2d_diag* diag, void* stream) {
+    int rc = check_mixed(cfg, mcfg);
+    if (rc) return rc;
+    MJ_REQUIRE(plan_T && ops && b && x0 && x && cg2dNorm && workspace && diag, "mj_cg2d_solve_mixed: null argument");
+    MJ_REQUIRE(ops->aW2d && ops->aS2d && ops->aC2d && ops->pW && ops->pS && ops->pC,
+               "mj_cg2d_solve_mixed: null operator");
+    MJ_REQUIRE(x != b && x != x0, "mj_cg2d_solve_mixed: x must not alias b or x0");
+    MJ_REQUIRE(((uintptr_t)workspace & 255) == 0, "mj_cg2d_solve_mixed: workspace must be 256-byte aligned");
+    MJ_REQUIRE(ops->member_stride % 2 == 0, "mj_cg2d_solve_mixed: operator member_stride must be even");
+    MJ_REQUIRE(ops->member_stride == 0 || ops->op_index == nullptr || mcfg->op_slots > 0,
+               "mj_cg2d_solve_mixed: op_slots (the number of operator slots) is required with op_index");
+    for (const void* q : {(const void*)x, (const void*)b, (const void*)ops->aW2d, (const void*)ops->aS2d,
+                          (const void*)ops->aC2d})
+        MJ_REQUIRE(((uintptr_t)q & 15) == 0, "mj_cg2d_solve_mixed: x, b and the operator must be 16-byte aligned");
+    const int nslots = nslots_of(cfg, ops, mcfg);
+    const MxLayout L = mx_layout(cfg, nslots);
+    MJ_REQUIRE(workspace_bytes >= L.total, "mj_cg2d_solve_mixed: workspace %zu < %zu bytes", workspace_bytes,
+               L.total);
+    mj_halo_plan* P = const_cast<mj_halo_plan*>(plan_T);
+    MJ_REQUIRE(P->ntiles == cfg->ntiles, "mj_cg2d_solve_mixed: plan has %d tiles, config %d", P->ntiles,
+               cfg->ntiles);
+    if ((rc = ensure_cg_tables(P, cfg->sNx, cfg->sNy, cfg->OL))) return rc;
+
+    char* w = (char*)workspace;
+    MxDev a{};
+    a.B = cfg->batch;
+    a.T = cfg->ntiles;
+    a.sNx = cfg->sNx;
+    a.sNy = cfg->sNy;
+    a.OL = cfg->OL;
+    a.nx = cfg->sNx + 2 * cfg->OL;
+    a.ny = cfg->sNy + 2 * cfg->OL;
+    a.plane = a.nx * a.ny;
+    a.ms = cfg->member_stride;
+    a.oms = ops->member_stride;
+    const int64_t tp = (int64_t)a.T * a.plane;
+    a.cms = ops->member_stride == 0 ? 0 : tp;
+    a.aW = ops->aW2d;
+    a.aS = ops->aS2d;
+    a.aC = ops->aC2d;
+    float* coef = (float*)(w + L.coef);
+    a.aW32 = coef;
+    a.aS32 = coef + 1 * nslots * tp;
+    a.aC32 = coef + 2 * nslots * tp;
+    a.pW32 = coef + 3 * nslots * tp;
+    a.pS32 = coef + 4 * nslots * tp;
+    a.pC32 = coef + 5 * nslots * tp;
+    a.b = b;
+    a.x0 = x0;
+    a.x = x;
+    a.norm = cg2dNorm;
+    a.tol_sq = cfg->tolerance * cfg->tolerance;
+    a.itol_sq = mcfg->inner_tolerance * mcfg->inner_tolerance;
+    a.max_iters = cfg->max_iters;
+    a.inner_max = mcfg->inner_max_iters;
+    a.max_outer = mcfg->max_outer;
+    a.full_off = P->cg_full_off;
+    a.full_dst = P->cg_full_dst;
+    a.full_src = P->cg_full_src;
+    a.full_sign = P->cg_full_sign;
+    a.ring_off = P->cg_ring_off;
+    a.ring_dst = P->cg_ring_dst;
+    a.ring_src = P->cg_ring_src;
+    a.ring_sign = P->cg_ring_sign;
+    a.r64 = (double*)(w + L.r64);
+    a.r0 = (float*)(w + L.r0);
+    a.r1 = (float*)(w + L.r1);
+    a.s0 = (float*)(w + L.s0);
+    a.s1 = (float*)(w + L.s1);
+    a.d = (float*)(w + L.d);
+    a.pmax = (double*)(w + L.pmax);
+    a.psum = (double*)(w + L.psum);
+    a.perr64 = (double*)(w + L.perr64);
+    a.perr = (double*)(w + L.perr);
+    a.peta = (double*)(w + L.peta);
+    a.psig = (double*)(w + L.psig);
+    a.op_index = ops->op_index;
+    a.rec = (MxRec*)(w + L.rec);
+    a.mdone = (int32_t*)(w + L.mdone);
+    a.nactive = (int*)(w + L.misc);
+    a.diag = diag;
+
+    hipStream_t st = as_stream(stream);
+    const int threads = cfg->block_threads ? cfg->block_threads : MJ_MX_THREADS;
+    const int nunits = a.B * a.T;
+    const dim3 grid((unsigned)((nunits + kXcds - 1) / kXcds * kXcds)), block((unsigned)threads);
+    hipLaunchKernelGGL(mx_reset_kernel, dim3(1), dim3(64), 0, st, a.nactive, a.B);
+    MJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mx_convert_kernel, dim3((unsigned)((tp + 255) / 256), (unsigned)nslots), dim3(256), 0, st,
+                       ops->aW2d, ops->aS2d, ops->aC2d, ops->pW, ops->pS, ops->pC, ops->member_stride, nslots, tp,
+                       coef);
+    MJ_HIP(hipGetLastError());
+    for (int ph = 0; ph < 2; ++ph) {
+        hipLaunchKernelGGL(mx_init_kernel, grid, block, 0, st, a, ph);
+        MJ_HIP(hipGetLastError());
+    }
+    // steps: every inner iteration takes two, every refinement three (RST, the stopping IB, TRUE), the last RST one
+    const int64_t kmax = 2 * (int64_t)cfg->max_iters + 3 * (int64_t)mcfg->max_outer + 2;
+    const int chunk = cfg->poll_iters > 0 ? 2 * cfg->poll_iters : 64;
+    int64_t k = 1;
+    while (k <= kmax) {
+        const int64_t kend = cfg->launch_mode == MJ_CG_LAUNCH_PHASES_FIXED ? kmax + 1 : std::min(kmax + 1, k + chunk);
+        for (; k < kend; ++k) {
+            hipLaunchKernelGGL(mx_step_kernel, grid, block, 0, st, a, (int)k);
+            MJ_HIP(hipGetLastError());
+        }
+        if (k > kmax) break;
+        int h = 0;
+        MJ_HIP(hipMemcpyAsync(&h, a.nactive, sizeof(int), hipMemcpyDeviceToHost, st));
+        MJ_HIP(hipStreamSynchronize(st));
+        if (h == 0) break;
+    }
+    return MJ_OK;
+}

@@ -156,6 +156,40 @@ class Exchanger:
                                                   ctypes.byref(g), _lib.stream_ptr()), "mj_halo_exchange_uv")
         return uo, vo
 
+    def scalar_adjoint(self, abar, kind="T"):
+        """Transpose of scalar(): the cotangent of the input for the cotangent abar of the exchanged array."""
+        abar = _require(abar, self.L, f"scalar_adjoint({kind})")
+        p = self.plan(kind, abar.device)
+        if p.vector:
+            raise ValueError(f"{kind} is a vector kind")
+        out = torch.empty_like(abar)
+        g = _geom(abar, self.L)
+        _lib.check(_lib.lib().mj_halo_exchange_adjoint(p.handle, _lib.ptr(abar), None, _lib.ptr(out), None,
+                                                       ctypes.byref(g), _lib.stream_ptr()), "mj_halo_exchange_adjoint")
+        return out
+
+    def vector_adjoint(self, ubar, vbar, kind="UVs"):
+        """Transpose of vector(): the cotangents of (u, v) for the cotangents of the exchanged pair."""
+        ubar = _require(ubar, self.L, f"vector_adjoint({kind}) u")
+        vbar = _require(vbar, self.L, f"vector_adjoint({kind}) v")
+        p = self.plan(kind, ubar.device)
+        if not p.vector:
+            raise ValueError(f"{kind} is a scalar kind")
+        uo, vo = torch.empty_like(ubar), torch.empty_like(vbar)
+        g = _geom(ubar, self.L)
+        _lib.check(_lib.lib().mj_halo_exchange_adjoint(p.handle, _lib.ptr(ubar), _lib.ptr(vbar), _lib.ptr(uo),
+                                                       _lib.ptr(vo), ctypes.byref(g), _lib.stream_ptr()),
+                   "mj_halo_exchange_adjoint")
+        return uo, vo
+
+    def scalar_ad(self, a, kind="T"):
+        """scalar() differentiable through torch.autograd (backward: scalar_adjoint)."""
+        return _ScalarExchange.apply(a, self, kind)
+
+    def vector_ad(self, u, v, kind="UVs"):
+        """vector() differentiable through torch.autograd (backward: vector_adjoint)."""
+        return _VectorExchange.apply(u, v, self, kind)
+
     def fill_(self, a, kind="T"):
         """In-place halo fill (only for kinds whose written points are never sources: T, 3D, UVs, UVn, UV3s)."""
         a = _require(a, self.L, f"fill_({kind})")
@@ -203,6 +237,33 @@ class Exchanger:
 
 
 _default = {}
+
+
+class _ScalarExchange(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, a, ex, kind):
+        ctx.ex, ctx.kind = ex, kind
+        return ex.scalar(a, kind)
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.ex.scalar_adjoint(g.contiguous(), ctx.kind), None, None
+
+
+class _VectorExchange(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u, v, ex, kind):
+        ctx.ex, ctx.kind = ex, kind
+        return ex.vector(u, v, kind)
+
+    @staticmethod
+    def backward(ctx, gu, gv):
+        if gu is None:
+            gu = torch.zeros_like(gv)
+        if gv is None:
+            gv = torch.zeros_like(gu)
+        ub, vb = ctx.ex.vector_adjoint(gu.contiguous(), gv.contiguous(), ctx.kind)
+        return ub, vb, None, None
 
 
 def default_exchanger(layout=None):

@@ -157,13 +157,8 @@ def _dev(a, device, name):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
 
 
-def advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, wFld=None,
-                 nlev=None, out=None):
-    """GAD_ADVECTION on all tiles (reference gad.py:460-539) -> gTracer, same shape as tracer.
-
-    tables: gad_tables(params) (or None). grid2d: dyG, dxG, recip_dxC, recip_dyC, recip_rA, maskInC [T, ny, nx]
-    and maskW, maskS [T, K, ny, nx] (or [B, T, K, ny, nx] per member); with wFld also rA [T, ny, nx] and maskC.
-    vgrid: drF, recip_drF, rhoFacC [K] (+ recip_drC with wFld). nlev: per-member level counts (ensemble)."""
+def _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, wFld, nlev):
+    """Checked device arguments of the DST3 entry points."""
     L = params.layout
     if not isinstance(tracer, torch.Tensor) or tracer.device.type != "cuda":
         raise TypeError("tracer: expected a CUDA tensor (the advection runs on the GPU only)")
@@ -228,13 +223,72 @@ def advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS
             raise ValueError("nlev: one value in 0..K per member")
     cfg = _lib.GadConfig(B, K, T * K * ny * nx, mms, _lib.ptr(nl))
     plan = tables if isinstance(tables, GadPlan) else gad_plan(params.topo, L, dev)
+    keep = (g, vt, nl)
+    return plan, grid_s, cfg, tr, fields, w, keep
+
+
+def advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, wFld=None,
+                 nlev=None, out=None):
+    """GAD_ADVECTION on all tiles (reference gad.py:460-539) -> gTracer, same shape as tracer.
+
+    tables: gad_tables(params) (or None). grid2d: dyG, dxG, recip_dxC, recip_dyC, recip_rA, maskInC [T, ny, nx]
+    and maskW, maskS [T, K, ny, nx] (or [B, T, K, ny, nx] per member); with wFld also rA [T, ny, nx] and maskC.
+    vgrid: drF, recip_drF, rhoFacC [K] (+ recip_drC with wFld). nlev: per-member level counts (ensemble)."""
+    plan, grid_s, cfg, tr, fields, w, keep = _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW,
+                                                      hFacS, recip_hFacC, wFld, nlev)
     gT = torch.empty_like(tr) if out is None else out
     work = torch.empty_like(tr) if w is not None else None
     u, v, hW, hS, rhC = fields
     _lib.check(_lib.lib().mj_gad_dst3(plan.handle, ctypes.byref(grid_s), ctypes.byref(cfg), _lib.ptr(u), _lib.ptr(v),
                                       _lib.ptr(w), _lib.ptr(tr), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(rhC),
                                       _lib.ptr(gT), _lib.ptr(work), _lib.stream_ptr()), "mj_gad_dst3")
+    del keep
     return gT
+
+
+def advect_tiles_adjoint(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, gT_bar,
+                         nlev=None):
+    """Reverse mode of advect_tiles without the R flux (libmjamd mj_gad_dst3_adjoint): cotangents
+    (tracer_bar, uFld_bar, vFld_bar) of the inputs for the cotangent gT_bar of gTracer (the reference's
+    jax.grad of advect_tiles in the tracer and the velocities, test_gad.py:297-346)."""
+    plan, grid_s, cfg, tr, fields, w, keep = _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW,
+                                                      hFacS, recip_hFacC, None, nlev)
+    gb = gT_bar.to(torch.float64).contiguous()
+    if tuple(gb.shape) != tuple(tr.shape):
+        raise ValueError("gT_bar shape differs from tracer")
+    bars = [torch.empty_like(tr) for _ in range(3)]
+    lib = _lib.lib()
+    wsb = lib.mj_gad_dst3_adjoint_workspace_bytes()
+    work = torch.empty(wsb, dtype=torch.uint8, device=tr.device)
+    u, v, hW, hS, rhC = fields
+    _lib.check(lib.mj_gad_dst3_adjoint(plan.handle, ctypes.byref(grid_s), ctypes.byref(cfg), _lib.ptr(u), _lib.ptr(v),
+                                       _lib.ptr(tr), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(rhC), _lib.ptr(gb),
+                                       *[_lib.ptr(b) for b in bars], _lib.ptr(work), ctypes.c_size_t(wsb),
+                                       _lib.stream_ptr()), "mj_gad_dst3_adjoint")
+    del keep, work
+    return tuple(bars)
+
+
+class _AdvectTiles(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, tracer, uFld, vFld, params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC, nlev):
+        ctx.save_for_backward(tracer, uFld, vFld)
+        ctx.rest = (params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC, nlev)
+        return advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, nlev=nlev)
+
+    @staticmethod
+    def backward(ctx, gT_bar):
+        tracer, uFld, vFld = ctx.saved_tensors
+        params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC, nlev = ctx.rest
+        tb, ub, vb = advect_tiles_adjoint(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS,
+                                          recip_hFacC, gT_bar, nlev=nlev)
+        return tb, ub, vb, None, None, None, None, None, None, None, None
+
+
+def advect_tiles_ad(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, nlev=None):
+    """advect_tiles (no R flux) differentiable in tracer, uFld and vFld through torch.autograd, with the
+    hand-written reverse mode (advect_tiles_adjoint)."""
+    return _AdvectTiles.apply(tracer, uFld, vFld, params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC, nlev)
 
 
 def gad_advection(params, g, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC):

@@ -122,6 +122,24 @@ def cg2d_matvec(Tmap, A, v, cg2dNorm, sNx=90, sNy=90, OL=4):
     return y
 
 
+def cg2d_coeff_grad(Tmap, x, lam, cg2dNorm, OL=4):
+    """Cotangent of (aW2d, aS2d, aC2d) in the implicit derivative of the solve (core/cg2d.py:326-348): the
+    transpose of dA -> -_matvec(dA, x) applied to lam (zero off the interior), xe = exch(where(inner, x, 0)):
+    gaW[q] = -(lam[q] xe[q-1] + lam[q-1] xe[q]) / cg2dNorm; gaS with q-nx; gaC[q] = -(lam[q] xe[q]) / cg2dNorm.
+    numpy, [T, ny, nx]."""
+    x = _f64(x)
+    T, ny, nx = x.shape
+    inner = np.zeros((ny, nx), bool)
+    inner[OL:ny - OL, OL:nx - OL] = True
+    xe = scalar(Tmap, np.where(inner[None], x, 0.0), T=T, plane=ny * nx)
+    lam = _f64(lam)
+    gW = np.zeros_like(x)
+    gS = np.zeros_like(x)
+    gW[:, :, 1:] = lam[:, :, 1:] * xe[:, :, :-1] + lam[:, :, :-1] * xe[:, :, 1:]
+    gS[:, 1:, :] = lam[:, 1:, :] * xe[:, :-1, :] + lam[:, :-1, :] * xe[:, 1:, :]
+    return -(gW / cg2dNorm), -(gS / cg2dNorm), -(lam * xe / cg2dNorm)
+
+
 # ------------------------------------------------------------------------------------------------------ DST3
 VERT_ORDER = ("dTlev", "deepFacC", "recip_deepFacC", "recip_deepFac2C", "recip_rhoFacC", "drF", "recip_drF",
               "rhoFacC")

@@ -169,3 +169,30 @@ def test_planted_errors_change_the_solution(case):
     bad[3].reshape(-1)[k] *= 1 + 1e-6
     x, _ = oracle.cg2d(Tm, bad, b, np.zeros_like(b), cg2dNorm=norm, tolerance=1e-9, max_iters=1000)
     assert not np.array_equal(x, ref)
+
+
+def test_coeff_grad_is_the_transpose_of_the_coefficient_matvec(case):
+    """The oracle's coefficient cotangent is the adjoint of dA -> -_matvec(dA, x) (core/cg2d.py:326-334): for random
+    directions dA (each of aW, aS, aC separately) <g_k, dA_k> = -<lam, _matvec(dA, x)>, where the right-hand side
+    is the oracle's C matvec (exchange + 5-point stencil), an independent path. _matvec is linear in dA, so the two
+    agree to rounding. A shifted neighbour (x(i+1) for x(i-1)) or a dropped term fails by O(1)."""
+    g, ops, norm, b = case
+    Tm = maps().full("T")
+    _, inner = _assemble(ops, norm)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=b.shape)  # halos arbitrary: the cotangent must only see exch(where(inner, x, 0))
+    lam = np.where(inner, rng.normal(size=b.shape), 0.0)
+    gW, gS, gC = oracle.cg2d_coeff_grad(Tm, x, lam, norm)
+    zero = np.zeros_like(b)
+    for k, gk in enumerate((gW, gS, gC)):
+        dA = rng.normal(size=b.shape)
+        A = [zero, zero, zero]
+        A[k] = dA
+        rhs = -(lam * oracle.cg2d_matvec(Tm, A, x, norm)).sum()
+        lhs = (gk * dA).sum()
+        assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs)), (k, lhs, rhs)
+    # entries no interior stencil reads carry no cotangent
+    ny, nx = inner.shape[1:]
+    read = np.zeros((ny, nx), bool)
+    read[4:94, 4:95] = True  # aW at i and i+1 of interior points
+    assert np.all(gW[:, ~read] == 0.0)
