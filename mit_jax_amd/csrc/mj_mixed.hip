@@ -79,6 +79,18 @@ __device__ __forceinline__ void mx_I1(const MxDev& a, int m, int t) {
     double* x = a.x + m * a.ms;
     double* r = a.r64 + m * a.ms;
     const int64_t ob = (int64_t)slot_of(a, m) * a.oms;
+    // halo points of the FP32 work vectors start at 0: the exchange leaves some halo points unwritten (open facet
+    // edges) and the stencils read them with zero coefficients
+    float *r0 = a.r0 + m * a.ms, *r1 = a.r1 + m * a.ms, *s0 = a.s0 + m * a.ms, *s1 = a.s1 + m * a.ms;
+    for (int qq = tid; qq < a.plane; qq += blockDim.x) {
+        const int j = qq / nx, i = qq - j * nx, p = tbase + qq;
+        if (!(j >= OL && j < OL + sNy && i >= OL && i < OL + sNx)) {
+            r0[p] = 0.0f;
+            r1[p] = 0.0f;
+            s0[p] = 0.0f;
+            s1[p] = 0.0f;
+        }
+    }
     first_guess(a, m, t, rhsNorm);
     MJ_ROWS({
         const double bn0 = b[p0] * nm * rhsNorm;
