@@ -1,16 +1,20 @@
 """Benchmark: ensemble member-steps/s of the CG2D + DST3 step on LLC90 (13 tiles x 90x90, 50 levels) on one MI355X.
 
-A member-step is one CG2D free-surface solve to the normalised target residual plus one GAD DST3 tracer step
-(X/Y flux passes, corner fills, flux divergence) of one ensemble member. One timed step runs the whole ensemble:
-one batched solve (mj_cg2d_solve, per-member operators and convergence) and one batched DST3 step (mj_gad_dst3,
-per-member level counts). Inputs are seeded synthetic fields resident in HBM before timing starts.
+A member-step is one CG2D free-surface solve to the normalised 1e-12 target plus one GAD DST3 tracer step (X/Y flux
+passes, corner fills, flux divergence) of one ensemble member. One timed step runs the whole ensemble: one batched
+solve (per-member operators through operator slots, per-member convergence) and one batched DST3 step (per-member
+level counts). Inputs are seeded synthetic fields resident in HBM before timing starts.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--members B] [--precision fp64|mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--members B] [--precision mixed|fp64]
 
-Prints one JSON line. `roofline` is the dominant kernel (the CG2D solve) timed with HIP events on its stream:
-achieved = algorithmic bytes per launch / average launch time, with algorithmic bytes = sum over members of
-numIters * N2 * (6 coefficient + 6 vector) * 8 B (N2 = 105,300 interior points). `cpu_baseline` is the FP64 C
-oracle (OpenMP, all host cores) on a bounded sample of the same member-steps.
+The headline (default) is the mixed-precision solve (FP32 inner PCG + FP64 refinement, mj_cg2d_solve_mixed); the
+FP64 solve (mj_cg2d_solve) is timed on the same step for `mixed_over_fp64`, and the two solutions are compared
+member by member (`error_vs_fp64`). Prints one JSON line. `roofline` is the solve (the dominant kernel), timed with
+HIP events on its stream: achieved = algorithmic bytes of the solve / solve time, with per inner iteration
+N2 * (6 coefficient + 6 vector) * s bytes (s = 4 mixed, 8 FP64; N2 = 105,300 interior points) and per refinement
+N2 * (5 + 3) * 8 bytes; `traffic` = measured HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
+profile of this workload. `cpu_baseline` is the FP64 C oracle (OpenMP, all host cores) on a bounded sample of the
+same member-steps.
 """
 
 import argparse
@@ -45,6 +49,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (the reference kernel for HBM counter scaling)")
+    ap.add_argument("--profile-solves", action="store_true",
+                    help="only run one FP64 and one mixed solve of --members right-hand sides on the shared 5-point "
+                         "operator (the workload of the hardware-counter profiles) and exit")
     return ap.parse_args()
 
 
@@ -139,12 +146,48 @@ def cpu_baseline(E, seconds):
                        f"{el:.1f} s")
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "mx_step_kernel_ensemble128_traffic.md"
+
+
+def measured_traffic(kernel, members):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE profile of the 128-member
+    workload (None for another workload or kernel)."""
+    if members != 128 or not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
+def profile_solves(B):
+    """One FP64 (phase launches) and one mixed solve of B right-hand sides on the shared operator."""
+    import torch
+    from mit_jax_amd import cg2d, mixed, synthetic as S
+    from mit_jax_amd.exchange import default_exchanger
+    dev = torch.device("cuda:0")
+    ex = default_exchanger()
+    g, ops, norm = S.cg2d_case(0, "5point")
+    opsd = [torch.as_tensor(o, device=dev) for o in ops]
+    b = torch.stack([torch.as_tensor(S.rhs(g, 100 + m), device=dev) for m in range(B)])
+    x0 = torch.zeros_like(b)
+    _, d = cg2d.cg2d_fortran(ex, opsd, b, x0, cg2dNorm=norm, tolerance=TOL, max_iters=2000, sum_order="tile",
+                             launch="phases")
+    _, d2 = mixed.cg2d_mixed(ex, opsd, b, x0, cg2dNorm=norm, tolerance=TOL, max_iters=4000)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_solves": B, "fp64_iterations": int(d["numIters"].max()),
+                      "mixed_iterations": int(d2["numIters"].max())}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the ensemble shards over GPUs with no communication, so --gpus {args.gpus}"
               " is not run here (one process per GPU would each run this same single-GPU step)", file=sys.stderr)
         sys.exit(2)
+    if args.profile_solves:
+        return profile_solves(args.members)
     import torch
     from mit_jax_amd import cg2d, gad, mixed
 
@@ -234,7 +277,9 @@ def main():
         "data": "synthetic",
         "config": cfg,
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kernel, B),
+                     "traffic_source": TRAFFIC_FILE.name if measured_traffic(kernel, B) else None,
                      "solve_ms": run["cg_ms"], "algorithmic_bytes": cg_bytes},
         "dst3": {"launch_ms": run["gad_ms"], "achieved_GBs": gad_bytes / (run["gad_ms"] * 1e-3) / 1e9,
                  "algorithmic_bytes": gad_bytes},

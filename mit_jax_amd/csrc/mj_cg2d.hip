@@ -207,7 +207,7 @@ __device__ __forceinline__ void unit_B(const CgDev& a, int m, int t, int n, doub
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         double2 Rm = ld2(r + p - nx), Rc = ld2(r + p), PS0 = ld2(pS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const double2 Rp = ld2(r + p + nx), PS1 = ld2(pS + p + nx);
             const double2 PC = ld2(pC + p), PW = ld2(pW + p), SO = ld2(so + p);
@@ -232,7 +232,7 @@ __device__ __forceinline__ void unit_B(const CgDev& a, int m, int t, int n, doub
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         double2 Xm = ld2(sn + p - nx), Xc = ld2(sn + p), AS0 = ld2(aS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const double2 Xp = ld2(sn + p + nx), AS1 = ld2(aS + p + nx);
             const double2 AW = ld2(aW + p), AC = ld2(aC + p);
@@ -269,7 +269,7 @@ __device__ __forceinline__ void unit_C(const CgDev& a, int m, int t, int n, doub
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         double2 Xm = ld2(sv + p - nx), Xc = ld2(sv + p), AS0 = ld2(aS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const double2 Xp = ld2(sv + p + nx), AS1 = ld2(aS + p + nx);
             const double2 AW = ld2(aW + p), AC = ld2(aC + p), RI = ld2(rin + p), XV = ld2(x + p);
@@ -299,7 +299,7 @@ __device__ __forceinline__ void unit_C(const CgDev& a, int m, int t, int n, doub
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         double2 Rm = ld2(rout + p - nx), Rc = ld2(rout + p), PS0 = ld2(pS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const double2 Rp = ld2(rout + p + nx), PS1 = ld2(pS + p + nx);
             const double2 PC = ld2(pC + p), PW = ld2(pW + p);

@@ -30,6 +30,9 @@ __device__ __forceinline__ T op_M(const T* pW, const T* pS, const T* pC, const T
 #ifndef MJ_ROW_UNROLL
 #define MJ_ROW_UNROLL 2
 #endif
+#ifndef MJ_SWEEP_UNROLL
+#define MJ_SWEEP_UNROLL 1
+#endif  // unroll of the row sweeps (more rows' loads in flight per wave)
 #define MJ_PRAGMA(x) _Pragma(#x)
 #define MJ_UNROLL(n) MJ_PRAGMA(unroll n)
 #define MJ_ROWS(...)                                                                                      \

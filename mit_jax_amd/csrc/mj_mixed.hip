@@ -150,7 +150,7 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         float2 Rm = ld2(r32 + p - nx), Rc = ld2(r32 + p), PS0 = ld2(pS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const float2 Rp = ld2(r32 + p + nx), PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
             const float hr = halo_of(r32, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
@@ -201,7 +201,7 @@ __device__ __forceinline__ void mx_true(const MxDev& a, int m, int t) {
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         double2 Xm = ld2(x + p - nx), Xc = ld2(x + p), AS0 = ld2(aS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const double2 Xp = ld2(x + p + nx), AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
             const double2 Bv = ld2(b + p);
@@ -238,7 +238,7 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         float2 Rm = ld2(r + p - nx), Rc = ld2(r + p), PS0 = ld2(pS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const float2 Rp = ld2(r + p + nx), PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
             const float hr = halo_of(r, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
@@ -266,7 +266,7 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         float2 Xm = ld2(sn + p - nx), Xc = ld2(sn + p), AS0 = ld2(aS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const float2 Xp = ld2(sn + p + nx), AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
             const float hx = halo_of(sn, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
@@ -299,7 +299,7 @@ __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, doubl
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         float2 Xm = ld2(sv + p - nx), Xc = ld2(sv + p), AS0 = ld2(aS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const float2 Xp = ld2(sv + p + nx), AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
             const float2 RI = ld2(rin + p), DV = ld2(d + p);
@@ -329,7 +329,7 @@ __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, doubl
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         float2 Rm = ld2(rout + p - nx), Rc = ld2(rout + p), PS0 = ld2(pS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
+        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
             const int prow = MJ_ROW(j);
             const float2 Rp = ld2(rout + p + nx), PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
             const float hr = halo_of(rout, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
@@ -363,7 +363,11 @@ __global__ void __launch_bounds__(MJ_MX_THREADS) mx_init_kernel(MxDev a, int ph)
 }
 
 // One step of every member's state machine.
-__global__ void __launch_bounds__(MJ_MX_THREADS) mx_step_kernel(MxDev a, int k) {
+#ifndef MJ_PH_WAVES_PER_EU
+#define MJ_PH_WAVES_PER_EU 8
+#endif
+__global__ void __launch_bounds__(MJ_MX_THREADS) __attribute__((amdgpu_waves_per_eu(MJ_PH_WAVES_PER_EU)))
+mx_step_kernel(MxDev a, int k) {
     int m, t;
     if (!phase_unit(a.B, a.T, m, t)) return;
     const int st = a.mdone[m];
