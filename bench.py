@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--members", type=int, default=128)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
+    ap.add_argument("--no-autotune", action="store_true", help="run the default launch configuration, no graph")
     ap.add_argument("--no-fp64-reference", action="store_true",
                     help="with --precision mixed: skip timing the FP64 solve (the ratio and the error vs FP64)")
     ap.add_argument("--sum-order", choices=("tile", "fortran"), default="tile")
@@ -189,49 +190,39 @@ def main():
     if args.profile_solves:
         return profile_solves(args.members)
     import torch
-    from mit_jax_amd import cg2d, gad, mixed
+    from mit_jax_amd import cg2d, gad
+    from mit_jax_amd.ensemble import EnsembleStep, StepConfig, autotune
 
     dev = torch.device("cuda:0")
     B = args.members
     print(f"bench: building {B} members", file=sys.stderr, flush=True)
     E = build_ensemble(B, dev)
     params = gad.GADParams.llc90()
-    gT = torch.empty_like(E["tr"])
-    x = torch.empty_like(E["rhs"])
+    log = lambda m: print("bench: autotune " + m, file=sys.stderr, flush=True)  # noqa: E731
 
-    def solve(precision, out):
-        if precision == "fp64":
-            return cg2d.cg2d_fortran(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
-                                     max_iters=2000, sum_order=args.sum_order, out=out, op_index=E["op_index"])
-        return mixed.cg2d_mixed(E["ex"], E["ops"], E["rhs"], E["x0"], cg2dNorm=E["norm"], tolerance=TOL,
-                                max_iters=4000, out=out, op_index=E["op_index"])
+    def make(precision):
+        st = EnsembleStep(E["ex"], E["ops"], E["op_index"], E["norm"], E["rhs"], E["x0"], params, E["g2d"],
+                          E["vg"], E["u"], E["v"], E["tr"], E["hW"], E["hS"], E["rhC"], nlev=E["nlev_t"], tolerance=TOL,
+                          max_iters=2000, cfg=StepConfig(precision=precision, sum_order=args.sum_order))
+        tuned = None
+        if not args.no_autotune:
+            best, results = autotune(st, precision, steps=2, warmup=1, log=log)
+            tuned = {"selected": {**best.config.__dict__}, "candidates": len(results),
+                     "ms": {json.dumps(r.config.__dict__, sort_keys=True): round(r.ms, 3) for r in results}}
+        return st, tuned
 
-    def step(precision, ev=None):
-        if ev is not None:
-            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-            e0.record()
-        _, d = solve(precision, x)
-        if ev is not None:
-            e1.record()
-        gad.advect_tiles(params, None, E["g2d"], E["vg"], E["u"], E["v"], E["tr"], E["hW"], E["hS"], E["rhC"],
-                         nlev=E["nlev_t"], out=gT)
-        if ev is not None:
-            e2.record()
-            ev.append((e0, e1, e2))
-        return d
-
-    def timed(precision):
+    def timed(st):
         for _ in range(args.warmup):
-            step(precision)
+            st.step()
         torch.cuda.synchronize()
         ev = []
-        print(f"bench: timing {precision}", file=sys.stderr, flush=True)
+        print(f"bench: timing {st.cfg.precision}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            d = step(precision, ev)
+            st.step(ev)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        d.check()
+        d = st.diag().check()
         return dict(el=el, d=d, cg_ms=float(np.mean([a.elapsed_time(b) for a, b, _ in ev])),
                     gad_ms=float(np.mean([b.elapsed_time(c) for _, b, c in ev])))
 
@@ -241,14 +232,16 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    run = timed(args.precision)
+    main_step, tuned = make(args.precision)
+    run = timed(main_step)
     d = run["d"]
     iters = d["numIters"].cpu().numpy().astype(np.int64)
     outer = d["outerIters"].cpu().numpy().astype(np.int64)
     conv = d["converged"].cpu().numpy()
+    sel = main_step.cfg
     if args.precision == "fp64":
         cg_bytes = float(iters.sum()) * N2 * (6 * 8 + 6 * 8)
-        kernel = "cg2d_phase_kernel"
+        kernel = "cg2d_phase_kernel" if sel.launch != "persistent" else "cg2d_persistent_kernel"
     else:
         # inner iterations: FP32 coefficients and vectors; refinements: FP64 true residual (5 coefficient/vector
         # reads: aW, aS, aC, x, b) and r written, plus the restart's FP32 conversion (r read, r32 written, d zeroed)
@@ -261,7 +254,8 @@ def main():
                        "step), 13x90x90 tiles, 50 levels, member level counts 15/30/50",
            "members": B, "precision": args.precision, "sum_order": args.sum_order if args.precision == "fp64" else
            "tile", "cg2d_iterations_mean": float(iters.mean()), "cg2d_iterations_max": int(iters.max()),
-           "refinements_mean": float(outer.mean()), "all_converged": bool((conv == 1).all())}
+           "refinements_mean": float(outer.mean()), "all_converged": bool((conv == 1).all()),
+           "launch": {k: v for k, v in sel.__dict__.items() if k != "precision"}, "autotune": tuned}
     out = {
         "metric": "member-steps/s, cg2d+DST3 at LLC90; mixed-refined vs FP64 at equal residual",
         "value": B * args.steps / run["el"],
@@ -285,19 +279,20 @@ def main():
                  "algorithmic_bytes": gad_bytes},
     }
     if args.precision == "mixed" and not args.no_fp64_reference:
-        ref = timed("fp64")
-        xr = torch.empty_like(x)
-        solve("fp64", xr)
-        solve("mixed", x)
-        torch.cuda.synchronize()
+        ref_step, ref_tuned = make("fp64")
+        ref = timed(ref_step)
         inner = cg2d.interior(E["ex"].L, dev)
+        x, xr = main_step.x, ref_step.x
         diff = torch.where(inner, (x - xr).abs(), 0.0).amax(dim=(1, 2, 3))
         scale = torch.where(inner, xr.abs(), 0.0).amax(dim=(1, 2, 3))
         rel = (diff / scale).cpu().numpy()
         fp64_value = B * args.steps / ref["el"]
         out["fp64"] = {"value": fp64_value, "ms_per_step": ref["el"] / args.steps * 1e3, "solve_ms": ref["cg_ms"],
-                       "cg2d_iterations_mean": float(ref["d"]["numIters"].float().mean())}
+                       "cg2d_iterations_mean": float(ref["d"]["numIters"].float().mean()),
+                       "launch": {k: v for k, v in ref_step.cfg.__dict__.items() if k != "precision"},
+                       "autotune": ref_tuned}
         out["mixed_over_fp64"] = out["value"] / fp64_value
+        out["mixed_over_fp64_solve"] = ref["cg_ms"] / run["cg_ms"]
         out["error_vs_fp64"] = {"max_rel": float(rel.max()), "mean_rel": float(rel.mean()),
                                 "measure": "per member max|x_mixed - x_fp64| / max|x_fp64| over the interior"}
     if not args.no_cpu_baseline:

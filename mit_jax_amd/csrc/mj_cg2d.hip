@@ -57,6 +57,8 @@ struct CgDev {
     double* prod;                              // MJ_SUM_FORTRAN staging: [B*T][2][sNy*sNx]
     int* nactive;                              // members not yet finished
     int32_t* mdone;                            // [B] iterations at which a member finished, -1 while active
+    int32_t* grp;                              // [B][kGroup] member groups (build_groups_kernel), -1 = empty
+    int* ngroups;
     mj_cg2d_diag* diag;
 };
 
@@ -65,10 +67,10 @@ struct CgDev {
 #endif  // largest block of the persistent launch
 #ifndef MJ_PH_THREADS
 #define MJ_PH_THREADS 256
-#endif  // largest block of the phase launches
+#endif  // block of the per-member phase launches (I0-I2); the group phases use MJ_PH_THREADS * kGroup
 #ifndef MJ_PH_WAVES_PER_EU
 #define MJ_PH_WAVES_PER_EU 8
-#endif  // occupancy target of the phase launches (waves per SIMD): bounds their register use
+#endif  // occupancy target of the phase launches (waves per SIMD): bounds their register use (a group's rows)
 __device__ __forceinline__ int64_t op_base(const CgDev& a, int m) {
     return (int64_t)(a.op_index ? a.op_index[m] : m) * a.oms;
 }
@@ -148,29 +150,8 @@ __device__ __forceinline__ void unit_I2(const CgDev& a, int m, int t) {
     }
 }
 
-// Status of member m at the start of B(n), the same in every block: 0 finished earlier, 1 finishing now
-// (stopping rule on err(n-1), cg2d.F:245-250), 2 active. Thread 0 computes it; the block reads g_ibcast, and for
-// status 1 g_bcast[0] = err(n-1).
-__device__ __forceinline__ int status_B(const CgDev& a, int m, int n) {
-    if (threadIdx.x == 0) {
-        int code = 0;
-        const int st = a.mdone[m];
-        if (!(st >= 0 && st < n - 1)) {
-            const double errp = tile_order_sum(a.perr + ((n - 1) & 1) * a.B * a.T + m * a.T, a.T);
-            code = (!(n - 1 < a.max_iters) || errp < a.tol_sq) ? 1 : 2;
-            g_bcast[0] = errp;
-            if (code == 2) {
-                const double* pe = a.peta + m * a.T;
-                const double eta = tile_order_sum(pe + (n & 1) * a.B * a.T, a.T);
-                const double etaNM1 = n == 1 ? 1.0 : tile_order_sum(pe + ((n - 1) & 1) * a.B * a.T, a.T);
-                g_bcast[1] = eta / etaNM1;  // beta (cg2d.F:268-271)
-            }
-        }
-        g_ibcast = code;
-    }
-    __syncthreads();
-    const int code = g_ibcast;
-    return code;
+__device__ __forceinline__ SweepGeo geo_of(const CgDev& a, int t) {
+    return SweepGeo{t, t * a.plane, a.sNx, a.sNy, a.OL, a.nx, a.ring_off, a.ring_dst, a.ring_src, a.ring_sign};
 }
 
 // Finishing member at B(n): un-normalise x on the tile (cg2d.F:380-395); tile 0 completes the diag.
@@ -191,163 +172,103 @@ __device__ __forceinline__ void unit_finish(const CgDev& a, int m, int t, int n,
     }
 }
 
-// Phase B(n) of an active member: s_n = M r_{n-1} + beta s_{n-1} on interior and ring (ring from its interior
-// sources), then the tile partial of sigma = sum s_n (A s_n).
+// Phase B(n) of a group (tile t): per member the status at the start of B(n), the same in every block — finished
+// earlier, finishing now (stopping rule on err(n-1), cg2d.F:245-250: x un-normalised), or active; the active members
+// then sweep together: s_n = M r_{n-1} + beta s_{n-1} on interior and ring, partial of sigma = sum s_n (A s_n).
 template <int MODE>
-__device__ __forceinline__ void unit_B(const CgDev& a, int m, int t, int n, double beta) {
-    MJ_UNIT_IDS
-    MJ_STRIP
-    const int64_t ob = op_base(a, m);
-    const double *__restrict__ pW = a.pW + ob, *__restrict__ pS = a.pS + ob, *__restrict__ pC = a.pC + ob;
-    const double *__restrict__ aW = a.aW + ob, *__restrict__ aS = a.aS + ob, *__restrict__ aC = a.aC + ob;
-    const double* __restrict__ r = R(a, n - 1) + m * a.ms;
-    const double* __restrict__ so = S(a, n - 1) + m * a.ms;
-    double* sn = S(a, n) + m * a.ms;
-    double* stage = stage_of(a, m, t);
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        double2 Rm = ld2(r + p - nx), Rc = ld2(r + p), PS0 = ld2(pS + p);
-        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const double2 Rp = ld2(r + p + nx), PS1 = ld2(pS + p + nx);
-            const double2 PC = ld2(pC + p), PW = ld2(pW + p), SO = ld2(so + p);
-            const double hr = halo_of(r, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
-            const double2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
-                                     right_of(Rc, hr, L));
-            double2 v;
-            v.x = q.x + beta * SO.x;
-            v.y = q.y + beta * SO.y;
-            if (L.act) st2(sn + p, v);
-            Rm = Rc;
-            Rc = Rp;
-            PS0 = PS1;
-        }
-    }
-    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const int s = a.ring_src[k];
-        const double v = op_M(pW, pS, pC, r, s, nx) + beta * so[s];
-        sn[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
-    }
-    __syncthreads();
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        double2 Xm = ld2(sn + p - nx), Xc = ld2(sn + p), AS0 = ld2(aS + p);
-        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const double2 Xp = ld2(sn + p + nx), AS1 = ld2(aS + p + nx);
-            const double2 AW = ld2(aW + p), AC = ld2(aC + p);
-            const double hx = halo_of(sn, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
-            const double2 q = pair_A(AW, right_of(AW, hw, L), AS0, AS1, AC, Xm, Xc, Xp, left_of(Xc, hx, L),
-                                     right_of(Xc, hx, L));
-            double2 v;
-            v.x = Xc.x * q.x;
-            v.y = Xc.y * q.y;
-            if (!L.act) v = make_double2(0.0, 0.0);
-            put_row2<MODE>(0, j, L, v, stage, sNx, sNy);
-            Xm = Xc;
-            Xc = Xp;
-            AS0 = AS1;
-        }
-    }
-    store_tile_sums<MODE>(a.psig + (n & 1) * a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, stage, sNx, sNy);
-}
-
-// Phase C(n) of an active member: x += alpha s_n; r_n = r_{n-1} - alpha A s_n on interior and ring; then the tile
-// partials of err(n) = sum r_n^2 and eta(n+1) = sum (M r_n) r_n.
-template <int MODE>
-__device__ __forceinline__ void unit_C(const CgDev& a, int m, int t, int n, double alpha) {
-    MJ_UNIT_IDS
-    MJ_STRIP
-    const int64_t ob = op_base(a, m);
-    const double *__restrict__ pW = a.pW + ob, *__restrict__ pS = a.pS + ob, *__restrict__ pC = a.pC + ob;
-    const double *__restrict__ aW = a.aW + ob, *__restrict__ aS = a.aS + ob, *__restrict__ aC = a.aC + ob;
-    const double* __restrict__ sv = S(a, n) + m * a.ms;
-    const double* __restrict__ rin = R(a, n - 1) + m * a.ms;
-    double* rout = R(a, n) + m * a.ms;
-    double* x = a.x + m * a.ms;
-    double* stage = stage_of(a, m, t);
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        double2 Xm = ld2(sv + p - nx), Xc = ld2(sv + p), AS0 = ld2(aS + p);
-        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const double2 Xp = ld2(sv + p + nx), AS1 = ld2(aS + p + nx);
-            const double2 AW = ld2(aW + p), AC = ld2(aC + p), RI = ld2(rin + p), XV = ld2(x + p);
-            const double hx = halo_of(sv, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
-            const double2 q = pair_A(AW, right_of(AW, hw, L), AS0, AS1, AC, Xm, Xc, Xp, left_of(Xc, hx, L),
-                                     right_of(Xc, hx, L));
-            double2 xn, rn;
-            xn.x = XV.x + alpha * Xc.x;
-            xn.y = XV.y + alpha * Xc.y;
-            rn.x = RI.x - alpha * q.x;
-            rn.y = RI.y - alpha * q.y;
-            if (L.act) {
-                st2(x + p, xn);
-                st2(rout + p, rn);
-            }
-            Xm = Xc;
-            Xc = Xp;
-            AS0 = AS1;
-        }
-    }
-    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const int s = a.ring_src[k];
-        const double v = rin[s] - alpha * op_A(aW, aS, aC, sv, s, nx);
-        rout[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
-    }
-    __syncthreads();
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        double2 Rm = ld2(rout + p - nx), Rc = ld2(rout + p), PS0 = ld2(pS + p);
-        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const double2 Rp = ld2(rout + p + nx), PS1 = ld2(pS + p + nx);
-            const double2 PC = ld2(pC + p), PW = ld2(pW + p);
-            const double hr = halo_of(rout, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
-            const double2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
-                                     right_of(Rc, hr, L));
-            double2 e, h;
-            e.x = Rc.x * Rc.x;
-            e.y = Rc.y * Rc.y;
-            h.x = q.x * Rc.x;
-            h.y = q.y * Rc.y;
-            if (!L.act) {
-                e = make_double2(0.0, 0.0);
-                h = e;
-            }
-            put_row2<MODE>(0, j, L, e, stage, sNx, sNy);
-            put_row2<MODE>(1, j, L, h, stage, sNx, sNy);
-            Rm = Rc;
-            Rc = Rp;
-            PS0 = PS1;
-        }
-    }
-    const int BT = a.B * a.T;
-    store_tile_sums<MODE>(a.perr + (n & 1) * BT, a.peta + ((n + 1) & 1) * BT, m * a.T + t, lane, wave, nwaves, stage,
-                          sNx, sNy);
-}
-
-// Whole phases of one unit (shared by both launch shapes).
-template <int MODE>
-__device__ __forceinline__ void unit_phase_B(const CgDev& a, int m, int t, int n) {
-    const int code = status_B(a, m, n);
-    if (code == 1) unit_finish(a, m, t, n, g_bcast[0]);
-    if (code == 2) unit_B<MODE>(a, m, t, n, g_bcast[1]);
-    __syncthreads();
-}
-template <int MODE>
-__device__ __forceinline__ void unit_phase_C(const CgDev& a, int m, int t, int n) {
-    if (threadIdx.x == 0) {
+__device__ __forceinline__ void group_phase_B(const CgDev& a, int gi, int t, int n) {
+    __shared__ int s_m[kGroup], s_code[kGroup], s_act[kGroup], s_nm;
+    __shared__ double s_beta[kGroup], s_err[kGroup];
+    const int tid = threadIdx.x, BT = a.B * a.T;
+    if (tid < kGroup) {
+        const int m = a.grp[gi * kGroup + tid];
         int code = 0;
-        if (a.mdone[m] < 0) {
-            const double* pe = a.peta + (n & 1) * a.B * a.T + m * a.T;
-            g_bcast[0] = tile_order_sum(pe, a.T) / tile_order_sum(a.psig + (n & 1) * a.B * a.T + m * a.T, a.T);
-            code = 1;
+        double beta = 0.0, errp = 0.0;
+        if (m >= 0) {
+            const int st = a.mdone[m];
+            if (!(st >= 0 && st < n - 1)) {
+                errp = tile_order_sum(a.perr + ((n - 1) & 1) * BT + m * a.T, a.T);
+                code = (!(n - 1 < a.max_iters) || errp < a.tol_sq) ? 1 : 2;
+                if (code == 2) {
+                    const double* pe = a.peta + m * a.T;
+                    const double eta = tile_order_sum(pe + (n & 1) * BT, a.T);
+                    const double etaNM1 = n == 1 ? 1.0 : tile_order_sum(pe + ((n - 1) & 1) * BT, a.T);
+                    beta = eta / etaNM1;  // cg2d.F:268-271
+                }
+            }
         }
-        g_ibcast = code;
+        s_m[tid] = m;
+        s_code[tid] = code;
+        s_beta[tid] = beta;
+        s_err[tid] = errp;
     }
     __syncthreads();
-    if (g_ibcast) unit_C<MODE>(a, m, t, n, g_bcast[0]);
+    if (tid == 0) {
+        int c = 0;
+        for (int k = 0; k < kGroup; ++k)
+            if (s_code[k] == 2) s_act[c++] = k;
+        s_nm = c;
+    }
+    __syncthreads();
+    for (int k = 0; k < kGroup; ++k)
+        if (s_code[k] == 1) unit_finish(a, s_m[k], t, n, s_err[k]);
+    const int nm = s_nm;
+    if (nm > 0) {
+        MemberB<double> mb[kGroup];
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k) {
+            const int m = s_m[s_act[k < nm ? k : 0]];
+            mb[k] = MemberB<double>{R(a, n - 1) + m * a.ms, S(a, n - 1) + m * a.ms, S(a, n) + m * a.ms,
+                                    s_beta[s_act[k < nm ? k : 0]], false, a.psig + (n & 1) * BT + m * a.T + t,
+                                    stage_of(a, m, t)};
+        }
+        const int64_t ob = op_base(a, s_m[s_act[0]]);
+        group_B<double, kGroup, MODE>(geo_of(a, t), a.pW + ob, a.pS + ob, a.pC + ob, a.aW + ob, a.aS + ob, a.aC + ob,
+                                      mb, nm);
+    }
+    __syncthreads();
+}
+
+// Phase C(n) of a group (tile t): the members still active sweep together: x += alpha s_n;
+// r_n = r_{n-1} - alpha A s_n on interior and ring; partials of err(n) = sum r_n^2 and eta(n+1) = sum (M r_n) r_n.
+template <int MODE>
+__device__ __forceinline__ void group_phase_C(const CgDev& a, int gi, int t, int n) {
+    __shared__ int s_m[kGroup], s_act[kGroup], s_nm;
+    __shared__ double s_alpha[kGroup];
+    const int tid = threadIdx.x, BT = a.B * a.T;
+    if (tid < kGroup) {
+        const int m = a.grp[gi * kGroup + tid];
+        s_m[tid] = m;
+        s_alpha[tid] = 0.0;
+        if (m >= 0 && a.mdone[m] < 0) {
+            const double* pe = a.peta + (n & 1) * BT + m * a.T;
+            s_alpha[tid] = tile_order_sum(pe, a.T) / tile_order_sum(a.psig + (n & 1) * BT + m * a.T, a.T);
+        } else {
+            s_m[tid] = -1;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int c = 0;
+        for (int k = 0; k < kGroup; ++k)
+            if (s_m[k] >= 0) s_act[c++] = k;
+        s_nm = c;
+    }
+    __syncthreads();
+    const int nm = s_nm;
+    if (nm > 0) {
+        MemberC<double> mc[kGroup];
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k) {
+            const int kk = s_act[k < nm ? k : 0], m = s_m[kk];
+            mc[k] = MemberC<double>{S(a, n) + m * a.ms, R(a, n - 1) + m * a.ms, R(a, n) + m * a.ms, a.x + m * a.ms,
+                                    s_alpha[kk], a.perr + (n & 1) * BT + m * a.T + t,
+                                    a.peta + ((n + 1) & 1) * BT + m * a.T + t, stage_of(a, m, t)};
+        }
+        const int64_t ob = op_base(a, s_m[s_act[0]]);
+        group_C<double, kGroup, MODE>(geo_of(a, t), a.pW + ob, a.pS + ob, a.pC + ob, a.aW + ob, a.aS + ob, a.aC + ob,
+                                      mc, nm);
+    }
     __syncthreads();
 }
 
@@ -369,11 +290,12 @@ __global__ void __launch_bounds__(MJ_CG_THREADS) cg2d_persistent_kernel(CgDev a,
     MJ_FOR_UNITS(unit_I2<MODE>(a, m, t); __syncthreads();)
     if (!grid_sync(gs)) goto done;
     for (int n = 1;; ++n) {
-        MJ_FOR_UNITS(unit_phase_B<MODE>(a, m, t, n);)
+        const int ng = *a.ngroups, ngu = ng * a.T;
+        for (int u = blockIdx.x; u < ngu; u += gridDim.x) group_phase_B<MODE>(a, u % ng, u / ng, n);
         if (!grid_sync(gs)) goto done;
         // nactive only changes in B phases: every block reads the same value here
         if (__hip_atomic_load(a.nactive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-        MJ_FOR_UNITS(unit_phase_C<MODE>(a, m, t, n);)
+        for (int u = blockIdx.x; u < ngu; u += gridDim.x) group_phase_C<MODE>(a, u % ng, u / ng, n);
         if (!grid_sync(gs)) goto done;
     }
 #undef MJ_FOR_UNITS
@@ -389,20 +311,19 @@ done:
 enum { PH_I0 = 0, PH_I1 = 1, PH_I2 = 2, PH_B = 3, PH_C = 4 };
 
 template <int MODE, int PH>
-__global__ void __launch_bounds__(MJ_PH_THREADS) __attribute__((amdgpu_waves_per_eu(MJ_PH_WAVES_PER_EU)))
+__global__ void __launch_bounds__(MJ_PH_THREADS * kGroup) __attribute__((amdgpu_waves_per_eu(MJ_PH_WAVES_PER_EU)))
 cg2d_phase_kernel(CgDev a, int n) {
     int m, t;
-    if (!phase_unit(a.B, a.T, m, t)) return;
     if (PH == PH_B || PH == PH_C) {
-        // units of members that finished earlier leave before touching anything else
-        const int st = a.mdone[m];
-        if (st >= 0 && (PH == PH_C || st < n - 1)) return;
+        if (!group_unit(*a.ngroups, a.T, m, t)) return;  // m: group index
+        if (PH == PH_B) group_phase_B<MODE>(a, m, t, n);
+        if (PH == PH_C) group_phase_C<MODE>(a, m, t, n);
+        return;
     }
+    if (!phase_unit(a.B, a.T, m, t)) return;
     if (PH == PH_I0) unit_I0(a, m, t);
     if (PH == PH_I1) unit_I1<MODE>(a, m, t);
     if (PH == PH_I2) unit_I2<MODE>(a, m, t);
-    if (PH == PH_B) unit_phase_B<MODE>(a, m, t, n);
-    if (PH == PH_C) unit_phase_C<MODE>(a, m, t, n);
 }
 
 __global__ void cg2d_init_kernel(unsigned int* counter, int* error, int* nactive, int B) {
@@ -505,7 +426,7 @@ using namespace mj;
 namespace {
 
 struct WsLayout {
-    size_t r0, r1, s0, s1, pmax, psum, perr, peta, psig, mdone, misc, prod, total;
+    size_t r0, r1, s0, s1, pmax, psum, perr, peta, psig, mdone, grp, misc, prod, total;
 };
 
 WsLayout ws_layout(const mj_cg2d_config* c) {
@@ -524,7 +445,8 @@ WsLayout ws_layout(const mj_cg2d_config* c) {
     L.peta = o; o += 2 * bt;
     L.psig = o; o += 2 * bt;
     L.mdone = o; o += al((size_t)c->batch * 4);
-    L.misc = o; o += 256;  // barrier counter, error flag, nactive
+    L.grp = o; o += al((size_t)c->batch * kGroup * 4);
+    L.misc = o; o += 256;  // barrier counter, error flag, nactive, ngroups
     L.prod = o;
     if (c->sum_order == MJ_SUM_FORTRAN) o += al((size_t)c->batch * c->ntiles * 2 * c->sNx * c->sNy * 8);
     L.total = o;
@@ -539,7 +461,7 @@ int launch_mode_of(const mj_cg2d_config* c) {
 }
 
 int block_threads_of(const mj_cg2d_config* c, int mode) {
-    const int limit = mode == MJ_CG_LAUNCH_PERSISTENT ? MJ_CG_THREADS : MJ_PH_THREADS;
+    const int limit = mode == MJ_CG_LAUNCH_PERSISTENT ? MJ_CG_THREADS : MJ_PH_THREADS * kGroup;
     if (c->block_threads == 0) return limit;
     MJ_REQUIRE(c->block_threads % 64 == 0 && c->block_threads >= 64 && c->block_threads <= limit,
                "cg2d: block_threads must be a multiple of 64 in 64..%d for this launch mode", limit);
@@ -573,11 +495,12 @@ template <int MODE>
 int launch_phases(const CgDev& a, const mj_cg2d_config* cfg, int threads, int mode, hipStream_t st) {
     const int nunits = a.B * a.T;
     const dim3 grid((unsigned)((nunits + kXcds - 1) / kXcds * kXcds)), block((unsigned)threads);
+    const dim3 iblock((unsigned)std::min(threads, MJ_PH_THREADS));
     auto go = [&](int ph, int n) -> int {
         switch (ph) {
-            case PH_I0: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I0>), grid, block, 0, st, a, n); break;
-            case PH_I1: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I1>), grid, block, 0, st, a, n); break;
-            case PH_I2: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I2>), grid, block, 0, st, a, n); break;
+            case PH_I0: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I0>), grid, iblock, 0, st, a, n); break;
+            case PH_I1: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I1>), grid, iblock, 0, st, a, n); break;
+            case PH_I2: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I2>), grid, iblock, 0, st, a, n); break;
             case PH_B: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_B>), grid, block, 0, st, a, n); break;
             default: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_C>), grid, block, 0, st, a, n); break;
         }
@@ -681,10 +604,14 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     unsigned int* counter = (unsigned int*)(w + L.misc);
     int* error = (int*)(w + L.misc + 64);
     a.nactive = (int*)(w + L.misc + 128);
+    a.ngroups = (int*)(w + L.misc + 192);
+    a.grp = (int32_t*)(w + L.grp);
     a.diag = diag;
 
     hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(cg2d_init_kernel, dim3(1), dim3(64), 0, st, counter, error, a.nactive, a.B);
+    MJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(build_groups_kernel, dim3(1), dim3(1024), 0, st, a.op_index, a.oms, a.B, a.grp, a.ngroups);
     MJ_HIP(hipGetLastError());
     if (mode != MJ_CG_LAUNCH_PERSISTENT)
         return cfg->sum_order == MJ_SUM_FORTRAN ? launch_phases<MJ_SUM_FORTRAN>(a, cfg, threads, mode, st)

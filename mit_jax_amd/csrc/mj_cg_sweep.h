@@ -54,8 +54,16 @@ __device__ __forceinline__ T op_M(const T* pW, const T* pS, const T* pC, const T
     (void)nx;                                                                                             \
     (void)tbase;
 
-// Per-block LDS: row sums of two reductions, and values broadcast from thread 0.
-static __shared__ double g_rows[2 * 128];
+#ifndef MJ_GROUP
+#define MJ_GROUP 1
+#endif
+// Members that share an operator slot swept together by one block (each coefficient row loaded once for all of
+// them). Measured on MI355X at 128 members the per-member blocks (1) are fastest: a group of 4 cuts the bytes the
+// sweeps request per member by a third but lengthens each wave's dependent row chain more than that saves.
+constexpr int kGroup = MJ_GROUP;
+
+// Per-block LDS: row sums of two reductions per group member (slot r + 2k), values broadcast from thread 0.
+static __shared__ double g_rows[2 * kGroup * 128];
 static __shared__ double g_bcast[4];
 static __shared__ int g_ibcast;
 
@@ -74,11 +82,11 @@ __device__ __forceinline__ void put_row(int r, int jl, int lane, double v0, doub
     }
 }
 
-// After a __syncthreads: tile partial of reduction r (valid in lane 0 of the calling wave).
+// After a __syncthreads: tile partial of reduction r of group member k (valid in lane 0 of the calling wave).
 template <int MODE>
-__device__ __forceinline__ double tile_of(int r, int lane, const double* stage, int sNx, int sNy) {
+__device__ __forceinline__ double tile_of(int r, int lane, const double* stage, int sNx, int sNy, int k = 0) {
     if (MODE == MJ_SUM_TREE) {
-        const double* rb = g_rows + r * 128;
+        const double* rb = g_rows + (r + 2 * k) * 128;
         const bool h = lane + 64 < sNy;
         return wave_tree(rb[lane], h ? rb[lane + 64] : 0.0, h);
     }
@@ -166,14 +174,15 @@ __device__ __forceinline__ P2<T> pair_M(P2<T> PC, P2<T> PW, T pwr, P2<T> PS0, P2
 // One row's per-point values of reduction r held as column pairs: tree order regroups them onto the canonical
 // lanes (lane l: v[l] and v[l + 64]) and reduces with wave_tree into the row sums; fortran order stages them.
 template <int MODE>
-__device__ __forceinline__ void put_row2(int r, int jl, const Lanes& L, double2 v, double* stage, int sNx, int sNy) {
+__device__ __forceinline__ void put_row2(int r, int jl, const Lanes& L, double2 v, double* stage, int sNx, int sNy,
+                                         int k = 0) {
     if (MODE == MJ_SUM_TREE) {
         const int l = L.lane, s0 = l >> 1, s1 = 32 + (l >> 1);
         const double a0 = __shfl(v.x, s0, 64), a1 = __shfl(v.y, s0, 64);
         const double b0 = __shfl(v.x, s1, 64), b1 = __shfl(v.y, s1, 64);
         const double lo = (l & 1) ? a1 : a0, hi = (l & 1) ? b1 : b0;
         const double w = wave_tree(lo, hi, l + 64 < sNx);
-        if (l == 0) g_rows[r * 128 + jl] = w;
+        if (l == 0) g_rows[(r + 2 * k) * 128 + jl] = w;
     } else {
         if (L.act) st2(stage + ((size_t)r * sNy + jl) * sNx + L.c, v);
     }
@@ -252,6 +261,301 @@ __device__ __forceinline__ void first_guess(const D& a, int m, int t, double rhs
     __syncthreads();
     exchanged_first_guess(a, m, t, rhsNorm);
     __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------ grouped sweeps
+// The two CG phases of up to G members that read the same operator slot, swept by one block: each coefficient row
+// is loaded once and applied to every member's rows. Per member the arithmetic is exactly the single-member one
+// (same expressions, same term order, same reduction trees), so results do not depend on the grouping.
+struct SweepGeo {
+    int t, tbase, sNx, sNy, OL, nx;
+    const int32_t *ring_off, *ring_dst, *ring_src;
+    const int8_t* ring_sign;
+};
+// Phase B of one member: s = z + beta s_old (z = M r; first: s = z), partial of sigma = (s, A s).
+template <typename T>
+struct MemberB {
+    const T* r;
+    const T* so;
+    T* sn;
+    T beta;
+    bool first;
+    double* sig;    // where the tile partial goes
+    double* stage;  // MJ_SUM_FORTRAN staging
+};
+// Phase C of one member: xs += alpha s; r = r_old - alpha A s; partials of err = (r, r) and eta = (M r, r).
+template <typename T>
+struct MemberC {
+    const T* sv;
+    const T* rin;
+    T* rout;
+    T* xs;
+    T alpha;
+    double* err;
+    double* eta;
+    double* stage;
+};
+
+template <typename T, int G, int MODE>
+__device__ __forceinline__ void group_B(const SweepGeo& g, const T* __restrict__ pW, const T* __restrict__ pS,
+                                        const T* __restrict__ pC, const T* __restrict__ aW, const T* __restrict__ aS,
+                                        const T* __restrict__ aC, const MemberB<T> (&mb)[G], int nm) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int sNx = g.sNx, sNy = g.sNy, OL = g.OL, nx = g.nx, tbase = g.tbase;
+    MJ_STRIP
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        P2<T> Rm[G], Rc[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (k < nm) {
+                Rm[k] = ld2(mb[k].r + p - nx);
+                Rc[k] = ld2(mb[k].r + p);
+            }
+        P2<T> PS0 = ld2(pS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const P2<T> PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
+            const T pwr = right_of(PW, halo_of(pW, prow, L, sNx), L);
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                if (k < nm) {
+                    const P2<T> Rp = ld2(mb[k].r + p + nx);
+                    const T hr = halo_of(mb[k].r, prow, L, sNx);
+                    const P2<T> q = pair_M(PC, PW, pwr, PS0, PS1, Rm[k], Rc[k], Rp, left_of(Rc[k], hr, L),
+                                           right_of(Rc[k], hr, L));
+                    P2<T> v = q;
+                    if (!mb[k].first) {
+                        const P2<T> SO = ld2(mb[k].so + p);
+                        v.x = q.x + mb[k].beta * SO.x;
+                        v.y = q.y + mb[k].beta * SO.y;
+                    }
+                    if (L.act) st2(mb[k].sn + p, v);
+                    Rm[k] = Rc[k];
+                    Rc[k] = Rp;
+                }
+            }
+            PS0 = PS1;
+        }
+    }
+    const int rk0 = g.ring_off[g.t], rk1 = g.ring_off[g.t + 1];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k < nm) {
+            for (int kk = rk0 + tid; kk < rk1; kk += blockDim.x) {
+                const int s = g.ring_src[kk];
+                const T q = op_M(pW, pS, pC, mb[k].r, s, nx);
+                const T v = mb[k].first ? q : q + mb[k].beta * mb[k].so[s];
+                mb[k].sn[g.ring_dst[kk]] = g.ring_sign[kk] < 0 ? -v : v;
+            }
+        }
+    }
+    __syncthreads();
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        P2<T> Xm[G], Xc[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (k < nm) {
+                Xm[k] = ld2(mb[k].sn + p - nx);
+                Xc[k] = ld2(mb[k].sn + p);
+            }
+        P2<T> AS0 = ld2(aS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const P2<T> AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
+            const T awr = right_of(AW, halo_of(aW, prow, L, sNx), L);
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                if (k < nm) {
+                    const P2<T> Xp = ld2(mb[k].sn + p + nx);
+                    const T hx = halo_of(mb[k].sn, prow, L, sNx);
+                    const P2<T> q = pair_A(AW, awr, AS0, AS1, AC, Xm[k], Xc[k], Xp, left_of(Xc[k], hx, L),
+                                           right_of(Xc[k], hx, L));
+                    double2 v = make_double2((double)Xc[k].x * (double)q.x, (double)Xc[k].y * (double)q.y);
+                    if (!L.act) v = make_double2(0.0, 0.0);
+                    put_row2<MODE>(0, j, L, v, mb[k].stage, sNx, sNy, k);
+                    Xm[k] = Xc[k];
+                    Xc[k] = Xp;
+                }
+            }
+            AS0 = AS1;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k < nm && k % nwaves == wave) {
+            const double v = tile_of<MODE>(0, lane, mb[k].stage, sNx, sNy, k);
+            if (lane == 0) *mb[k].sig = v;
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int G, int MODE>
+__device__ __forceinline__ void group_C(const SweepGeo& g, const T* __restrict__ pW, const T* __restrict__ pS,
+                                        const T* __restrict__ pC, const T* __restrict__ aW, const T* __restrict__ aS,
+                                        const T* __restrict__ aC, const MemberC<T> (&mc)[G], int nm) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int sNx = g.sNx, sNy = g.sNy, OL = g.OL, nx = g.nx, tbase = g.tbase;
+    MJ_STRIP
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        P2<T> Xm[G], Xc[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (k < nm) {
+                Xm[k] = ld2(mc[k].sv + p - nx);
+                Xc[k] = ld2(mc[k].sv + p);
+            }
+        P2<T> AS0 = ld2(aS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const P2<T> AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
+            const T awr = right_of(AW, halo_of(aW, prow, L, sNx), L);
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                if (k < nm) {
+                    const P2<T> Xp = ld2(mc[k].sv + p + nx), RI = ld2(mc[k].rin + p), XV = ld2(mc[k].xs + p);
+                    const T hx = halo_of(mc[k].sv, prow, L, sNx);
+                    const P2<T> q = pair_A(AW, awr, AS0, AS1, AC, Xm[k], Xc[k], Xp, left_of(Xc[k], hx, L),
+                                           right_of(Xc[k], hx, L));
+                    const T alpha = mc[k].alpha;
+                    P2<T> xn, rn;
+                    xn.x = XV.x + alpha * Xc[k].x;
+                    xn.y = XV.y + alpha * Xc[k].y;
+                    rn.x = RI.x - alpha * q.x;
+                    rn.y = RI.y - alpha * q.y;
+                    if (L.act) {
+                        st2(mc[k].xs + p, xn);
+                        st2(mc[k].rout + p, rn);
+                    }
+                    Xm[k] = Xc[k];
+                    Xc[k] = Xp;
+                }
+            }
+            AS0 = AS1;
+        }
+    }
+    const int rk0 = g.ring_off[g.t], rk1 = g.ring_off[g.t + 1];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k < nm) {
+            for (int kk = rk0 + tid; kk < rk1; kk += blockDim.x) {
+                const int s = g.ring_src[kk];
+                const T v = mc[k].rin[s] - mc[k].alpha * op_A(aW, aS, aC, mc[k].sv, s, nx);
+                mc[k].rout[g.ring_dst[kk]] = g.ring_sign[kk] < 0 ? -v : v;
+            }
+        }
+    }
+    __syncthreads();
+    if (j0 < j1) {
+        int p = MJ_ROW(j0) + L.c;
+        P2<T> Rm[G], Rc[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (k < nm) {
+                Rm[k] = ld2(mc[k].rout + p - nx);
+                Rc[k] = ld2(mc[k].rout + p);
+            }
+        P2<T> PS0 = ld2(pS + p);
+        for (int j = j0; j < j1; ++j, p += nx) {
+            const int prow = MJ_ROW(j);
+            const P2<T> PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
+            const T pwr = right_of(PW, halo_of(pW, prow, L, sNx), L);
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                if (k < nm) {
+                    const P2<T> Rp = ld2(mc[k].rout + p + nx);
+                    const T hr = halo_of(mc[k].rout, prow, L, sNx);
+                    const P2<T> q = pair_M(PC, PW, pwr, PS0, PS1, Rm[k], Rc[k], Rp, left_of(Rc[k], hr, L),
+                                           right_of(Rc[k], hr, L));
+                    double2 e = make_double2((double)Rc[k].x * (double)Rc[k].x, (double)Rc[k].y * (double)Rc[k].y);
+                    double2 h = make_double2((double)q.x * (double)Rc[k].x, (double)q.y * (double)Rc[k].y);
+                    if (!L.act) {
+                        e = make_double2(0.0, 0.0);
+                        h = e;
+                    }
+                    put_row2<MODE>(0, j, L, e, mc[k].stage, sNx, sNy, k);
+                    put_row2<MODE>(1, j, L, h, mc[k].stage, sNx, sNy, k);
+                    Rm[k] = Rc[k];
+                    Rc[k] = Rp;
+                }
+            }
+            PS0 = PS1;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (k < nm && (2 * k + r) % nwaves == wave) {
+                const double v = tile_of<MODE>(r, lane, mc[k].stage, sNx, sNy, k);
+                if (lane == 0) *(r == 0 ? mc[k].err : mc[k].eta) = v;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------ groups
+// Member groups: members of one operator slot, in member order, cut into groups of up to kGroup; groups are
+// numbered by their first member. grp[g * kGroup + k] = member or -1; *ngroups = number of groups. One block.
+// slot(m) = 0 for a shared operator, op_index[m] with slots, m without (every member its own group).
+constexpr int kGroupBuildMax = 4096;  // larger batches get one group per member
+static __global__ void __launch_bounds__(1024) build_groups_kernel(const int32_t* op_index, int64_t oms, int B,
+                                                            int32_t* grp, int* ngroups) {
+    __shared__ int s_slot[kGroupBuildMax];
+    __shared__ short s_rank[kGroupBuildMax];
+    __shared__ int s_n;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int q = tid; q < B * kGroup; q += nt) grp[q] = -1;
+    if (B > kGroupBuildMax) {
+        __syncthreads();
+        for (int m = tid; m < B; m += nt) grp[m * kGroup] = m;
+        if (tid == 0) *ngroups = B;
+        return;
+    }
+    for (int m = tid; m < B; m += nt) s_slot[m] = oms == 0 ? 0 : (op_index ? op_index[m] : m);
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+    for (int m = tid; m < B; m += nt) {
+        int rank = 0;
+        for (int q = 0; q < m; ++q) rank += s_slot[q] == s_slot[m];
+        s_rank[m] = (short)rank;
+    }
+    __syncthreads();
+    for (int m = tid; m < B; m += nt) {
+        const int want = s_rank[m] - s_rank[m] % kGroup;  // rank of the group's first member
+        int lead = m;
+        for (int q = m; q >= 0; --q)
+            if (s_slot[q] == s_slot[m] && s_rank[q] == want) {
+                lead = q;
+                break;
+            }
+        int gid = 0;  // groups whose first member precedes lead
+        for (int q = 0; q < lead; ++q) gid += s_rank[q] % kGroup == 0;
+        grp[gid * kGroup + s_rank[m] % kGroup] = m;
+        if (s_rank[m] % kGroup == 0) atomicAdd(&s_n, 1);
+    }
+    __syncthreads();
+    if (tid == 0) *ngroups = s_n;
+}
+
+// Unit (tile t, group gi) of a grouped phase launch of gridDim.x >= ngroups * T blocks (a multiple of kXcds):
+// the ngroups * T units, tile-major, are cut into kXcds contiguous ranges, one per XCD (block b runs on XCD
+// b % kXcds). Returns false for the blocks beyond.
+__device__ __forceinline__ bool group_unit(int ngroups, int T, int& gi, int& t) {
+    const int n = ngroups * T;
+    const int chunk = (n + kXcds - 1) / kXcds;
+    const int pos = blockIdx.x / kXcds;
+    const int u = (blockIdx.x % kXcds) * chunk + pos;
+    if (pos >= chunk || u >= n) return false;
+    t = u / ngroups;
+    gi = u - t * ngroups;
+    return true;
 }
 
 // Unit of a phase launch: gridDim.x is a multiple of kXcds; block b runs on XCD b % kXcds and takes unit
