@@ -132,9 +132,50 @@ __device__ __forceinline__ double dst3_flux(double tr_, double vel_cfl_abs, doub
     return 0.5 * (tr_ + fabs(tr_)) * (t_m1 + (d0 * Rj + d1 * Rjm)) + 0.5 * (tr_ - fabs(tr_)) * (t_0 - (d0 * Rj + d1 * Rjp));
 }
 
+// maskLocW / maskLocS of one (member, tile, level) as LDS bit planes, built once per block from mask_loc (the masks
+// are 0/1 by construction, hFac != 0, so a bit and its double 0.0/1.0 carry the same value).
+constexpr int kMaskWords = (GPLANE + 63) / 64;
+__device__ __forceinline__ void build_mask_bits(const GadDev& a, const MjTilePlan& tp, const double* mW,
+                                                const double* mS, unsigned long long* bw, unsigned long long* bs) {
+    const int lane = threadIdx.x & 63;
+    for (int q0 = threadIdx.x - lane; q0 < kMaskWords * 64; q0 += blockDim.x) {  // wave-uniform
+        const int q = q0 + lane;
+        bool w = false, sm = false;
+        if (q < GPLANE) {
+            const int j = q / GNX, i = q - j * GNX;
+            w = mask_loc(a, tp, mW, mS, j, i, true) != 0.0;
+            sm = mask_loc(a, tp, mW, mS, j, i, false) != 0.0;
+        }
+        const unsigned long long vw = __ballot(w), vs = __ballot(sm);
+        if (lane == 0) {
+            bw[q0 >> 6] = vw;
+            bs[q0 >> 6] = vs;
+        }
+    }
+    __syncthreads();
+}
+__device__ __forceinline__ double mbit(const unsigned long long* b, int q) {
+    return (double)((b[q >> 6] >> (q & 63)) & 1ull);
+}
+
+// The interface fluxes a sweep's update reads: faces q and q + d (d = 1 in X, GNX in Y) of every box point.
+template <class F>
+__device__ __forceinline__ void flux_boxes(const MjSweep& S, int dir, double* af, F&& flux) {
+    for (int bx = 0; bx < S.nbox; ++bx) {
+        const MjBox B = S.box[bx];
+        const int i1 = B.i1 + (dir == 0 ? 1 : 0), j1 = B.j1 + (dir == 1 ? 1 : 0);
+        const int bw = i1 - B.i0, n = bw * (j1 - B.j0);
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+            const int q = (B.j0 + idx / bw) * GNX + B.i0 + idx % bw;
+            af[q] = flux(q);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
     __shared__ double loc[GPLANE];
     __shared__ double af[GPLANE];
+    __shared__ unsigned long long mlw[kMaskWords], mls[kMaskWords];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int k = blockIdx.x % a.K;
     const int mt = blockIdx.x / a.K;
@@ -160,13 +201,27 @@ __global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
     const double oneSixth = a.oneSixth;
 
     for (int q = tid; q < GPLANE; q += nthr) loc[q] = tr[q];
-    __syncthreads();
+    build_mask_bits(a, tp, mW, mS, mlw, mls);
 
     // uTrans = uFld * (dyG*deepFacC*drF*hFacW) * rhoFacC (gad.py:482-485), left to right
     auto uTrans = [&](int q) { return u[q] * (dyG[q] * deepFacC * drF * hW[q]) * rhoFacC; };
     auto vTrans = [&](int q) { return v[q] * (dxG[q] * deepFacC * drF * hS[q]) * rhoFacC; };
     // update coefficient dt*recip_rhoFacC*recip_hFacC*recip_drF*recip_rA*recip_deepFac2C (gad.py:494)
     auto coef = [&](int q) { return dt * rrhoFacC * rhC[q] * rdrF * rrA[q] * rdeep2C; };
+    // interface value at q from the tile state in loc; 0 outside the Fortran flux loops
+    auto flux = [&](int dir, int q) {
+        const int j = q / GNX, i = q - j * GNX;
+        if (dir == 0) {
+            if (i < 2 || i > GNX - 2) return 0.0;  // i = 1-OLx+2 .. sNx+OLx-1 (gad_dst3_adv_x.F:80)
+            const double cfl = fabs(u[q] * dt * rdxC[q] * rdeepC);
+            return dst3_flux(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1], mbit(mlw, q - 1),
+                             mbit(mlw, q), mbit(mlw, q + 1));
+        }
+        if (j < 2 || j > GNY - 2) return 0.0;  // j = 1-OLy+2 .. sNy+OLy-1 (gad_dst3_adv_y.F:78)
+        const double cfl = fabs(v[q] * dt * rdyC[q] * rdeepC);
+        return dst3_flux(vTrans(q), cfl, oneSixth, loc[q - 2 * GNX], loc[q - GNX], loc[q], loc[q + GNX],
+                         mbit(mls, q - GNX), mbit(mls, q), mbit(mls, q + GNX));
+    };
 
     for (int pass = 0; pass < 3; ++pass) {
         for (int dir = 0; dir < 2; ++dir) {
@@ -174,26 +229,7 @@ __global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
             if (!S.run) continue;
             if (S.fill_before) corner_fill(loc, a, tp, S.fill_before, tid);
             if (S.nbox) {
-                for (int q = tid; q < GPLANE; q += nthr) {
-                    const int j = q / GNX, i = q - j * GNX;
-                    double f = 0.0;
-                    if (dir == 0) {
-                        if (i >= 2 && i <= GNX - 2) {  // i = 1-OLx+2 .. sNx+OLx-1 (gad_dst3_adv_x.F:80)
-                            const double cfl = fabs(u[q] * dt * rdxC[q] * rdeepC);
-                            f = dst3_flux(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1],
-                                          mask_loc(a, tp, mW, mS, j, i - 1, true), mask_loc(a, tp, mW, mS, j, i, true),
-                                          mask_loc(a, tp, mW, mS, j, i + 1, true));
-                        }
-                    } else {
-                        if (j >= 2 && j <= GNY - 2) {  // j = 1-OLy+2 .. sNy+OLy-1 (gad_dst3_adv_y.F:78)
-                            const double cfl = fabs(v[q] * dt * rdyC[q] * rdeepC);
-                            f = dst3_flux(vTrans(q), cfl, oneSixth, loc[q - 2 * GNX], loc[q - GNX], loc[q],
-                                          loc[q + GNX], mask_loc(a, tp, mW, mS, j - 1, i, false),
-                                          mask_loc(a, tp, mW, mS, j, i, false), mask_loc(a, tp, mW, mS, j + 1, i, false));
-                        }
-                    }
-                    af[q] = f;
-                }
+                flux_boxes(S, dir, af, [&](int q) { return flux(dir, q); });
                 __syncthreads();
             }
             if (S.fill_after) corner_fill(loc, a, tp, S.fill_after, tid);
@@ -266,6 +302,7 @@ __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const do
                                                               double* vb, double* work, int blk0) {
     __shared__ double loc[GPLANE];
     __shared__ double fb[GPLANE];
+    __shared__ unsigned long long mlw[kMaskWords], mls[kMaskWords];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int blk = blk0 + blockIdx.x;
     const int k = blk % a.K;
@@ -318,23 +355,34 @@ __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const do
         if (dir == 0) {
             if (i < 2 || i > GNX - 2) return false;
             const double cfl = fabs(u[q] * dt * rdxC[q] * rdeepC);
-            P = dst3_partials(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1],
-                              mask_loc(a, tp, mW, mS, j, i - 1, true), mask_loc(a, tp, mW, mS, j, i, true),
-                              mask_loc(a, tp, mW, mS, j, i + 1, true));
+            P = dst3_partials(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1], mbit(mlw, q - 1),
+                              mbit(mlw, q), mbit(mlw, q + 1));
         } else {
             if (j < 2 || j > GNY - 2) return false;
             const double cfl = fabs(v[q] * dt * rdyC[q] * rdeepC);
             P = dst3_partials(vTrans(q), cfl, oneSixth, loc[q - 2 * GNX], loc[q - GNX], loc[q], loc[q + GNX],
-                              mask_loc(a, tp, mW, mS, j - 1, i, false), mask_loc(a, tp, mW, mS, j, i, false),
-                              mask_loc(a, tp, mW, mS, j + 1, i, false));
+                              mbit(mls, q - GNX), mbit(mls, q), mbit(mls, q + GNX));
         }
         return true;
+    };
+    auto flux = [&](int dir, int q) {
+        const int j = q / GNX, i = q - j * GNX;
+        if (dir == 0) {
+            if (i < 2 || i > GNX - 2) return 0.0;
+            const double cfl = fabs(u[q] * dt * rdxC[q] * rdeepC);
+            return dst3_flux(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1], mbit(mlw, q - 1),
+                             mbit(mlw, q), mbit(mlw, q + 1));
+        }
+        if (j < 2 || j > GNY - 2) return 0.0;
+        const double cfl = fabs(v[q] * dt * rdyC[q] * rdeepC);
+        return dst3_flux(vTrans(q), cfl, oneSixth, loc[q - 2 * GNX], loc[q - GNX], loc[q], loc[q + GNX],
+                         mbit(mls, q - GNX), mbit(mls, q), mbit(mls, q + GNX));
     };
 
     // ---- forward replay, snapshot before each sweep
     int order[6], ns = 0;
     for (int q = tid; q < GPLANE; q += nthr) loc[q] = tr[q];
-    __syncthreads();
+    build_mask_bits(a, tp, mW, mS, mlw, mls);
     for (int pass = 0; pass < 3; ++pass) {
         for (int dir = 0; dir < 2; ++dir) {
             const MjSweep& S = tp.sweep[pass][dir];
@@ -344,26 +392,7 @@ __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const do
             order[ns++] = pass * 2 + dir;
             if (S.fill_before) corner_fill(loc, a, tp, S.fill_before, tid);
             if (S.nbox) {
-                for (int q = tid; q < GPLANE; q += nthr) {
-                    const int j = q / GNX, i = q - j * GNX;
-                    double f = 0.0;
-                    if (dir == 0) {
-                        if (i >= 2 && i <= GNX - 2) {
-                            const double cfl = fabs(u[q] * dt * rdxC[q] * rdeepC);
-                            f = dst3_flux(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1],
-                                          mask_loc(a, tp, mW, mS, j, i - 1, true), mask_loc(a, tp, mW, mS, j, i, true),
-                                          mask_loc(a, tp, mW, mS, j, i + 1, true));
-                        }
-                    } else {
-                        if (j >= 2 && j <= GNY - 2) {
-                            const double cfl = fabs(v[q] * dt * rdyC[q] * rdeepC);
-                            f = dst3_flux(vTrans(q), cfl, oneSixth, loc[q - 2 * GNX], loc[q - GNX], loc[q],
-                                          loc[q + GNX], mask_loc(a, tp, mW, mS, j - 1, i, false),
-                                          mask_loc(a, tp, mW, mS, j, i, false), mask_loc(a, tp, mW, mS, j + 1, i, false));
-                        }
-                    }
-                    fb[q] = f;
-                }
+                flux_boxes(S, dir, fb, [&](int q) { return flux(dir, q); });
                 __syncthreads();
             }
             if (S.fill_after) corner_fill(loc, a, tp, S.fill_after, tid);
