@@ -132,6 +132,22 @@ __device__ __forceinline__ double dst3_flux(double tr_, double vel_cfl_abs, doub
     return 0.5 * (tr_ + fabs(tr_)) * (t_m1 + (d0 * Rj + d1 * Rjm)) + 0.5 * (tr_ - fabs(tr_)) * (t_0 - (d0 * Rj + d1 * Rjp));
 }
 
+// Unit (tile, member, level) of a DST3 launch of gridDim.x >= B*T*K blocks (a multiple of 8): units ordered tile-major
+// (tile, member, level) and cut into 8 contiguous ranges, one per XCD (block b runs on XCD b % 8), so each XCD
+// works through about T/8 tiles and their 2-D metrics stay in its L2 across members and levels.
+__device__ __forceinline__ bool gad_unit(const GadDev& a, int& t, int& m, int& k) {
+    const int n = a.B * a.T * a.K;
+    const int chunk = (n + 7) / 8;
+    const int pos = blockIdx.x / 8;
+    const int u = (blockIdx.x % 8) * chunk + pos;
+    if (pos >= chunk || u >= n) return false;
+    k = u % a.K;
+    const int tm = u / a.K;
+    m = tm % a.B;
+    t = tm / a.B;
+    return true;
+}
+
 // maskLocW / maskLocS of one (member, tile, level) as LDS bit planes, built once per block from mask_loc (the masks
 // are 0/1 by construction, hFac != 0, so a bit and its double 0.0/1.0 carry the same value).
 constexpr int kMaskWords = (GPLANE + 63) / 64;
@@ -177,9 +193,8 @@ __global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
     __shared__ double af[GPLANE];
     __shared__ unsigned long long mlw[kMaskWords], mls[kMaskWords];
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int k = blockIdx.x % a.K;
-    const int mt = blockIdx.x / a.K;
-    const int t = mt % a.T, m = mt / a.T;
+    int t, m, k;
+    if (!gad_unit(a, t, m, k)) return;
     if (a.nlev && k >= a.nlev[m]) return;
     const MjTilePlan& tp = a.tiles[t];
     const int64_t b3 = m * a.ms3 + ((int64_t)t * a.K + k) * GPLANE;
@@ -771,7 +786,7 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
     GadDev a = gad_dev(P, g, c, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC);
     a.out = wFld ? work : gTracer;
     hipStream_t st = as_stream(stream);
-    const unsigned nblk = (unsigned)(c->batch * P->ntiles * c->nr);
+    const unsigned nblk = (unsigned)((c->batch * P->ntiles * c->nr + 7) / 8 * 8);
     hipLaunchKernelGGL(gad_hz_kernel, dim3(nblk), dim3(1024), 0, st, a);
     MJ_HIP(hipGetLastError());
     if (wFld) {
