@@ -207,8 +207,10 @@ def main():
         tuned = None
         if not args.no_autotune:
             best, results = autotune(st, precision, steps=2, warmup=1, log=log)
+            keys = ("block_threads", "inner_tolerance", "overlap", "graph")
             tuned = {"selected": {**best.config.__dict__}, "candidates": len(results),
-                     "ms": {json.dumps(r.config.__dict__, sort_keys=True): round(r.ms, 3) for r in results}}
+                     "ms_per_step": [[*(r.config.__dict__[k] for k in keys), round(r.ms, 3)] for r in results],
+                     "columns": [*keys, "ms"]}
         return st, tuned
 
     def timed(st):
@@ -232,8 +234,27 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    def kernel_times(st, reps=3):
+        """The solve alone and the DST3 step alone (eager, polled, not overlapped), HIP events on the current
+        stream: the per-kernel times behind `roofline` and `dst3` (the timed steps may overlap the two)."""
+        from dataclasses import replace as _replace
+        st.set_config(_replace(st.cfg, graph=False))
+        ts, ta = [], []
+        for _ in range(reps):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record()
+            st._solve()
+            e[1].record()
+            st._advect()
+            e[2].record()
+            torch.cuda.synchronize()
+            ts.append(e[0].elapsed_time(e[1]))
+            ta.append(e[1].elapsed_time(e[2]))
+        return float(np.median(ts)), float(np.median(ta))
+
     main_step, tuned = make(args.precision)
     run = timed(main_step)
+    run["cg_ms"], run["gad_ms"] = kernel_times(main_step)
     d = run["d"]
     iters = d["numIters"].cpu().numpy().astype(np.int64)
     outer = d["outerIters"].cpu().numpy().astype(np.int64)
@@ -281,6 +302,7 @@ def main():
     if args.precision == "mixed" and not args.no_fp64_reference:
         ref_step, ref_tuned = make("fp64")
         ref = timed(ref_step)
+        ref["cg_ms"], _ = kernel_times(ref_step)
         inner = cg2d.interior(E["ex"].L, dev)
         x, xr = main_step.x, ref_step.x
         diff = torch.where(inner, (x - xr).abs(), 0.0).amax(dim=(1, 2, 3))

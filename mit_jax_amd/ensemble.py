@@ -38,6 +38,7 @@ class StepConfig:
     max_outer: int = 40
     sum_order: str = "tile"       # fp64 only
     graph: bool = False           # replay a captured graph
+    overlap: bool = False         # run the DST3 step on a second stream, concurrently with the solve
 
 
 @dataclass
@@ -122,22 +123,48 @@ class EnsembleStep:
                                           _lib.ptr(v), None, _lib.ptr(trc), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(rhC),
                                           _lib.ptr(self.gT), None, _lib.stream_ptr()), "mj_gad_dst3")
 
+    def _side(self):
+        if getattr(self, "side", None) is None:
+            self.side = torch.cuda.Stream()  # default priority
+        return self.side
+
+    def _both(self, e1=None):
+        """Solve then DST3, or with overlap the DST3 step forked onto the side stream and joined after the solve."""
+        if self.cfg.overlap:
+            cur = torch.cuda.current_stream()
+            side = self._side()
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):
+                self._advect()
+            self._solve()
+            if e1 is not None:
+                e1.record()
+            cur.wait_stream(side)
+        else:
+            self._solve()
+            if e1 is not None:
+                e1.record()
+            self._advect()
+
     def step(self, events=None):
         """One step: the graph if one is captured and selected, else eager launches. events: optional list that
-        receives (start, after-solve, end) HIP events."""
+        receives (start, after-solve, end) HIP events (with overlap, after-solve marks the solve's own stream)."""
+        e1 = None
         if events is not None:
             e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
             e0.record()
         if self.graph is not None and self.cfg.graph:
-            self.graph[0].replay()
-            if events is not None:
-                e1.record()
-            self.graph[1].replay()
+            if len(self.graph) == 1:
+                self.graph[0].replay()
+                if e1 is not None:
+                    e1.record()
+            else:
+                self.graph[0].replay()
+                if e1 is not None:
+                    e1.record()
+                self.graph[1].replay()
         else:
-            self._solve()
-            if events is not None:
-                e1.record()
-            self._advect()
+            self._both(e1)
         if events is not None:
             e2.record()
             events.append((e0, e1, e2))
@@ -160,17 +187,22 @@ class EnsembleStep:
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):  # warm the fixed-length path once outside the capture
-            self._solve()
-            self._advect()
+            self._both()
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
-        gs, ga = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
-        with torch.cuda.graph(gs):
-            self._solve()
-        with torch.cuda.graph(ga):
-            self._advect()
+        if cfg.overlap:  # one graph with the DST3 branch forked off the solve
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._both()
+            self.graph = (g,)
+        else:
+            gs, ga = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gs):
+                self._solve()
+            with torch.cuda.graph(ga):
+                self._advect()
+            self.graph = (gs, ga)
         torch.cuda.synchronize()
-        self.graph = (gs, ga)
         self.graph_cfg = cfg
         return self.graph
 
@@ -194,18 +226,17 @@ def _time(step: EnsembleStep, steps, warmup):
 def candidates(precision, B, T=13):
     """Launch candidates for a batch of B members: persistent shapes only where the units fit one launch."""
     out = []
-    if precision == "mixed":
-        for itol in (1e-4, 1e-5):
+    for ov in (False, True):
+        if precision == "mixed":
+            for itol in (1e-4, 1e-5):
+                for bt in (128, 256):
+                    out.append(StepConfig("mixed", "phases", bt, 64, itol, overlap=ov))
+        else:
             for bt in (128, 256):
-                for poll in (32, 64):
-                    out.append(StepConfig("mixed", "phases", bt, poll, itol))
-    else:
-        for bt in (128, 256):
-            for poll in (32, 64):
-                out.append(StepConfig("fp64", "phases", bt, poll))
-        if B * T <= 512:
-            for bt in (512, 1024):
-                out.append(StepConfig("fp64", "persistent", bt))
+                out.append(StepConfig("fp64", "phases", bt, 64, overlap=ov))
+            if B * T <= 512:
+                for bt in (512, 1024):
+                    out.append(StepConfig("fp64", "persistent", bt, overlap=ov))
     return out
 
 

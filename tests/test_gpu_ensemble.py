@@ -32,13 +32,13 @@ def _step(E, cfg):
                         E["u"], E["v"], E["tr"], E["hW"], E["hS"], E["rhC"], cfg=cfg)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_graph_replays_equal_eager(dev, precision):
+@pytest.mark.parametrize("precision,overlap", [("fp64", False), ("mixed", False), ("mixed", True)])
+def test_graph_replays_equal_eager(dev, precision, overlap):
     import torch
     from mit_jax_amd import cg2d, gad, mixed
     from mit_jax_amd.ensemble import StepConfig
     E = _ensemble(dev)
-    s = _step(E, StepConfig(precision=precision))
+    s = _step(E, StepConfig(precision=precision, overlap=overlap))
     s.step()
     torch.cuda.synchronize()
     x_e, gT_e, d = s.x.clone(), s.gT.clone(), s.diag()
