@@ -18,10 +18,11 @@
 // source's owner uses, which gives the same bits as the exchanged copy. s and r are ping-pong buffers so a source
 // is never overwritten while another block reads it. Per iteration two phases:
 //   phase B(n): stopping rule on err(n-1) (finished members un-normalise x and leave); beta = eta(n)/eta(n-1);
-//               s_n = M r_{n-1} + beta s_{n-1} on interior + ring; sigma = sum s_n A s_n
-//   phase C(n): alpha = eta(n)/sigma; x += alpha s_n; r_n = r_{n-1} - alpha A s_n on interior + ring;
-//               err(n) = sum r_n^2; eta(n+1) = sum (M r_n) r_n
-// which is the reference recurrence with eta(n+1) computed one phase early (the value is the same).
+//               s_n = z + beta s_{n-1} on interior + ring; q = A s_n; sigma = sum s_n q
+//   phase C(n): alpha = eta(n)/sigma; x += alpha s_n; r_n = r_{n-1} - alpha q on interior + ring;
+//               z = M r_n; err(n) = sum r_n^2; eta(n+1) = sum z r_n
+// which is the reference recurrence with z = M r_n and eta(n+1) computed one phase early (the values are the same).
+// z and q are kept as vectors, so each phase applies one stencil and reads one coefficient set.
 //
 // Two launch shapes run the same unit functions, so their results are bitwise identical:
 //   persistent: one cooperative launch, phases separated by grid barriers (small batches: no launch gaps, no host
@@ -52,6 +53,7 @@ struct CgDev {
     const int32_t *ring_off, *ring_dst, *ring_src;
     const int8_t* ring_sign;
     double *r0, *r1, *s0, *s1;
+    double *z, *q;                             // M r and A s of the current iteration (interior points)
     double *pmax, *psum, *perr, *peta, *psig;  // partials [m * T + t]: pmax, psum [B*T]; perr, peta, psig [2][B*T]
     const int32_t* op_index;                   // operator slot per member (NULL: slot = member)
     double* prod;                              // MJ_SUM_FORTRAN staging: [B*T][2][sNy*sNx]
@@ -121,11 +123,12 @@ __device__ __forceinline__ void unit_I1(const CgDev& a, int m, int t) {
     store_tile_sums<MODE>(a.perr, a.psum, m * a.T + t, lane, wave, nwaves, stage, sNx, sNy);  // perr slot 0: err(0)
 }
 
-// I2: exchange r (ring), tile partial of eta(1) = sum (M r) r; tile 0 initialises the member's diag and status.
+// I2: exchange r (ring), z = M r, tile partial of eta(1) = sum z r; tile 0 initialises the member's diag and status.
 template <int MODE>
 __device__ __forceinline__ void unit_I2(const CgDev& a, int m, int t) {
     MJ_UNIT_IDS
     double* r0 = a.r0 + m * a.ms;
+    double* z = a.z + m * a.ms;
     double* stage = stage_of(a, m, t);
     const int64_t ob = op_base(a, m);
     for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
@@ -134,9 +137,14 @@ __device__ __forceinline__ void unit_I2(const CgDev& a, int m, int t) {
     }
     __syncthreads();
     MJ_ROWS({
-        const double q0 = op_M(a.pW + ob, a.pS + ob, a.pC + ob, r0, p0, nx) * r0[p0];
-        const double q1 = has1 ? op_M(a.pW + ob, a.pS + ob, a.pC + ob, r0, p1, nx) * r0[p1] : 0.0;
-        put_row<MODE>(0, jl, lane, q0, q1, has1, stage, sNx, sNy);
+        const double z0 = op_M(a.pW + ob, a.pS + ob, a.pC + ob, r0, p0, nx);
+        z[p0] = z0;
+        double z1 = 0.0;
+        if (has1) {
+            z1 = op_M(a.pW + ob, a.pS + ob, a.pC + ob, r0, p1, nx);
+            z[p1] = z1;
+        }
+        put_row<MODE>(0, jl, lane, z0 * r0[p0], has1 ? z1 * r0[p1] : 0.0, has1, stage, sNx, sNy);
     })
     store_tile_sums<MODE>(a.peta + a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, stage, sNx, sNy);  // eta(1)
     if (t == 0 && tid == 0) {
@@ -174,7 +182,7 @@ __device__ __forceinline__ void unit_finish(const CgDev& a, int m, int t, int n,
 
 // Phase B(n) of a group (tile t): per member the status at the start of B(n), the same in every block — finished
 // earlier, finishing now (stopping rule on err(n-1), cg2d.F:245-250: x un-normalised), or active; the active members
-// then sweep together: s_n = M r_{n-1} + beta s_{n-1} on interior and ring, partial of sigma = sum s_n (A s_n).
+// then sweep together: s_n = z + beta s_{n-1} on interior and ring, q = A s_n, partial of sigma = sum s_n q.
 template <int MODE>
 __device__ __forceinline__ void group_phase_B(const CgDev& a, int gi, int t, int n) {
     __shared__ int s_m[kGroup], s_code[kGroup], s_act[kGroup], s_nm;
@@ -218,19 +226,18 @@ __device__ __forceinline__ void group_phase_B(const CgDev& a, int gi, int t, int
 #pragma unroll
         for (int k = 0; k < kGroup; ++k) {
             const int m = s_m[s_act[k < nm ? k : 0]];
-            mb[k] = MemberB<double>{R(a, n - 1) + m * a.ms, S(a, n - 1) + m * a.ms, S(a, n) + m * a.ms,
+            mb[k] = MemberB<double>{a.z + m * a.ms, S(a, n - 1) + m * a.ms, S(a, n) + m * a.ms, a.q + m * a.ms,
                                     s_beta[s_act[k < nm ? k : 0]], false, a.psig + (n & 1) * BT + m * a.T + t,
                                     stage_of(a, m, t)};
         }
         const int64_t ob = op_base(a, s_m[s_act[0]]);
-        group_B<double, kGroup, MODE>(geo_of(a, t), a.pW + ob, a.pS + ob, a.pC + ob, a.aW + ob, a.aS + ob, a.aC + ob,
-                                      mb, nm);
+        group_B<double, kGroup, MODE>(geo_of(a, t), a.aW + ob, a.aS + ob, a.aC + ob, mb, nm);
     }
     __syncthreads();
 }
 
 // Phase C(n) of a group (tile t): the members still active sweep together: x += alpha s_n;
-// r_n = r_{n-1} - alpha A s_n on interior and ring; partials of err(n) = sum r_n^2 and eta(n+1) = sum (M r_n) r_n.
+// r_n = r_{n-1} - alpha q on interior and ring; z = M r_n; partials of err(n) = sum r_n^2 and eta(n+1) = sum z r_n.
 template <int MODE>
 __device__ __forceinline__ void group_phase_C(const CgDev& a, int gi, int t, int n) {
     __shared__ int s_m[kGroup], s_act[kGroup], s_nm;
@@ -261,13 +268,12 @@ __device__ __forceinline__ void group_phase_C(const CgDev& a, int gi, int t, int
 #pragma unroll
         for (int k = 0; k < kGroup; ++k) {
             const int kk = s_act[k < nm ? k : 0], m = s_m[kk];
-            mc[k] = MemberC<double>{S(a, n) + m * a.ms, R(a, n - 1) + m * a.ms, R(a, n) + m * a.ms, a.x + m * a.ms,
-                                    s_alpha[kk], a.perr + (n & 1) * BT + m * a.T + t,
+            mc[k] = MemberC<double>{S(a, n) + m * a.ms, a.q + m * a.ms, R(a, n - 1) + m * a.ms, R(a, n) + m * a.ms,
+                                    a.x + m * a.ms, a.z + m * a.ms, s_alpha[kk], a.perr + (n & 1) * BT + m * a.T + t,
                                     a.peta + ((n + 1) & 1) * BT + m * a.T + t, stage_of(a, m, t)};
         }
         const int64_t ob = op_base(a, s_m[s_act[0]]);
-        group_C<double, kGroup, MODE>(geo_of(a, t), a.pW + ob, a.pS + ob, a.pC + ob, a.aW + ob, a.aS + ob, a.aC + ob,
-                                      mc, nm);
+        group_C<double, kGroup, MODE>(geo_of(a, t), a.pW + ob, a.pS + ob, a.pC + ob, mc, nm);
     }
     __syncthreads();
 }
@@ -426,7 +432,7 @@ using namespace mj;
 namespace {
 
 struct WsLayout {
-    size_t r0, r1, s0, s1, pmax, psum, perr, peta, psig, mdone, grp, misc, prod, total;
+    size_t r0, r1, s0, s1, z, q, pmax, psum, perr, peta, psig, mdone, grp, misc, prod, total;
 };
 
 WsLayout ws_layout(const mj_cg2d_config* c) {
@@ -439,6 +445,8 @@ WsLayout ws_layout(const mj_cg2d_config* c) {
     L.r1 = o; o += vec;
     L.s0 = o; o += vec;
     L.s1 = o; o += vec;
+    L.z = o; o += vec;
+    L.q = o; o += vec;
     L.pmax = o; o += bt;
     L.psum = o; o += bt;
     L.perr = o; o += 2 * bt;
@@ -593,6 +601,8 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     a.r1 = (double*)(w + L.r1);
     a.s0 = (double*)(w + L.s0);
     a.s1 = (double*)(w + L.s1);
+    a.z = (double*)(w + L.z);
+    a.q = (double*)(w + L.q);
     a.pmax = (double*)(w + L.pmax);
     a.psum = (double*)(w + L.psum);
     a.perr = (double*)(w + L.perr);

@@ -265,31 +265,36 @@ __device__ __forceinline__ void first_guess(const D& a, int m, int t, double rhs
 
 // ------------------------------------------------------------------------------------------------ grouped sweeps
 // The two CG phases of up to G members that read the same operator slot, swept by one block: each coefficient row
-// is loaded once and applied to every member's rows. Per member the arithmetic is exactly the single-member one
-// (same expressions, same term order, same reduction trees), so results do not depend on the grouping.
+// is loaded once and applied to every member's rows. The recurrence keeps z = M r and q = A s as vectors, as
+// cg2d.F does: phase B forms s = z + beta s_old pointwise (ring from the sources), then q = A s by the stencil and
+// the partial of sigma = (s, q); phase C forms x += alpha s and r = r_old - alpha q pointwise (ring from the
+// sources), then z = M r by the stencil and the partials of err = (r, r) and eta = (z, r). Per member the arithmetic
+// is exactly the single-member one (same expressions, same term order, same reduction trees), so results do not
+// depend on the grouping.
 struct SweepGeo {
     int t, tbase, sNx, sNy, OL, nx;
     const int32_t *ring_off, *ring_dst, *ring_src;
     const int8_t* ring_sign;
 };
-// Phase B of one member: s = z + beta s_old (z = M r; first: s = z), partial of sigma = (s, A s).
 template <typename T>
 struct MemberB {
-    const T* r;
-    const T* so;
-    T* sn;
+    const T* z;     // M r_{n-1}
+    const T* so;    // s_{n-1} (unused when first)
+    T* sn;          // s_n
+    T* q;           // out: A s_n
     T beta;
-    bool first;
-    double* sig;    // where the tile partial goes
+    bool first;     // s_n = z (no s_{n-1} term)
+    double* sig;    // where the tile partial of sigma goes
     double* stage;  // MJ_SUM_FORTRAN staging
 };
-// Phase C of one member: xs += alpha s; r = r_old - alpha A s; partials of err = (r, r) and eta = (M r, r).
 template <typename T>
 struct MemberC {
-    const T* sv;
-    const T* rin;
-    T* rout;
-    T* xs;
+    const T* sv;   // s_n
+    const T* q;    // A s_n
+    const T* rin;  // r_{n-1}
+    T* rout;       // r_n
+    T* xs;         // x (FP64) or d (FP32): xs += alpha s
+    T* z;          // out: M r_n
     T alpha;
     double* err;
     double* eta;
@@ -297,45 +302,26 @@ struct MemberC {
 };
 
 template <typename T, int G, int MODE>
-__device__ __forceinline__ void group_B(const SweepGeo& g, const T* __restrict__ pW, const T* __restrict__ pS,
-                                        const T* __restrict__ pC, const T* __restrict__ aW, const T* __restrict__ aS,
+__device__ __forceinline__ void group_B(const SweepGeo& g, const T* __restrict__ aW, const T* __restrict__ aS,
                                         const T* __restrict__ aC, const MemberB<T> (&mb)[G], int nm) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int sNx = g.sNx, sNy = g.sNy, OL = g.OL, nx = g.nx, tbase = g.tbase;
     MJ_STRIP
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        P2<T> Rm[G], Rc[G];
+    // s_n = z + beta s_{n-1}: interior rows of the strip, then the ring from the sources
+    for (int j = j0; j < j1; ++j) {
+        const int p = MJ_ROW(j) + L.c;
 #pragma unroll
-        for (int k = 0; k < G; ++k)
+        for (int k = 0; k < G; ++k) {
             if (k < nm) {
-                Rm[k] = ld2(mb[k].r + p - nx);
-                Rc[k] = ld2(mb[k].r + p);
-            }
-        P2<T> PS0 = ld2(pS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const P2<T> PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
-            const T pwr = right_of(PW, halo_of(pW, prow, L, sNx), L);
-#pragma unroll
-            for (int k = 0; k < G; ++k) {
-                if (k < nm) {
-                    const P2<T> Rp = ld2(mb[k].r + p + nx);
-                    const T hr = halo_of(mb[k].r, prow, L, sNx);
-                    const P2<T> q = pair_M(PC, PW, pwr, PS0, PS1, Rm[k], Rc[k], Rp, left_of(Rc[k], hr, L),
-                                           right_of(Rc[k], hr, L));
-                    P2<T> v = q;
-                    if (!mb[k].first) {
-                        const P2<T> SO = ld2(mb[k].so + p);
-                        v.x = q.x + mb[k].beta * SO.x;
-                        v.y = q.y + mb[k].beta * SO.y;
-                    }
-                    if (L.act) st2(mb[k].sn + p, v);
-                    Rm[k] = Rc[k];
-                    Rc[k] = Rp;
+                const P2<T> Z = ld2(mb[k].z + p);
+                P2<T> v = Z;
+                if (!mb[k].first) {
+                    const P2<T> SO = ld2(mb[k].so + p);
+                    v.x = Z.x + mb[k].beta * SO.x;
+                    v.y = Z.y + mb[k].beta * SO.y;
                 }
+                if (L.act) st2(mb[k].sn + p, v);
             }
-            PS0 = PS1;
         }
     }
     const int rk0 = g.ring_off[g.t], rk1 = g.ring_off[g.t + 1];
@@ -344,13 +330,13 @@ __device__ __forceinline__ void group_B(const SweepGeo& g, const T* __restrict__
         if (k < nm) {
             for (int kk = rk0 + tid; kk < rk1; kk += blockDim.x) {
                 const int s = g.ring_src[kk];
-                const T q = op_M(pW, pS, pC, mb[k].r, s, nx);
-                const T v = mb[k].first ? q : q + mb[k].beta * mb[k].so[s];
+                const T v = mb[k].first ? mb[k].z[s] : mb[k].z[s] + mb[k].beta * mb[k].so[s];
                 mb[k].sn[g.ring_dst[kk]] = g.ring_sign[kk] < 0 ? -v : v;
             }
         }
     }
     __syncthreads();
+    // q = A s_n, sigma partial (s_n, q)
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         P2<T> Xm[G], Xc[G];
@@ -372,6 +358,7 @@ __device__ __forceinline__ void group_B(const SweepGeo& g, const T* __restrict__
                     const T hx = halo_of(mb[k].sn, prow, L, sNx);
                     const P2<T> q = pair_A(AW, awr, AS0, AS1, AC, Xm[k], Xc[k], Xp, left_of(Xc[k], hx, L),
                                            right_of(Xc[k], hx, L));
+                    if (L.act) st2(mb[k].q + p, q);
                     double2 v = make_double2((double)Xc[k].x * (double)q.x, (double)Xc[k].y * (double)q.y);
                     if (!L.act) v = make_double2(0.0, 0.0);
                     put_row2<MODE>(0, j, L, v, mb[k].stage, sNx, sNy, k);
@@ -395,47 +382,29 @@ __device__ __forceinline__ void group_B(const SweepGeo& g, const T* __restrict__
 
 template <typename T, int G, int MODE>
 __device__ __forceinline__ void group_C(const SweepGeo& g, const T* __restrict__ pW, const T* __restrict__ pS,
-                                        const T* __restrict__ pC, const T* __restrict__ aW, const T* __restrict__ aS,
-                                        const T* __restrict__ aC, const MemberC<T> (&mc)[G], int nm) {
+                                        const T* __restrict__ pC, const MemberC<T> (&mc)[G], int nm) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int sNx = g.sNx, sNy = g.sNy, OL = g.OL, nx = g.nx, tbase = g.tbase;
     MJ_STRIP
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        P2<T> Xm[G], Xc[G];
+    // x += alpha s_n, r_n = r_{n-1} - alpha q: interior rows of the strip, then the ring of r_n from the sources
+    for (int j = j0; j < j1; ++j) {
+        const int p = MJ_ROW(j) + L.c;
 #pragma unroll
-        for (int k = 0; k < G; ++k)
+        for (int k = 0; k < G; ++k) {
             if (k < nm) {
-                Xm[k] = ld2(mc[k].sv + p - nx);
-                Xc[k] = ld2(mc[k].sv + p);
-            }
-        P2<T> AS0 = ld2(aS + p);
-        for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const P2<T> AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
-            const T awr = right_of(AW, halo_of(aW, prow, L, sNx), L);
-#pragma unroll
-            for (int k = 0; k < G; ++k) {
-                if (k < nm) {
-                    const P2<T> Xp = ld2(mc[k].sv + p + nx), RI = ld2(mc[k].rin + p), XV = ld2(mc[k].xs + p);
-                    const T hx = halo_of(mc[k].sv, prow, L, sNx);
-                    const P2<T> q = pair_A(AW, awr, AS0, AS1, AC, Xm[k], Xc[k], Xp, left_of(Xc[k], hx, L),
-                                           right_of(Xc[k], hx, L));
-                    const T alpha = mc[k].alpha;
-                    P2<T> xn, rn;
-                    xn.x = XV.x + alpha * Xc[k].x;
-                    xn.y = XV.y + alpha * Xc[k].y;
-                    rn.x = RI.x - alpha * q.x;
-                    rn.y = RI.y - alpha * q.y;
-                    if (L.act) {
-                        st2(mc[k].xs + p, xn);
-                        st2(mc[k].rout + p, rn);
-                    }
-                    Xm[k] = Xc[k];
-                    Xc[k] = Xp;
+                const P2<T> SV = ld2(mc[k].sv + p), Q = ld2(mc[k].q + p), RI = ld2(mc[k].rin + p),
+                            XV = ld2(mc[k].xs + p);
+                const T alpha = mc[k].alpha;
+                P2<T> xn, rn;
+                xn.x = XV.x + alpha * SV.x;
+                xn.y = XV.y + alpha * SV.y;
+                rn.x = RI.x - alpha * Q.x;
+                rn.y = RI.y - alpha * Q.y;
+                if (L.act) {
+                    st2(mc[k].xs + p, xn);
+                    st2(mc[k].rout + p, rn);
                 }
             }
-            AS0 = AS1;
         }
     }
     const int rk0 = g.ring_off[g.t], rk1 = g.ring_off[g.t + 1];
@@ -444,12 +413,13 @@ __device__ __forceinline__ void group_C(const SweepGeo& g, const T* __restrict__
         if (k < nm) {
             for (int kk = rk0 + tid; kk < rk1; kk += blockDim.x) {
                 const int s = g.ring_src[kk];
-                const T v = mc[k].rin[s] - mc[k].alpha * op_A(aW, aS, aC, mc[k].sv, s, nx);
+                const T v = mc[k].rin[s] - mc[k].alpha * mc[k].q[s];
                 mc[k].rout[g.ring_dst[kk]] = g.ring_sign[kk] < 0 ? -v : v;
             }
         }
     }
     __syncthreads();
+    // z = M r_n, partials err = (r_n, r_n), eta = (z, r_n)
     if (j0 < j1) {
         int p = MJ_ROW(j0) + L.c;
         P2<T> Rm[G], Rc[G];
@@ -469,10 +439,11 @@ __device__ __forceinline__ void group_C(const SweepGeo& g, const T* __restrict__
                 if (k < nm) {
                     const P2<T> Rp = ld2(mc[k].rout + p + nx);
                     const T hr = halo_of(mc[k].rout, prow, L, sNx);
-                    const P2<T> q = pair_M(PC, PW, pwr, PS0, PS1, Rm[k], Rc[k], Rp, left_of(Rc[k], hr, L),
-                                           right_of(Rc[k], hr, L));
+                    const P2<T> zz = pair_M(PC, PW, pwr, PS0, PS1, Rm[k], Rc[k], Rp, left_of(Rc[k], hr, L),
+                                            right_of(Rc[k], hr, L));
+                    if (L.act) st2(mc[k].z + p, zz);
                     double2 e = make_double2((double)Rc[k].x * (double)Rc[k].x, (double)Rc[k].y * (double)Rc[k].y);
-                    double2 h = make_double2((double)q.x * (double)Rc[k].x, (double)q.y * (double)Rc[k].y);
+                    double2 h = make_double2((double)zz.x * (double)Rc[k].x, (double)zz.y * (double)Rc[k].y);
                     if (!L.act) {
                         e = make_double2(0.0, 0.0);
                         h = e;

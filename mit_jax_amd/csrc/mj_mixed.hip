@@ -19,8 +19,9 @@
 // the member's record of the previous step and the partials, so all tiles of a member act alike; tile 0 writes
 // the record of the step (records are double-buffered by step parity). States:
 //   RST   err from the true-residual partials; finish, or restart the inner solve (r32, d = 0, eta(1))
-//   IB    inner phase B(n): inner stop -> x += sigma d (next: TRUE); else s = z + beta s, sigma_n = s.A32 s
-//   IC    inner phase C(n): d += alpha s; r32 -= alpha A32 s; err(n), eta(n+1)
+//   IB    inner phase B(n): inner stop -> x += sigma d (next: TRUE); else s = z + beta s, q = A32 s, sigma_n = s.q
+//   IC    inner phase C(n): d += alpha s; r32 -= alpha q; z = M32 r32; err(n) = r32.r32, eta(n+1) = z.r32
+// (z and q are kept as FP32 vectors, so each inner phase applies one stencil and reads one coefficient set.)
 //   TRUE  x's ring from its sources; r = bn - A x; err partials
 #include <algorithm>
 
@@ -52,7 +53,7 @@ struct MxDev {
     const int32_t *ring_off, *ring_dst, *ring_src;
     const int8_t* ring_sign;
     double* r64;
-    float *r0, *r1, *s0, *s1, *d;
+    float *r0, *r1, *s0, *s1, *d, *z, *q;
     double *pmax, *psum, *perr64, *perr, *peta, *psig;  // [B*T] x3, [2][B*T] x3
     const int32_t* op_index;
     MxRec* rec;  // [2][B]
@@ -125,13 +126,15 @@ __device__ __forceinline__ void mx_finish(const MxDev& a, int m, int t) {
     exchanged_first_guess(a, m, t, rhsNorm);
 }
 
-// Restart: r32 = r / sigma on interior and ring (ring from the sources' r), d = 0, partial of eta(1).
+// Restart: r32 = r / sigma on interior and ring (ring from the sources' r), d = 0, z = M32 r32, partial of
+// eta(1) = z.r32.
 __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double sigma) {
     MJ_UNIT_IDS
     MJ_STRIP
     const double* r = a.r64 + m * a.ms;
     float* r32 = R32(a, 0) + m * a.ms;
     float* d = a.d + m * a.ms;
+    float* z = a.z + m * a.ms;
     const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
     const float *__restrict__ pW = a.pW32 + oc, *__restrict__ pS = a.pS32 + oc, *__restrict__ pC = a.pC32 + oc;
     MJ_ROWS({
@@ -156,6 +159,7 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
             const float hr = halo_of(r32, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
             const float2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
                                     right_of(Rc, hr, L));
+            if (L.act) st2(z + p, q);
             double2 h = make_double2((double)q.x * (double)Rc.x, (double)q.y * (double)Rc.y);
             if (!L.act) h = make_double2(0.0, 0.0);
             put_row2<MJ_SUM_TREE>(0, j, L, h, nullptr, sNx, sNy);
@@ -223,134 +227,29 @@ __device__ __forceinline__ void mx_true(const MxDev& a, int m, int t) {
     store_tile_sums<MJ_SUM_TREE>(a.perr64, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
 }
 
-// Inner phase B(n): s_n = z + beta s_{n-1} (z = M32 r32_{n-1}; s_1 = z) on interior and ring, sigma_n partial.
-__device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64) {
-    MJ_UNIT_IDS
-    MJ_STRIP
-    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
-    const float *__restrict__ pW = a.pW32 + oc, *__restrict__ pS = a.pS32 + oc, *__restrict__ pC = a.pC32 + oc;
-    const float *__restrict__ aW = a.aW32 + oc, *__restrict__ aS = a.aS32 + oc, *__restrict__ aC = a.aC32 + oc;
-    const float* __restrict__ r = R32(a, n - 1) + m * a.ms;
-    const float* __restrict__ so = S32(a, n - 1) + m * a.ms;
-    float* sn = S32(a, n) + m * a.ms;
-    const float beta = (float)beta64;
-    const bool first = n == 1;
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        float2 Rm = ld2(r + p - nx), Rc = ld2(r + p), PS0 = ld2(pS + p);
-        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const float2 Rp = ld2(r + p + nx), PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
-            const float hr = halo_of(r, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
-            const float2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
-                                    right_of(Rc, hr, L));
-            float2 v = q;
-            if (!first) {
-                const float2 SO = ld2(so + p);
-                v.x = q.x + beta * SO.x;
-                v.y = q.y + beta * SO.y;
-            }
-            if (L.act) st2(sn + p, v);
-            Rm = Rc;
-            Rc = Rp;
-            PS0 = PS1;
-        }
-    }
-    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const int s = a.ring_src[k];
-        const float q = op_M(pW, pS, pC, r, s, nx);
-        const float v = first ? q : q + beta * so[s];
-        sn[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
-    }
-    __syncthreads();
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        float2 Xm = ld2(sn + p - nx), Xc = ld2(sn + p), AS0 = ld2(aS + p);
-        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const float2 Xp = ld2(sn + p + nx), AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
-            const float hx = halo_of(sn, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
-            const float2 q = pair_A(AW, right_of(AW, hw, L), AS0, AS1, AC, Xm, Xc, Xp, left_of(Xc, hx, L),
-                                    right_of(Xc, hx, L));
-            double2 v = make_double2((double)Xc.x * (double)q.x, (double)Xc.y * (double)q.y);
-            if (!L.act) v = make_double2(0.0, 0.0);
-            put_row2<MJ_SUM_TREE>(0, j, L, v, nullptr, sNx, sNy);
-            Xm = Xc;
-            Xc = Xp;
-            AS0 = AS1;
-        }
-    }
-    store_tile_sums<MJ_SUM_TREE>(a.psig + (n & 1) * a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx,
-                                 sNy);
+__device__ __forceinline__ SweepGeo mx_geo(const MxDev& a, int t) {
+    return SweepGeo{t, t * a.plane, a.sNx, a.sNy, a.OL, a.nx, a.ring_off, a.ring_dst, a.ring_src, a.ring_sign};
 }
 
-// Inner phase C(n): d += alpha s_n; r32_n = r32_{n-1} - alpha A32 s_n on interior and ring; err(n), eta(n+1).
-__device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, double alpha64) {
-    MJ_UNIT_IDS
-    MJ_STRIP
+// Inner phase B(n): s_n = z + beta s_{n-1} (s_1 = z) on interior and ring, q = A32 s_n, sigma_n partial.
+__device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64) {
     const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
-    const float *__restrict__ pW = a.pW32 + oc, *__restrict__ pS = a.pS32 + oc, *__restrict__ pC = a.pC32 + oc;
-    const float *__restrict__ aW = a.aW32 + oc, *__restrict__ aS = a.aS32 + oc, *__restrict__ aC = a.aC32 + oc;
-    const float* __restrict__ sv = S32(a, n) + m * a.ms;
-    const float* __restrict__ rin = R32(a, n - 1) + m * a.ms;
-    float* rout = R32(a, n) + m * a.ms;
-    float* d = a.d + m * a.ms;
-    const float alpha = (float)alpha64;
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        float2 Xm = ld2(sv + p - nx), Xc = ld2(sv + p), AS0 = ld2(aS + p);
-        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const float2 Xp = ld2(sv + p + nx), AS1 = ld2(aS + p + nx), AW = ld2(aW + p), AC = ld2(aC + p);
-            const float2 RI = ld2(rin + p), DV = ld2(d + p);
-            const float hx = halo_of(sv, prow, L, sNx), hw = halo_of(aW, prow, L, sNx);
-            const float2 q = pair_A(AW, right_of(AW, hw, L), AS0, AS1, AC, Xm, Xc, Xp, left_of(Xc, hx, L),
-                                    right_of(Xc, hx, L));
-            float2 dn, rn;
-            dn.x = DV.x + alpha * Xc.x;
-            dn.y = DV.y + alpha * Xc.y;
-            rn.x = RI.x - alpha * q.x;
-            rn.y = RI.y - alpha * q.y;
-            if (L.act) {
-                st2(d + p, dn);
-                st2(rout + p, rn);
-            }
-            Xm = Xc;
-            Xc = Xp;
-            AS0 = AS1;
-        }
-    }
-    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const int s = a.ring_src[k];
-        const float v = rin[s] - alpha * op_A(aW, aS, aC, sv, s, nx);
-        rout[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
-    }
-    __syncthreads();
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        float2 Rm = ld2(rout + p - nx), Rc = ld2(rout + p), PS0 = ld2(pS + p);
-        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const float2 Rp = ld2(rout + p + nx), PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
-            const float hr = halo_of(rout, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
-            const float2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
-                                    right_of(Rc, hr, L));
-            double2 e = make_double2((double)Rc.x * (double)Rc.x, (double)Rc.y * (double)Rc.y);
-            double2 h = make_double2((double)q.x * (double)Rc.x, (double)q.y * (double)Rc.y);
-            if (!L.act) {
-                e = make_double2(0.0, 0.0);
-                h = e;
-            }
-            put_row2<MJ_SUM_TREE>(0, j, L, e, nullptr, sNx, sNy);
-            put_row2<MJ_SUM_TREE>(1, j, L, h, nullptr, sNx, sNy);
-            Rm = Rc;
-            Rc = Rp;
-            PS0 = PS1;
-        }
-    }
     const int BT = a.B * a.T;
-    store_tile_sums<MJ_SUM_TREE>(a.perr + (n & 1) * BT, a.peta + ((n + 1) & 1) * BT, m * a.T + t, lane, wave, nwaves,
-                                 nullptr, sNx, sNy);
+    const MemberB<float> mb[1] = {{a.z + m * a.ms, S32(a, n - 1) + m * a.ms, S32(a, n) + m * a.ms, a.q + m * a.ms,
+                                   (float)beta64, n == 1, a.psig + (n & 1) * BT + m * a.T + t, nullptr}};
+    group_B<float, 1, MJ_SUM_TREE>(mx_geo(a, t), a.aW32 + oc, a.aS32 + oc, a.aC32 + oc, mb, 1);
+}
+
+// Inner phase C(n): d += alpha s_n; r32_n = r32_{n-1} - alpha q on interior and ring; z = M32 r32_n; partials of
+// err(n) and eta(n+1).
+__device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, double alpha64) {
+    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
+    const int BT = a.B * a.T;
+    const MemberC<float> mc[1] = {{S32(a, n) + m * a.ms, a.q + m * a.ms, R32(a, n - 1) + m * a.ms,
+                                   R32(a, n) + m * a.ms, a.d + m * a.ms, a.z + m * a.ms, (float)alpha64,
+                                   a.perr + (n & 1) * BT + m * a.T + t, a.peta + ((n + 1) & 1) * BT + m * a.T + t,
+                                   nullptr}};
+    group_C<float, 1, MJ_SUM_TREE>(mx_geo(a, t), a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, mc, 1);
 }
 
 __global__ void __launch_bounds__(MJ_MX_THREADS) mx_init_kernel(MxDev a, int ph) {
@@ -473,7 +372,7 @@ using namespace mj;
 namespace {
 
 struct MxLayout {
-    size_t r64, r0, r1, s0, s1, d, coef, pmax, psum, perr64, perr, peta, psig, rec, mdone, misc, total;
+    size_t r64, r0, r1, s0, s1, d, z, q, coef, pmax, psum, perr64, perr, peta, psig, rec, mdone, misc, total;
 };
 
 int nslots_of(const mj_cg2d_config* c, const mj_cg2d_ops* ops, const mj_cg2d_mixed_config* mc) {
@@ -494,6 +393,8 @@ MxLayout mx_layout(const mj_cg2d_config* c, int nslots) {
     L.s0 = o; o += al(n * 4);
     L.s1 = o; o += al(n * 4);
     L.d = o; o += al(n * 4);
+    L.z = o; o += al(n * 4);
+    L.q = o; o += al(n * 4);
     L.coef = o; o += al(6 * (size_t)nslots * tp * 4);
     L.pmax = o; o += bt;
     L.psum = o; o += bt;
@@ -617,6 +518,8 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     a.s0 = (float*)(w + L.s0);
     a.s1 = (float*)(w + L.s1);
     a.d = (float*)(w + L.d);
+    a.z = (float*)(w + L.z);
+    a.q = (float*)(w + L.q);
     a.pmax = (double*)(w + L.pmax);
     a.psum = (double*)(w + L.psum);
     a.perr64 = (double*)(w + L.perr64);
