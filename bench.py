@@ -147,6 +147,35 @@ def cpu_baseline(E, seconds):
                        f"{el:.1f} s")
 
 
+def plain_fp64(E, params, members):
+    """The plain FP64 GPU form: one member per call, as a caller of the reference-signature ops would run the
+    ensemble: cg2d_fortran (literal FP64 PCG, launch shape picked for one member) then advect_tiles for that
+    member's levels. Returns (member-steps/s, the solutions of the first `members` members)."""
+    import torch
+    from mit_jax_amd import cg2d, gad
+    ex = E["ex"]
+    plan = gad.gad_tables(params)
+    xs = []
+
+    def one(m):
+        li = LEVEL_CHOICES.index(int(E["nlev"][m]))
+        ops = [o[li] for o in E["ops"]]
+        x, _ = cg2d.cg2d_fortran(ex, ops, E["rhs"][m], E["x0"][m], cg2dNorm=E["norm"][m:m + 1], tolerance=TOL,
+                                 max_iters=2000, sum_order="tile")
+        g2d = dict(E["g2d"], maskW=E["g2d"]["maskW"][m:m + 1], maskS=E["g2d"]["maskS"][m:m + 1])
+        gad.advect_tiles(params, plan, g2d, E["vg"], E["u"][m:m + 1], E["v"][m:m + 1], E["tr"][m:m + 1],
+                         E["hW"][m:m + 1], E["hS"][m:m + 1], E["rhC"][m:m + 1], nlev=E["nlev_t"][m:m + 1])
+        return x
+
+    one(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m in range(members):
+        xs.append(one(m))
+    torch.cuda.synchronize()
+    return members / (time.perf_counter() - t0), xs
+
+
 TRAFFIC_FILE = ROOT / "profiles" / "mx_step_kernel_ensemble128_traffic.md"
 
 
@@ -317,6 +346,14 @@ def main():
         out["mixed_over_fp64_solve"] = ref["cg_ms"] / run["cg_ms"]
         out["error_vs_fp64"] = {"max_rel": float(rel.max()), "mean_rel": float(rel.mean()),
                                 "measure": "per member max|x_mixed - x_fp64| / max|x_fp64| over the interior"}
+        print("bench: timing the plain FP64 form (one member per call)", file=sys.stderr, flush=True)
+        pv, xs = plain_fp64(E, params, B)
+        same = all(bool(torch.equal(torch.where(inner, xp, 0.0), torch.where(inner, xr[m], 0.0)))
+                   for m, xp in enumerate(xs))
+        out["plain_fp64"] = {"value": pv, "form": "one member per call: cg2d_fortran (FP64, launch auto) + "
+                                                  "advect_tiles (member's levels)",
+                             "solutions_equal_batched_fp64": same}
+        out["mixed_over_plain_fp64"] = out["value"] / pv
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(E, args.cpu_seconds)

@@ -68,8 +68,11 @@ struct CgDev {
 #define MJ_CG_THREADS 1024
 #endif  // largest block of the persistent launch
 #ifndef MJ_PH_THREADS
-#define MJ_PH_THREADS 256
-#endif  // block of the per-member phase launches (I0-I2); the group phases use MJ_PH_THREADS * kGroup
+#define MJ_PH_THREADS 1024
+#endif  // largest block of the phase launches (the group phases: MJ_PH_THREADS * kGroup)
+#ifndef MJ_PH_DEFAULT_THREADS
+#define MJ_PH_DEFAULT_THREADS 256
+#endif  // block of the phase launches when the config leaves it open
 #ifndef MJ_PH_WAVES_PER_EU
 #define MJ_PH_WAVES_PER_EU 8
 #endif  // occupancy target of the phase launches (waves per SIMD): bounds their register use (a group's rows)
@@ -470,7 +473,7 @@ int launch_mode_of(const mj_cg2d_config* c) {
 
 int block_threads_of(const mj_cg2d_config* c, int mode) {
     const int limit = mode == MJ_CG_LAUNCH_PERSISTENT ? MJ_CG_THREADS : MJ_PH_THREADS * kGroup;
-    if (c->block_threads == 0) return limit;
+    if (c->block_threads == 0) return mode == MJ_CG_LAUNCH_PERSISTENT ? limit : MJ_PH_DEFAULT_THREADS * kGroup;
     MJ_REQUIRE(c->block_threads % 64 == 0 && c->block_threads >= 64 && c->block_threads <= limit,
                "cg2d: block_threads must be a multiple of 64 in 64..%d for this launch mode", limit);
     return c->block_threads;

@@ -63,8 +63,11 @@ struct MxDev {
 };
 
 #ifndef MJ_MX_THREADS
-#define MJ_MX_THREADS 256
-#endif
+#define MJ_MX_THREADS 1024
+#endif  // largest block
+#ifndef MJ_MX_DEFAULT_THREADS
+#define MJ_MX_DEFAULT_THREADS 256
+#endif  // block when the config leaves it open
 
 __device__ __forceinline__ int slot_of(const MxDev& a, int m) { return a.op_index ? a.op_index[m] : m; }
 __device__ __forceinline__ float* R32(const MxDev& a, int n) { return n & 1 ? a.r1 : a.r0; }
@@ -533,7 +536,7 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     a.diag = diag;
 
     hipStream_t st = as_stream(stream);
-    const int threads = cfg->block_threads ? cfg->block_threads : MJ_MX_THREADS;
+    const int threads = cfg->block_threads ? cfg->block_threads : MJ_MX_DEFAULT_THREADS;
     const int nunits = a.B * a.T;
     const dim3 grid((unsigned)((nunits + kXcds - 1) / kXcds * kXcds)), block((unsigned)threads);
     hipLaunchKernelGGL(mx_reset_kernel, dim3(1), dim3(64), 0, st, a.nactive, a.B);

@@ -229,10 +229,10 @@ def candidates(precision, B, T=13):
     for ov in (False, True):
         if precision == "mixed":
             for itol in (1e-4, 1e-5):
-                for bt in (128, 256):
+                for bt in (128, 256, 512, 1024):
                     out.append(StepConfig("mixed", "phases", bt, 64, itol, overlap=ov))
         else:
-            for bt in (128, 256):
+            for bt in (128, 256, 512, 1024):
                 out.append(StepConfig("fp64", "phases", bt, 64, overlap=ov))
             if B * T <= 512:
                 for bt in (512, 1024):

@@ -97,3 +97,21 @@ def test_mixed_operator_slots_and_caps(dev, ex):
     x1, d1 = mixed.cg2d_mixed(ex, slots, b, x0, cg2dNorm=norm, tolerance=TOL, max_iters=20, op_index=op_index,
                               launch="phases_fixed")
     assert np.array_equal(x1.cpu().numpy(), x.cpu().numpy())
+
+
+def test_mixed_block_size_independent(dev, ex):
+    """Every block size gives the same bits (fixed reduction trees, no floating-point atomics)."""
+    from mit_jax_amd import mixed, synthetic as S
+    g, ops, norm = S.cg2d_case(0, "5point")
+    b = to_dev(np.stack([S.rhs(g, 50 + m) for m in range(2)]), dev)
+    x0 = to_dev(np.zeros((2,) + g.L.shape2d), dev)
+    opsd = [to_dev(o, dev) for o in ops]
+    outs = []
+    for bt in (64, 256, 512, 1024):
+        print(f"mixed block {bt}", flush=True)
+        x, d = mixed.cg2d_mixed(ex, opsd, b, x0, cg2dNorm=norm, tolerance=TOL, max_iters=4000, block_threads=bt)
+        assert d["converged"].cpu().tolist() == [1, 1]
+        outs.append((x.cpu().numpy(), d["numIters"].cpu().tolist()))
+    for x, it in outs[1:]:
+        assert it == outs[0][1]
+        assert np.array_equal(x, outs[0][0])

@@ -149,7 +149,7 @@ def test_cg2d_block_size_independent(dev, ex):
     x0 = np.zeros_like(b)
     outs = []
     for launch, bt in (("persistent", 128), ("persistent", 512), ("persistent", 1024), ("phases", 64),
-                       ("phases", 256)):
+                       ("phases", 256), ("phases", 512), ("phases", 1024)):
         print(f"cg2d geometry {launch} {bt}", flush=True)
         x, d = cg2d.cg2d_fortran(ex, [to_dev(o, dev) for o in ops], to_dev(b, dev), to_dev(x0, dev), cg2dNorm=norm,
                                  tolerance=1e-12, max_iters=2000, sum_order="tile", block_threads=bt, launch=launch)
