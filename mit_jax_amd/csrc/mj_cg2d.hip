@@ -135,7 +135,8 @@ __device__ __forceinline__ void unit_I2(const CgDev& a, int m, int t) {
     double* stage = stage_of(a, m, t);
     const int64_t ob = op_base(a, m);
     for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const double v = r0[a.ring_src[k]];
+        const int s = a.ring_src[k];
+        const double v = s < 0 ? 0.0 : r0[s];
         r0[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
     }
     __syncthreads();
@@ -186,7 +187,7 @@ __device__ __forceinline__ void unit_finish(const CgDev& a, int m, int t, int n,
 // Phase B(n) of a group (tile t): per member the status at the start of B(n), the same in every block — finished
 // earlier, finishing now (stopping rule on err(n-1), cg2d.F:245-250: x un-normalised), or active; the active members
 // then sweep together: s_n = z + beta s_{n-1} on interior and ring, q = A s_n, partial of sigma = sum s_n q.
-template <int MODE>
+template <int MODE, bool LDS>
 __device__ __forceinline__ void group_phase_B(const CgDev& a, int gi, int t, int n) {
     __shared__ int s_m[kGroup], s_code[kGroup], s_act[kGroup], s_nm;
     __shared__ double s_beta[kGroup], s_err[kGroup];
@@ -234,14 +235,17 @@ __device__ __forceinline__ void group_phase_B(const CgDev& a, int gi, int t, int
                                     stage_of(a, m, t)};
         }
         const int64_t ob = op_base(a, s_m[s_act[0]]);
-        group_B<double, kGroup, MODE>(geo_of(a, t), a.aW + ob, a.aS + ob, a.aC + ob, mb, nm);
+        if constexpr (LDS)
+            lds_B<double, MODE>(geo_of(a, t), a.aW + ob, a.aS + ob, a.aC + ob, mb[0]);
+        else
+            group_B<double, kGroup, MODE>(geo_of(a, t), a.aW + ob, a.aS + ob, a.aC + ob, mb, nm);
     }
     __syncthreads();
 }
 
 // Phase C(n) of a group (tile t): the members still active sweep together: x += alpha s_n;
 // r_n = r_{n-1} - alpha q on interior and ring; z = M r_n; partials of err(n) = sum r_n^2 and eta(n+1) = sum z r_n.
-template <int MODE>
+template <int MODE, bool LDS>
 __device__ __forceinline__ void group_phase_C(const CgDev& a, int gi, int t, int n) {
     __shared__ int s_m[kGroup], s_act[kGroup], s_nm;
     __shared__ double s_alpha[kGroup];
@@ -276,7 +280,10 @@ __device__ __forceinline__ void group_phase_C(const CgDev& a, int gi, int t, int
                                     a.peta + ((n + 1) & 1) * BT + m * a.T + t, stage_of(a, m, t)};
         }
         const int64_t ob = op_base(a, s_m[s_act[0]]);
-        group_C<double, kGroup, MODE>(geo_of(a, t), a.pW + ob, a.pS + ob, a.pC + ob, mc, nm);
+        if constexpr (LDS)
+            lds_C<double, MODE>(geo_of(a, t), a.pW + ob, a.pS + ob, a.pC + ob, mc[0]);
+        else
+            group_C<double, kGroup, MODE>(geo_of(a, t), a.pW + ob, a.pS + ob, a.pC + ob, mc, nm);
     }
     __syncthreads();
 }
@@ -300,11 +307,11 @@ __global__ void __launch_bounds__(MJ_CG_THREADS) cg2d_persistent_kernel(CgDev a,
     if (!grid_sync(gs)) goto done;
     for (int n = 1;; ++n) {
         const int ng = *a.ngroups, ngu = ng * a.T;
-        for (int u = blockIdx.x; u < ngu; u += gridDim.x) group_phase_B<MODE>(a, u % ng, u / ng, n);
+        for (int u = blockIdx.x; u < ngu; u += gridDim.x) group_phase_B<MODE, false>(a, u % ng, u / ng, n);
         if (!grid_sync(gs)) goto done;
         // nactive only changes in B phases: every block reads the same value here
         if (__hip_atomic_load(a.nactive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-        for (int u = blockIdx.x; u < ngu; u += gridDim.x) group_phase_C<MODE>(a, u % ng, u / ng, n);
+        for (int u = blockIdx.x; u < ngu; u += gridDim.x) group_phase_C<MODE, false>(a, u % ng, u / ng, n);
         if (!grid_sync(gs)) goto done;
     }
 #undef MJ_FOR_UNITS
@@ -318,6 +325,8 @@ done:
 
 // ------------------------------------------------------------------------------------------------ phases
 enum { PH_I0 = 0, PH_I1 = 1, PH_I2 = 2, PH_B = 3, PH_C = 4 };
+// the iteration phases of the phase launches stage the stencil operand in LDS (one member per block)
+constexpr bool kPhaseLds = MJ_LDS_SWEEPS && kGroup == 1;
 
 template <int MODE, int PH>
 __global__ void __launch_bounds__(MJ_PH_THREADS * kGroup) __attribute__((amdgpu_waves_per_eu(MJ_PH_WAVES_PER_EU)))
@@ -325,8 +334,8 @@ cg2d_phase_kernel(CgDev a, int n) {
     int m, t;
     if (PH == PH_B || PH == PH_C) {
         if (!group_unit(*a.ngroups, a.T, m, t)) return;  // m: group index
-        if (PH == PH_B) group_phase_B<MODE>(a, m, t, n);
-        if (PH == PH_C) group_phase_C<MODE>(a, m, t, n);
+        if (PH == PH_B) group_phase_B<MODE, kPhaseLds>(a, m, t, n);
+        if (PH == PH_C) group_phase_C<MODE, kPhaseLds>(a, m, t, n);
         return;
     }
     if (!phase_unit(a.B, a.T, m, t)) return;
@@ -401,6 +410,25 @@ int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
                 rsrc.push_back(P->h_src[0][k]);
                 rsg.push_back(P->h_sign[0][k]);
             }
+        }
+        // ring points the exchange leaves unwritten (open facet edges): entries with no source (value 0), so that
+        // every ring point of the tile has an entry
+        std::vector<char> seen((size_t)plane, 0);
+        for (int k = roff[t]; k < (int)rdst.size(); ++k) seen[rdst[k] - t * plane] = 1;
+        auto add = [&](int j, int i) {
+            const int r = j * nx + i;
+            if (seen[r]) return;
+            rdst.push_back(t * plane + r);
+            rsrc.push_back(-1);
+            rsg.push_back(1);
+        };
+        for (int i = OL; i < OL + sNx; ++i) {
+            add(OL - 1, i);
+            add(OL + sNy, i);
+        }
+        for (int j = OL; j < OL + sNy; ++j) {
+            add(j, OL - 1);
+            add(j, OL + sNx);
         }
         foff[t + 1] = (int32_t)fdst.size();
         roff[t + 1] = (int32_t)rdst.size();
@@ -507,13 +535,20 @@ int launch_phases(const CgDev& a, const mj_cg2d_config* cfg, int threads, int mo
     const int nunits = a.B * a.T;
     const dim3 grid((unsigned)((nunits + kXcds - 1) / kXcds * kXcds)), block((unsigned)threads);
     const dim3 iblock((unsigned)std::min(threads, MJ_PH_THREADS));
+    const size_t lds = kPhaseLds ? lds_tile_bytes(a.sNx, a.sNy, (int)sizeof(double)) : 0;
+    if (lds > 65536) {
+        MJ_HIP(hipFuncSetAttribute((const void*)cg2d_phase_kernel<MODE, PH_B>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        MJ_HIP(hipFuncSetAttribute((const void*)cg2d_phase_kernel<MODE, PH_C>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     auto go = [&](int ph, int n) -> int {
         switch (ph) {
             case PH_I0: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I0>), grid, iblock, 0, st, a, n); break;
             case PH_I1: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I1>), grid, iblock, 0, st, a, n); break;
             case PH_I2: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_I2>), grid, iblock, 0, st, a, n); break;
-            case PH_B: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_B>), grid, block, 0, st, a, n); break;
-            default: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_C>), grid, block, 0, st, a, n); break;
+            case PH_B: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_B>), grid, block, lds, st, a, n); break;
+            default: hipLaunchKernelGGL((cg2d_phase_kernel<MODE, PH_C>), grid, block, lds, st, a, n); break;
         }
         MJ_HIP(hipGetLastError());
         return MJ_OK;

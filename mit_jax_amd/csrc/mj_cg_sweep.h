@@ -330,7 +330,7 @@ __device__ __forceinline__ void group_B(const SweepGeo& g, const T* __restrict__
         if (k < nm) {
             for (int kk = rk0 + tid; kk < rk1; kk += blockDim.x) {
                 const int s = g.ring_src[kk];
-                const T v = mb[k].first ? mb[k].z[s] : mb[k].z[s] + mb[k].beta * mb[k].so[s];
+                const T v = s < 0 ? T(0) : mb[k].first ? mb[k].z[s] : mb[k].z[s] + mb[k].beta * mb[k].so[s];
                 mb[k].sn[g.ring_dst[kk]] = g.ring_sign[kk] < 0 ? -v : v;
             }
         }
@@ -413,7 +413,7 @@ __device__ __forceinline__ void group_C(const SweepGeo& g, const T* __restrict__
         if (k < nm) {
             for (int kk = rk0 + tid; kk < rk1; kk += blockDim.x) {
                 const int s = g.ring_src[kk];
-                const T v = mc[k].rin[s] - mc[k].alpha * mc[k].q[s];
+                const T v = s < 0 ? T(0) : mc[k].rin[s] - mc[k].alpha * mc[k].q[s];
                 mc[k].rout[g.ring_dst[kk]] = g.ring_sign[kk] < 0 ? -v : v;
             }
         }
@@ -467,6 +467,189 @@ __device__ __forceinline__ void group_C(const SweepGeo& g, const T* __restrict__
                 if (lane == 0) *(r == 0 ? mc[k].err : mc[k].eta) = v;
             }
         }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------ LDS-staged sweeps
+// The two CG phases of one member with the stencil operand staged in LDS: the pointwise update writes the new
+// vector's interior to global memory (read by the next phase and iteration) and to an LDS tile, the ring is
+// computed from its sources into the LDS tile only (no global ring writes: nothing reads them), and the stencil
+// reads its neighbours from LDS. The coefficients stream from global memory down each wave's strip of rows (the
+// south coefficient row rolls). Every value is the expression group_B / group_C use, in the same term order and
+// reduction trees, so the results are bitwise theirs.
+//
+// LDS tile: rows -1..sNy, columns -1..sNx at lds_at(j, i) = (j + 1) * (sNx + 4) + i + 2 (even offsets for the
+// column pairs). Ring entries with no source (open facet edges) carry src < 0 and give 0.
+#ifndef MJ_LDS_SWEEPS
+#define MJ_LDS_SWEEPS 1
+#endif  // 1: the phase launches use the LDS-staged sweeps (0: the global-ring sweeps)
+extern __shared__ __align__(16) double mj_lds[];
+__host__ __device__ constexpr size_t lds_tile_bytes(int sNx, int sNy, int elem) {
+    return (size_t)(sNy + 2) * (sNx + 4) * elem;
+}
+struct LdsGeo {
+    int W;
+    __device__ __forceinline__ int at(int j, int i) const { return (j + 1) * W + i + 2; }
+};
+// Column of this lane's coefficient pair load: the lane after the last one with a column pair loads the east halo
+// column sNx (its first value is the last lane's right neighbour), every other lane its own pair.
+__device__ __forceinline__ int east_col(const Lanes& L, int sNx) {
+    return L.lane == L.last + 1 ? sNx : L.c;
+}
+// Ring position of a tile-relative halo index r (row-major over ny x nx): LDS offset.
+__device__ __forceinline__ int lds_ring(const LdsGeo& G, int r, int nx, int OL) {
+    const int jh = r / nx;
+    return G.at(jh - OL, r - jh * nx - OL);
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ void lds_B(const SweepGeo& g, const T* __restrict__ aW, const T* __restrict__ aS,
+                                      const T* __restrict__ aC, const MemberB<T>& mb) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int sNx = g.sNx, sNy = g.sNy, OL = g.OL, nx = g.nx, tbase = g.tbase;
+    T* tile = reinterpret_cast<T*>(mj_lds);
+    const LdsGeo G{sNx + 4};
+    MJ_STRIP
+    const T* __restrict__ z = mb.z;
+    const T* __restrict__ so = mb.so;
+    T* __restrict__ sn = mb.sn;
+    const T beta = mb.beta;
+    // s_n = z + beta s_{n-1} on the strip's interior rows, two rows' loads in flight
+    for (int j = j0; j < j1; j += 2) {
+        const bool two = j + 1 < j1;
+        const int p = MJ_ROW(j) + L.c, p2 = two ? p + nx : p;
+        const P2<T> Z0 = ld2(z + p), Z1 = ld2(z + p2);
+        P2<T> S0 = Z0, S1 = Z1;
+        if (!mb.first) {
+            const P2<T> O0 = ld2(so + p), O1 = ld2(so + p2);
+            S0.x = Z0.x + beta * O0.x;
+            S0.y = Z0.y + beta * O0.y;
+            S1.x = Z1.x + beta * O1.x;
+            S1.y = Z1.y + beta * O1.y;
+        }
+        if (L.act) {
+            st2(sn + p, S0);
+            st2(tile + G.at(j, L.c), S0);
+            if (two) {
+                st2(sn + p2, S1);
+                st2(tile + G.at(j + 1, L.c), S1);
+            }
+        }
+    }
+    for (int kk = g.ring_off[g.t] + tid; kk < g.ring_off[g.t + 1]; kk += blockDim.x) {
+        const int s = g.ring_src[kk];
+        T v = T(0);
+        if (s >= 0) v = mb.first ? z[s] : z[s] + beta * so[s];
+        tile[lds_ring(G, g.ring_dst[kk] - tbase, nx, OL)] = g.ring_sign[kk] < 0 ? -v : v;
+    }
+    __syncthreads();
+    // q = A s_n from the LDS tile; sigma partial (s_n, q)
+    if (j0 < j1) {
+        const int ce = east_col(L, sNx);
+        int pr = MJ_ROW(j0);
+        P2<T> AS0 = ld2(aS + pr + L.c);
+        for (int j = j0; j < j1; ++j, pr += nx) {
+            const int p = pr + L.c;
+            const P2<T> AS1 = ld2(aS + p + nx), AC = ld2(aC + p), AW = ld2(aW + pr + ce);
+            T awr = __shfl_down(AW.x, 1, 64);
+            if (L.last == 63 && lane == 63) awr = aW[pr + sNx];  // sNx = 128: no lane after the last
+            const int q0 = G.at(j, L.c);
+            const P2<T> Xc = ld2(tile + q0), Xm = ld2(tile + q0 - G.W), Xp = ld2(tile + q0 + G.W);
+            const T xl = tile[q0 - 1], xr = tile[q0 + 2];
+            const P2<T> q = pair_A(AW, awr, AS0, AS1, AC, Xm, Xc, Xp, xl, xr);
+            if (L.act) st2(mb.q + p, q);
+            double2 v = make_double2((double)Xc.x * (double)q.x, (double)Xc.y * (double)q.y);
+            if (!L.act) v = make_double2(0.0, 0.0);
+            put_row2<MODE>(0, j, L, v, mb.stage, sNx, sNy, 0);
+            AS0 = AS1;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const double v = tile_of<MODE>(0, lane, mb.stage, sNx, sNy, 0);
+        if (lane == 0) *mb.sig = v;
+    }
+    __syncthreads();
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ void lds_C(const SweepGeo& g, const T* __restrict__ pW, const T* __restrict__ pS,
+                                      const T* __restrict__ pC, const MemberC<T>& mc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int sNx = g.sNx, sNy = g.sNy, OL = g.OL, nx = g.nx, tbase = g.tbase;
+    T* tile = reinterpret_cast<T*>(mj_lds);
+    const LdsGeo G{sNx + 4};
+    MJ_STRIP
+    const T* __restrict__ sv = mc.sv;
+    const T* __restrict__ qv = mc.q;
+    const T* __restrict__ rin = mc.rin;
+    T* __restrict__ rout = mc.rout;
+    T* __restrict__ xs = mc.xs;
+    const T alpha = mc.alpha;
+    // x += alpha s_n, r_n = r_{n-1} - alpha q on the strip's interior rows, two rows' loads in flight
+    for (int j = j0; j < j1; j += 2) {
+        const bool two = j + 1 < j1;
+        const int p = MJ_ROW(j) + L.c, p2 = two ? p + nx : p;
+        const P2<T> S0 = ld2(sv + p), Q0 = ld2(qv + p), R0 = ld2(rin + p), X0 = ld2(xs + p);
+        const P2<T> S1 = ld2(sv + p2), Q1 = ld2(qv + p2), R1 = ld2(rin + p2), X1 = ld2(xs + p2);
+        P2<T> x0, r0, x1, r1;
+        x0.x = X0.x + alpha * S0.x;
+        x0.y = X0.y + alpha * S0.y;
+        r0.x = R0.x - alpha * Q0.x;
+        r0.y = R0.y - alpha * Q0.y;
+        x1.x = X1.x + alpha * S1.x;
+        x1.y = X1.y + alpha * S1.y;
+        r1.x = R1.x - alpha * Q1.x;
+        r1.y = R1.y - alpha * Q1.y;
+        if (L.act) {
+            st2(xs + p, x0);
+            st2(rout + p, r0);
+            st2(tile + G.at(j, L.c), r0);
+            if (two) {
+                st2(xs + p2, x1);
+                st2(rout + p2, r1);
+                st2(tile + G.at(j + 1, L.c), r1);
+            }
+        }
+    }
+    for (int kk = g.ring_off[g.t] + tid; kk < g.ring_off[g.t + 1]; kk += blockDim.x) {
+        const int s = g.ring_src[kk];
+        T v = T(0);
+        if (s >= 0) v = rin[s] - alpha * qv[s];
+        tile[lds_ring(G, g.ring_dst[kk] - tbase, nx, OL)] = g.ring_sign[kk] < 0 ? -v : v;
+    }
+    __syncthreads();
+    // z = M r_n from the LDS tile; partials err = (r_n, r_n), eta = (z, r_n)
+    if (j0 < j1) {
+        const int ce = east_col(L, sNx);
+        int pr = MJ_ROW(j0);
+        P2<T> PS0 = ld2(pS + pr + L.c);
+        for (int j = j0; j < j1; ++j, pr += nx) {
+            const int p = pr + L.c;
+            const P2<T> PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + pr + ce);
+            T pwr = __shfl_down(PW.x, 1, 64);
+            if (L.last == 63 && lane == 63) pwr = pW[pr + sNx];
+            const int q0 = G.at(j, L.c);
+            const P2<T> Rc = ld2(tile + q0), Rm = ld2(tile + q0 - G.W), Rp = ld2(tile + q0 + G.W);
+            const T rl = tile[q0 - 1], rr = tile[q0 + 2];
+            const P2<T> zz = pair_M(PC, PW, pwr, PS0, PS1, Rm, Rc, Rp, rl, rr);
+            if (L.act) st2(mc.z + p, zz);
+            double2 e = make_double2((double)Rc.x * (double)Rc.x, (double)Rc.y * (double)Rc.y);
+            double2 h = make_double2((double)zz.x * (double)Rc.x, (double)zz.y * (double)Rc.y);
+            if (!L.act) {
+                e = make_double2(0.0, 0.0);
+                h = e;
+            }
+            put_row2<MODE>(0, j, L, e, mc.stage, sNx, sNy, 0);
+            put_row2<MODE>(1, j, L, h, mc.stage, sNx, sNy, 0);
+            PS0 = PS1;
+        }
+    }
+    __syncthreads();
+    for (int r = wave; r < 2; r += nwaves) {
+        const double v = tile_of<MODE>(r, lane, mc.stage, sNx, sNy, 0);
+        if (lane == 0) *(r == 0 ? mc.err : mc.eta) = v;
     }
     __syncthreads();
 }

@@ -149,7 +149,8 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
         }
     })
     for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const float v = (float)(r[a.ring_src[k]] / sigma);
+        const int s = a.ring_src[k];
+        const float v = s < 0 ? 0.0f : (float)(r[s] / sigma);
         r32[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
     }
     __syncthreads();
@@ -201,7 +202,9 @@ __device__ __forceinline__ void mx_true(const MxDev& a, int m, int t) {
     const int64_t ob = (int64_t)slot_of(a, m) * a.oms;
     const double *__restrict__ aW = a.aW + ob, *__restrict__ aS = a.aS + ob, *__restrict__ aC = a.aC + ob;
     for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const double v = x[a.ring_src[k]];
+        const int s = a.ring_src[k];
+        if (s < 0) continue;  // no source: x keeps its halo value
+        const double v = x[s];
         x[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
     }
     __syncthreads();
@@ -240,7 +243,11 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
     const int BT = a.B * a.T;
     const MemberB<float> mb[1] = {{a.z + m * a.ms, S32(a, n - 1) + m * a.ms, S32(a, n) + m * a.ms, a.q + m * a.ms,
                                    (float)beta64, n == 1, a.psig + (n & 1) * BT + m * a.T + t, nullptr}};
+#if MJ_LDS_SWEEPS
+    lds_B<float, MJ_SUM_TREE>(mx_geo(a, t), a.aW32 + oc, a.aS32 + oc, a.aC32 + oc, mb[0]);
+#else
     group_B<float, 1, MJ_SUM_TREE>(mx_geo(a, t), a.aW32 + oc, a.aS32 + oc, a.aC32 + oc, mb, 1);
+#endif
 }
 
 // Inner phase C(n): d += alpha s_n; r32_n = r32_{n-1} - alpha q on interior and ring; z = M32 r32_n; partials of
@@ -252,7 +259,11 @@ __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, doubl
                                    R32(a, n) + m * a.ms, a.d + m * a.ms, a.z + m * a.ms, (float)alpha64,
                                    a.perr + (n & 1) * BT + m * a.T + t, a.peta + ((n + 1) & 1) * BT + m * a.T + t,
                                    nullptr}};
+#if MJ_LDS_SWEEPS
+    lds_C<float, MJ_SUM_TREE>(mx_geo(a, t), a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, mc[0]);
+#else
     group_C<float, 1, MJ_SUM_TREE>(mx_geo(a, t), a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, mc, 1);
+#endif
 }
 
 __global__ void __launch_bounds__(MJ_MX_THREADS) mx_init_kernel(MxDev a, int ph) {
@@ -539,6 +550,10 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     const int threads = cfg->block_threads ? cfg->block_threads : MJ_MX_DEFAULT_THREADS;
     const int nunits = a.B * a.T;
     const dim3 grid((unsigned)((nunits + kXcds - 1) / kXcds * kXcds)), block((unsigned)threads);
+    // the step kernel's LDS tile of the stencil operand (FP32)
+    const size_t lds = MJ_LDS_SWEEPS ? lds_tile_bytes(cfg->sNx, cfg->sNy, (int)sizeof(float)) : 0;
+    if (lds > 65536)
+        MJ_HIP(hipFuncSetAttribute((const void*)mx_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(mx_reset_kernel, dim3(1), dim3(64), 0, st, a.nactive, a.B);
     MJ_HIP(hipGetLastError());
     hipLaunchKernelGGL(mx_convert_kernel, dim3((unsigned)((tp + 255) / 256), (unsigned)nslots), dim3(256), 0, st,
@@ -556,7 +571,7 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     while (k <= kmax) {
         const int64_t kend = cfg->launch_mode == MJ_CG_LAUNCH_PHASES_FIXED ? kmax + 1 : std::min(kmax + 1, k + chunk);
         for (; k < kend; ++k) {
-            hipLaunchKernelGGL(mx_step_kernel, grid, block, 0, st, a, (int)k);
+            hipLaunchKernelGGL(mx_step_kernel, grid, block, lds, st, a, (int)k);
             MJ_HIP(hipGetLastError());
         }
         if (k > kmax) break;
