@@ -449,6 +449,23 @@ int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
         (rc = up(rsrc.data(), rsrc.size() * 4, (void**)&P->cg_ring_src)) ||
         (rc = up(rsg.data(), rsg.size(), (void**)&P->cg_ring_sign)))
         return rc;
+    // compact form: LDS offset of each ring point, compact source index
+    const int TS = compact_tile_stride(sNx, sNy);
+    std::vector<int32_t> clds(rdst.size()), csrc(rdst.size());
+    for (size_t k = 0; k < rdst.size(); ++k) {
+        const int r = rdst[k] % plane, jh = r / nx, ih = r % nx;
+        clds[k] = (jh - OL + 1) * (sNx + 4) + (ih - OL + 2);
+        const int s = rsrc[k];
+        if (s < 0) {
+            csrc[k] = -1;
+        } else {
+            const int ts = s / plane, rs = s % plane, js = rs / nx - OL, is = rs % nx - OL;
+            csrc[k] = ts * TS + js * sNx + is;
+        }
+    }
+    if ((rc = up(clds.data(), clds.size() * 4, (void**)&P->cg_cring_lds)) ||
+        (rc = up(csrc.data(), csrc.size() * 4, (void**)&P->cg_cring_src)))
+        return rc;
     P->cg_sNx = sNx;
     P->cg_sNy = sNy;
     P->cg_OL = OL;

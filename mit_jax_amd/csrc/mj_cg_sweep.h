@@ -171,18 +171,14 @@ __device__ __forceinline__ P2<T> pair_M(P2<T> PC, P2<T> PW, T pwr, P2<T> PS0, P2
     q.y = PC.y * Rc.y + PW.y * Rc.x + pwr * rr + PS0.y * Rm.y + PS1.y * Rp.y;
     return q;
 }
-// One row's per-point values of reduction r held as column pairs: tree order regroups them onto the canonical
-// lanes (lane l: v[l] and v[l + 64]) and reduces with wave_tree into the row sums; fortran order stages them.
+// One row's per-point values of reduction r held as column pairs: tree order reduces them with pair_tree (the
+// canonical row tree) into the row sums; fortran order stages them.
 template <int MODE>
 __device__ __forceinline__ void put_row2(int r, int jl, const Lanes& L, double2 v, double* stage, int sNx, int sNy,
                                          int k = 0) {
     if (MODE == MJ_SUM_TREE) {
-        const int l = L.lane, s0 = l >> 1, s1 = 32 + (l >> 1);
-        const double a0 = __shfl(v.x, s0, 64), a1 = __shfl(v.y, s0, 64);
-        const double b0 = __shfl(v.x, s1, 64), b1 = __shfl(v.y, s1, 64);
-        const double lo = (l & 1) ? a1 : a0, hi = (l & 1) ? b1 : b0;
-        const double w = wave_tree(lo, hi, l + 64 < sNx);
-        if (l == 0) g_rows[(r + 2 * k) * 128 + jl] = w;
+        const double w = pair_tree(v, sNx, L.lane);
+        if (L.lane == 0) g_rows[(r + 2 * k) * 128 + jl] = w;
     } else {
         if (L.act) st2(stage + ((size_t)r * sNy + jl) * sNx + L.c, v);
     }
@@ -488,6 +484,8 @@ extern __shared__ __align__(16) double mj_lds[];
 __host__ __device__ constexpr size_t lds_tile_bytes(int sNx, int sNy, int elem) {
     return (size_t)(sNy + 2) * (sNx + 4) * elem;
 }
+// Tile stride of the compact work vectors (interior only, row-major sNy x sNx): rounded up to 128 bytes of FP32.
+__host__ __device__ constexpr int compact_tile_stride(int sNx, int sNy) { return (sNx * sNy + 31) / 32 * 32; }
 struct LdsGeo {
     int W;
     __device__ __forceinline__ int at(int j, int i) const { return (j + 1) * W + i + 2; }

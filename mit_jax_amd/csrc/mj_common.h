@@ -37,13 +37,62 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // ---------------------------------------------------------------------------------------------------- reductions
 // The fixed reduction tree of the "tile" sum order (documented in mjamd.h): n values (64 <= n <= 128) are folded
 // onto the 64 lanes of a wavefront, lane l holding v[l] + v[l+64] (v[l] alone when l+64 >= n), then a 6-step
-// butterfly w[l] = w[l] + w[l ^ m], m = 32, 16, ..., 1. Addition is commutative, so every lane ends with the same
-// bits; lane 0's value is the result. A row of 90 points and a tile of 90 row sums are both reduced this way.
+// butterfly w[l] = w[l] + w[l ^ m], m = 32, 16, ..., 1; lane 0's value is the result. A row of 90 points and a
+// tile of 90 row sums are both reduced this way.
+//
+// Lane 0 of the butterfly only ever adds its partner l + m, so the lower half of each step suffices: lane l < m
+// takes the value of lane l + m with the cross-lane moves of the VALU (gfx950 permlane swaps for m = 32, 16, DPP
+// row shifts for m = 8 .. 1) instead of LDS permutes. The other lanes end with partial values.
+__device__ __forceinline__ double lane_up32(double v) {  // lanes 0..31: the value of lane l + 32
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double((int)b[1], (int)a[1]);
+}
+__device__ __forceinline__ double lane_up16(double v) {  // lanes 0..15 (and 32..47): the value of lane l + 16
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double((int)b[1], (int)a[1]);
+}
+template <int K>  // lanes with l % 16 < 16 - K: the value of lane l + K (DPP row_shl:K)
+__device__ __forceinline__ double lane_up_row(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int l2 = __builtin_amdgcn_update_dpp(lo, lo, 0x100 + K, 0xf, 0xf, false);
+    const int h2 = __builtin_amdgcn_update_dpp(hi, hi, 0x100 + K, 0xf, 0xf, false);
+    return __hiloint2double(h2, l2);
+}
+// Steps m = 16 .. 1 of the butterfly (lane 0's chain).
+__device__ __forceinline__ double tree_from16(double w) {
+    w = w + lane_up16(w);
+    w = w + lane_up_row<8>(w);
+    w = w + lane_up_row<4>(w);
+    w = w + lane_up_row<2>(w);
+    return w + lane_up_row<1>(w);
+}
 __device__ __forceinline__ double wave_tree(double lo, double hi, bool has_hi) {
     double w = has_hi ? lo + hi : lo;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) w = w + __shfl_xor(w, m, 64);
-    return w;
+    w = w + lane_up32(w);
+    return tree_from16(w);
+}
+// The same tree for n values (64 <= n <= 128, n even) held as column pairs: lane L holds v[2L], v[2L+1] (zeros
+// beyond the row). In pair terms the fold is v[2L] + v[2L+64], the steps m = 32 .. 2 pair lane L with L + m/2 and
+// the last step adds the two halves of lane 0's pair.
+__device__ __forceinline__ double pair_tree(double2 v, int n, int lane) {
+    const double ux = lane_up32(v.x), uy = lane_up32(v.y);
+    const bool hi = 2 * lane + 64 < n;
+    double x = hi ? v.x + ux : v.x, y = hi ? v.y + uy : v.y;
+    x = x + lane_up16(x);
+    y = y + lane_up16(y);
+    x = x + lane_up_row<8>(x);
+    y = y + lane_up_row<8>(y);
+    x = x + lane_up_row<4>(x);
+    y = y + lane_up_row<4>(y);
+    x = x + lane_up_row<2>(x);
+    y = y + lane_up_row<2>(y);
+    x = x + lane_up_row<1>(x);
+    y = y + lane_up_row<1>(y);
+    return x + y;
 }
 
 // Sum of the per-tile partials in tile order starting from 0 (GLOBAL_SUM_TILE_RL, reference

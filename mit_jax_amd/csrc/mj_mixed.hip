@@ -13,8 +13,11 @@
 // and repeat. max_iters caps the total inner iterations and max_outer the refinements; a member that reaches a
 // cap finishes with converged = 0. Un-normalisation and the halos of x are as cg2d_fortran leaves them.
 //
-// Work decomposition as the FP64 solve: one block per (member, tile), row sweeps over column pairs, rings computed
-// from their interior sources, per-tile partials summed in tile order. Members move through their own state
+// Work decomposition as the FP64 solve: one block per (member, tile), rings computed from their interior sources,
+// per-tile partials summed in tile order. The FP32 work vectors (r32, s, d, z, q) are the solver's own and live in a
+// compact layout: per member and tile the interior only, row-major sNy x sNx, tiles 128-byte aligned. The pointwise
+// updates stream them as flat 16-byte accesses; the stencil operand is staged in an LDS tile whose ring is computed
+// from the sources' compact values; the stencils stream the FP32 coefficient rows down each wave's strip. Members move through their own state
 // machine, one state per launch ("step"): every block of a member derives the member's action for the step from
 // the member's record of the previous step and the partials, so all tiles of a member act alike; tile 0 writes
 // the record of the step (records are double-buffered by step parity). States:
@@ -52,8 +55,11 @@ struct MxDev {
     const int8_t* full_sign;
     const int32_t *ring_off, *ring_dst, *ring_src;
     const int8_t* ring_sign;
+    const int32_t *cring_lds, *cring_src;  // ring entries in compact terms (LDS offset, compact source or -1)
     double* r64;
-    float *r0, *r1, *s0, *s1, *d, *z, *q;
+    int TS;      // compact tile stride (elements)
+    int64_t cs;  // compact member stride = T * TS
+    float *r0, *r1, *s0, *s1, *d, *z, *q;  // compact
     double *pmax, *psum, *perr64, *perr, *peta, *psig;  // [B*T] x3, [2][B*T] x3
     const int32_t* op_index;
     MxRec* rec;  // [2][B]
@@ -72,6 +78,50 @@ struct MxDev {
 __device__ __forceinline__ int slot_of(const MxDev& a, int m) { return a.op_index ? a.op_index[m] : m; }
 __device__ __forceinline__ float* R32(const MxDev& a, int n) { return n & 1 ? a.r1 : a.r0; }
 __device__ __forceinline__ float* S32(const MxDev& a, int n) { return n & 1 ? a.s1 : a.s0; }
+// compact offset of member m, tile t
+__device__ __forceinline__ int64_t cbase(const MxDev& a, int m, int t) { return m * a.cs + (int64_t)t * a.TS; }
+
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ void st4(float* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
+// Row and column of compact element e of a tile (exact: e < 2^14, the fraction of (e + 0.5) / sNx stays >= 1/256
+// away from an integer).
+__device__ __forceinline__ int crow(int e, float inv_sNx) { return (int)((float(e) + 0.5f) * inv_sNx); }
+// The four compact elements e .. e+3 (e a multiple of 4, sNx even) into the LDS tile: two column pairs, each within
+// one row.
+__device__ __forceinline__ void lds_put4(float* tile, const LdsGeo& G, int e, float4 v, int sNx, float inv_sNx) {
+    const int j0 = crow(e, inv_sNx), j1 = crow(e + 2, inv_sNx);
+    st2(tile + G.at(j0, e - j0 * sNx), make_float2(v.x, v.y));
+    st2(tile + G.at(j1, e + 2 - j1 * sNx), make_float2(v.z, v.w));
+}
+
+// Stencil over this wave's strip of the LDS tile: IS_A: A32 = (W, S, C) = (aW, aS, aC); else M32 = (pW, pS, pC).
+// The result of each row goes to out (compact, row j at j * sNx) and row_cb(j, L, Xc, v) forms the row's partials.
+template <bool IS_A, class F>
+__device__ __forceinline__ void mx_stencil(const MxDev& a, int t, const float* __restrict__ cW,
+                                           const float* __restrict__ cS, const float* __restrict__ cC,
+                                           const float* tile, float* __restrict__ out, F&& row_cb) {
+    MJ_UNIT_IDS
+    MJ_STRIP
+    const LdsGeo G{sNx + 4};
+    if (j0 >= j1) return;
+    const int ce = east_col(L, sNx);
+    int pr = MJ_ROW(j0);
+    float2 S0 = ld2(cS + pr + L.c);
+    for (int j = j0; j < j1; ++j, pr += nx) {
+        const int p = pr + L.c;
+        const float2 S1 = ld2(cS + p + nx), C = ld2(cC + p), Wc = ld2(cW + pr + ce);
+        float wr = __shfl_down(Wc.x, 1, 64);
+        if (L.last == 63 && lane == 63) wr = cW[pr + sNx];
+        const int q0 = G.at(j, L.c);
+        const float2 Xc = ld2(tile + q0), Xm = ld2(tile + q0 - G.W), Xp = ld2(tile + q0 + G.W);
+        const float xl = tile[q0 - 1], xr = tile[q0 + 2];
+        const float2 v = IS_A ? pair_A(Wc, wr, S0, S1, C, Xm, Xc, Xp, xl, xr)
+                              : pair_M(C, Wc, wr, S0, S1, Xm, Xc, Xp, xl, xr);
+        if (L.act) st2(out + j * sNx + L.c, v);
+        row_cb(j, L, Xc, v);
+        S0 = S1;
+    }
+}
 
 // I1: x = normalised first guess with exchanged halos; r = bn - A x (FP64); partials of err and sumRHS; the
 // member's record starts in RST.
@@ -83,18 +133,6 @@ __device__ __forceinline__ void mx_I1(const MxDev& a, int m, int t) {
     double* x = a.x + m * a.ms;
     double* r = a.r64 + m * a.ms;
     const int64_t ob = (int64_t)slot_of(a, m) * a.oms;
-    // halo points of the FP32 work vectors start at 0: the exchange leaves some halo points unwritten (open facet
-    // edges) and the stencils read them with zero coefficients
-    float *r0 = a.r0 + m * a.ms, *r1 = a.r1 + m * a.ms, *s0 = a.s0 + m * a.ms, *s1 = a.s1 + m * a.ms;
-    for (int qq = tid; qq < a.plane; qq += blockDim.x) {
-        const int j = qq / nx, i = qq - j * nx, p = tbase + qq;
-        if (!(j >= OL && j < OL + sNy && i >= OL && i < OL + sNx)) {
-            r0[p] = 0.0f;
-            r1[p] = 0.0f;
-            s0[p] = 0.0f;
-            s1[p] = 0.0f;
-        }
-    }
     first_guess(a, m, t, rhsNorm);
     MJ_ROWS({
         const double bn0 = b[p0] * nm * rhsNorm;
@@ -129,49 +167,40 @@ __device__ __forceinline__ void mx_finish(const MxDev& a, int m, int t) {
     exchanged_first_guess(a, m, t, rhsNorm);
 }
 
-// Restart: r32 = r / sigma on interior and ring (ring from the sources' r), d = 0, z = M32 r32, partial of
-// eta(1) = z.r32.
+// Restart: r32 = r / sigma on the interior (compact) and the ring (LDS, from the sources' r), d = 0, z = M32 r32,
+// partial of eta(1) = z.r32.
 __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double sigma) {
     MJ_UNIT_IDS
     MJ_STRIP
+    float* tile = reinterpret_cast<float*>(mj_lds);
+    const LdsGeo G{sNx + 4};
     const double* r = a.r64 + m * a.ms;
-    float* r32 = R32(a, 0) + m * a.ms;
-    float* d = a.d + m * a.ms;
-    float* z = a.z + m * a.ms;
-    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
-    const float *__restrict__ pW = a.pW32 + oc, *__restrict__ pS = a.pS32 + oc, *__restrict__ pC = a.pC32 + oc;
-    MJ_ROWS({
-        r32[p0] = (float)(r[p0] / sigma);
-        d[p0] = 0.0f;
-        if (has1) {
-            r32[p1] = (float)(r[p1] / sigma);
-            d[p1] = 0.0f;
+    const int64_t cb = cbase(a, m, t);
+    float* r32 = R32(a, 0) + cb;
+    float* d = a.d + cb;
+    for (int j = j0; j < j1; ++j) {
+        const double2 R = ld2(r + MJ_ROW(j) + L.c);
+        const float2 v = make_float2((float)(R.x / sigma), (float)(R.y / sigma));
+        if (L.act) {
+            st2(r32 + j * sNx + L.c, v);
+            st2(tile + G.at(j, L.c), v);
         }
-    })
+    }
+    const int nf4 = sNx * sNy / 4;
+    for (int f = tid; f < nf4; f += blockDim.x) st4(d + 4 * f, make_float4(0.f, 0.f, 0.f, 0.f));
     for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
         const int s = a.ring_src[k];
         const float v = s < 0 ? 0.0f : (float)(r[s] / sigma);
-        r32[a.ring_dst[k]] = a.ring_sign[k] < 0 ? -v : v;
+        tile[a.cring_lds[k]] = a.ring_sign[k] < 0 ? -v : v;
     }
     __syncthreads();
-    if (j0 < j1) {
-        int p = MJ_ROW(j0) + L.c;
-        float2 Rm = ld2(r32 + p - nx), Rc = ld2(r32 + p), PS0 = ld2(pS + p);
-        MJ_UNROLL(MJ_SWEEP_UNROLL) for (int j = j0; j < j1; ++j, p += nx) {
-            const int prow = MJ_ROW(j);
-            const float2 Rp = ld2(r32 + p + nx), PS1 = ld2(pS + p + nx), PC = ld2(pC + p), PW = ld2(pW + p);
-            const float hr = halo_of(r32, prow, L, sNx), hw = halo_of(pW, prow, L, sNx);
-            const float2 q = pair_M(PC, PW, right_of(PW, hw, L), PS0, PS1, Rm, Rc, Rp, left_of(Rc, hr, L),
-                                    right_of(Rc, hr, L));
-            if (L.act) st2(z + p, q);
-            double2 h = make_double2((double)q.x * (double)Rc.x, (double)q.y * (double)Rc.y);
-            if (!L.act) h = make_double2(0.0, 0.0);
-            put_row2<MJ_SUM_TREE>(0, j, L, h, nullptr, sNx, sNy);
-            Rm = Rc;
-            Rc = Rp;
-            PS0 = PS1;
-        }
-    }
+    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
+    mx_stencil<false>(a, t, a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, tile, a.z + cb,
+                      [&](int j, const Lanes& L2, float2 Rc, float2 q) {
+                          double2 h = make_double2((double)q.x * (double)Rc.x, (double)q.y * (double)Rc.y);
+                          if (!L2.act) h = make_double2(0.0, 0.0);
+                          put_row2<MJ_SUM_TREE>(0, j, L2, h, nullptr, sNx, sNy);
+                      });
     store_tile_sums<MJ_SUM_TREE>(a.peta + a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
 }
 
@@ -180,11 +209,11 @@ __device__ __forceinline__ void mx_update(const MxDev& a, int m, int t, double s
     MJ_UNIT_IDS
     MJ_STRIP
     double* x = a.x + m * a.ms;
-    const float* d = a.d + m * a.ms;
+    const float* d = a.d + cbase(a, m, t);
     for (int j = j0; j < j1; ++j) {
         const int p = MJ_ROW(j) + L.c;
         const double2 X = ld2(x + p);
-        const float2 D = ld2(d + p);
+        const float2 D = ld2(d + j * sNx + L.c);
         if (L.act) st2(x + p, make_double2(X.x + sigma * (double)D.x, X.y + sigma * (double)D.y));
     }
 }
@@ -233,37 +262,101 @@ __device__ __forceinline__ void mx_true(const MxDev& a, int m, int t) {
     store_tile_sums<MJ_SUM_TREE>(a.perr64, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
 }
 
-__device__ __forceinline__ SweepGeo mx_geo(const MxDev& a, int t) {
-    return SweepGeo{t, t * a.plane, a.sNx, a.sNy, a.OL, a.nx, a.ring_off, a.ring_dst, a.ring_src, a.ring_sign};
-}
-
-// Inner phase B(n): s_n = z + beta s_{n-1} (s_1 = z) on interior and ring, q = A32 s_n, sigma_n partial.
+// Inner phase B(n): s_n = z + beta s_{n-1} (s_1 = z) on the interior (compact, flat) and the ring (LDS),
+// q = A32 s_n, sigma_n partial (s_n, q).
 __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64) {
+    MJ_UNIT_IDS
+    float* tile = reinterpret_cast<float*>(mj_lds);
+    const LdsGeo G{sNx + 4};
+    const float inv = 1.0f / (float)sNx;
+    const int64_t cm = m * a.cs, cb = cbase(a, m, t);
+    const float *z = a.z + cm, *so = S32(a, n - 1) + cm;
+    float* sn = S32(a, n) + cb;
+    const float beta = (float)beta64;
+    const bool first = n == 1;
+    const int nf4 = sNx * sNy / 4;
+    for (int f = tid; f < nf4; f += blockDim.x) {
+        const int e = 4 * f;
+        const int64_t o = (int64_t)t * a.TS + e;
+        float4 v = ld4(z + o);
+        if (!first) {
+            const float4 O = ld4(so + o);
+            v.x = v.x + beta * O.x;
+            v.y = v.y + beta * O.y;
+            v.z = v.z + beta * O.z;
+            v.w = v.w + beta * O.w;
+        }
+        st4(sn + e, v);
+        lds_put4(tile, G, e, v, sNx, inv);
+    }
+    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const int s = a.cring_src[k];
+        float v = 0.0f;
+        if (s >= 0) v = first ? z[s] : z[s] + beta * so[s];
+        tile[a.cring_lds[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
     const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
-    const int BT = a.B * a.T;
-    const MemberB<float> mb[1] = {{a.z + m * a.ms, S32(a, n - 1) + m * a.ms, S32(a, n) + m * a.ms, a.q + m * a.ms,
-                                   (float)beta64, n == 1, a.psig + (n & 1) * BT + m * a.T + t, nullptr}};
-#if MJ_LDS_SWEEPS
-    lds_B<float, MJ_SUM_TREE>(mx_geo(a, t), a.aW32 + oc, a.aS32 + oc, a.aC32 + oc, mb[0]);
-#else
-    group_B<float, 1, MJ_SUM_TREE>(mx_geo(a, t), a.aW32 + oc, a.aS32 + oc, a.aC32 + oc, mb, 1);
-#endif
+    mx_stencil<true>(a, t, a.aW32 + oc, a.aS32 + oc, a.aC32 + oc, tile, a.q + cb,
+                     [&](int j, const Lanes& L2, float2 Xc, float2 q) {
+                         double2 v = make_double2((double)Xc.x * (double)q.x, (double)Xc.y * (double)q.y);
+                         if (!L2.act) v = make_double2(0.0, 0.0);
+                         put_row2<MJ_SUM_TREE>(0, j, L2, v, nullptr, sNx, sNy);
+                     });
+    store_tile_sums<MJ_SUM_TREE>(a.psig + (n & 1) * a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx,
+                                 sNy);
 }
 
-// Inner phase C(n): d += alpha s_n; r32_n = r32_{n-1} - alpha q on interior and ring; z = M32 r32_n; partials of
-// err(n) and eta(n+1).
+// Inner phase C(n): d += alpha s_n; r32_n = r32_{n-1} - alpha q on the interior (compact, flat) and the ring (LDS);
+// z = M32 r32_n; partials of err(n) = (r32_n, r32_n) and eta(n+1) = (z, r32_n).
 __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, double alpha64) {
+    MJ_UNIT_IDS
+    float* tile = reinterpret_cast<float*>(mj_lds);
+    const LdsGeo G{sNx + 4};
+    const float inv = 1.0f / (float)sNx;
+    const int64_t cm = m * a.cs, cb = cbase(a, m, t);
+    const float *sv = S32(a, n) + cb, *qv = a.q + cm, *rin = R32(a, n - 1) + cm;
+    float *rout = R32(a, n) + cb, *d = a.d + cb;
+    const float alpha = (float)alpha64;
+    const int nf4 = sNx * sNy / 4;
+    const int64_t ot = (int64_t)t * a.TS;
+    for (int f = tid; f < nf4; f += blockDim.x) {
+        const int e = 4 * f;
+        const float4 S = ld4(sv + e), Q = ld4(qv + ot + e), R = ld4(rin + ot + e), D = ld4(d + e);
+        float4 dn, rn;
+        dn.x = D.x + alpha * S.x;
+        dn.y = D.y + alpha * S.y;
+        dn.z = D.z + alpha * S.z;
+        dn.w = D.w + alpha * S.w;
+        rn.x = R.x - alpha * Q.x;
+        rn.y = R.y - alpha * Q.y;
+        rn.z = R.z - alpha * Q.z;
+        rn.w = R.w - alpha * Q.w;
+        st4(d + e, dn);
+        st4(rout + e, rn);
+        lds_put4(tile, G, e, rn, sNx, inv);
+    }
+    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const int s = a.cring_src[k];
+        const float v = s < 0 ? 0.0f : rin[s] - alpha * qv[s];
+        tile[a.cring_lds[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
+    __syncthreads();
     const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
+    mx_stencil<false>(a, t, a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, tile, a.z + cb,
+                      [&](int j, const Lanes& L2, float2 Rc, float2 zz) {
+                          double2 e = make_double2((double)Rc.x * (double)Rc.x, (double)Rc.y * (double)Rc.y);
+                          double2 h = make_double2((double)zz.x * (double)Rc.x, (double)zz.y * (double)Rc.y);
+                          if (!L2.act) {
+                              e = make_double2(0.0, 0.0);
+                              h = e;
+                          }
+                          put_row2<MJ_SUM_TREE>(0, j, L2, e, nullptr, sNx, sNy);
+                          put_row2<MJ_SUM_TREE>(1, j, L2, h, nullptr, sNx, sNy);
+                      });
     const int BT = a.B * a.T;
-    const MemberC<float> mc[1] = {{S32(a, n) + m * a.ms, a.q + m * a.ms, R32(a, n - 1) + m * a.ms,
-                                   R32(a, n) + m * a.ms, a.d + m * a.ms, a.z + m * a.ms, (float)alpha64,
-                                   a.perr + (n & 1) * BT + m * a.T + t, a.peta + ((n + 1) & 1) * BT + m * a.T + t,
-                                   nullptr}};
-#if MJ_LDS_SWEEPS
-    lds_C<float, MJ_SUM_TREE>(mx_geo(a, t), a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, mc[0]);
-#else
-    group_C<float, 1, MJ_SUM_TREE>(mx_geo(a, t), a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, mc, 1);
-#endif
+    store_tile_sums<MJ_SUM_TREE>(a.perr + (n & 1) * BT, a.peta + ((n + 1) & 1) * BT, m * a.T + t, lane, wave, nwaves,
+                                 nullptr, sNx, sNy);
 }
 
 __global__ void __launch_bounds__(MJ_MX_THREADS) mx_init_kernel(MxDev a, int ph) {
@@ -398,17 +491,18 @@ MxLayout mx_layout(const mj_cg2d_config* c, int nslots) {
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     MxLayout L{};
     const size_t n = (size_t)c->batch * c->member_stride;
+    const size_t nc = (size_t)c->batch * c->ntiles * compact_tile_stride(c->sNx, c->sNy);  // compact vectors
     const size_t tp = (size_t)c->ntiles * (c->sNx + 2 * c->OL) * (c->sNy + 2 * c->OL);
     const size_t bt = al((size_t)c->batch * c->ntiles * 8);
     size_t o = 0;
     L.r64 = o; o += al(n * 8);
-    L.r0 = o; o += al(n * 4);
-    L.r1 = o; o += al(n * 4);
-    L.s0 = o; o += al(n * 4);
-    L.s1 = o; o += al(n * 4);
-    L.d = o; o += al(n * 4);
-    L.z = o; o += al(n * 4);
-    L.q = o; o += al(n * 4);
+    L.r0 = o; o += al(nc * 4);
+    L.r1 = o; o += al(nc * 4);
+    L.s0 = o; o += al(nc * 4);
+    L.s1 = o; o += al(nc * 4);
+    L.d = o; o += al(nc * 4);
+    L.z = o; o += al(nc * 4);
+    L.q = o; o += al(nc * 4);
     L.coef = o; o += al(6 * (size_t)nslots * tp * 4);
     L.pmax = o; o += bt;
     L.psum = o; o += bt;
@@ -426,9 +520,9 @@ MxLayout mx_layout(const mj_cg2d_config* c, int nslots) {
 int check_mixed(const mj_cg2d_config* c, const mj_cg2d_mixed_config* mc) {
     MJ_REQUIRE(c && mc, "cg2d mixed: null config");
     MJ_REQUIRE(c->batch >= 1 && c->ntiles >= 1, "cg2d mixed: bad batch/layout");
-    MJ_REQUIRE(c->sNx >= 64 && c->sNx <= 128 && c->sNy >= 64 && c->sNy <= 128 && c->sNx % 2 == 0 && c->OL >= 2 &&
-                   c->OL % 2 == 0,
-               "cg2d mixed: sNx, sNy must be in 64..128, sNx and OL even");
+    MJ_REQUIRE(c->sNx >= 64 && c->sNx <= 128 && c->sNy >= 64 && c->sNy <= 128 && c->sNx % 2 == 0 &&
+                   c->sNy % 2 == 0 && c->OL >= 2 && c->OL % 2 == 0,
+               "cg2d mixed: sNx, sNy must be even in 64..128, OL even and >= 2");
     const int64_t plane = (int64_t)(c->sNx + 2 * c->OL) * (c->sNy + 2 * c->OL);
     MJ_REQUIRE(c->member_stride >= c->ntiles * plane && c->member_stride % 2 == 0,
                "cg2d mixed: member_stride must be even and >= ntiles * ny * nx");
@@ -526,6 +620,10 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     a.ring_dst = P->cg_ring_dst;
     a.ring_src = P->cg_ring_src;
     a.ring_sign = P->cg_ring_sign;
+    a.cring_lds = P->cg_cring_lds;
+    a.cring_src = P->cg_cring_src;
+    a.TS = compact_tile_stride(a.sNx, a.sNy);
+    a.cs = (int64_t)a.T * a.TS;
     a.r64 = (double*)(w + L.r64);
     a.r0 = (float*)(w + L.r0);
     a.r1 = (float*)(w + L.r1);
@@ -551,7 +649,7 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     const int nunits = a.B * a.T;
     const dim3 grid((unsigned)((nunits + kXcds - 1) / kXcds * kXcds)), block((unsigned)threads);
     // the step kernel's LDS tile of the stencil operand (FP32)
-    const size_t lds = MJ_LDS_SWEEPS ? lds_tile_bytes(cfg->sNx, cfg->sNy, (int)sizeof(float)) : 0;
+    const size_t lds = lds_tile_bytes(cfg->sNx, cfg->sNy, (int)sizeof(float));
     if (lds > 65536)
         MJ_HIP(hipFuncSetAttribute((const void*)mx_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(mx_reset_kernel, dim3(1), dim3(64), 0, st, a.nactive, a.B);
