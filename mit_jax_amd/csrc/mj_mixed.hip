@@ -7,8 +7,8 @@
 //   err = sum r^2                                    (FP64); stop when err < tol^2
 //   sigma = sqrt(err); r32 = r / sigma               (inner right-hand side of unit norm)
 //   PCG on A32 d = r32 from d = 0 in FP32 with FP32 operator/preconditioner copies; every dot product is formed
-//   and summed in FP64 (the fixed MJ_SUM_TREE order) and alpha/beta are FP64 ratios; the inner solve stops when
-//   its recursive residual falls below inner_tolerance (or after inner_max_iters)
+//   and summed in FP64 in a fixed order (below) and alpha/beta are FP64 ratios; the inner solve stops when its
+//   recursive residual falls below inner_tolerance (or after inner_max_iters)
 //   x = x + sigma * d                                (FP64 solution update)
 // and repeat. max_iters caps the total inner iterations and max_outer the refinements; a member that reaches a
 // cap finishes with converged = 0. Un-normalisation and the halos of x are as cg2d_fortran leaves them.
@@ -17,15 +17,23 @@
 // per-tile partials summed in tile order. The FP32 work vectors (r32, s, d, z, q) are the solver's own and live in a
 // compact layout: per member and tile the interior only, row-major sNy x sNx, tiles 128-byte aligned. The pointwise
 // updates stream them as flat 16-byte accesses; the stencil operand is staged in an LDS tile whose ring is computed
-// from the sources' compact values; the stencils stream the FP32 coefficient rows down each wave's strip. Members move through their own state
-// machine, one state per launch ("step"): every block of a member derives the member's action for the step from
-// the member's record of the previous step and the partials, so all tiles of a member act alike; tile 0 writes
-// the record of the step (records are double-buffered by step parity). States:
+// from the sources' compact values; the stencils stream the FP32 coefficient rows down each wave's strip.
+//
+// Members move through their own state machine, one state per launch ("step"): every block of a member derives the
+// member's action for the step from the member's record of the previous step and the partials, so all tiles of a
+// member act alike; tile 0 writes the record of the step (records are double-buffered by step parity). States:
 //   RST   err from the true-residual partials; finish, or restart the inner solve (r32, d = 0, eta(1))
-//   IB    inner phase B(n): inner stop -> x += sigma d (next: TRUE); else s = z + beta s, q = A32 s, sigma_n = s.q
-//   IC    inner phase C(n): d += alpha s; r32 -= alpha q; z = M32 r32; err(n) = r32.r32, eta(n+1) = z.r32
+//   IB    inner phase B(n): d += alpha_{n-1} s_{n-1} (the solution update of iteration n-1, moved here where
+//         s_{n-1} is read anyway); inner stop -> x += sigma d (next: TRUE); else s = z + beta s, q = A32 s,
+//         sigma_n = s.q
+//   IC    inner phase C(n): r32 -= alpha q; z = M32 r32; err(n) = r32.r32, eta(n+1) = z.r32
 // (z and q are kept as FP32 vectors, so each inner phase applies one stencil and reads one coefficient set.)
 //   TRUE  x's ring from its sources; r = bn - A x; err partials
+//
+// Inner dot products (FP64): per tile, the rows in fixed groups of kRowGroup consecutive rows; each column pair is
+// accumulated down the group's rows in row order, the group's column sums are reduced by the row tree of the
+// "tile" order (pair_tree), and the group sums are added in group order. The order depends on the tile shape only,
+// not on the launch geometry. The true-residual sums (FP64 refinement) use the "tile" order itself.
 #include <algorithm>
 
 #include "mj_cg_sweep.h"
@@ -94,33 +102,67 @@ __device__ __forceinline__ void lds_put4(float* tile, const LdsGeo& G, int e, fl
     st2(tile + G.at(j1, e + 2 - j1 * sNx), make_float2(v.z, v.w));
 }
 
-// Stencil over this wave's strip of the LDS tile: IS_A: A32 = (W, S, C) = (aW, aS, aC); else M32 = (pW, pS, pC).
-// The result of each row goes to out (compact, row j at j * sNx) and row_cb(j, L, Xc, v) forms the row's partials.
-template <bool IS_A, class F>
+// Stencil over the LDS tile: IS_A: A32 = (W, S, C) = (aW, aS, aC); else M32 = (pW, pS, pC). Rows go in groups of
+// kRowGroup, group g to wave g % nwaves; the result of each row goes to out (compact, row j at j * sNx) and
+// prods(Xc, v, pv) gives the row's NR per-point products, reduced in the inner dot-product order; tile sum r of
+// the block is written to *dst[r].
+#ifndef MJ_ROW_GROUP
+#define MJ_ROW_GROUP 6
+#endif
+constexpr int kRowGroup = MJ_ROW_GROUP;
+template <bool IS_A, int NR, class F>
 __device__ __forceinline__ void mx_stencil(const MxDev& a, int t, const float* __restrict__ cW,
                                            const float* __restrict__ cS, const float* __restrict__ cC,
-                                           const float* tile, float* __restrict__ out, F&& row_cb) {
+                                           const float* tile, float* __restrict__ out, F&& prods,
+                                           double* const (&dst)[NR]) {
     MJ_UNIT_IDS
-    MJ_STRIP
+    const Lanes L = lanes_of(lane, sNx);
     const LdsGeo G{sNx + 4};
-    if (j0 >= j1) return;
     const int ce = east_col(L, sNx);
-    int pr = MJ_ROW(j0);
-    float2 S0 = ld2(cS + pr + L.c);
-    for (int j = j0; j < j1; ++j, pr += nx) {
-        const int p = pr + L.c;
-        const float2 S1 = ld2(cS + p + nx), C = ld2(cC + p), Wc = ld2(cW + pr + ce);
-        float wr = __shfl_down(Wc.x, 1, 64);
-        if (L.last == 63 && lane == 63) wr = cW[pr + sNx];
-        const int q0 = G.at(j, L.c);
-        const float2 Xc = ld2(tile + q0), Xm = ld2(tile + q0 - G.W), Xp = ld2(tile + q0 + G.W);
-        const float xl = tile[q0 - 1], xr = tile[q0 + 2];
-        const float2 v = IS_A ? pair_A(Wc, wr, S0, S1, C, Xm, Xc, Xp, xl, xr)
-                              : pair_M(C, Wc, wr, S0, S1, Xm, Xc, Xp, xl, xr);
-        if (L.act) st2(out + j * sNx + L.c, v);
-        row_cb(j, L, Xc, v);
-        S0 = S1;
+    const int ng = (sNy + kRowGroup - 1) / kRowGroup;
+    for (int g = wave; g < ng; g += nwaves) {
+        const int j0 = g * kRowGroup, j1 = min(j0 + kRowGroup, sNy);
+        double2 acc[NR];
+        int pr = MJ_ROW(j0);
+        float2 S0 = ld2(cS + pr + L.c);
+        for (int j = j0; j < j1; ++j, pr += nx) {
+            const int p = pr + L.c;
+            const float2 S1 = ld2(cS + p + nx), C = ld2(cC + p), Wc = ld2(cW + pr + ce);
+            float wr = __shfl_down(Wc.x, 1, 64);
+            if (L.last == 63 && lane == 63) wr = cW[pr + sNx];
+            const int q0 = G.at(j, L.c);
+            const float2 Xc = ld2(tile + q0), Xm = ld2(tile + q0 - G.W), Xp = ld2(tile + q0 + G.W);
+            const float xl = tile[q0 - 1], xr = tile[q0 + 2];
+            const float2 v = IS_A ? pair_A(Wc, wr, S0, S1, C, Xm, Xc, Xp, xl, xr)
+                                  : pair_M(C, Wc, wr, S0, S1, Xm, Xc, Xp, xl, xr);
+            if (L.act) st2(out + j * sNx + L.c, v);
+            double2 pv[NR];
+            prods(Xc, v, pv);
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (!L.act) pv[k] = make_double2(0.0, 0.0);
+                if (j == j0) {
+                    acc[k] = pv[k];
+                } else {
+                    acc[k].x = acc[k].x + pv[k].x;
+                    acc[k].y = acc[k].y + pv[k].y;
+                }
+            }
+            S0 = S1;
+        }
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const double w = pair_tree(acc[k], sNx, lane);
+            if (lane == 0) g_rows[k * 128 + g] = w;
+        }
     }
+    __syncthreads();
+    if (tid < NR) {
+        double sum = g_rows[tid * 128];
+        for (int g = 1; g < ng; ++g) sum = sum + g_rows[tid * 128 + g];
+        *dst[tid] = sum;
+    }
+    __syncthreads();
 }
 
 // I1: x = normalised first guess with exchanged halos; r = bn - A x (FP64); partials of err and sumRHS; the
@@ -195,25 +237,34 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
     }
     __syncthreads();
     const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
-    mx_stencil<false>(a, t, a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, tile, a.z + cb,
-                      [&](int j, const Lanes& L2, float2 Rc, float2 q) {
-                          double2 h = make_double2((double)q.x * (double)Rc.x, (double)q.y * (double)Rc.y);
-                          if (!L2.act) h = make_double2(0.0, 0.0);
-                          put_row2<MJ_SUM_TREE>(0, j, L2, h, nullptr, sNx, sNy);
-                      });
-    store_tile_sums<MJ_SUM_TREE>(a.peta + a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
+    double* const dst[1] = {a.peta + a.B * a.T + m * a.T + t};  // eta(1)
+    mx_stencil<false, 1>(
+        a, t, a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, tile, a.z + cb,
+        [](float2 Rc, float2 q, double2(&pv)[1]) {
+            pv[0] = make_double2((double)q.x * (double)Rc.x, (double)q.y * (double)Rc.y);
+        },
+        dst);
 }
 
-// Solution update of a finished inner solve: x = x + sigma * d on the interior (FP64).
-__device__ __forceinline__ void mx_update(const MxDev& a, int m, int t, double sigma) {
+// Solution update of a finished inner solve at B(n): the pending d += alpha_{n-1} s_{n-1} (n > 1), then
+// x = x + sigma * d on the interior (FP64).
+__device__ __forceinline__ void mx_update(const MxDev& a, int m, int t, int n, double sigma, double alpha64) {
     MJ_UNIT_IDS
     MJ_STRIP
     double* x = a.x + m * a.ms;
-    const float* d = a.d + cbase(a, m, t);
+    const int64_t cb = cbase(a, m, t);
+    const float* d = a.d + cb;
+    const float* so = S32(a, n - 1) + cb;
+    const float alpha = (float)alpha64;
     for (int j = j0; j < j1; ++j) {
         const int p = MJ_ROW(j) + L.c;
         const double2 X = ld2(x + p);
-        const float2 D = ld2(d + j * sNx + L.c);
+        float2 D = ld2(d + j * sNx + L.c);
+        if (n > 1) {
+            const float2 S = ld2(so + j * sNx + L.c);
+            D.x = D.x + alpha * S.x;
+            D.y = D.y + alpha * S.y;
+        }
         if (L.act) st2(x + p, make_double2(X.x + sigma * (double)D.x, X.y + sigma * (double)D.y));
     }
 }
@@ -262,25 +313,30 @@ __device__ __forceinline__ void mx_true(const MxDev& a, int m, int t) {
     store_tile_sums<MJ_SUM_TREE>(a.perr64, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
 }
 
-// Inner phase B(n): s_n = z + beta s_{n-1} (s_1 = z) on the interior (compact, flat) and the ring (LDS),
-// q = A32 s_n, sigma_n partial (s_n, q).
-__device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64) {
+// Inner phase B(n): d += alpha_{n-1} s_{n-1} and s_n = z + beta s_{n-1} (s_1 = z) on the interior (compact, flat),
+// s_n on the ring (LDS), q = A32 s_n, sigma_n partial (s_n, q).
+__device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64, double alpha64) {
     MJ_UNIT_IDS
     float* tile = reinterpret_cast<float*>(mj_lds);
     const LdsGeo G{sNx + 4};
     const float inv = 1.0f / (float)sNx;
-    const int64_t cm = m * a.cs, cb = cbase(a, m, t);
+    const int64_t cm = m * a.cs, cb = cbase(a, m, t), ot = (int64_t)t * a.TS;
     const float *z = a.z + cm, *so = S32(a, n - 1) + cm;
-    float* sn = S32(a, n) + cb;
-    const float beta = (float)beta64;
+    float *sn = S32(a, n) + cb, *d = a.d + cb;
+    const float beta = (float)beta64, alpha = (float)alpha64;
     const bool first = n == 1;
     const int nf4 = sNx * sNy / 4;
     for (int f = tid; f < nf4; f += blockDim.x) {
         const int e = 4 * f;
-        const int64_t o = (int64_t)t * a.TS + e;
-        float4 v = ld4(z + o);
+        float4 v = ld4(z + ot + e);
         if (!first) {
-            const float4 O = ld4(so + o);
+            const float4 O = ld4(so + ot + e), D = ld4(d + e);
+            float4 dn;
+            dn.x = D.x + alpha * O.x;
+            dn.y = D.y + alpha * O.y;
+            dn.z = D.z + alpha * O.z;
+            dn.w = D.w + alpha * O.w;
+            st4(d + e, dn);
             v.x = v.x + beta * O.x;
             v.y = v.y + beta * O.y;
             v.z = v.z + beta * O.z;
@@ -297,42 +353,35 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
     }
     __syncthreads();
     const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
-    mx_stencil<true>(a, t, a.aW32 + oc, a.aS32 + oc, a.aC32 + oc, tile, a.q + cb,
-                     [&](int j, const Lanes& L2, float2 Xc, float2 q) {
-                         double2 v = make_double2((double)Xc.x * (double)q.x, (double)Xc.y * (double)q.y);
-                         if (!L2.act) v = make_double2(0.0, 0.0);
-                         put_row2<MJ_SUM_TREE>(0, j, L2, v, nullptr, sNx, sNy);
-                     });
-    store_tile_sums<MJ_SUM_TREE>(a.psig + (n & 1) * a.B * a.T, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx,
-                                 sNy);
+    double* const dst[1] = {a.psig + (n & 1) * a.B * a.T + m * a.T + t};
+    mx_stencil<true, 1>(
+        a, t, a.aW32 + oc, a.aS32 + oc, a.aC32 + oc, tile, a.q + cb,
+        [](float2 Xc, float2 q, double2(&pv)[1]) {
+            pv[0] = make_double2((double)Xc.x * (double)q.x, (double)Xc.y * (double)q.y);
+        },
+        dst);
 }
 
-// Inner phase C(n): d += alpha s_n; r32_n = r32_{n-1} - alpha q on the interior (compact, flat) and the ring (LDS);
-// z = M32 r32_n; partials of err(n) = (r32_n, r32_n) and eta(n+1) = (z, r32_n).
+// Inner phase C(n): r32_n = r32_{n-1} - alpha q on the interior (compact, flat) and the ring (LDS); z = M32 r32_n;
+// partials of err(n) = (r32_n, r32_n) and eta(n+1) = (z, r32_n).
 __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, double alpha64) {
     MJ_UNIT_IDS
     float* tile = reinterpret_cast<float*>(mj_lds);
     const LdsGeo G{sNx + 4};
     const float inv = 1.0f / (float)sNx;
-    const int64_t cm = m * a.cs, cb = cbase(a, m, t);
-    const float *sv = S32(a, n) + cb, *qv = a.q + cm, *rin = R32(a, n - 1) + cm;
-    float *rout = R32(a, n) + cb, *d = a.d + cb;
+    const int64_t cm = m * a.cs, cb = cbase(a, m, t), ot = (int64_t)t * a.TS;
+    const float *qv = a.q + cm, *rin = R32(a, n - 1) + cm;
+    float* rout = R32(a, n) + cb;
     const float alpha = (float)alpha64;
     const int nf4 = sNx * sNy / 4;
-    const int64_t ot = (int64_t)t * a.TS;
     for (int f = tid; f < nf4; f += blockDim.x) {
         const int e = 4 * f;
-        const float4 S = ld4(sv + e), Q = ld4(qv + ot + e), R = ld4(rin + ot + e), D = ld4(d + e);
-        float4 dn, rn;
-        dn.x = D.x + alpha * S.x;
-        dn.y = D.y + alpha * S.y;
-        dn.z = D.z + alpha * S.z;
-        dn.w = D.w + alpha * S.w;
+        const float4 Q = ld4(qv + ot + e), R = ld4(rin + ot + e);
+        float4 rn;
         rn.x = R.x - alpha * Q.x;
         rn.y = R.y - alpha * Q.y;
         rn.z = R.z - alpha * Q.z;
         rn.w = R.w - alpha * Q.w;
-        st4(d + e, dn);
         st4(rout + e, rn);
         lds_put4(tile, G, e, rn, sNx, inv);
     }
@@ -343,20 +392,15 @@ __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, doubl
     }
     __syncthreads();
     const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
-    mx_stencil<false>(a, t, a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, tile, a.z + cb,
-                      [&](int j, const Lanes& L2, float2 Rc, float2 zz) {
-                          double2 e = make_double2((double)Rc.x * (double)Rc.x, (double)Rc.y * (double)Rc.y);
-                          double2 h = make_double2((double)zz.x * (double)Rc.x, (double)zz.y * (double)Rc.y);
-                          if (!L2.act) {
-                              e = make_double2(0.0, 0.0);
-                              h = e;
-                          }
-                          put_row2<MJ_SUM_TREE>(0, j, L2, e, nullptr, sNx, sNy);
-                          put_row2<MJ_SUM_TREE>(1, j, L2, h, nullptr, sNx, sNy);
-                      });
     const int BT = a.B * a.T;
-    store_tile_sums<MJ_SUM_TREE>(a.perr + (n & 1) * BT, a.peta + ((n + 1) & 1) * BT, m * a.T + t, lane, wave, nwaves,
-                                 nullptr, sNx, sNy);
+    double* const dst[2] = {a.perr + (n & 1) * BT + m * a.T + t, a.peta + ((n + 1) & 1) * BT + m * a.T + t};
+    mx_stencil<false, 2>(
+        a, t, a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, tile, a.z + cb,
+        [](float2 Rc, float2 zz, double2(&pv)[2]) {
+            pv[0] = make_double2((double)Rc.x * (double)Rc.x, (double)Rc.y * (double)Rc.y);
+            pv[1] = make_double2((double)zz.x * (double)Rc.x, (double)zz.y * (double)Rc.y);
+        },
+        dst);
 }
 
 __global__ void __launch_bounds__(MJ_MX_THREADS) mx_init_kernel(MxDev a, int ph) {
@@ -406,6 +450,10 @@ mx_step_kernel(MxDev a, int k) {
             case MX_IB: {
                 bool stop = r.iters >= a.max_iters || r.n - 1 >= a.inner_max;
                 if (!stop && r.n > 1) stop = tile_order_sum(a.perr + ((r.n - 1) & 1) * BT + m * a.T, a.T) < a.itol_sq;
+                // alpha_{n-1} of the pending solution update (the value C(n-1) used)
+                if (r.n > 1)
+                    g_bcast[3] = tile_order_sum(pe + ((r.n - 1) & 1) * BT, a.T) /
+                                 tile_order_sum(a.psig + ((r.n - 1) & 1) * BT + m * a.T, a.T);
                 if (stop) {
                     act = AX_UPD;
                     nr.mode = MX_TRUE;
@@ -445,12 +493,12 @@ mx_step_kernel(MxDev a, int k) {
     }
     __syncthreads();
     const int act = g_ibcast, n = (int)g_bcast[2];
-    const double c = g_bcast[0], sigma = g_bcast[1];
+    const double c = g_bcast[0], sigma = g_bcast[1], alpha_prev = g_bcast[3];
     switch (act) {
         case AX_FINISH: mx_finish(a, m, t); break;
         case AX_RESTART: mx_restart(a, m, t, sigma); break;
-        case AX_UPD: mx_update(a, m, t, sigma); break;
-        case AX_IB: mx_IB(a, m, t, n, c); break;
+        case AX_UPD: mx_update(a, m, t, n, sigma, alpha_prev); break;
+        case AX_IB: mx_IB(a, m, t, n, c, alpha_prev); break;
         case AX_IC: mx_IC(a, m, t, n, c); break;
         default: mx_true(a, m, t); break;
     }
