@@ -192,27 +192,33 @@ __device__ __forceinline__ void group_phase_B(const CgDev& a, int gi, int t, int
     __shared__ int s_m[kGroup], s_code[kGroup], s_act[kGroup], s_nm;
     __shared__ double s_beta[kGroup], s_err[kGroup];
     const int tid = threadIdx.x, BT = a.B * a.T;
-    if (tid < kGroup) {
-        const int m = a.grp[gi * kGroup + tid];
+    // wave 0 derives each member's status (the partial sums loaded by the lanes in parallel), lane 0 records it
+    for (int k = 0; k < kGroup && tid < 64; ++k) {
+        const int m = a.grp[gi * kGroup + k];
         int code = 0;
         double beta = 0.0, errp = 0.0;
         if (m >= 0) {
             const int st = a.mdone[m];
             if (!(st >= 0 && st < n - 1)) {
-                errp = tile_order_sum(a.perr + ((n - 1) & 1) * BT + m * a.T, a.T);
+                const double* pe = a.peta + m * a.T;
+                const double* const src[3] = {a.perr + ((n - 1) & 1) * BT + m * a.T, pe + (n & 1) * BT,
+                                              pe + ((n - 1) & 1) * BT};
+                double v[3];
+                wave_tile_sums<3>(src, a.T, v);
+                errp = v[0];
                 code = (!(n - 1 < a.max_iters) || errp < a.tol_sq) ? 1 : 2;
                 if (code == 2) {
-                    const double* pe = a.peta + m * a.T;
-                    const double eta = tile_order_sum(pe + (n & 1) * BT, a.T);
-                    const double etaNM1 = n == 1 ? 1.0 : tile_order_sum(pe + ((n - 1) & 1) * BT, a.T);
-                    beta = eta / etaNM1;  // cg2d.F:268-271
+                    const double etaNM1 = n == 1 ? 1.0 : v[2];
+                    beta = v[1] / etaNM1;  // cg2d.F:268-271
                 }
             }
         }
-        s_m[tid] = m;
-        s_code[tid] = code;
-        s_beta[tid] = beta;
-        s_err[tid] = errp;
+        if (tid == 0) {
+            s_m[k] = m;
+            s_code[k] = code;
+            s_beta[k] = beta;
+            s_err[k] = errp;
+        }
     }
     __syncthreads();
     if (tid == 0) {
@@ -250,15 +256,19 @@ __device__ __forceinline__ void group_phase_C(const CgDev& a, int gi, int t, int
     __shared__ int s_m[kGroup], s_act[kGroup], s_nm;
     __shared__ double s_alpha[kGroup];
     const int tid = threadIdx.x, BT = a.B * a.T;
-    if (tid < kGroup) {
-        const int m = a.grp[gi * kGroup + tid];
-        s_m[tid] = m;
-        s_alpha[tid] = 0.0;
-        if (m >= 0 && a.mdone[m] < 0) {
-            const double* pe = a.peta + (n & 1) * BT + m * a.T;
-            s_alpha[tid] = tile_order_sum(pe, a.T) / tile_order_sum(a.psig + (n & 1) * BT + m * a.T, a.T);
-        } else {
-            s_m[tid] = -1;
+    for (int k = 0; k < kGroup && tid < 64; ++k) {
+        const int m = a.grp[gi * kGroup + k];
+        const bool act = m >= 0 && a.mdone[m] < 0;
+        double alpha = 0.0;
+        if (act) {
+            const double* const src[2] = {a.peta + (n & 1) * BT + m * a.T, a.psig + (n & 1) * BT + m * a.T};
+            double v[2];
+            wave_tile_sums<2>(src, a.T, v);
+            alpha = v[0] / v[1];
+        }
+        if (tid == 0) {
+            s_m[k] = act ? m : -1;
+            s_alpha[k] = alpha;
         }
     }
     __syncthreads();

@@ -102,6 +102,30 @@ __device__ __forceinline__ double tile_order_sum(const double* p, int ntiles) {
     for (int t = 0; t < ntiles; ++t) s = s + p[t];
     return s;
 }
+// N tile-order sums computed by one whole wave: the lanes load the partials of all N sums at once (one memory
+// latency instead of N * ntiles dependent loads), then every lane adds them in tile order from lane reads. Same
+// order and bits as tile_order_sum; all 64 lanes of the wave must call it with the same arguments.
+template <int N>
+__device__ __forceinline__ void wave_tile_sums(const double* const (&p)[N], int ntiles, double (&out)[N]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = 0.0;
+    for (int t0 = 0; t0 < ntiles; t0 += 64) {
+        const int t = t0 + lane, n = ntiles - t0 < 64 ? ntiles - t0 : 64;
+        int lo[N], hi[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double v = t < ntiles ? p[k][t] : 0.0;
+            lo[k] = __double2loint(v);
+            hi[k] = __double2hiint(v);
+        }
+        for (int i = 0; i < n; ++i) {
+#pragma unroll
+            for (int k = 0; k < N; ++k)
+                out[k] = out[k] + __hiloint2double(__builtin_amdgcn_readlane(hi[k], i), __builtin_amdgcn_readlane(lo[k], i));
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------------------------------- grid barrier
 // A monotonic arrival counter: barrier number k (1, 2, ...) completes when the counter reaches k * gridDim.x.

@@ -423,19 +423,25 @@ mx_step_kernel(MxDev a, int k) {
     const int st = a.mdone[m];
     if (st >= 0 && st < k) return;  // finished in an earlier step
     const int BT = a.B * a.T;
-    if (threadIdx.x == 0) {
+    // wave 0 derives the member's action (every lane alike; the partial sums are loaded by the lanes in parallel),
+    // lane 0 publishes it
+    if (threadIdx.x < 64) {
         const MxRec r = a.rec[((k - 1) & 1) * a.B + m];
         MxRec nr = r;
         int act = 0;
-        double c = 0.0;
+        double c = 0.0, alpha_prev = 0.0;
         const double* pe = a.peta + m * a.T;
+        const double* ps = a.psig + m * a.T;
         switch (r.mode) {
             case MX_RST: {
-                const double e = tile_order_sum(a.perr64 + m * a.T, a.T);
+                const double* const src[2] = {a.perr64 + m * a.T, a.psum + m * a.T};
+                double v[2];
+                wave_tile_sums<2>(src, a.T, v);
+                const double e = v[0];
                 c = e;
-                if (t == 0 && r.outer == 0) {
+                if (t == 0 && r.outer == 0 && threadIdx.x == 0) {
                     a.diag[m].firstResidual = sqrt(e);
-                    a.diag[m].sumRHS = tile_order_sum(a.psum + m * a.T, a.T);
+                    a.diag[m].sumRHS = v[1];
                     a.diag[m].rhsMax = member_rhs_max(a, m);
                     a.diag[m].pad_ = 0;
                 }
@@ -449,45 +455,57 @@ mx_step_kernel(MxDev a, int k) {
             }
             case MX_IB: {
                 bool stop = r.iters >= a.max_iters || r.n - 1 >= a.inner_max;
-                if (!stop && r.n > 1) stop = tile_order_sum(a.perr + ((r.n - 1) & 1) * BT + m * a.T, a.T) < a.itol_sq;
-                // alpha_{n-1} of the pending solution update (the value C(n-1) used)
-                if (r.n > 1)
-                    g_bcast[3] = tile_order_sum(pe + ((r.n - 1) & 1) * BT, a.T) /
-                                 tile_order_sum(a.psig + ((r.n - 1) & 1) * BT + m * a.T, a.T);
+                if (r.n > 1) {
+                    // err(n-1) (inner stopping rule), eta(n), eta(n-1) (beta), sigma(n-1) (alpha_{n-1} of the
+                    // pending solution update: the value C(n-1) used)
+                    const double* const src[4] = {a.perr + ((r.n - 1) & 1) * BT + m * a.T, pe + (r.n & 1) * BT,
+                                                   pe + ((r.n - 1) & 1) * BT, ps + ((r.n - 1) & 1) * BT};
+                    double v[4];
+                    wave_tile_sums<4>(src, a.T, v);
+                    if (!stop) stop = v[0] < a.itol_sq;
+                    alpha_prev = v[2] / v[3];
+                    c = v[1] / v[2];
+                }
                 if (stop) {
                     act = AX_UPD;
                     nr.mode = MX_TRUE;
                 } else {
                     act = AX_IB;
-                    c = r.n == 1 ? 0.0
-                                 : tile_order_sum(pe + (r.n & 1) * BT, a.T) / tile_order_sum(pe + ((r.n - 1) & 1) * BT, a.T);
                     nr.mode = MX_IC;
                 }
+                if (r.n == 1) c = 0.0;
                 break;
             }
-            case MX_IC:
+            case MX_IC: {
+                const double* const src[2] = {pe + (r.n & 1) * BT, ps + (r.n & 1) * BT};
+                double v[2];
+                wave_tile_sums<2>(src, a.T, v);
                 act = AX_IC;
-                c = tile_order_sum(pe + (r.n & 1) * BT, a.T) / tile_order_sum(a.psig + (r.n & 1) * BT + m * a.T, a.T);
+                c = v[0] / v[1];
                 nr = MxRec{MX_IB, r.n + 1, r.iters + 1, r.outer, r.sigma};
                 break;
+            }
             default:
                 act = AX_TRUE;
                 nr.mode = MX_RST;
                 break;
         }
-        g_ibcast = act;
-        g_bcast[0] = c;
-        g_bcast[1] = act == AX_RESTART ? nr.sigma : r.sigma;
-        g_bcast[2] = (double)r.n;
-        if (t == 0) {
-            a.rec[(k & 1) * a.B + m] = nr;
-            if (act == AX_FINISH) {
-                a.diag[m].lastResidual = sqrt(c);
-                a.diag[m].numIters = r.iters;
-                a.diag[m].outerIters = r.outer;
-                a.diag[m].converged = c < a.tol_sq ? 1 : 0;
-                a.mdone[m] = k;
-                atomicSub(a.nactive, 1);
+        if (threadIdx.x == 0) {
+            g_bcast[3] = alpha_prev;
+            g_ibcast = act;
+            g_bcast[0] = c;
+            g_bcast[1] = act == AX_RESTART ? nr.sigma : r.sigma;
+            g_bcast[2] = (double)r.n;
+            if (t == 0) {
+                a.rec[(k & 1) * a.B + m] = nr;
+                if (act == AX_FINISH) {
+                    a.diag[m].lastResidual = sqrt(c);
+                    a.diag[m].numIters = r.iters;
+                    a.diag[m].outerIters = r.outer;
+                    a.diag[m].converged = c < a.tol_sq ? 1 : 0;
+                    a.mdone[m] = k;
+                    atomicSub(a.nactive, 1);
+                }
             }
         }
     }
