@@ -10,9 +10,16 @@ level counts). Inputs are seeded synthetic fields resident in HBM before timing 
 The headline (default) is the mixed-precision solve (FP32 inner PCG + FP64 refinement, mj_cg2d_solve_mixed); the
 FP64 solve (mj_cg2d_solve) is timed on the same step for `mixed_over_fp64`, and the two solutions are compared
 member by member (`error_vs_fp64`). Prints one JSON line. `roofline` is the solve (the dominant kernel), timed with
-HIP events on its stream: achieved = algorithmic bytes of the solve / solve time, with per inner iteration
-N2 * (6 coefficient + 6 vector) * s bytes (s = 4 mixed, 8 FP64; N2 = 105,300 interior points) and per refinement
-N2 * (5 + 3) * 8 bytes; `traffic` = measured HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
+HIP events on its stream: achieved = algorithmic bytes of the solve / solve time. Algorithmic bytes count the
+vector streams the fused recurrence must move per member (N2 = 105,300 interior points): mixed, per inner
+iteration 10 FP32 streams (B: z, s_{n-1}, d read, d, s_n, q written; C: q, r_{n-1} read, r_n, z written), per
+refinement 68 B per point (true residual x, b read and r written in FP64; restart r read, r32, d, z written;
+solution update x read and written, d, s read); FP64, per iteration 11 FP64 streams (B: z, s_{n-1} read, s_n, q
+written; C: s, q, r_{n-1}, x read, x, r_n, z written); both, once per member 48 B per point (first guess, initial
+residual, un-normalisation). The operator and preconditioner coefficients are shared by the members of an operator
+slot and served from cache: not counted. DST3: per (member, tile, level) 9 FP64 planes of 98 x 98 (tracer, u, v,
+hFacW, hFacS, recip_hFacC, maskW, maskS read, the tendency written), and the 2-D metrics (6 planes per tile)
+once. `traffic` = measured HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
 profile of this workload. `cpu_baseline` is the FP64 C oracle (OpenMP, all host cores) on a bounded sample of the
 same member-steps.
 """
@@ -42,7 +49,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--members", type=int, default=128)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
-    ap.add_argument("--no-autotune", action="store_true", help="run the default launch configuration, no graph")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the launch configuration of --block-threads / --inner-tolerance / --overlap, no graph")
+    ap.add_argument("--block-threads", type=int, default=0, help="with --no-autotune (0: the library default)")
+    ap.add_argument("--inner-tolerance", type=float, default=1e-5, help="with --no-autotune")
+    ap.add_argument("--overlap", action="store_true", help="with --no-autotune: DST3 concurrent with the solve")
     ap.add_argument("--no-fp64-reference", action="store_true",
                     help="with --precision mixed: skip timing the FP64 solve (the ratio and the error vs FP64)")
     ap.add_argument("--sum-order", choices=("tile", "fortran"), default="tile")
@@ -232,7 +243,10 @@ def main():
     def make(precision):
         st = EnsembleStep(E["ex"], E["ops"], E["op_index"], E["norm"], E["rhs"], E["x0"], params, E["g2d"],
                           E["vg"], E["u"], E["v"], E["tr"], E["hW"], E["hS"], E["rhC"], nlev=E["nlev_t"], tolerance=TOL,
-                          max_iters=2000, cfg=StepConfig(precision=precision, sum_order=args.sum_order))
+                          max_iters=2000, cfg=StepConfig(precision=precision, sum_order=args.sum_order,
+                                                         block_threads=args.block_threads,
+                                                         inner_tolerance=args.inner_tolerance,
+                                                         overlap=args.overlap))
         tuned = None
         if not args.no_autotune:
             best, results = autotune(st, precision, steps=2, warmup=1, log=log)
@@ -290,14 +304,12 @@ def main():
     conv = d["converged"].cpu().numpy()
     sel = main_step.cfg
     if args.precision == "fp64":
-        cg_bytes = float(iters.sum()) * N2 * (6 * 8 + 6 * 8)
+        cg_bytes = float(iters.sum()) * N2 * 11 * 8 + B * N2 * 48
         kernel = "cg2d_phase_kernel" if sel.launch != "persistent" else "cg2d_persistent_kernel"
     else:
-        # inner iterations: FP32 coefficients and vectors; refinements: FP64 true residual (5 coefficient/vector
-        # reads: aW, aS, aC, x, b) and r written, plus the restart's FP32 conversion (r read, r32 written, d zeroed)
-        cg_bytes = float(iters.sum()) * N2 * (6 * 4 + 6 * 4) + float(outer.sum()) * N2 * (5 * 8 + 3 * 8)
+        cg_bytes = float(iters.sum()) * N2 * 10 * 4 + float(outer.sum()) * N2 * 68 + B * N2 * 48
         kernel = "mx_step_kernel"
-    gad_bytes = float(sum(80 * 13 * 98 * 98 * int(n) + 6 * 13 * 98 * 98 * 8 for n in E["nlev"]))
+    gad_bytes = float(sum(9 * 8 * 13 * 98 * 98 * int(n) for n in E["nlev"])) + 6 * 8 * 13 * 98 * 98
     achieved = cg_bytes / (run["cg_ms"] * 1e-3) / 1e9
     ms = run["el"] / args.steps * 1e3
     cfg = {"workload": f"LLC90 ensemble: {B} members x (CG2D to 1e-12 normalised residual + one DST3 "
