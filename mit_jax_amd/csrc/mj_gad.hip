@@ -188,10 +188,21 @@ __device__ __forceinline__ void flux_boxes(const MjSweep& S, int dir, double* af
     }
 }
 
+// Box point idx (row-major, width bw) of box B: array index.
+__device__ __forceinline__ int box_point(const MjBox& B, int bw, float inv_bw, int idx) {
+    const int r = (int)(((float)idx + 0.5f) * inv_bw);  // idx / bw, exact for idx < 2^14 and bw <= 128
+    return (B.j0 + r) * GNX + B.i0 + (idx - r * bw);
+}
+
+#ifndef MJ_GAD_BATCH
+#define MJ_GAD_BATCH 2
+#endif  // points per thread whose global loads are issued together (the kernel is bound by load latency)
+
 __global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
     __shared__ double loc[GPLANE];
     __shared__ double af[GPLANE];
     __shared__ unsigned long long mlw[kMaskWords], mls[kMaskWords];
+    constexpr int NB = MJ_GAD_BATCH;
     const int tid = threadIdx.x, nthr = blockDim.x;
     int t, m, k;
     if (!gad_unit(a, t, m, k)) return;
@@ -200,79 +211,131 @@ __global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
     const int64_t b3 = m * a.ms3 + ((int64_t)t * a.K + k) * GPLANE;
     const int64_t bm = m * a.mms + ((int64_t)t * a.K + k) * GPLANE;
     const int64_t b2 = (int64_t)t * GPLANE;
-    const double* tr = a.tr + b3;
-    const double* u = a.u + b3;
-    const double* v = a.v + b3;
-    const double* hW = a.hFacW + b3;
-    const double* hS = a.hFacS + b3;
-    const double* rhC = a.recip_hFacC + b3;
+    const double* __restrict__ tr = a.tr + b3;
     const double* mW = a.maskW + bm;
     const double* mS = a.maskS + bm;
-    const double *dyG = a.dyG + b2, *dxG = a.dxG + b2, *rdxC = a.recip_dxC + b2, *rdyC = a.recip_dyC + b2;
-    const double *rrA = a.recip_rA + b2, *mInC = a.maskInC + b2;
+    const double* __restrict__ rhC = a.recip_hFacC + b3;
+    const double *__restrict__ rrA = a.recip_rA + b2, *__restrict__ mInC = a.maskInC + b2;
     const double dt = a.dTlev[k];
     const double deepFacC = a.deepFacC[k], rdeepC = a.recip_deepFacC[k], rdeep2C = a.recip_deepFac2C[k];
     const double rhoFacC = a.rhoFacC[k], rrhoFacC = a.recip_rhoFacC[k], drF = a.drF[k], rdrF = a.recip_drF[k];
     const double oneSixth = a.oneSixth;
 
-    for (int q = tid; q < GPLANE; q += nthr) loc[q] = tr[q];
-    build_mask_bits(a, tp, mW, mS, mlw, mls);
-
-    // uTrans = uFld * (dyG*deepFacC*drF*hFacW) * rhoFacC (gad.py:482-485), left to right
-    auto uTrans = [&](int q) { return u[q] * (dyG[q] * deepFacC * drF * hW[q]) * rhoFacC; };
-    auto vTrans = [&](int q) { return v[q] * (dxG[q] * deepFacC * drF * hS[q]) * rhoFacC; };
-    // update coefficient dt*recip_rhoFacC*recip_hFacC*recip_drF*recip_rA*recip_deepFac2C (gad.py:494)
-    auto coef = [&](int q) { return dt * rrhoFacC * rhC[q] * rdrF * rrA[q] * rdeep2C; };
-    // interface value at q from the tile state in loc; 0 outside the Fortran flux loops
-    auto flux = [&](int dir, int q) {
-        const int j = q / GNX, i = q - j * GNX;
-        if (dir == 0) {
-            if (i < 2 || i > GNX - 2) return 0.0;  // i = 1-OLx+2 .. sNx+OLx-1 (gad_dst3_adv_x.F:80)
-            const double cfl = fabs(u[q] * dt * rdxC[q] * rdeepC);
-            return dst3_flux(uTrans(q), cfl, oneSixth, loc[q - 2], loc[q - 1], loc[q], loc[q + 1], mbit(mlw, q - 1),
-                             mbit(mlw, q), mbit(mlw, q + 1));
+    for (int q0 = tid; q0 < GPLANE; q0 += NB * nthr) {
+        double v[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int q = q0 + b * nthr;
+            v[b] = q < GPLANE ? tr[q] : 0.0;
         }
-        if (j < 2 || j > GNY - 2) return 0.0;  // j = 1-OLy+2 .. sNy+OLy-1 (gad_dst3_adv_y.F:78)
-        const double cfl = fabs(v[q] * dt * rdyC[q] * rdeepC);
-        return dst3_flux(vTrans(q), cfl, oneSixth, loc[q - 2 * GNX], loc[q - GNX], loc[q], loc[q + GNX],
-                         mbit(mls, q - GNX), mbit(mls, q), mbit(mls, q + GNX));
-    };
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (q0 + b * nthr < GPLANE) loc[q0 + b * nthr] = v[b];
+    }
+    build_mask_bits(a, tp, mW, mS, mlw, mls);
 
     for (int pass = 0; pass < 3; ++pass) {
         for (int dir = 0; dir < 2; ++dir) {
             const MjSweep& S = tp.sweep[pass][dir];
             if (!S.run) continue;
+            // X: uFld, hFacW, dyG, recip_dxC along i; Y: vFld, hFacS, dxG, recip_dyC along j
+            const double* __restrict__ vel = (dir == 0 ? a.u : a.v) + b3;
+            const double* __restrict__ hf = (dir == 0 ? a.hFacW : a.hFacS) + b3;
+            const double* __restrict__ gl = (dir == 0 ? a.dyG : a.dxG) + b2;
+            const double* __restrict__ rdc = (dir == 0 ? a.recip_dxC : a.recip_dyC) + b2;
+            const unsigned long long* mb = dir == 0 ? mlw : mls;
+            const int d = dir == 0 ? 1 : GNX;
+            // uTrans / vTrans = vel * (gl*deepFacC*drF*hf) * rhoFacC (gad.py:482-485), left to right
+            auto trans = [&](double vv, double g, double h) { return vv * (g * deepFacC * drF * h) * rhoFacC; };
             if (S.fill_before) corner_fill(loc, a, tp, S.fill_before, tid);
-            if (S.nbox) {
-                flux_boxes(S, dir, af, [&](int q) { return flux(dir, q); });
-                __syncthreads();
+            // interface values over the faces the update boxes read (faces q and q + d of every box point)
+            for (int bx = 0; bx < S.nbox; ++bx) {
+                const MjBox B = S.box[bx];
+                const int i1 = B.i1 + (dir == 0 ? 1 : 0), j1 = B.j1 + (dir == 1 ? 1 : 0);
+                const int bw = i1 - B.i0, n = bw * (j1 - B.j0);
+                const float inv = 1.0f / (float)bw;
+                for (int i0 = tid; i0 < n; i0 += NB * nthr) {
+                    int q[NB];
+                    double V[NB], G[NB], H[NB], R[NB];
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        const int idx = i0 + b * nthr;
+                        q[b] = box_point(B, bw, inv, idx < n ? idx : i0);
+                        V[b] = vel[q[b]];
+                        G[b] = gl[q[b]];
+                        H[b] = hf[q[b]];
+                        R[b] = rdc[q[b]];
+                    }
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        if (i0 + b * nthr >= n) continue;
+                        const int qq = q[b], j = qq / GNX, i = qq - j * GNX;
+                        double f = 0.0;
+                        // 0 outside the Fortran flux loops: i = 1-OLx+2 .. sNx+OLx-1 (gad_dst3_adv_x.F:80),
+                        // j = 1-OLy+2 .. sNy+OLy-1 (gad_dst3_adv_y.F:78)
+                        const int c = dir == 0 ? i : j, cmax = dir == 0 ? GNX - 2 : GNY - 2;
+                        if (c >= 2 && c <= cmax) {
+                            const double cfl = fabs(V[b] * dt * R[b] * rdeepC);
+                            f = dst3_flux(trans(V[b], G[b], H[b]), cfl, oneSixth, loc[qq - 2 * d], loc[qq - d], loc[qq],
+                                          loc[qq + d], mbit(mb, qq - d), mbit(mb, qq), mbit(mb, qq + d));
+                        }
+                        af[qq] = f;
+                    }
+                }
             }
+            if (S.nbox) __syncthreads();
             if (S.fill_after) corner_fill(loc, a, tp, S.fill_after, tid);
+            // local - coef*(d(af) - tracer*d(trans))*maskInC (gad.py:511-514); update coefficient
+            // dt*recip_rhoFacC*recip_hFacC*recip_drF*recip_rA*recip_deepFac2C (gad.py:494)
             for (int bx = 0; bx < S.nbox; ++bx) {
                 const MjBox B = S.box[bx];
                 const int bw = B.i1 - B.i0, n = bw * (B.j1 - B.j0);
-                for (int idx = tid; idx < n; idx += nthr) {
-                    const int q = (B.j0 + idx / bw) * GNX + B.i0 + idx % bw;
-                    // local - coef*(d(af) - tracer*d(trans))*maskInC (gad.py:511-514)
-                    double daf, dtr;
-                    if (dir == 0) {
-                        daf = af[q + 1] - af[q];
-                        dtr = uTrans(q + 1) - uTrans(q);
-                    } else {
-                        daf = af[q + GNX] - af[q];
-                        dtr = vTrans(q + GNX) - vTrans(q);
+                const float inv = 1.0f / (float)bw;
+                for (int i0 = tid; i0 < n; i0 += NB * nthr) {
+                    int q[NB];
+                    double T0[NB], T1[NB], TR[NB], RH[NB], RA[NB], MI[NB];
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        const int idx = i0 + b * nthr;
+                        const int qq = box_point(B, bw, inv, idx < n ? idx : i0);
+                        q[b] = qq;
+                        T0[b] = trans(vel[qq], gl[qq], hf[qq]);
+                        T1[b] = trans(vel[qq + d], gl[qq + d], hf[qq + d]);
+                        TR[b] = tr[qq];
+                        RH[b] = rhC[qq];
+                        RA[b] = rrA[qq];
+                        MI[b] = mInC[qq];
                     }
-                    loc[q] = loc[q] - coef(q) * (daf - tr[q] * dtr) * mInC[q];
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        if (i0 + b * nthr >= n) continue;
+                        const int qq = q[b];
+                        const double daf = af[qq + d] - af[qq], dtr = T1[b] - T0[b];
+                        const double coef = dt * rrhoFacC * RH[b] * rdrF * RA[b] * rdeep2C;
+                        loc[qq] = loc[qq] - coef * (daf - TR[b] * dtr) * MI[b];
+                    }
                 }
             }
             if (S.nbox) __syncthreads();
         }
     }
     double* out = a.out + b3;
-    if (a.w) {
-        for (int q = tid; q < GPLANE; q += nthr) out[q] = loc[q];
-    } else {
-        for (int q = tid; q < GPLANE; q += nthr) out[q] = (loc[q] - tr[q]) / dt;  // gad.py:539
+    for (int q0 = tid; q0 < GPLANE; q0 += NB * nthr) {
+        double v[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int q = q0 + b * nthr;
+            v[b] = q < GPLANE ? tr[q] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int q = q0 + b * nthr;
+            if (q >= GPLANE) continue;
+            if (a.w)
+                out[q] = loc[q];
+            else
+                out[q] = (loc[q] - v[b]) / dt;  // gad.py:539
+        }
     }
 }
 

@@ -423,27 +423,50 @@ mx_step_kernel(MxDev a, int k) {
     const int st = a.mdone[m];
     if (st >= 0 && st < k) return;  // finished in an earlier step
     const int BT = a.B * a.T;
-    // wave 0 derives the member's action (every lane alike; the partial sums are loaded by the lanes in parallel),
-    // lane 0 publishes it
+    // wave 0 derives the member's action (every lane alike), lane 0 publishes it. The member's record and every
+    // partial a decision may need are loaded at once (lane t: tile t), so the decision costs one memory latency;
+    // the tile-order sums are then formed from lane reads.
     if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
         const MxRec r = a.rec[((k - 1) & 1) * a.B + m];
+        const bool lanes = a.T <= 64;  // partials held by the lanes (else summed from memory)
+        double P[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // perr64, psum, perr[0..1], peta[0..1], psig[0..1]
+        if (lanes && lane < a.T) {
+            const int i = m * a.T + lane;
+            P[0] = a.perr64[i];
+            P[1] = a.psum[i];
+            P[2] = a.perr[i];
+            P[3] = a.perr[BT + i];
+            P[4] = a.peta[i];
+            P[5] = a.peta[BT + i];
+            P[6] = a.psig[i];
+            P[7] = a.psig[BT + i];
+        }
+        // tile-order sum of partial array `p` (one of the above, v its lane values)
+        auto tsum = [&](double v, const double* p) {
+            if (!lanes) return tile_order_sum(p, a.T);
+            double sum = 0.0;
+            const int lo = __double2loint(v), hi = __double2hiint(v);
+            for (int i = 0; i < a.T; ++i)
+                sum = sum + __hiloint2double(__builtin_amdgcn_readlane(hi, i), __builtin_amdgcn_readlane(lo, i));
+            return sum;
+        };
         MxRec nr = r;
         int act = 0;
         double c = 0.0, alpha_prev = 0.0;
-        const double* pe = a.peta + m * a.T;
-        const double* ps = a.psig + m * a.T;
+        const int o = m * a.T;
         switch (r.mode) {
             case MX_RST: {
-                const double* const src[2] = {a.perr64 + m * a.T, a.psum + m * a.T};
-                double v[2];
-                wave_tile_sums<2>(src, a.T, v);
-                const double e = v[0];
+                const double e = tsum(P[0], a.perr64 + o);
                 c = e;
-                if (t == 0 && r.outer == 0 && threadIdx.x == 0) {
-                    a.diag[m].firstResidual = sqrt(e);
-                    a.diag[m].sumRHS = v[1];
-                    a.diag[m].rhsMax = member_rhs_max(a, m);
-                    a.diag[m].pad_ = 0;
+                if (t == 0 && r.outer == 0) {
+                    const double sr = tsum(P[1], a.psum + o);
+                    if (lane == 0) {
+                        a.diag[m].firstResidual = sqrt(e);
+                        a.diag[m].sumRHS = sr;
+                        a.diag[m].rhsMax = member_rhs_max(a, m);
+                        a.diag[m].pad_ = 0;
+                    }
                 }
                 if (e < a.tol_sq || r.iters >= a.max_iters || r.outer >= a.max_outer) {
                     act = AX_FINISH;
@@ -458,13 +481,14 @@ mx_step_kernel(MxDev a, int k) {
                 if (r.n > 1) {
                     // err(n-1) (inner stopping rule), eta(n), eta(n-1) (beta), sigma(n-1) (alpha_{n-1} of the
                     // pending solution update: the value C(n-1) used)
-                    const double* const src[4] = {a.perr + ((r.n - 1) & 1) * BT + m * a.T, pe + (r.n & 1) * BT,
-                                                   pe + ((r.n - 1) & 1) * BT, ps + ((r.n - 1) & 1) * BT};
-                    double v[4];
-                    wave_tile_sums<4>(src, a.T, v);
-                    if (!stop) stop = v[0] < a.itol_sq;
-                    alpha_prev = v[2] / v[3];
-                    c = v[1] / v[2];
+                    const int cp = r.n & 1, pp = (r.n - 1) & 1;
+                    const double errp = tsum(pp ? P[3] : P[2], a.perr + pp * BT + o);
+                    const double etan = tsum(cp ? P[5] : P[4], a.peta + cp * BT + o);
+                    const double etap = tsum(pp ? P[5] : P[4], a.peta + pp * BT + o);
+                    const double sigp = tsum(pp ? P[7] : P[6], a.psig + pp * BT + o);
+                    if (!stop) stop = errp < a.itol_sq;
+                    alpha_prev = etap / sigp;
+                    c = etan / etap;
                 }
                 if (stop) {
                     act = AX_UPD;
@@ -477,11 +501,11 @@ mx_step_kernel(MxDev a, int k) {
                 break;
             }
             case MX_IC: {
-                const double* const src[2] = {pe + (r.n & 1) * BT, ps + (r.n & 1) * BT};
-                double v[2];
-                wave_tile_sums<2>(src, a.T, v);
+                const int cp = r.n & 1;
+                const double etan = tsum(cp ? P[5] : P[4], a.peta + cp * BT + o);
+                const double sign = tsum(cp ? P[7] : P[6], a.psig + cp * BT + o);
                 act = AX_IC;
-                c = v[0] / v[1];
+                c = etan / sign;
                 nr = MxRec{MX_IB, r.n + 1, r.iters + 1, r.outer, r.sigma};
                 break;
             }
