@@ -412,6 +412,15 @@ __global__ void __launch_bounds__(MJ_MX_THREADS) mx_init_kernel(MxDev a, int ph)
         mx_I1(a, m, t);
 }
 
+// Tile-order sum of T <= 64 partials held by the lanes of one wave (lane t: tile t); every lane gets it.
+__device__ __forceinline__ double lane_tile_sum(double v, int T) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    double sum = 0.0;
+    for (int i = 0; i < T; ++i)
+        sum = sum + __hiloint2double(__builtin_amdgcn_readlane(hi, i), __builtin_amdgcn_readlane(lo, i));
+    return sum;
+}
+
 // One step of every member's state machine.
 #ifndef MJ_PH_WAVES_PER_EU
 #define MJ_PH_WAVES_PER_EU 8
@@ -423,6 +432,43 @@ mx_step_kernel(MxDev a, int k) {
     const int st = a.mdone[m];
     if (st >= 0 && st < k) return;  // finished in an earlier step
     const int BT = a.B * a.T;
+    const MxRec r0 = a.rec[((k - 1) & 1) * a.B + m];
+    if ((r0.mode == MX_IB || r0.mode == MX_IC) && a.T <= 64) {
+        // inner phases: every wave derives the action itself from the partials its lanes load (the same loads and
+        // sums in every wave, so every wave takes the same branch) and goes on without a barrier
+        const int lane = threadIdx.x & 63, o = m * a.T, n = r0.n, cp = n & 1, pp = (n - 1) & 1;
+        const bool tl = lane < a.T;
+        if (r0.mode == MX_IC) {
+            const double E = tl ? a.peta[cp * BT + o + lane] : 0.0, S = tl ? a.psig[cp * BT + o + lane] : 0.0;
+            const double alpha = lane_tile_sum(E, a.T) / lane_tile_sum(S, a.T);
+            if (t == 0 && threadIdx.x == 0) a.rec[(k & 1) * a.B + m] = MxRec{MX_IB, n + 1, r0.iters + 1, r0.outer, r0.sigma};
+            mx_IC(a, m, t, n, alpha);
+            return;
+        }
+        bool stop = r0.iters >= a.max_iters || n - 1 >= a.inner_max;
+        double beta = 0.0, alpha_prev = 0.0;
+        if (n > 1) {
+            // err(n-1) (inner stopping rule), eta(n), eta(n-1) (beta), sigma(n-1) (alpha_{n-1} of the pending
+            // solution update: the value C(n-1) used)
+            const double ER = tl ? a.perr[pp * BT + o + lane] : 0.0, EN = tl ? a.peta[cp * BT + o + lane] : 0.0;
+            const double EP = tl ? a.peta[pp * BT + o + lane] : 0.0, SP = tl ? a.psig[pp * BT + o + lane] : 0.0;
+            const double errp = lane_tile_sum(ER, a.T), etan = lane_tile_sum(EN, a.T);
+            const double etap = lane_tile_sum(EP, a.T), sigp = lane_tile_sum(SP, a.T);
+            if (!stop) stop = errp < a.itol_sq;
+            alpha_prev = etap / sigp;
+            beta = etan / etap;
+        }
+        if (t == 0 && threadIdx.x == 0) {
+            MxRec nr = r0;
+            nr.mode = stop ? MX_TRUE : MX_IC;
+            a.rec[(k & 1) * a.B + m] = nr;
+        }
+        if (stop)
+            mx_update(a, m, t, n, r0.sigma, alpha_prev);
+        else
+            mx_IB(a, m, t, n, beta, alpha_prev);
+        return;
+    }
     // wave 0 derives the member's action (every lane alike), lane 0 publishes it. The member's record and every
     // partial a decision may need are loaded at once (lane t: tile t), so the decision costs one memory latency;
     // the tile-order sums are then formed from lane reads.
