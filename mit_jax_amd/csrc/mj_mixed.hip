@@ -124,10 +124,16 @@ __device__ __forceinline__ void mx_stencil(const MxDev& a, int t, const float* _
         const int j0 = g * kRowGroup, j1 = min(j0 + kRowGroup, sNy);
         double2 acc[NR];
         int pr = MJ_ROW(j0);
-        float2 S0 = ld2(cS + pr + L.c);
+        // the coefficient rows of row j + 1 are loaded while row j is computed
+        float2 S0 = ld2(cS + pr + L.c), S1n = ld2(cS + pr + nx + L.c), Cn = ld2(cC + pr + L.c), Wn = ld2(cW + pr + ce);
         for (int j = j0; j < j1; ++j, pr += nx) {
             const int p = pr + L.c;
-            const float2 S1 = ld2(cS + p + nx), C = ld2(cC + p), Wc = ld2(cW + pr + ce);
+            const float2 S1 = S1n, C = Cn, Wc = Wn;
+            if (j + 1 < j1) {
+                S1n = ld2(cS + p + 2 * nx);
+                Cn = ld2(cC + p + nx);
+                Wn = ld2(cW + pr + nx + ce);
+            }
             float wr = __shfl_down(Wc.x, 1, 64);
             if (L.last == 63 && lane == 63) wr = cW[pr + sNx];
             const int q0 = G.at(j, L.c);
