@@ -36,11 +36,36 @@ def test_workspace_query_and_config_validation():
     from mit_jax_amd.layout import LLC90
     cfg = cg2d.config(LLC90, 4, 1e-12, 100, "tile")
     n = _lib.lib().mj_cg2d_workspace_bytes(ctypes.byref(cfg))
-    # r, s ping-pong buffers: 4 x 4 members x 124,852 doubles, plus partials
-    assert n >= 4 * 4 * LLC90.npad * 8
+    # r, s ping-pong buffers, z = M r and q = A s: 6 x 4 members x 124,852 doubles, plus partials
+    assert n >= 6 * 4 * LLC90.npad * 8
     bad = cg2d.config(LLC90, 4, 1e-12, 100, "tile", block_threads=100)
     assert _lib.lib().mj_cg2d_workspace_bytes(ctypes.byref(bad)) == 0
     assert "block_threads" in _lib.last_error()
+
+
+def test_mixed_workspace_and_config_validation():
+    """The mixed solve's workspace query: compact FP32 work vectors (7 x members x tiles x 8,128 values) besides the
+    FP64 residual; invalid configurations are refused with a message, before any device work."""
+    import ctypes
+    from mit_jax_amd import _lib, cg2d
+    from mit_jax_amd.layout import LLC90
+    L = _lib.lib()
+    B = 4
+    cfg = cg2d.config(LLC90, B, 1e-12, 100, "tile", launch="phases")
+    mcfg = _lib.Cg2dMixedConfig(1, 1e-5, 400, 40)
+    n = L.mj_cg2d_mixed_workspace_bytes(ctypes.byref(cfg), ctypes.byref(mcfg), 0)
+    assert n >= 7 * B * LLC90.nTiles * 8128 * 4 + B * LLC90.npad * 8
+    for field, value, word in (("sNy", 89, "sNy"), ("OL", 0, "OL"), ("sum_order", 0, "MJ_SUM_TREE"),
+                               ("tolerance", 0.0, "tolerance"), ("block_threads", 2048, "block_threads")):
+        bad = cg2d.config(LLC90, B, 1e-12, 100, "tile", launch="phases")
+        setattr(bad, field, value)
+        assert L.mj_cg2d_mixed_workspace_bytes(ctypes.byref(bad), ctypes.byref(mcfg), 0) == 0, field
+        assert word in _lib.last_error(), (field, _lib.last_error())
+    for mbad, word in ((_lib.Cg2dMixedConfig(1, 0.0, 400, 40), "inner_tolerance"),
+                       (_lib.Cg2dMixedConfig(1, 1e-5, 0, 40), "inner_max_iters"),
+                       (_lib.Cg2dMixedConfig(-1, 1e-5, 400, 40), "op_slots")):
+        assert L.mj_cg2d_mixed_workspace_bytes(ctypes.byref(cfg), ctypes.byref(mbad), 0) == 0
+        assert word in _lib.last_error(), (word, _lib.last_error())
 
 
 def test_unported_branches_raise():
