@@ -148,24 +148,51 @@ __device__ __forceinline__ bool gad_unit(const GadDev& a, int& t, int& m, int& k
     return true;
 }
 
-// maskLocW / maskLocS of one (member, tile, level) as LDS bit planes, built once per block from mask_loc (the masks
-// are 0/1 by construction, hFac != 0, so a bit and its double 0.0/1.0 carry the same value).
+// maskLocW / maskLocS of one (member, tile, level) as LDS bit planes (the masks are 0/1 by construction, hFac != 0,
+// so a bit and its double 0.0/1.0 carry the same value). Off the four corner halo squares mask_loc is the plain
+// mask, so every wave ballots the plain maskW / maskS of its words (all its loads in flight at once); then the
+// points of the corner squares of the corners the tile has are set from mask_loc.
 constexpr int kMaskWords = (GPLANE + 63) / 64;
 __device__ __forceinline__ void build_mask_bits(const GadDev& a, const MjTilePlan& tp, const double* mW,
                                                 const double* mS, unsigned long long* bw, unsigned long long* bs) {
-    const int lane = threadIdx.x & 63;
-    for (int q0 = threadIdx.x - lane; q0 < kMaskWords * 64; q0 += blockDim.x) {  // wave-uniform
-        const int q = q0 + lane;
-        bool w = false, sm = false;
-        if (q < GPLANE) {
-            const int j = q / GNX, i = q - j * GNX;
-            w = mask_loc(a, tp, mW, mS, j, i, true) != 0.0;
-            sm = mask_loc(a, tp, mW, mS, j, i, false) != 0.0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    constexpr int kWordsPerWave = 16;  // words of one wave whose loads are issued together
+    for (int w0 = wave; w0 < kMaskWords; w0 += nwaves * kWordsPerWave) {
+        double vw[kWordsPerWave], vs[kWordsPerWave];
+#pragma unroll
+        for (int r = 0; r < kWordsPerWave; ++r) {
+            const int q = (w0 + r * nwaves) * 64 + lane;
+            const bool in = w0 + r * nwaves < kMaskWords && q < GPLANE;
+            vw[r] = in ? mW[q] : 0.0;
+            vs[r] = in ? mS[q] : 0.0;
         }
-        const unsigned long long vw = __ballot(w), vs = __ballot(sm);
-        if (lane == 0) {
-            bw[q0 >> 6] = vw;
-            bs[q0 >> 6] = vs;
+#pragma unroll
+        for (int r = 0; r < kWordsPerWave; ++r) {
+            const unsigned long long bwv = __ballot(vw[r] != 0.0), bsv = __ballot(vs[r] != 0.0);
+            if (lane == 0 && w0 + r * nwaves < kMaskWords) {
+                bw[w0 + r * nwaves] = bwv;
+                bs[w0 + r * nwaves] = bsv;
+            }
+        }
+    }
+    __syncthreads();
+    // corner squares: corner c (sw, se, nw, ne), point p of its OL x OL square, mask u (W) or v (S)
+    const int OL = a.OLx, n = 4 * OL * OL;
+    const int tid = threadIdx.x;
+    if (tid < 2 * n) {
+        const int want_u = tid < n, r = tid % n, c = r / (OL * OL), pp = r % (OL * OL);
+        const bool on = c == 0 ? tp.sw : c == 1 ? tp.se : c == 2 ? tp.nw : tp.ne;
+        if (on) {
+            const int dj = pp / OL, di = pp % OL;
+            const int ja = (c < 2) ? dj : a.OLy + a.sNy + dj, ia = (c % 2 == 0) ? di : a.OLx + a.sNx + di;
+            const int q = ja * GNX + ia;
+            const bool v = mask_loc(a, tp, mW, mS, ja, ia, want_u) != 0.0;
+            unsigned long long* word = (want_u ? bw : bs) + (q >> 6);
+            const unsigned long long bit = 1ull << (q & 63);
+            if (v)
+                atomicOr(word, bit);
+            else
+                atomicAnd(word, ~bit);
         }
     }
     __syncthreads();
