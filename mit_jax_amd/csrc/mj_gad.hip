@@ -148,6 +148,31 @@ __device__ __forceinline__ bool gad_unit(const GadDev& a, int& t, int& m, int& k
     return true;
 }
 
+// The points of the corner squares of the corners the tile has: mask_loc (corner-filled) into the bit planes.
+__device__ __forceinline__ void fix_mask_corners(const GadDev& a, const MjTilePlan& tp, const double* mW,
+                                                 const double* mS, unsigned long long* bw, unsigned long long* bs) {
+    // corner c (sw, se, nw, ne), point p of its OL x OL square, mask u (W) or v (S)
+    const int OL = a.OLx, n = 4 * OL * OL;
+    const int tid = threadIdx.x;
+    if (tid < 2 * n) {
+        const int want_u = tid < n, r = tid % n, c = r / (OL * OL), pp = r % (OL * OL);
+        const bool on = c == 0 ? tp.sw : c == 1 ? tp.se : c == 2 ? tp.nw : tp.ne;
+        if (on) {
+            const int dj = pp / OL, di = pp % OL;
+            const int ja = (c < 2) ? dj : a.OLy + a.sNy + dj, ia = (c % 2 == 0) ? di : a.OLx + a.sNx + di;
+            const int q = ja * GNX + ia;
+            const bool v = mask_loc(a, tp, mW, mS, ja, ia, want_u) != 0.0;
+            unsigned long long* word = (want_u ? bw : bs) + (q >> 6);
+            const unsigned long long bit = 1ull << (q & 63);
+            if (v)
+                atomicOr(word, bit);
+            else
+                atomicAnd(word, ~bit);
+        }
+    }
+    __syncthreads();
+}
+
 // maskLocW / maskLocS of one (member, tile, level) as LDS bit planes (the masks are 0/1 by construction, hFac != 0,
 // so a bit and its double 0.0/1.0 carry the same value). Off the four corner halo squares mask_loc is the plain
 // mask, so every wave ballots the plain maskW / maskS of its words (all its loads in flight at once); then the
@@ -176,27 +201,48 @@ __device__ __forceinline__ void build_mask_bits(const GadDev& a, const MjTilePla
         }
     }
     __syncthreads();
-    // corner squares: corner c (sw, se, nw, ne), point p of its OL x OL square, mask u (W) or v (S)
-    const int OL = a.OLx, n = 4 * OL * OL;
-    const int tid = threadIdx.x;
-    if (tid < 2 * n) {
-        const int want_u = tid < n, r = tid % n, c = r / (OL * OL), pp = r % (OL * OL);
-        const bool on = c == 0 ? tp.sw : c == 1 ? tp.se : c == 2 ? tp.nw : tp.ne;
-        if (on) {
-            const int dj = pp / OL, di = pp % OL;
-            const int ja = (c < 2) ? dj : a.OLy + a.sNy + dj, ia = (c % 2 == 0) ? di : a.OLx + a.sNx + di;
-            const int q = ja * GNX + ia;
-            const bool v = mask_loc(a, tp, mW, mS, ja, ia, want_u) != 0.0;
-            unsigned long long* word = (want_u ? bw : bs) + (q >> 6);
-            const unsigned long long bit = 1ull << (q & 63);
-            if (v)
-                atomicOr(word, bit);
-            else
-                atomicAnd(word, ~bit);
+    fix_mask_corners(a, tp, mW, mS, bw, bs);
+}
+// The tracer plane into LDS and the mask bit planes, with the tracer's and the plain masks' loads in flight
+// together (one memory latency for the three planes of a 1024-thread block), then the corner-square fix-up.
+__device__ __forceinline__ void load_tile_and_masks(const GadDev& a, const MjTilePlan& tp, const double* tr,
+                                                    const double* mW, const double* mS, double* loc,
+                                                    unsigned long long* bw, unsigned long long* bs) {
+    const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
+    constexpr int KP = (GPLANE + 1023) / 1024, KW = (kMaskWords + 15) / 16;  // per thread / per wave at 1024
+    for (int it = 0; it * KP * nthr < GPLANE || it * KW * nwaves < kMaskWords; ++it) {
+        double v[KP], vw[KW], vs[KW];
+#pragma unroll
+        for (int b = 0; b < KP; ++b) {
+            const int q = (it * KP + b) * nthr + tid;
+            v[b] = q < GPLANE ? tr[q] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < KW; ++r) {
+            const int wd = (it * KW + r) * nwaves + wave, q = wd * 64 + lane;
+            const bool in = wd < kMaskWords && q < GPLANE;
+            vw[r] = in ? mW[q] : 0.0;
+            vs[r] = in ? mS[q] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < KP; ++b) {
+            const int q = (it * KP + b) * nthr + tid;
+            if (q < GPLANE) loc[q] = v[b];
+        }
+#pragma unroll
+        for (int r = 0; r < KW; ++r) {
+            const int wd = (it * KW + r) * nwaves + wave;
+            const unsigned long long bwv = __ballot(vw[r] != 0.0), bsv = __ballot(vs[r] != 0.0);
+            if (lane == 0 && wd < kMaskWords) {
+                bw[wd] = bwv;
+                bs[wd] = bsv;
+            }
         }
     }
     __syncthreads();
+    fix_mask_corners(a, tp, mW, mS, bw, bs);
 }
+
 __device__ __forceinline__ double mbit(const unsigned long long* b, int q) {
     return (double)((b[q >> 6] >> (q & 63)) & 1ull);
 }
@@ -248,18 +294,7 @@ __global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
     const double rhoFacC = a.rhoFacC[k], rrhoFacC = a.recip_rhoFacC[k], drF = a.drF[k], rdrF = a.recip_drF[k];
     const double oneSixth = a.oneSixth;
 
-    for (int q0 = tid; q0 < GPLANE; q0 += NB * nthr) {
-        double v[NB];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int q = q0 + b * nthr;
-            v[b] = q < GPLANE ? tr[q] : 0.0;
-        }
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-            if (q0 + b * nthr < GPLANE) loc[q0 + b * nthr] = v[b];
-    }
-    build_mask_bits(a, tp, mW, mS, mlw, mls);
+    load_tile_and_masks(a, tp, tr, mW, mS, loc, mlw, mls);
 
     for (int pass = 0; pass < 3; ++pass) {
         for (int dir = 0; dir < 2; ++dir) {
