@@ -21,7 +21,8 @@ slot and served from cache: not counted. DST3: per (member, tile, level) 9 FP64 
 hFacW, hFacS, recip_hFacC, maskW, maskS read, the tendency written), and the 2-D metrics (6 planes per tile)
 once. `traffic` = measured HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
 profile of this workload. `cpu_baseline` is the FP64 C oracle (OpenMP, all host cores) on a bounded sample of the
-same member-steps.
+same member-steps. `producers` times the operator builder (UPDATE_CG2D) and the rhs builder (cg2d_rhs) on the same
+ensemble, each alone (not part of the headline step, whose operators and rhs are inputs).
 """
 
 import argparse
@@ -59,6 +60,7 @@ def parse():
     ap.add_argument("--sum-order", choices=("tile", "fortran"), default="tile")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-producers", action="store_true", help="skip the UPDATE_CG2D / cg2d_rhs measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (the reference kernel for HBM counter scaling)")
     ap.add_argument("--profile-solves", action="store_true",
@@ -221,6 +223,78 @@ def profile_solves(B):
                       "mixed_iterations": int(d2["numIters"].max())}), flush=True)
 
 
+def producers(E, dev, reps=5):
+    """UPDATE_CG2D and cg2d_rhs (the producers either side of the solve) on the whole ensemble through the C ABI,
+    outputs preallocated, each timed alone with HIP events on the current stream (median of reps). Algorithmic
+    bytes per member: UPDATE_CG2D reads hFacW, hFacS on the (sNy+1) x (sNx+1) points of every level and writes
+    aW2d, aS2d, aC2d (whole planes) and pW, pS, pC ((sNy+1) x (sNx+1)); cg2d_rhs reads hFacW, gU on sNy x (sNx+1)
+    and hFacS, gV on (sNy+1) x sNx points of every level, etaN (whole plane), EmPmR, etaH (interior), and writes
+    cg2d_b, cg2d_x (whole planes). The GPU operators are checked bitwise against the CPU builder's."""
+    import ctypes
+    import torch
+    import mit_jax_amd._lib as L
+    from mit_jax_amd import synthetic as S
+    B, T, K, ny, nx = E["hW"].shape
+    sN = nx - 8
+    g0 = E["cls"][50][0]
+    f64 = dict(dtype=torch.float64, device=dev)
+    keep = {k: torch.as_tensor(np.ascontiguousarray(g0.f[k]), device=dev)
+            for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "rA", "maskInC")}
+    keep["drF"] = torch.as_tensor(np.ascontiguousarray(g0.v["drF"]), device=dev)
+    grid = L.SfpGrid(*(keep[k].data_ptr() for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "rA", "maskInC")), None,
+                     keep["drF"].data_ptr(), None, None, None)
+    cfg = L.SfpConfig(B, T, sN, sN, 4, K, T * K * ny * nx, T * ny * nx)
+    fs = L.FsParams(S.DT, S.DT, 1.0, 1.0, 1.0, 1.0 / 1029.0, 1)
+    rBo = torch.full((T, ny, nx), 1.0 / S.G, **f64)
+    Bo = torch.full((T, ny, nx), S.G, **f64)
+    outs = [torch.zeros((B, T, ny, nx), **f64) for _ in range(6)]
+    b, x = torch.empty((B, T, ny, nx), **f64), torch.empty((B, T, ny, nx), **f64)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4001)
+    eta, etaH = (torch.randn((B, T, ny, nx), generator=gen, **f64) for _ in range(2))
+    em = torch.randn((B, T, ny, nx), generator=gen, **f64) * 1e-5
+    plan = E["ex"].plan("T", dev)
+    P = L.ptr
+    lib = L.lib()
+
+    def upd():
+        L.check(lib.mj_update_cg2d(plan.handle, ctypes.byref(grid), ctypes.byref(fs), ctypes.byref(cfg), P(E["norm"]),
+                                   0.51, P(E["hW"]), P(E["hS"]), P(rBo), *(P(o) for o in outs), L.stream_ptr()),
+                "mj_update_cg2d")
+
+    def rhs():
+        L.check(lib.mj_cg2d_rhs(ctypes.byref(grid), ctypes.byref(fs), ctypes.byref(cfg), P(E["u"]), P(E["v"]),
+                                P(E["hW"]), P(E["hS"]), P(em), P(eta), P(etaH), P(Bo), P(b), P(x), L.stream_ptr()),
+                "mj_cg2d_rhs")
+
+    def t(fn):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        return float(np.median(ms))
+
+    print("bench: producers (UPDATE_CG2D, cg2d_rhs)", file=sys.stderr, flush=True)
+    tu, tr = t(upd), t(rhs)
+    same = all(bool(torch.equal(outs[k][m], E["ops"][k][int(E["op_index"][m])])) for k in range(6) for m in range(B))
+    p1, pu, pv, plane = (sN + 1) * (sN + 1), sN * (sN + 1), (sN + 1) * sN, ny * nx
+    ub = B * T * 8 * (2 * K * p1 + 3 * plane + 3 * p1)
+    rb = B * T * 8 * (2 * K * pu + 2 * K * pv + plane + 2 * sN * sN + 2 * plane)
+    res = {}
+    for name, ms, nb, kern in (("update_cg2d", tu, ub, "sfp_opsum_kernel"), ("cg2d_rhs", tr, rb, "sfp_rhs_kernel")):
+        gbs = nb / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "member_calls_per_s": B / (ms * 1e-3), "algorithmic_bytes": nb, "achieved_GBs": gbs,
+                     "frac": gbs / HBM_PEAK_GBS, "dominant_kernel": kern}
+    res["update_cg2d"]["operators_equal_cpu_builder"] = same
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -366,6 +440,8 @@ def main():
                                                   "advect_tiles (member's levels)",
                              "solutions_equal_batched_fp64": same}
         out["mixed_over_plain_fp64"] = out["value"] / pv
+    if not args.no_producers:
+        out["producers"] = producers(E, dev)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(E, args.cpu_seconds)

@@ -180,6 +180,58 @@ int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, cons
                         const mj_cg2d_mixed_config* mcfg, void* workspace, size_t workspace_bytes,
                         mj_cg2d_diag* diag, void* stream);
 
+/* ======================================================================================= CG2D operator and rhs
+ * The producers either side of the solve in SOLVE_FOR_PRESSURE (core/solve_for_pressure.py): the operator and
+ * preconditioner of UPDATE_CG2D (:29-65) and the right-hand side and first guess of cg2d_rhs (:68-106). Both are
+ * column sums over k in the reference's order (aW/aS: k = 1..Nr; the divergence of cg2d_b: k = Nr..1), FP64,
+ * bitwise equal to the literal loops. */
+typedef struct {
+    /* FreeSurfParams fields the two producers read (core/free_surface.py:33-52) */
+    double deltaTMom, deltaTFreeSurf, freeSurfFac, implicSurfPress, implicDiv2DFlow, mass2rUnit;
+    int useRealFreshWaterFlux;
+} mj_fs_params;
+
+typedef struct {
+    /* 2-D [T, ny, nx], shared by all members */
+    const double *dyG, *dxG, *recip_dxC, *recip_dyC, *rA, *maskInC;
+    const int32_t* kSurfC; /* 1-based first wet level of each column, nr + 1 when dry (k_surf of maskC); cg2d_rhs */
+    /* per level: drF [nr]; deepFacC, rhoFacC [nr] and deepFac2F [nr + 1] (NULL = all ones, the V4r4 values of
+       vertical_factors, core/free_surface.py:121-125) */
+    const double *drF, *deepFacC, *rhoFacC, *deepFac2F;
+} mj_sfp_grid;
+
+typedef struct {
+    int batch;               /* members */
+    int ntiles, sNx, sNy, OL; /* tile layout of the T plan */
+    int nr;                  /* levels of the 3-D arrays */
+    int64_t member_stride3d; /* elements between members' 3-D arrays ([T, nr, ny, nx]) */
+    int64_t member_stride2d; /* elements between members' 2-D arrays ([T, ny, nx]) */
+} mj_sfp_config;
+
+/* UPDATE_CG2D (core/solve_for_pressure.py:29-65). Per member m: aW2d = ((sum_k dyG drF(k) hFacW recip_dxC) *
+ * cg2dNorm[m]) * implicSurfPress * implicDiv2DFlow on j, i = 1..sNy+1, 1..sNx+1 and 0 elsewhere (aS2d likewise);
+ * aC2d = -(aW(i) + aW(i+1) + aS(j) + aS(j+1) + freeSurfFac cg2dNorm[m] recip_Bo rA / deltaTMom / deltaTFreeSurf) on
+ * the interior, 0 elsewhere, then exchanged with plan_T; pC, pW, pS rewritten on j, i = 1..sNy+1, 1..sNx+1 only
+ * (pC = 1/aC, 1 where aC = 0; pW = -aW / (cg2dpcOffDFac (aC(i) + aC(i-1)))^2, 0 where the sum is 0; pS likewise):
+ * their other points keep the values they hold. hFacW, hFacS: [B][T, nr, ny, nx]; recip_Bo: [T, ny, nx] shared;
+ * cg2dNorm: device [B]; outputs [B][T, ny, nx] (member_stride2d apart), none aliasing an input. */
+int mj_update_cg2d(const mj_halo_plan* plan_T, const mj_sfp_grid* grid, const mj_fs_params* p,
+                   const mj_sfp_config* cfg, const double* cg2dNorm, double cg2dpcOffDFac, const double* hFacW,
+                   const double* hFacS, const double* recip_Bo, double* aW2d, double* aS2d, double* aC2d, double* pW,
+                   double* pS, double* pC, void* stream);
+
+/* cg2d_rhs (core/solve_for_pressure.py:68-106; solve_for_pressure.F:124-255 with CALC_DIV_GHAT): cg2d_x = Bo_surf *
+ * etaN on the whole array; cg2d_b = 0 off the interior and on the interior
+ *   b = useRealFreshWaterFlux ? freeSurfFac mass2rUnit implicDiv2DFlow rA EmPmR / deltaTMom maskInC : 0
+ *   for k = nr..1: b = b + pfu(i+1) - pfu(i);  b = b + pfv(j+1) - pfv(j)
+ *     pfu = dyG deepFacC(k) drF(k) hFacW rhoFacC(k) gU / deltaTMom   (pfv with dxG, hFacS, gV)
+ *   b = b - freeSurfFac rA deepFac2F(kSurfC) / deltaTMom / deltaTFreeSurf etaH
+ * gU, gV, hFacW, hFacS: [B][T, nr, ny, nx]; EmPmR, etaN, etaH, cg2d_b, cg2d_x: [B][T, ny, nx]; Bo_surf: [T, ny,
+ * nx] shared; EmPmR may be NULL when useRealFreshWaterFlux is 0. Outputs must not alias inputs. */
+int mj_cg2d_rhs(const mj_sfp_grid* grid, const mj_fs_params* p, const mj_sfp_config* cfg, const double* gU,
+                const double* gV, const double* hFacW, const double* hFacS, const double* EmPmR, const double* etaN,
+                const double* etaH, const double* Bo_surf, double* cg2d_b, double* cg2d_x, void* stream);
+
 /* ======================================================================================= GAD DST3
  * Replaces gad_advection / advect_tiles (pkgs/gad.py:460-560) with dst3_adv_x/y (:409-453): the 3-pass
  * multi-dimensional DST3 on the LLC cubed sphere with per-facet sweep order, overlap/interior update regions and

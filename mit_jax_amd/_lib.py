@@ -61,6 +61,21 @@ class GadConfig(ctypes.Structure):
     _fields_ = [("batch", I32), ("nr", I32), ("member_stride3d", I64), ("mask_member_stride", I64), ("nlev", P)]
 
 
+class FsParams(ctypes.Structure):
+    _fields_ = [(k, F64) for k in ("deltaTMom", "deltaTFreeSurf", "freeSurfFac", "implicSurfPress", "implicDiv2DFlow",
+                                   "mass2rUnit")] + [("useRealFreshWaterFlux", I32)]
+
+
+class SfpGrid(ctypes.Structure):
+    _fields_ = [(k, P) for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "rA", "maskInC", "kSurfC", "drF",
+                                 "deepFacC", "rhoFacC", "deepFac2F")]
+
+
+class SfpConfig(ctypes.Structure):
+    _fields_ = [("batch", I32), ("ntiles", I32), ("sNx", I32), ("sNy", I32), ("OL", I32), ("nr", I32),
+                ("member_stride3d", I64), ("member_stride2d", I64)]
+
+
 # name -> (restype, argtypes): every entry point declared in include/mjamd.h
 SIGNATURES = {
     "mj_version": (I32, []),
@@ -80,6 +95,10 @@ SIGNATURES = {
                                                         I64]),
     "mj_cg2d_solve_mixed": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, P, ctypes.POINTER(Cg2dConfig),
                                   ctypes.POINTER(Cg2dMixedConfig), P, ctypes.c_size_t, P, P]),
+    "mj_update_cg2d": (I32, [P, ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, F64,
+                             P, P, P, P, P, P, P, P, P, P]),
+    "mj_cg2d_rhs": (I32, [ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, P, P, P,
+                          P, P, P, P, P, P, P]),
     "mj_gad_plan_create": (I32, [I32, P, P, I32, I32, I32, I32, ctypes.POINTER(P)]),
     "mj_gad_plan_destroy": (I32, [P]),
     "mj_gad_plan_describe": (I32, [P, P, P]),

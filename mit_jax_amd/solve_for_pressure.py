@@ -1,0 +1,179 @@
+"""The producers either side of the CG2D solve in SOLVE_FOR_PRESSURE, on the GPU, with the reference's interface
+(mitgcm_jax core/solve_for_pressure.py).
+
+`update_cg2d(p, cp, g, ex, hFacW, hFacS, recip_Bo, pW, pS, pC) -> (aW2d, aS2d, aC2d, pW, pS, pC)`
+(reference :29-65, update_cg2d.F:59-195): the 5-point operator from the column sums of the r* hFacW/S and the
+5-point preconditioner, through libmjamd's mj_update_cg2d. Returns new arrays; pW, pS, pC are copies of the
+arguments with j, i = 1..sNy+1, 1..sNx+1 rewritten, as the reference's `.at[].set`.
+
+`cg2d_rhs(p, g, gU, gV, hFacW, hFacS, EmPmR, etaN, etaH, Bo_surf) -> (cg2d_b, cg2d_x)` (reference :68-106,
+solve_for_pressure.F:124-255 with CALC_DIV_GHAT): through mj_cg2d_rhs.
+
+Both are bitwise equal to the literal Fortran-order loops. Field arrays are CUDA float64 [T, K, ny, nx] / [T, ny, nx],
+or with a leading member axis [B, ...] for an ensemble (grid fields, recip_Bo and Bo_surf shared). `g` is a grid
+object with `layout` (or `L`), `dyG, dxG, recip_dxC, recip_dyC, rA, maskInC, drF` (numpy or CUDA tensors); the
+vertical factors are V4r4's (all 1, reference core/free_surface.py:121-125), so the library skips them.
+"""
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .cg2d import _norm_tensor
+
+
+@dataclass(frozen=True)
+class FreeSurfParams:
+    """The FreeSurfParams fields these producers read (reference core/free_surface.py:33-52); V4r4 defaults.
+    A reference FreeSurfParams object with the same attribute names is accepted as well."""
+    deltaTMom: float
+    deltaTFreeSurf: float
+    freeSurfFac: float = 1.0
+    implicSurfPress: float = 1.0
+    implicDiv2DFlow: float = 1.0
+    mass2rUnit: float = 1.0 / 1029.0
+    useRealFreshWaterFlux: bool = True
+
+
+def _layout(g):
+    L = getattr(g, "layout", None)
+    return L if L is not None else g.L
+
+
+def _fs(p):
+    return _lib.FsParams(float(p.deltaTMom), float(p.deltaTFreeSurf), float(p.freeSurfFac), float(p.implicSurfPress),
+                         float(p.implicDiv2DFlow), float(p.mass2rUnit), 1 if p.useRealFreshWaterFlux else 0)
+
+
+def _dev2(a, device, name):
+    if isinstance(a, torch.Tensor):
+        if a.device.type != "cuda":
+            raise TypeError(f"{name}: expected a CUDA tensor")
+        return a.to(dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+
+
+_GRID_FIELDS = ("dyG", "dxG", "recip_dxC", "recip_dyC", "rA", "maskInC")
+
+
+class _GridDev:
+    """Device copies of the grid fields the producers read, and the mj_sfp_grid pointing at them."""
+
+    def __init__(self, g, device):
+        L = _layout(g)
+        self.keep = {k: _dev2(getattr(g, k), device, k) for k in _GRID_FIELDS}
+        for k, t in self.keep.items():
+            if tuple(t.shape) != (L.nTiles, L.ny, L.nx):
+                raise ValueError(f"grid.{k}: expected [{L.nTiles}, {L.ny}, {L.nx}], got {tuple(t.shape)}")
+        self.keep["drF"] = _dev2(np.asarray(getattr(g, "drF")).reshape(-1) if not isinstance(g.drF, torch.Tensor)
+                                 else g.drF.reshape(-1), device, "drF")
+        k = self.keep
+        self.s = _lib.SfpGrid(*(k[n].data_ptr() for n in _GRID_FIELDS), None, k["drF"].data_ptr(), None, None, None)
+
+
+_grid_cache = {}
+
+
+def _grid(g, device):
+    key = (id(g), str(device))
+    hit = _grid_cache.get(key)
+    if hit is None or hit[0] is not g:
+        hit = (g, _GridDev(g, device))
+        _grid_cache[key] = hit
+    return hit[1]
+
+
+def _batch3(a, L, name):
+    if not isinstance(a, torch.Tensor) or a.device.type != "cuda" or a.dtype != torch.float64:
+        raise TypeError(f"{name}: expected a CUDA float64 tensor")
+    if a.dim() == 4:
+        a = a.unsqueeze(0)
+    if a.dim() != 5 or tuple(a.shape[1:2]) != (L.nTiles,) or tuple(a.shape[3:]) != (L.ny, L.nx):
+        raise ValueError(f"{name}: expected [{L.nTiles}, K, {L.ny}, {L.nx}] or [B, ...], got {tuple(a.shape)}")
+    return a.contiguous()
+
+
+def _batch2(a, L, B, name):
+    if not isinstance(a, torch.Tensor) or a.device.type != "cuda" or a.dtype != torch.float64:
+        raise TypeError(f"{name}: expected a CUDA float64 tensor")
+    if a.dim() == 3:
+        a = a.unsqueeze(0)
+    if a.dim() != 4 or tuple(a.shape[1:]) != L.shape2d:
+        raise ValueError(f"{name}: expected [{L.nTiles}, {L.ny}, {L.nx}] or [B, ...], got {tuple(a.shape)}")
+    if a.shape[0] == 1 and B > 1:
+        a = a.expand(B, *a.shape[1:])
+    if a.shape[0] != B:
+        raise ValueError(f"{name}: {a.shape[0]} members, expected {B}")
+    return a.contiguous()
+
+
+def _shared2(a, L, device, name):
+    t = _dev2(a, device, name)
+    if tuple(t.shape) != L.shape2d:
+        raise ValueError(f"{name}: expected [{L.nTiles}, {L.ny}, {L.nx}] (shared by all members)")
+    return t
+
+
+def _config(L, B, K):
+    return _lib.SfpConfig(B, L.nTiles, L.sNx, L.sNy, L.OLx, K, L.nTiles * K * L.ny * L.nx, L.npad)
+
+
+def _check_layout(L):
+    if L.OLx != L.OLy:
+        raise ValueError("the library's tile layout has OLx == OLy")
+
+
+def update_cg2d(p, cp, g, ex, hFacW, hFacS, recip_Bo, pW, pS, pC):
+    """UPDATE_CG2D (reference core/solve_for_pressure.py:29-65). cp: Cg2dParams (cg2dNorm: a scalar or one value
+    per member; cg2dpcOffDFac). Returns (aW2d, aS2d, aC2d, pW, pS, pC), batched like hFacW."""
+    L = _layout(g)
+    _check_layout(L)
+    batched = hFacW.dim() == 5
+    hW, hS = _batch3(hFacW, L, "hFacW"), _batch3(hFacS, L, "hFacS")
+    if hW.shape != hS.shape:
+        raise ValueError("hFacW and hFacS differ in shape")
+    B, K, dev = hW.shape[0], hW.shape[2], hW.device
+    gd = _grid(g, dev)
+    if gd.keep["drF"].numel() < K:
+        raise ValueError(f"grid.drF has {gd.keep['drF'].numel()} levels, the hFac arrays {K}")
+    rBo = _shared2(recip_Bo, L, dev, "recip_Bo")
+    norm = _norm_tensor(cp.cg2dNorm, B, dev)
+    aW, aS, aC = (torch.empty((B,) + L.shape2d, dtype=torch.float64, device=dev) for _ in range(3))
+    pWn, pSn, pCn = (_batch2(a, L, B, n).clone() for a, n in ((pW, "pW"), (pS, "pS"), (pC, "pC")))
+    cfg, fs = _config(L, B, K), _fs(p)
+    plan = ex.plan("T", dev)
+    P = _lib.ptr
+    _lib.check(_lib.lib().mj_update_cg2d(plan.handle, ctypes.byref(gd.s), ctypes.byref(fs), ctypes.byref(cfg),
+                                         P(norm), float(cp.cg2dpcOffDFac), P(hW), P(hS), P(rBo), P(aW), P(aS), P(aC),
+                                         P(pWn), P(pSn), P(pCn), _lib.stream_ptr()), "mj_update_cg2d")
+    out = (aW, aS, aC, pWn, pSn, pCn)
+    return out if batched else tuple(o[0] for o in out)
+
+
+def cg2d_rhs(p, g, gU, gV, hFacW, hFacS, EmPmR, etaN, etaH, Bo_surf):
+    """cg2d_b and the first guess cg2d_x (reference core/solve_for_pressure.py:68-106). Returns (cg2d_b, cg2d_x),
+    batched like gU. EmPmR may be None when p.useRealFreshWaterFlux is false."""
+    L = _layout(g)
+    _check_layout(L)
+    batched = gU.dim() == 5
+    arrs = [_batch3(a, L, n) for a, n in ((gU, "gU"), (gV, "gV"), (hFacW, "hFacW"), (hFacS, "hFacS"))]
+    if len({tuple(a.shape) for a in arrs}) != 1:
+        raise ValueError("gU, gV, hFacW, hFacS differ in shape")
+    B, K, dev = arrs[0].shape[0], arrs[0].shape[2], arrs[0].device
+    gd = _grid(g, dev)
+    if gd.keep["drF"].numel() < K:
+        raise ValueError(f"grid.drF has {gd.keep['drF'].numel()} levels, the 3-D arrays {K}")
+    if EmPmR is None and p.useRealFreshWaterFlux:
+        raise ValueError("useRealFreshWaterFlux needs EmPmR")
+    em = _batch2(EmPmR, L, B, "EmPmR") if EmPmR is not None else None
+    eN, eH = _batch2(etaN, L, B, "etaN"), _batch2(etaH, L, B, "etaH")
+    Bo = _shared2(Bo_surf, L, dev, "Bo_surf")
+    b, x = (torch.empty((B,) + L.shape2d, dtype=torch.float64, device=dev) for _ in range(2))
+    cfg, fs = _config(L, B, K), _fs(p)
+    P = _lib.ptr
+    _lib.check(_lib.lib().mj_cg2d_rhs(ctypes.byref(gd.s), ctypes.byref(fs), ctypes.byref(cfg), *(P(a) for a in arrs),
+                                      P(em), P(eN), P(eH), P(Bo), P(b), P(x), _lib.stream_ptr()), "mj_cg2d_rhs")
+    return (b, x) if batched else (b[0], x[0])

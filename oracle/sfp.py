@@ -1,0 +1,100 @@
+"""CPU oracle of UPDATE_CG2D and cg2d_rhs (reference core/solve_for_pressure.py:29-106) — TEST INFRASTRUCTURE ONLY.
+
+numpy FP64, one elementwise operation at a time in the reference's written order (numpy never fuses a multiply
+and an add), the level sums as explicit sequential loops. The halo exchange of aC2d is the oracle's exchange
+(oracle.scalar with the T map). Imported only by tests/ as the checker.
+
+Arrays: [T, K, ny, nx] 3-D and [T, ny, nx] 2-D, one member; grid: an object with dyG, dxG, recip_dxC, recip_dyC, rA,
+maskInC ([T, ny, nx]) and drF ([K]); vertical factors all 1 (reference core/free_surface.py:121-125) unless given.
+"""
+
+import numpy as np
+
+from . import scalar
+
+
+def _sl(lo, hi, OL):
+    """Array slice of the Fortran index range lo..hi (inclusive) for halo width OL (reference layout.py)."""
+    return slice(lo - 1 + OL, hi + OL)
+
+
+def update_cg2d(p, cg2dNorm, cg2dpcOffDFac, g, Tmap, hFacW, hFacS, recip_Bo, pW, pS, pC, sNx=90, sNy=90, OL=4):
+    """UPDATE_CG2D (reference core/solve_for_pressure.py:29-65). p: deltaTMom, deltaTFreeSurf, freeSurfFac,
+    implicSurfPress, implicDiv2DFlow. Returns (aW2d, aS2d, aC2d, pW, pS, pC), new arrays."""
+    T, K = hFacW.shape[:2]
+    J1, I1 = _sl(1, sNy + 1, OL), _sl(1, sNx + 1, OL)
+    z2 = np.zeros_like(hFacW[:, 0])                                              # :38
+    # :41-43 terms ((dyG * drF) * hFacW) * recip_dxC, summed k = 1..Nr in order from 0 (_ksum :23-26)
+    aW = z2[:, J1, I1].copy()
+    aS = z2[:, J1, I1].copy()
+    for k in range(K):
+        aW = aW + ((g.dyG[:, J1, I1] * g.drF[k]) * hFacW[:, k, J1, I1]) * g.recip_dxC[:, J1, I1]
+        aS = aS + ((g.dxG[:, J1, I1] * g.drF[k]) * hFacS[:, k, J1, I1]) * g.recip_dyC[:, J1, I1]
+    aW = ((aW * cg2dNorm) * p.implicSurfPress) * p.implicDiv2DFlow                # :44-45
+    aS = ((aS * cg2dNorm) * p.implicSurfPress) * p.implicDiv2DFlow
+    aW2d = z2.copy()
+    aS2d = z2.copy()
+    aW2d[:, J1, I1] = aW                                                          # :46-47
+    aS2d[:, J1, I1] = aS
+    J, I, Jp, Ip = _sl(1, sNy, OL), _sl(1, sNx, OL), _sl(2, sNy + 1, OL), _sl(2, sNx + 1, OL)
+    d = ((((p.freeSurfFac * cg2dNorm) * recip_Bo[:, J, I]) * g.rA[:, J, I]) / p.deltaTMom) / p.deltaTFreeSurf
+    aC = -((((aW2d[:, J, I] + aW2d[:, J, Ip]) + aS2d[:, J, I]) + aS2d[:, Jp, I]) + d)  # :50-51
+    aC2d = z2.copy()
+    aC2d[:, J, I] = aC                                                            # :52
+    aC2d = scalar(Tmap, aC2d, T, aC2d.shape[1] * aC2d.shape[2])                  # :53 exch_xy
+    Im, Jm = _sl(0, sNx, OL), _sl(0, sNy, OL)
+    a = aC2d[:, J1, I1]
+    pC_n = np.where(a == 0.0, 1.0, 1.0 / np.where(a == 0.0, 1.0, a))              # :57
+    tW = a + aC2d[:, J1, Im]                                                      # :58
+    fW = cg2dpcOffDFac * np.where(tW == 0.0, 1.0, tW)
+    pW_n = np.where(tW == 0.0, 0.0, -aW2d[:, J1, I1] / (fW * fW))                 # :59-60 (x**2 = x*x)
+    tS = a + aC2d[:, Jm, I1]                                                      # :61
+    fS = cg2dpcOffDFac * np.where(tS == 0.0, 1.0, tS)
+    pS_n = np.where(tS == 0.0, 0.0, -aS2d[:, J1, I1] / (fS * fS))                 # :62-63
+    pW, pS, pC = np.array(pW, np.float64), np.array(pS, np.float64), np.array(pC, np.float64)
+    pW[:, J1, I1] = pW_n
+    pS[:, J1, I1] = pS_n
+    pC[:, J1, I1] = pC_n
+    return aW2d, aS2d, aC2d, pW, pS, pC
+
+
+def k_surf(maskC):
+    """1-based first wet level, Nr+1 for a dry column (reference core/free_surface.py:128-134)."""
+    Nr = maskC.shape[1]
+    wet = maskC != 0
+    return np.where(wet.any(axis=1), np.argmax(wet, axis=1) + 1, Nr + 1).astype(np.int32)
+
+
+def cg2d_rhs(p, g, gU, gV, hFacW, hFacS, EmPmR, etaN, etaH, Bo_surf, sNx=90, sNy=90, OL=4, deepFacC=None,
+             rhoFacC=None, deepFac2F=None, maskC=None):
+    """cg2d_b, cg2d_x (reference core/solve_for_pressure.py:68-106). p: deltaTMom, deltaTFreeSurf, freeSurfFac,
+    implicDiv2DFlow, mass2rUnit, useRealFreshWaterFlux. The vertical factors default to 1 (deepFac2F then needs no
+    kSurf); with deepFac2F given, maskC gives kSurf."""
+    T, K = gU.shape[:2]
+    J, I = _sl(1, sNy, OL), _sl(1, sNx, OL)
+    cg2d_x = Bo_surf * etaN                                                       # :76
+    cg2d_b = np.zeros_like(etaN)                                                  # :77
+    if p.useRealFreshWaterFlux:                                                   # :78-80
+        tmpFac = (p.freeSurfFac * p.mass2rUnit) * p.implicDiv2DFlow
+        cg2d_b[:, J, I] = (((tmpFac * g.rA[:, J, I]) * EmPmR[:, J, I]) / p.deltaTMom) * g.maskInC[:, J, I]
+    b = cg2d_b[:, J, I].copy()
+    J1, I1 = _sl(1, sNy + 1, OL), _sl(1, sNx + 1, OL)
+    Ju, Iu = _sl(1, sNy, OL), _sl(1, sNx + 1, OL)
+    Jv, Iv = _sl(1, sNy + 1, OL), _sl(1, sNx, OL)
+    one = np.ones(K)
+    dfc = one if deepFacC is None else deepFacC
+    rfc = one if rhoFacC is None else rhoFacC
+    for k in range(K - 1, -1, -1):                                                # :101 scan, reverse
+        # :90-93 xA = (((dyG * deepFacC) * drF) * hFacW) * rhoFacC; pfu = (xA * gU) / deltaTMom
+        xA = (((g.dyG[:, J1, I1] * dfc[k]) * g.drF[k]) * hFacW[:, k, J1, I1]) * rfc[k]
+        yA = (((g.dxG[:, J1, I1] * dfc[k]) * g.drF[k]) * hFacS[:, k, J1, I1]) * rfc[k]
+        pu = (xA[:, :-1, :] * gU[:, k, Ju, Iu]) / p.deltaTMom                    # [T, sNy, sNx+1]
+        pv = (yA[:, :, :-1] * gV[:, k, Jv, Iv]) / p.deltaTMom                    # [T, sNy+1, sNx]
+        b = (b + pu[:, :, 1:]) - pu[:, :, :-1]                                    # :97
+        b = (b + pv[:, 1:, :]) - pv[:, :-1, :]                                    # :98
+    e = p.freeSurfFac * g.rA[:, J, I]
+    if deepFac2F is not None:                                                     # :103 deepFac2F(kSurfC)
+        e = e * np.asarray(deepFac2F)[k_surf(maskC)[:, J, I] - 1]
+    b = b - ((e / p.deltaTMom) / p.deltaTFreeSurf) * etaH[:, J, I]               # :103-105
+    cg2d_b[:, J, I] = b
+    return cg2d_b, cg2d_x

@@ -68,6 +68,49 @@ def test_mixed_workspace_and_config_validation():
         assert word in _lib.last_error(), (word, _lib.last_error())
 
 
+def test_operator_and_rhs_producers_validate_before_device_work():
+    """mj_update_cg2d / mj_cg2d_rhs refuse null arguments, bad layouts and strides with a message (MJ_ERR_ARG)
+    before touching the device."""
+    import ctypes
+    from mit_jax_amd import _lib
+    L = _lib.lib()
+    fs = _lib.FsParams(3600.0, 3600.0, 1.0, 1.0, 1.0, 1.0 / 1029.0, 1)
+    dummy = ctypes.c_void_p(16)
+    grid = _lib.SfpGrid(*([dummy] * 6), None, dummy, None, None, None)
+    good = _lib.SfpConfig(2, 13, 90, 90, 4, 50, 13 * 50 * 98 * 98, 13 * 98 * 98)
+    args13 = [dummy] * 8
+
+    def rhs(g, p, c, *arrs):
+        return L.mj_cg2d_rhs(g, p, c, *arrs, None)
+
+    assert rhs(None, ctypes.byref(fs), ctypes.byref(good), *args13, dummy, dummy) == -1
+    assert "null" in _lib.last_error()
+    for field, value, word in (("batch", 0, "batch"), ("nr", 0, "nr"), ("OL", 0, "OL"), ("sNx", 125, "sNx"),
+                               ("member_stride3d", 5, "member strides")):
+        c = _lib.SfpConfig(2, 13, 90, 90, 4, 50, 13 * 50 * 98 * 98, 13 * 98 * 98)
+        setattr(c, field, value)
+        assert rhs(ctypes.byref(grid), ctypes.byref(fs), ctypes.byref(c), *args13, dummy, dummy) == -1, field
+        assert word in _lib.last_error(), (field, _lib.last_error())
+    # useRealFreshWaterFlux without EmPmR; outputs aliasing inputs; deepFac2F without kSurfC
+    a = [ctypes.c_void_p(16 * (n + 1)) for n in range(10)]
+    assert rhs(ctypes.byref(grid), ctypes.byref(fs), ctypes.byref(good), *a[:4], None, *a[5:8], a[8], a[9]) == -1
+    assert "EmPmR" in _lib.last_error()
+    assert rhs(ctypes.byref(grid), ctypes.byref(fs), ctypes.byref(good), *a[:8], a[0], a[9]) == -1
+    assert "alias" in _lib.last_error()
+    g2 = _lib.SfpGrid(*([dummy] * 6), None, dummy, None, None, dummy)
+    assert rhs(ctypes.byref(g2), ctypes.byref(fs), ctypes.byref(good), *a[:8], a[8], a[9]) == -1
+    assert "kSurfC" in _lib.last_error()
+    # update_cg2d: null plan, aliasing outputs
+    upd = [ctypes.c_void_p(16 * (n + 1)) for n in range(10)]
+    assert L.mj_update_cg2d(None, ctypes.byref(grid), ctypes.byref(fs), ctypes.byref(good), upd[0], 0.51, *upd[1:4],
+                            *upd[4:10], None) == -1
+    assert "plan" in _lib.last_error()
+    bad_fs = _lib.FsParams(0.0, 3600.0, 1.0, 1.0, 1.0, 1.0, 0)
+    assert L.mj_update_cg2d(None, ctypes.byref(grid), ctypes.byref(bad_fs), ctypes.byref(good), upd[0], 0.51,
+                            *upd[1:4], *upd[4:10], None) == -1
+    assert "deltaTMom" in _lib.last_error()
+
+
 def test_unported_branches_raise():
     """As the reference raises at setup for the branches it does not port (gad.py:159-182, cg2d.py:212-213)."""
     from mit_jax_amd import cg2d, gad
