@@ -4,9 +4,10 @@
 // Both are column sums over the levels of 3-D [T, nr, ny, nx] FP64 fields into 2-D [T, ny, nx] results: HBM-bound
 // streams with a few flops per byte. One thread owns one (j, i) column of one member and walks the levels in the
 // reference's order (UPDATE_CG2D k = 1..Nr, the divergence of cg2d_b k = Nr..1), so every sum is the literal
-// sequential one and the results are bitwise those of the reference's loops (-ffp-contract=off). A block covers
-// kRows rows x 128 columns of one (member, tile): the level loads of a row are contiguous 8-byte words across the
-// lanes, and the i+1 / j+1 neighbours the divergence reads are mostly the block's own loads (L1 hits).
+// sequential one and the results are bitwise those of the reference's loops (-ffp-contract=off). The level-sum
+// kernels give each thread two horizontally adjacent points (flat over the padded plane), so every level is read
+// with contiguous 16-byte loads across the lanes; the j+1 row the divergence reads is the neighbouring threads'
+// (L1/L2 hits). The 2-D kernels (aC2d, the preconditioner) cover kRows rows x 128 columns per block.
 //
 // UPDATE_CG2D runs as: sums -> aW2d, aS2d (whole arrays); aC2d on the interior (whole array); the T exchange of
 // aC2d; the preconditioner on j, i = 1..sNy+1, 1..sNx+1 (it reads aC2d at i-1 and j-1, i.e. exchanged halo points).
@@ -16,8 +17,17 @@ namespace mj {
 
 namespace {
 
-constexpr int kCols = 128;  // threads across a row (nx <= 128)
-constexpr int kRows = 8;    // rows per block
+constexpr int kCols = 128;        // threads across a row (nx <= 128), 2-D kernels
+constexpr int kRows = 8;          // rows per block, 2-D kernels
+constexpr int kPairThreads = 256; // level-sum kernels: one thread per pair of adjacent points
+#ifndef MJ_SFP_LV
+#define MJ_SFP_LV 5
+#endif
+#ifndef MJ_SFP_RHS_LV
+#define MJ_SFP_RHS_LV 2
+#endif
+constexpr int kLv = MJ_SFP_LV;         // levels per load batch (UPDATE_CG2D sums)
+constexpr int kRhsLv = MJ_SFP_RHS_LV;  // levels per load batch (cg2d_rhs)
 
 struct SfpGeo {
     int B, T, nx, ny, sNx, sNy, OL, nr;
@@ -26,40 +36,70 @@ struct SfpGeo {
 
 __device__ __forceinline__ bool in_range(int v, int lo, int hi) { return v >= lo && v <= hi; }
 
+__device__ __forceinline__ double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
+__device__ __forceinline__ void st2(double* p, double2 v) { *reinterpret_cast<double2*>(p) = v; }
+
 // aW2d, aS2d: ((sum_k ((dyG * drF(k)) * hFacW(k)) * recip_dxC) * cg2dNorm) * implicSurfPress * implicDiv2DFlow on
-// Fortran j, i = 1..sNy+1, 1..sNx+1 (reference solve_for_pressure.py:72-80, :101-115 with _ksum :23-26, :119-132),
-// 0 elsewhere.
-__global__ void __launch_bounds__(kCols * kRows)
+// Fortran j, i = 1..sNy+1, 1..sNx+1, 0 elsewhere (reference solve_for_pressure.py:38-47 with _ksum :23-26).
+// One thread per pair of horizontally adjacent points (flat over the padded plane, nx even: a pair never straddles
+// a row), 16-byte loads of each level.
+__global__ void __launch_bounds__(kPairThreads)
     sfp_opsum_kernel(SfpGeo g, const double* __restrict__ dyG, const double* __restrict__ dxG,
                      const double* __restrict__ recip_dxC, const double* __restrict__ recip_dyC,
                      const double* __restrict__ drF, const double* __restrict__ hFacW,
                      const double* __restrict__ hFacS, const double* __restrict__ norm, double isp, double idf,
                      double* __restrict__ aW2d, double* __restrict__ aS2d) {
-    const int i = threadIdx.x, j = blockIdx.x * kRows + threadIdx.y, t = blockIdx.y, m = blockIdx.z;
-    if (i >= g.nx || j >= g.ny) return;
-    const int64_t q = (int64_t)t * g.plane + (int64_t)j * g.nx + i;
+    const int p = blockIdx.x * kPairThreads + threadIdx.x, t = blockIdx.y, m = blockIdx.z;
+    const int q0 = 2 * p;
+    if (q0 >= g.plane) return;
+    const int j = q0 / g.nx, i0 = q0 - j * g.nx;
+    const int64_t q = (int64_t)t * g.plane + q0;
     const int64_t o2 = m * g.ms2 + q;
-    double w = 0.0, s = 0.0;
-    if (in_range(j, g.OL, g.OL + g.sNy) && in_range(i, g.OL, g.OL + g.sNx)) {
-        const double cW = dyG[q], cS = dxG[q], rW = recip_dxC[q], rS = recip_dyC[q];
-        const double* hW = hFacW + m * g.ms3 + (int64_t)t * g.nr * g.plane + (int64_t)j * g.nx + i;
-        const double* hS = hFacS + m * g.ms3 + (int64_t)t * g.nr * g.plane + (int64_t)j * g.nx + i;
-#pragma unroll 10
-        for (int k = 0; k < g.nr; ++k) {
+    const bool jin = in_range(j, g.OL, g.OL + g.sNy);
+    const bool in0 = jin && in_range(i0, g.OL, g.OL + g.sNx), in1 = jin && in_range(i0 + 1, g.OL, g.OL + g.sNx);
+    double w0 = 0.0, w1 = 0.0, s0 = 0.0, s1 = 0.0;
+    if (in0 || in1) {
+        const double2 cW = ld2(dyG + q), cS = ld2(dxG + q), rW = ld2(recip_dxC + q), rS = ld2(recip_dyC + q);
+        const int64_t o3 = m * g.ms3 + (int64_t)t * g.nr * g.plane + q0;
+        const double *hW = hFacW + o3, *hS = hFacS + o3;
+        // levels in chunks of kLv: the chunk's loads are all issued before its sums (the sums stay in k order)
+        int k = 0;
+        for (; k + kLv <= g.nr; k += kLv) {
+            double2 hw[kLv], hs[kLv];
+#pragma unroll
+            for (int c = 0; c < kLv; ++c) {
+                hw[c] = ld2(hW + (int64_t)(k + c) * g.plane);
+                hs[c] = ld2(hS + (int64_t)(k + c) * g.plane);
+            }
+#pragma unroll
+            for (int c = 0; c < kLv; ++c) {
+                const double dz = drF[k + c];
+                w0 = w0 + ((cW.x * dz) * hw[c].x) * rW.x;
+                w1 = w1 + ((cW.y * dz) * hw[c].y) * rW.y;
+                s0 = s0 + ((cS.x * dz) * hs[c].x) * rS.x;
+                s1 = s1 + ((cS.y * dz) * hs[c].y) * rS.y;
+            }
+        }
+        for (; k < g.nr; ++k) {
             const double dz = drF[k];
-            w = w + ((cW * dz) * hW[k * g.plane]) * rW;
-            s = s + ((cS * dz) * hS[k * g.plane]) * rS;
+            const double2 hw = ld2(hW + (int64_t)k * g.plane), hs = ld2(hS + (int64_t)k * g.plane);
+            w0 = w0 + ((cW.x * dz) * hw.x) * rW.x;
+            w1 = w1 + ((cW.y * dz) * hw.y) * rW.y;
+            s0 = s0 + ((cS.x * dz) * hs.x) * rS.x;
+            s1 = s1 + ((cS.y * dz) * hs.y) * rS.y;
         }
         const double nm = norm[m];
-        w = ((w * nm) * isp) * idf;
-        s = ((s * nm) * isp) * idf;
+        w0 = in0 ? ((w0 * nm) * isp) * idf : 0.0;
+        s0 = in0 ? ((s0 * nm) * isp) * idf : 0.0;
+        w1 = in1 ? ((w1 * nm) * isp) * idf : 0.0;
+        s1 = in1 ? ((s1 * nm) * isp) * idf : 0.0;
     }
-    aW2d[o2] = w;
-    aS2d[o2] = s;
+    st2(aW2d + o2, make_double2(w0, w1));
+    st2(aS2d + o2, make_double2(s0, s1));
 }
 
 // aC2d = -(aW(i) + aW(i+1) + aS(j) + aS(j+1) + freeSurfFac cg2dNorm recip_Bo rA / deltaTMom / deltaTFreeSurf) on
-// the interior, 0 elsewhere (reference solve_for_pressure.py:134-138).
+// the interior, 0 elsewhere (reference solve_for_pressure.py:48-52).
 __global__ void __launch_bounds__(kCols * kRows)
     sfp_aC_kernel(SfpGeo g, const double* __restrict__ aW2d, const double* __restrict__ aS2d,
                   const double* __restrict__ recip_Bo, const double* __restrict__ rA, const double* __restrict__ norm,
@@ -77,7 +117,7 @@ __global__ void __launch_bounds__(kCols * kRows)
 }
 
 // The preconditioner on Fortran j, i = 1..sNy+1, 1..sNx+1 from the exchanged aC2d (reference
-// solve_for_pressure.py:139-149); other points untouched.
+// solve_for_pressure.py:54-65); other points untouched.
 __global__ void __launch_bounds__(kCols * kRows)
     sfp_precond_kernel(SfpGeo g, const double* __restrict__ aW2d, const double* __restrict__ aS2d,
                        const double* __restrict__ aC2d, double off, double* __restrict__ pW, double* __restrict__ pS,
@@ -105,7 +145,7 @@ struct RhsArgs {
     int rfw;
 };
 
-// pfu = (((((dyG deepFacC(k)) drF(k)) hFacW) rhoFacC(k)) gU) / deltaTMom (reference solve_for_pressure.py:89-94);
+// pfu = (((((dyG deepFacC(k)) drF(k)) hFacW) rhoFacC(k)) gU) / deltaTMom (reference solve_for_pressure.py:90-93);
 // the factors of a NULL vertical array are 1 and skipped (x * 1 == x).
 __device__ __forceinline__ double pf(double c, const double* dfc, const double* rfc, double dz, int k, double h,
                                      double v, double dtm) {
@@ -116,49 +156,102 @@ __device__ __forceinline__ double pf(double c, const double* dfc, const double* 
     return (xa * v) / dtm;
 }
 
+__device__ __forceinline__ double eta_term(const RhsArgs& a, const SfpGeo& g, int64_t q, double rA, double etaH) {
+    double e = a.fsf * rA;
+    if (a.d2f) {
+        int ks = a.kSurfC[q];
+        ks = ks < 1 ? 1 : (ks > g.nr + 1 ? g.nr + 1 : ks);
+        e = e * a.d2f[ks - 1];
+    }
+    return ((e / a.dtm) / a.dtfs) * etaH;
+}
+
 // cg2d_x = Bo_surf etaN on the whole array; cg2d_b on the interior (0 elsewhere): the fresh-water term, the
 // divergence of the pf fluxes for k = nr..1 in order, the exactConserv etaH term (reference
-// solve_for_pressure.py:68-106).
-__global__ void __launch_bounds__(kCols * kRows) sfp_rhs_kernel(SfpGeo g, RhsArgs a) {
-    const int i = threadIdx.x, j = blockIdx.x * kRows + threadIdx.y, t = blockIdx.y, m = blockIdx.z;
-    if (i >= g.nx || j >= g.ny) return;
-    const int64_t q = (int64_t)t * g.plane + (int64_t)j * g.nx + i;
+// solve_for_pressure.py:68-106). One thread per pair of horizontally adjacent points (i0, i0+1) of row j: per level
+// it reads hFacW, gU at i0..i0+2 and hFacS, gV at rows j and j+1 (16-byte loads but for i0+2), and the flux at
+// i0+1 serves both points.
+__global__ void __launch_bounds__(kPairThreads) sfp_rhs_kernel(SfpGeo g, RhsArgs a) {
+    const int p = blockIdx.x * kPairThreads + threadIdx.x, t = blockIdx.y, m = blockIdx.z;
+    const int q0 = 2 * p;
+    if (q0 >= g.plane) return;
+    const int j = q0 / g.nx, i0 = q0 - j * g.nx;
+    const int64_t q = (int64_t)t * g.plane + q0;
     const int64_t o2 = m * g.ms2 + q;
-    a.x[o2] = a.Bo_surf[q] * a.etaN[o2];
-    double b = 0.0;
-    if (in_range(j, g.OL, g.OL + g.sNy - 1) && in_range(i, g.OL, g.OL + g.sNx - 1)) {
-        if (a.rfw) b = (((a.tmpFac * a.rA[q]) * a.EmPmR[o2]) / a.dtm) * a.maskInC[q];
-        const double cu0 = a.dyG[q], cu1 = a.dyG[q + 1], cv0 = a.dxG[q], cv1 = a.dxG[q + g.nx];
-        const int64_t o3 = m * g.ms3 + (int64_t)t * g.nr * g.plane + (int64_t)j * g.nx + i;
-        const double *hW = a.hFacW + o3, *hS = a.hFacS + o3, *gU = a.gU + o3, *gV = a.gV + o3;
-#pragma unroll 4
-        for (int k = g.nr - 1; k >= 0; --k) {
-            const int64_t ok = (int64_t)k * g.plane;
-            const double dz = a.drF[k];
-            const double pu1 = pf(cu1, a.dfc, a.rfc, dz, k, hW[ok + 1], gU[ok + 1], a.dtm);
-            const double pu0 = pf(cu0, a.dfc, a.rfc, dz, k, hW[ok], gU[ok], a.dtm);
-            const double pv1 = pf(cv1, a.dfc, a.rfc, dz, k, hS[ok + g.nx], gV[ok + g.nx], a.dtm);
-            const double pv0 = pf(cv0, a.dfc, a.rfc, dz, k, hS[ok], gV[ok], a.dtm);
-            b = (b + pu1) - pu0;
-            b = (b + pv1) - pv0;
-        }
-        double e = a.fsf * a.rA[q];
-        if (a.d2f) {
-            int ks = a.kSurfC[q];
-            ks = ks < 1 ? 1 : (ks > g.nr + 1 ? g.nr + 1 : ks);
-            e = e * a.d2f[ks - 1];
-        }
-        b = b - ((e / a.dtm) / a.dtfs) * a.etaH[o2];
+    {
+        const double2 bo = ld2(a.Bo_surf + q), en = ld2(a.etaN + o2);
+        st2(a.x + o2, make_double2(bo.x * en.x, bo.y * en.y));
     }
-    a.b[o2] = b;
+    const bool jin = in_range(j, g.OL, g.OL + g.sNy - 1);
+    const bool in0 = jin && in_range(i0, g.OL, g.OL + g.sNx - 1);
+    const bool in1 = jin && in_range(i0 + 1, g.OL, g.OL + g.sNx - 1);
+    double b0 = 0.0, b1 = 0.0;
+    if (in0 || in1) {
+        const double2 rA = ld2(a.rA + q);
+        if (a.rfw) {
+            const double2 em = ld2(a.EmPmR + o2), mk = ld2(a.maskInC + q);
+            b0 = (((a.tmpFac * rA.x) * em.x) / a.dtm) * mk.x;
+            b1 = (((a.tmpFac * rA.y) * em.y) / a.dtm) * mk.y;
+        }
+        const double2 cu = ld2(a.dyG + q), cv0 = ld2(a.dxG + q), cv1 = ld2(a.dxG + q + g.nx);
+        const double cu2 = a.dyG[q + 2];
+        const int64_t o3 = m * g.ms3 + (int64_t)t * g.nr * g.plane + q0;
+        const double *hW = a.hFacW + o3, *hS = a.hFacS + o3, *gU = a.gU + o3, *gV = a.gV + o3;
+        // levels k = nr-1 .. 0 in chunks of kRhsLv: the chunk's loads are issued before its terms
+        for (int k = g.nr - 1; k >= 0; k -= kRhsLv) {
+            const int nc = k + 1 < kRhsLv ? k + 1 : kRhsLv;
+            double2 h01[kRhsLv], u01[kRhsLv], s0[kRhsLv], s1[kRhsLv], v0[kRhsLv], v1[kRhsLv];
+            double h2[kRhsLv], u2[kRhsLv];
+#pragma unroll
+            for (int c = 0; c < kRhsLv; ++c) {
+                if (c < nc) {
+                    const int64_t ok = (int64_t)(k - c) * g.plane;
+                    h01[c] = ld2(hW + ok);
+                    h2[c] = hW[ok + 2];
+                    u01[c] = ld2(gU + ok);
+                    u2[c] = gU[ok + 2];
+                    s0[c] = ld2(hS + ok);
+                    s1[c] = ld2(hS + ok + g.nx);
+                    v0[c] = ld2(gV + ok);
+                    v1[c] = ld2(gV + ok + g.nx);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kRhsLv; ++c) {
+                if (c < nc) {
+                    const int kk = k - c;
+                    const double dz = a.drF[kk];
+                    const double pu0 = pf(cu.x, a.dfc, a.rfc, dz, kk, h01[c].x, u01[c].x, a.dtm);
+                    const double pu1 = pf(cu.y, a.dfc, a.rfc, dz, kk, h01[c].y, u01[c].y, a.dtm);
+                    const double pu2 = pf(cu2, a.dfc, a.rfc, dz, kk, h2[c], u2[c], a.dtm);
+                    const double pv00 = pf(cv0.x, a.dfc, a.rfc, dz, kk, s0[c].x, v0[c].x, a.dtm);
+                    const double pv10 = pf(cv1.x, a.dfc, a.rfc, dz, kk, s1[c].x, v1[c].x, a.dtm);
+                    const double pv01 = pf(cv0.y, a.dfc, a.rfc, dz, kk, s0[c].y, v0[c].y, a.dtm);
+                    const double pv11 = pf(cv1.y, a.dfc, a.rfc, dz, kk, s1[c].y, v1[c].y, a.dtm);
+                    b0 = (b0 + pu1) - pu0;
+                    b0 = (b0 + pv10) - pv00;
+                    b1 = (b1 + pu2) - pu1;
+                    b1 = (b1 + pv11) - pv01;
+                }
+            }
+        }
+        const double2 eH = ld2(a.etaH + o2);
+        b0 = in0 ? b0 - eta_term(a, g, q, rA.x, eH.x) : 0.0;
+        b1 = in1 ? b1 - eta_term(a, g, q + 1, rA.y, eH.y) : 0.0;
+    }
+    st2(a.b + o2, make_double2(b0, b1));
 }
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 int check_sfp(const mj_sfp_grid* gr, const mj_fs_params* p, const mj_sfp_config* c, SfpGeo* g, const char* who) {
     MJ_REQUIRE(gr && p && c, "%s: null grid/params/config", who);
     MJ_REQUIRE(c->batch >= 1 && c->batch <= 65535 && c->ntiles >= 1 && c->ntiles <= 65535 && c->nr >= 1,
                "%s: batch and ntiles must be in 1..65535, nr >= 1", who);
-    MJ_REQUIRE(c->sNx >= 1 && c->sNy >= 1 && c->OL >= 1 && c->sNx + 2 * c->OL <= kCols,
-               "%s: need sNx, sNy, OL >= 1 and sNx + 2 OL <= %d", who, kCols);
+    MJ_REQUIRE(c->sNx >= 2 && c->sNx % 2 == 0 && c->sNy >= 1 && c->OL >= 1 && c->sNx + 2 * c->OL <= kCols,
+               "%s: need sNx even, sNy, OL >= 1 and sNx + 2 OL <= %d", who, kCols);
+    MJ_REQUIRE(c->batch == 1 || (c->member_stride3d % 2 == 0 && c->member_stride2d % 2 == 0),
+               "%s: member strides must be even (16-byte aligned members)", who);
     g->B = c->batch;
     g->T = c->ntiles;
     g->sNx = c->sNx;
@@ -173,12 +266,18 @@ int check_sfp(const mj_sfp_grid* gr, const mj_fs_params* p, const mj_sfp_config*
     MJ_REQUIRE(c->batch == 1 || (g->ms3 >= (int64_t)g->T * g->nr * g->plane && g->ms2 >= (int64_t)g->T * g->plane),
                "%s: member strides smaller than one member's arrays", who);
     MJ_REQUIRE(gr->dyG && gr->dxG && gr->drF, "%s: null grid field", who);
+    MJ_REQUIRE(aligned16(gr->dyG) && aligned16(gr->dxG) && aligned16(gr->recip_dxC) && aligned16(gr->recip_dyC) &&
+                   aligned16(gr->rA) && aligned16(gr->maskInC),
+               "%s: 2-D grid fields must be 16-byte aligned", who);
     MJ_REQUIRE(p->deltaTMom != 0.0 && p->deltaTFreeSurf != 0.0, "%s: deltaTMom and deltaTFreeSurf must be nonzero",
                who);
     return MJ_OK;
 }
 
 dim3 sfp_grid(const SfpGeo& g) { return dim3((unsigned)((g.ny + kRows - 1) / kRows), (unsigned)g.T, (unsigned)g.B); }
+dim3 pair_grid(const SfpGeo& g) {
+    return dim3((unsigned)((g.plane / 2 + kPairThreads - 1) / kPairThreads), (unsigned)g.T, (unsigned)g.B);
+}
 
 }  // namespace
 }  // namespace mj
@@ -203,9 +302,11 @@ extern "C" int mj_update_cg2d(const mj_halo_plan* plan_T, const mj_sfp_grid* gri
         for (const double* in : ins) MJ_REQUIRE(o != in, "mj_update_cg2d: an output aliases an input");
     for (int x = 0; x < 6; ++x)
         for (int y = x + 1; y < 6; ++y) MJ_REQUIRE(outs[x] != outs[y], "mj_update_cg2d: two outputs alias");
+    MJ_REQUIRE(aligned16(hFacW) && aligned16(hFacS) && aligned16(aW2d) && aligned16(aS2d),
+               "mj_update_cg2d: hFacW, hFacS, aW2d, aS2d must be 16-byte aligned");
     hipStream_t st = as_stream(stream);
     const dim3 grd = sfp_grid(g), blk(kCols, kRows);
-    hipLaunchKernelGGL(sfp_opsum_kernel, grd, blk, 0, st, g, grid->dyG, grid->dxG, grid->recip_dxC, grid->recip_dyC,
+    hipLaunchKernelGGL(sfp_opsum_kernel, pair_grid(g), dim3(kPairThreads), 0, st, g, grid->dyG, grid->dxG, grid->recip_dxC, grid->recip_dyC,
                        grid->drF, hFacW, hFacS, cg2dNorm, p->implicSurfPress, p->implicDiv2DFlow, aW2d, aS2d);
     MJ_HIP(hipGetLastError());
     hipLaunchKernelGGL(sfp_aC_kernel, grd, blk, 0, st, g, (const double*)aW2d, (const double*)aS2d, recip_Bo, grid->rA,
@@ -232,8 +333,11 @@ extern "C" int mj_cg2d_rhs(const mj_sfp_grid* grid, const mj_fs_params* p, const
     MJ_REQUIRE(EmPmR || !p->useRealFreshWaterFlux, "mj_cg2d_rhs: useRealFreshWaterFlux needs EmPmR");
     MJ_REQUIRE(cg2d_b != cg2d_x, "mj_cg2d_rhs: cg2d_b and cg2d_x alias");
     const double* ins[] = {gU, gV, hFacW, hFacS, EmPmR, etaN, etaH, Bo_surf};
-    for (const double* in : ins)
+    for (const double* in : ins) {
         MJ_REQUIRE(in != cg2d_b && in != cg2d_x, "mj_cg2d_rhs: an output aliases an input");
+        MJ_REQUIRE(aligned16(in), "mj_cg2d_rhs: arrays must be 16-byte aligned");
+    }
+    MJ_REQUIRE(aligned16(cg2d_b) && aligned16(cg2d_x), "mj_cg2d_rhs: arrays must be 16-byte aligned");
     RhsArgs a;
     a.dyG = grid->dyG;
     a.dxG = grid->dxG;
@@ -259,7 +363,7 @@ extern "C" int mj_cg2d_rhs(const mj_sfp_grid* grid, const mj_fs_params* p, const
     a.dtm = p->deltaTMom;
     a.dtfs = p->deltaTFreeSurf;
     a.rfw = p->useRealFreshWaterFlux ? 1 : 0;
-    hipLaunchKernelGGL(sfp_rhs_kernel, sfp_grid(g), dim3(kCols, kRows), 0, as_stream(stream), g, a);
+    hipLaunchKernelGGL(sfp_rhs_kernel, pair_grid(g), dim3(kPairThreads), 0, as_stream(stream), g, a);
     MJ_HIP(hipGetLastError());
     return MJ_OK;
 }
