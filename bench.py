@@ -22,7 +22,8 @@ hFacW, hFacS, recip_hFacC, maskW, maskS read, the tendency written), and the 2-D
 once. `traffic` = measured HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
 profile of this workload. `cpu_baseline` is the FP64 C oracle (OpenMP, all host cores) on a bounded sample of the
 same member-steps. `producers` times the operator builder (UPDATE_CG2D) and the rhs builder (cg2d_rhs) on the same
-ensemble, each alone (not part of the headline step, whose operators and rhs are inputs).
+ensemble, each alone (not part of the headline step, whose operators and rhs are inputs); `implicit_r` likewise
+times GAD_IMPLICIT_R (the implicit vertical advection/diffusion solve) on the ensemble's tracer.
 """
 
 import argparse
@@ -60,7 +61,8 @@ def parse():
     ap.add_argument("--sum-order", choices=("tile", "fortran"), default="tile")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-producers", action="store_true", help="skip the UPDATE_CG2D / cg2d_rhs measurement")
+    ap.add_argument("--no-producers", action="store_true",
+                    help="skip the UPDATE_CG2D / cg2d_rhs / GAD_IMPLICIT_R measurements")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (the reference kernel for HBM counter scaling)")
     ap.add_argument("--profile-solves", action="store_true",
@@ -86,7 +88,7 @@ def build_ensemble(B, dev):
         g, ops, norm = S.cg2d_case(0, "5point", nlev=int(n))
         u, v, w = S.flow(g, 2)
         tr = S.tracer(g, 3)
-        cls[int(n)] = (g, ops, norm, u, v, tr)
+        cls[int(n)] = (g, ops, norm, u, v, tr, w)
     T = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
     idx = torch.as_tensor([LEVEL_CHOICES.index(int(n)) for n in nlev], device=dev)
 
@@ -125,7 +127,7 @@ def build_ensemble(B, dev):
     g2d.update(maskW=mW, maskS=mS)
     vg = {k: g0.v[k] for k in ("drF", "recip_drF", "rhoFacC", "recip_drC")}
     return dict(ex=ex, nlev=nlev, nlev_t=torch.as_tensor(nlev.astype(np.int32), device=dev), ops=ops,
-                op_index=op_index, norm=norm,
+                op_index=op_index, norm=norm, mC=mC, per_member=per_member,
                 rhs=rhs, x0=x0, u=u, v=v, tr=tr, hW=hW, hS=hS, rhC=rhC, g2d=g2d, vg=vg, cls=cls)
 
 
@@ -143,7 +145,7 @@ def cpu_baseline(E, seconds):
     done = []
     while True:
         lv = LEVEL_CHOICES[n % 3]
-        g, ops, norm, u, v, tr = E["cls"][lv]
+        g, ops, norm, u, v, tr = E["cls"][lv][:6]
         b = S.rhs(g, 100 + n)
         ta = time.perf_counter()
         oracle.cg2d(Tm, ops, b, np.zeros_like(b), cg2dNorm=norm, tolerance=TOL, max_iters=2000,
@@ -295,6 +297,56 @@ def producers(E, dev, reps=5):
     return res
 
 
+def implicit_r(E, dev, reps=5):
+    """GAD_IMPLICIT_R (implicit U3 vertical advection + diffusion, pentadiagonal solve per column) on the whole
+    ensemble through the C ABI, timed alone with HIP events (median of reps). Inputs: each member's maskC,
+    recip_hFacC and w of its level class, the perturbed tracer, a seeded kappaRX. Algorithmic bytes per member:
+    maskC, recip_hFac, kappaRX, wFld and the tracer read on every interior column and level, the tracer read on the
+    halo columns, the result written everywhere."""
+    import ctypes
+    import torch
+    import mit_jax_amd._lib as L
+    B, T, K, ny, nx = E["tr"].shape
+    g0 = E["cls"][50][0]
+    w = E["per_member"](lambda c: c[6])
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4002)
+    kap = (1e-5 + 1e-2 * torch.rand(E["tr"].shape, generator=gen, device=dev, dtype=torch.float64)) * E["mC"]
+    keep = [torch.as_tensor(np.ascontiguousarray(a, np.float64), device=dev) for a in
+            (g0.f["rA"], g0.f["recip_rA"], g0.v["dTlev"], g0.v["recip_drF"], g0.v["recip_drC"])]
+    gs = L.ImprGrid(E["mC"].data_ptr(), *(k.data_ptr() for k in keep), -1.0)
+    n3 = T * K * ny * nx
+    cfg = L.ImprConfig(B, T, nx - 8, ny - 8, 4, K, n3, n3, 1)
+    out = torch.empty_like(E["tr"])
+    lib = L.lib()
+
+    def run():
+        L.check(lib.mj_gad_implicit_r(ctypes.byref(gs), ctypes.byref(cfg), L.ptr(kap), L.ptr(E["rhC"]), L.ptr(w),
+                                      L.ptr(E["tr"]), L.ptr(out), L.stream_ptr()), "mj_gad_implicit_r")
+
+    print("bench: GAD_IMPLICIT_R", file=sys.stderr, flush=True)
+    run()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    t = float(np.median(ms))
+    interior, cols = T * (nx - 8) * (ny - 8), T * ny * nx
+    nb = B * K * 8 * (6 * interior + 2 * (cols - interior))
+    gbs = nb / (t * 1e-3) / 1e9
+    wt = E["tr"].new_tensor(np.asarray(g0.v["drF"]))[None, None, :, None, None] / E["rhC"].clamp(min=1e-300) * E["mC"]
+    cons = float(((wt * out).sum(dim=2) - (wt * E["tr"]).sum(dim=2)).abs().max() /
+                 (wt * E["tr"]).sum(dim=2).abs().max())
+    del kap, w, out
+    return {"ms": t, "member_calls_per_s": B / (t * 1e-3), "algorithmic_bytes": nb, "achieved_GBs": gbs,
+            "frac": gbs / HBM_PEAK_GBS, "kernel": "impr_kernel", "column_integral_max_rel_change": cons}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -442,6 +494,7 @@ def main():
         out["mixed_over_plain_fp64"] = out["value"] / pv
     if not args.no_producers:
         out["producers"] = producers(E, dev)
+        out["implicit_r"] = implicit_r(E, dev)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(E, args.cpu_seconds)

@@ -291,6 +291,31 @@ int mj_gad_dst3_adjoint(const mj_gad_plan* plan, const mj_gad_grid* grid, const 
                         double* tracer_bar, double* uFld_bar, double* vFld_bar, double* work, size_t work_bytes,
                         void* stream);
 
+/* ======================================================================================= GAD_IMPLICIT_R
+ * Replaces gad_implicit_r (core/tracer_implicit.py:165-173) with gad_implicit_r_matrix (:34-114) and
+ * solve_pentadiagonal (:116-162): per water column, the implicit vertical diffusion (kappaRX) and, with
+ * implicitAdvection, the 3rd-order upwind implicit vertical advection (wFld) as a pentadiagonal system on the tile
+ * interior (halo columns: identity rows), solved by the elimination of SOLVE_DIAGONAL_KINNER with its zero-pivot
+ * guard. Bitwise the reference's expressions. With nr = 1 the tracer is returned unchanged. */
+typedef struct {
+    const double* maskC;          /* [T, nr, ny, nx] (mask_member_stride apart per member; 0 = shared) */
+    const double *rA, *recip_rA;  /* [T, ny, nx], shared (advection only) */
+    const double *dTtracerLev, *recip_drF, *recip_drC; /* [nr] */
+    double rkSign;                /* -1 for the ocean's z coordinate */
+} mj_impr_grid;
+
+typedef struct {
+    int batch, ntiles, sNx, sNy, OL, nr;
+    int64_t member_stride3d;      /* elements between members' kappaRX, recip_hFac, wFld, gTracer, out */
+    int64_t mask_member_stride;   /* elements between members' maskC; 0 = one maskC for all */
+    int implicitAdvection;        /* 1: U3 implicit vertical advection (wFld, rA, recip_rA needed) */
+} mj_impr_config;
+
+/* kappaRX, recip_hFac, wFld, gTracer (the tracer after the explicit step), out: [B][T, nr, ny, nx]; nr <= 150.
+ * out may equal gTracer (in place), no other input. */
+int mj_gad_implicit_r(const mj_impr_grid* grid, const mj_impr_config* cfg, const double* kappaRX,
+                      const double* recip_hFac, const double* wFld, const double* gTracer, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

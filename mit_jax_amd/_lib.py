@@ -76,6 +76,16 @@ class SfpConfig(ctypes.Structure):
                 ("member_stride3d", I64), ("member_stride2d", I64)]
 
 
+class ImprGrid(ctypes.Structure):
+    _fields_ = [(k, P) for k in ("maskC", "rA", "recip_rA", "dTtracerLev", "recip_drF", "recip_drC")] + \
+        [("rkSign", F64)]
+
+
+class ImprConfig(ctypes.Structure):
+    _fields_ = [("batch", I32), ("ntiles", I32), ("sNx", I32), ("sNy", I32), ("OL", I32), ("nr", I32),
+                ("member_stride3d", I64), ("mask_member_stride", I64), ("implicitAdvection", I32)]
+
+
 # name -> (restype, argtypes): every entry point declared in include/mjamd.h
 SIGNATURES = {
     "mj_version": (I32, []),
@@ -99,6 +109,7 @@ SIGNATURES = {
                              P, P, P, P, P, P, P, P, P, P]),
     "mj_cg2d_rhs": (I32, [ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, P, P, P,
                           P, P, P, P, P, P, P]),
+    "mj_gad_implicit_r": (I32, [ctypes.POINTER(ImprGrid), ctypes.POINTER(ImprConfig), P, P, P, P, P, P]),
     "mj_gad_plan_create": (I32, [I32, P, P, I32, I32, I32, I32, ctypes.POINTER(P)]),
     "mj_gad_plan_destroy": (I32, [P]),
     "mj_gad_plan_describe": (I32, [P, P, P]),

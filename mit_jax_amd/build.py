@@ -12,7 +12,7 @@ import sys
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SOURCES = ["mj_halo.hip", "mj_cg2d.hip", "mj_gad.hip", "mj_adjoint.hip", "mj_mixed.hip", "mj_sfp.hip"]
+SOURCES = ["mj_halo.hip", "mj_cg2d.hip", "mj_gad.hip", "mj_adjoint.hip", "mj_mixed.hip", "mj_sfp.hip", "mj_implicit_r.hip"]
 OUT = HERE / "libmjamd.so"
 ARCH = os.environ.get("MJ_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",

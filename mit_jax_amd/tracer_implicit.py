@@ -1,0 +1,84 @@
+"""GAD_IMPLICIT_R on the GPU with the reference's interface (mitgcm_jax core/tracer_implicit.py).
+
+`gad_implicit_r(p, g, implicitAdvection, advectionScheme, kappaRX, recip_hFac, wFld, gTracer)` (reference
+:165-173): the tracer after the implicit vertical diffusion and, with implicitAdvection, the 3rd-order upwind
+implicit vertical advection, solved per water column (pentadiagonal elimination with the zero-pivot guard of
+solve_pentadiagonal.F), through libmjamd's mj_gad_implicit_r; bitwise the reference's expressions. Arrays are
+CUDA float64 [T, Nr, ny, nx] or [B, T, Nr, ny, nx]; maskC [T, Nr, ny, nx] (shared) or per member. p supplies
+dTtracerLev (Nr values), implicitDiffusion and rkSign; g supplies layout (or L), maskC, rA, recip_rA, recip_drF and
+recip_drC. As the reference, only implicitDiffusion = True and the U3 scheme (advectionScheme 3) are ported.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+ENUM_UPWIND_3RD = 3
+
+
+def _f64(a, device, name):
+    if isinstance(a, torch.Tensor):
+        if a.device.type != "cuda":
+            raise TypeError(f"{name}: expected a CUDA tensor")
+        return a.to(dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+
+
+def _batch(a, name):
+    if not isinstance(a, torch.Tensor) or a.device.type != "cuda" or a.dtype != torch.float64:
+        raise TypeError(f"{name}: expected a CUDA float64 tensor")
+    if a.dim() == 4:
+        a = a.unsqueeze(0)
+    if a.dim() != 5:
+        raise ValueError(f"{name}: expected [T, Nr, ny, nx] or [B, T, Nr, ny, nx], got {tuple(a.shape)}")
+    return a.contiguous()
+
+
+def gad_implicit_r(p, g, implicitAdvection, advectionScheme, kappaRX, recip_hFac, wFld, gTracer):
+    """GAD_IMPLICIT_R (reference core/tracer_implicit.py:165-173). Returns a new array shaped like gTracer."""
+    if not p.implicitDiffusion:
+        raise NotImplementedError("GAD_IMPLICIT_R without implicitDiffusion: the V4r4 run always has it")
+    if implicitAdvection and advectionScheme != ENUM_UPWIND_3RD:
+        raise NotImplementedError(f"GAD_IMPLICIT_R: vertical scheme {advectionScheme} not ported (V4r4: 3)")
+    L = getattr(g, "layout", None) or g.L
+    batched = gTracer.dim() == 5
+    y = _batch(gTracer, "gTracer")
+    B, T, K, ny, nx = y.shape
+    if (T, ny, nx) != (L.nTiles, L.ny, L.nx):
+        raise ValueError(f"gTracer: tile layout {(T, ny, nx)} is not {(L.nTiles, L.ny, L.nx)}")
+    out = torch.empty_like(y)
+    if K <= 1:
+        out.copy_(y)
+        return out if batched else out[0]
+    kap, rh = _batch(kappaRX, "kappaRX"), _batch(recip_hFac, "recip_hFac")
+    w = _batch(wFld, "wFld") if implicitAdvection else None
+    for t, n in ((kap, "kappaRX"), (rh, "recip_hFac")) + (((w, "wFld"),) if w is not None else ()):
+        if t.shape != y.shape:
+            raise ValueError(f"{n}: shape {tuple(t.shape)} differs from gTracer's {tuple(y.shape)}")
+    dev = y.device
+    mC = _f64(g.maskC, dev, "maskC")
+    if mC.dim() == 4:
+        mms = 0
+        if tuple(mC.shape) != (T, K, ny, nx):
+            raise ValueError("maskC: expected [T, Nr, ny, nx] or [B, T, Nr, ny, nx]")
+    else:
+        if tuple(mC.shape) != (B, T, K, ny, nx):
+            raise ValueError("maskC: expected [T, Nr, ny, nx] or [B, T, Nr, ny, nx]")
+        mms = T * K * ny * nx
+    keep = dict(maskC=mC, rA=_f64(g.rA, dev, "rA"), recip_rA=_f64(g.recip_rA, dev, "recip_rA"),
+                dt=_f64(np.asarray(p.dTtracerLev, np.float64).reshape(-1)[:K], dev, "dTtracerLev"),
+                rdrF=_f64(np.asarray(g.recip_drF, np.float64).reshape(-1)[:K], dev, "recip_drF"),
+                rdrC=_f64(np.asarray(g.recip_drC, np.float64).reshape(-1)[:K], dev, "recip_drC"))
+    for n in ("dt", "rdrF", "rdrC"):
+        if keep[n].numel() != K:
+            raise ValueError(f"{n}: fewer than {K} levels")
+    gs = _lib.ImprGrid(keep["maskC"].data_ptr(), keep["rA"].data_ptr(), keep["recip_rA"].data_ptr(),
+                       keep["dt"].data_ptr(), keep["rdrF"].data_ptr(), keep["rdrC"].data_ptr(), float(p.rkSign))
+    cfg = _lib.ImprConfig(B, T, L.sNx, L.sNy, L.OLx, K, T * K * ny * nx, mms, 1 if implicitAdvection else 0)
+    P = _lib.ptr
+    _lib.check(_lib.lib().mj_gad_implicit_r(ctypes.byref(gs), ctypes.byref(cfg), P(kap), P(rh), P(w), P(y), P(out),
+                                            _lib.stream_ptr()), "mj_gad_implicit_r")
+    return out if batched else out[0]
