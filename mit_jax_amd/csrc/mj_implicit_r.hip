@@ -9,15 +9,16 @@
 // output column; the back substitution then walks k = Nr..1. Every expression is the reference's, in its order
 // (-ffp-contract=off), so the result is bitwise the reference's. Level data are loaded kPf levels ahead of the row
 // being built (a register ring with compile-time slots), so each lane keeps up to 5 x kPf loads in flight; the
-// columns of a wave are consecutive points of a row (coalesced).
+// columns of a wave are consecutive points of a row (coalesced). The per-level constants sit in LDS and the rows
+// between the first two and the last two run branch-free, so that no wait for a load is forced before its use.
 #include "mj_common.h"
 
 namespace mj {
 namespace {
 
-constexpr int kImprThreads = 64;  // one wave per workgroup (LDS: 2 * nr * 64 doubles)
+constexpr int kImprThreads = 64;  // one wave per workgroup (LDS: 2 * nr * 64 + 3 * nr doubles)
 #ifndef MJ_IMPR_PF
-#define MJ_IMPR_PF 8
+#define MJ_IMPR_PF 16
 #endif
 constexpr int kPf = MJ_IMPR_PF;  // prefetch distance in levels
 static_assert(kPf >= 3, "the ring must hold levels m .. m+2");
@@ -60,11 +61,92 @@ __device__ __forceinline__ void norm3(double cp, double& dp, double& ep, double&
     yp = nz ? yp * rec : 0.0;
 }
 
+// Per-thread state of the sweep down a column.
+struct ColState {
+    double mC1;            // maskC of level m-1
+    Iface up;              // interface m (above level m), built by row m-1
+    double dp1, ep1, yp1;  // row m-1 after elimination
+    double dp2, ep2, yp2;  // row m-2
+};
+
+// Row m of the matrix and its elimination. GEN: any m (the branches of the first and last rows); otherwise
+// 2 <= m <= nr-3, branch-free, so that the loads of the ring stay in flight across rows. Level data: mC0, rh0, kap0,
+// y0 of level m; mCn, kapn, wn of level m+1; mCk = maskC(min(nr-1, m+2)). Halo columns (in = false) get identity
+// rows; their level data are loaded like the others' and ignored.
+template <bool GEN, bool ADV>
+__device__ __forceinline__ void impr_row(const ImprArgs& a, int m, bool in, double rA, double rrA, double mC0,
+                                         double rh0, double kap0, double y0, double mCn, double kapn, double wn,
+                                         double mCk, ColState& c, double* dpL, double* epL, double* out,
+                                         const double* lv) {
+    const int nr = a.nr;
+    const double dtm = lv[m], rdf = lv[nr + m];
+    const double* rdrC = lv + 2 * nr;
+    const bool hasU = !GEN || m >= 1, hasL = !GEN || m + 1 < nr;
+    // diffusion, reference tracer_implicit.py:54-64
+    double rb = 0.0, rd = 0.0, ra = 0.0, re = 0.0;
+    if (hasU) rb = -(((((dtm * c.mC1) * rh0) * rdf) * kap0) * rdrC[m]);
+    if (hasL) rd = -(((((dtm * mCn) * rh0) * rdf) * kapn) * rdrC[m + 1]);
+    double rc = 1.0 - (rb + rd);
+    if (ADV) {
+        // U3 advection, :95-105: level m takes the updates of interface m+1 (its bottom) first, then those of
+        // interface m (its top)
+        Iface lo{0.0, 0.0, 0.0, 0.0};
+        if (hasL) {
+            lo = iface(a, m + 1, wn, rA, rrA, mC0, (!GEN || m >= 1) ? c.mC1 : mC0, mCk);
+            rb = rb - ((lo.km * dtm) * rh0) * rdf;
+            rc = rc + (((((lo.rC + lo.rUp) + lo.km) * dtm) * rh0) * rdf);
+            rd = rd + (((((lo.rC - lo.rUp) + lo.kp) * dtm) * rh0) * rdf);
+            re = re - ((lo.kp * dtm) * rh0) * rdf;
+        }
+        if (hasU) {
+            ra = ra + ((c.up.km * dtm) * rh0) * rdf;
+            rb = rb - (((((c.up.rC + c.up.rUp) + c.up.km) * dtm) * rh0) * rdf);
+            rc = rc - (((((c.up.rC - c.up.rUp) + c.up.kp) * dtm) * rh0) * rdf);
+            rd = rd + ((c.up.kp * dtm) * rh0) * rdf;
+        }
+        c.up = lo;
+    }
+    if (!in) ra = 0.0, rb = 0.0, rc = 1.0, rd = 0.0, re = 0.0;
+    // forward elimination, reference tracer_implicit.py:126-143 (rows 0 and 1 through their own formulas)
+    double dq, eq = re, yq;
+    if (GEN && m == 0) {
+        dq = rd;
+        yq = y0;
+        norm3(rc, dq, eq, yq);
+    } else if (GEN && m == 1) {
+        const double c2 = rc - rb * c.dp1;
+        dq = rd - rb * c.ep1;
+        yq = y0 - rb * c.yp1;
+        norm3(c2, dq, eq, yq);
+    } else {
+        const double bb = rb - ra * c.dp2;
+        const double cq = (rc - ra * c.ep2) - bb * c.dp1;
+        dq = rd - bb * c.ep1;
+        yq = (y0 - ra * c.yp2) - bb * c.yp1;
+        norm3(cq, dq, eq, yq);
+    }
+    dpL[(int64_t)m * kImprThreads] = dq;
+    epL[(int64_t)m * kImprThreads] = eq;
+    out[(int64_t)m * a.plane] = yq;
+    c.dp2 = c.dp1, c.ep2 = c.ep1, c.yp2 = c.yp1;
+    c.dp1 = dq, c.ep1 = eq, c.yp1 = yq;
+    c.mC1 = mC0;
+}
+
+template <bool ADV>
 __global__ void __launch_bounds__(kImprThreads) impr_kernel(ImprArgs a) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const int64_t col = (int64_t)blockIdx.x * kImprThreads + lane;
     const int mem = blockIdx.y;
+    // the per-level constants (deltaT, recip_drF, recip_drC) in LDS: read per row without vector-memory waits
+    double* lv = lds + (int64_t)2 * a.nr * kImprThreads;
+    for (int k = lane; k < a.nr; k += kImprThreads) {
+        lv[k] = a.dt[k];
+        lv[a.nr + k] = a.rdrF[k];
+        lv[2 * a.nr + k] = a.rdrC[k];
+    }
+    __syncthreads();
     if (col >= (int64_t)a.T * a.plane) return;  // no barriers below
     const int t = (int)(col / a.plane), q = (int)(col - (int64_t)t * a.plane);
     const int j = q / a.nx, i = q - j * a.nx;
@@ -77,114 +159,80 @@ __global__ void __launch_bounds__(kImprThreads) impr_kernel(ImprArgs a) {
     double* dpL = lds + lane;
     double* epL = lds + (int64_t)nr * kImprThreads + lane;
     const int64_t o2 = (int64_t)t * a.plane + q;
-    const double rA = in && a.adv ? a.rA[o2] : 0.0;
-    const double rrA = in && a.adv ? a.recip_rA[o2] : 0.0;
+    const double rA = ADV ? a.rA[o2] : 0.0;
+    const double rrA = ADV ? a.recip_rA[o2] : 0.0;
+    const double* mCp = a.maskC + om;
+    const double *rhp = a.rh + o3, *kp = a.kap + o3, *wp = a.w + o3;
 
-    double rmC[kPf], rrh[kPf], rkap[kPf], rw[kPf], ry[kPf];  // slot s: level m with m % kPf == s
+    // ring of level data: slot s holds a level m with m % kPf == s; loads are unconditional (levels clamped to
+    // nr-1) so that no branch separates a load from its use
+    double rmC[kPf], rrh[kPf], rkap[kPf], rw[kPf], ry[kPf];
     auto load = [&](int s, int lev) {
-        const int64_t ok = (int64_t)lev * a.plane;
+        const int64_t ok = (int64_t)min(lev, nr - 1) * a.plane;
         ry[s] = y[ok];
-        if (in) {
-            rmC[s] = a.maskC[om + ok];
-            rrh[s] = a.rh[o3 + ok];
-            rkap[s] = a.kap[o3 + ok];
-            rw[s] = a.adv ? a.w[o3 + ok] : 0.0;
-        }
+        rmC[s] = mCp[ok];
+        rrh[s] = rhp[ok];
+        rkap[s] = kp[ok];
+        rw[s] = wp[ok];
     };
 #pragma unroll
-    for (int s = 0; s < kPf; ++s) {
-        rmC[s] = rrh[s] = rkap[s] = rw[s] = ry[s] = 0.0;
-        if (s < nr) load(s, s);
+    for (int s = 0; s < kPf; ++s) load(s, s);
+    ColState c{0.0, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    auto mck = [&](int m, int s2, int s1) { return m + 2 < nr ? rmC[s2] : rmC[s1]; };
+    // rows 0 and 1
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (s < nr) {
+            const int s1 = (s + 1) % kPf, s2 = (s + 2) % kPf;
+            impr_row<true, ADV>(a, s, in, rA, rrA, rmC[s], rrh[s], rkap[s], ry[s], rmC[s1], rkap[s1], rw[s1],
+                           mck(s, s2, s1), c, dpL, epL, out, lv);
+            load(s, s + kPf);
+        }
     }
-    double mC1 = 0.0;                // maskC of level m-1
-    Iface up{0.0, 0.0, 0.0, 0.0};   // interface m (above level m), built by row m-1
-    double dp1 = 0.0, ep1 = 0.0, yp1 = 0.0, dp2 = 0.0, ep2 = 0.0, yp2 = 0.0;  // rows m-1, m-2 after elimination
-    for (int m0 = 0; m0 < nr; m0 += kPf) {
+    // rows 2 .. nr-3 in laps of kPf (a lap starts at m = 2 mod kPf, so slot (2 + s) % kPf is level m0 + s)
+    int m0 = 2;
+    for (; m0 + kPf - 1 <= nr - 3; m0 += kPf) {
 #pragma unroll
         for (int s = 0; s < kPf; ++s) {
             const int m = m0 + s;
-            if (m < nr) {
-                const int s1 = (s + 1) % kPf, s2 = (s + 2) % kPf;
-                // row m of the matrix (identity on halo columns)
-                double ra = 0.0, rb = 0.0, rc = 1.0, rd = 0.0, re = 0.0;
-                const double mC0 = rmC[s];
-                if (in) {
-                    const double dtm = a.dt[m], rdf = a.rdrF[m], rhm = rrh[s];
-                    const double mCn = m + 1 < nr ? rmC[s1] : 0.0;
-                    // diffusion, reference tracer_implicit.py:54-64
-                    if (m >= 1) rb = -(((((dtm * mC1) * rhm) * rdf) * rkap[s]) * a.rdrC[m]);
-                    if (m + 1 < nr) rd = -(((((dtm * mCn) * rhm) * rdf) * rkap[s1]) * a.rdrC[m + 1]);
-                    rc = 1.0 - (rb + rd);
-                    if (a.adv) {
-                        // U3 advection, :95-105: level m takes the updates of interface m+1 (its bottom) first,
-                        // then those of interface m (its top)
-                        Iface lo{0.0, 0.0, 0.0, 0.0};
-                        if (m + 1 < nr) {
-                            const double mCk = m + 2 < nr ? rmC[s2] : mCn;
-                            lo = iface(a, m + 1, rw[s1], rA, rrA, mC0, m >= 1 ? mC1 : mC0, mCk);
-                            rb = rb - ((lo.km * dtm) * rhm) * rdf;
-                            rc = rc + (((((lo.rC + lo.rUp) + lo.km) * dtm) * rhm) * rdf);
-                            rd = rd + (((((lo.rC - lo.rUp) + lo.kp) * dtm) * rhm) * rdf);
-                            re = re - ((lo.kp * dtm) * rhm) * rdf;
-                        }
-                        if (m >= 1) {
-                            ra = ra + ((up.km * dtm) * rhm) * rdf;
-                            rb = rb - (((((up.rC + up.rUp) + up.km) * dtm) * rhm) * rdf);
-                            rc = rc - (((((up.rC - up.rUp) + up.kp) * dtm) * rhm) * rdf);
-                            rd = rd + ((up.kp * dtm) * rhm) * rdf;
-                        }
-                        up = lo;
-                    }
-                }
-                // forward elimination of row m, reference tracer_implicit.py:126-143
-                double dq, eq = re, yq;
-                if (m == 0) {
-                    dq = rd;
-                    yq = ry[s];
-                    norm3(rc, dq, eq, yq);
-                } else if (m == 1) {
-                    const double c2 = rc - rb * dp1;
-                    dq = rd - rb * ep1;
-                    yq = ry[s] - rb * yp1;
-                    norm3(c2, dq, eq, yq);
-                } else {
-                    const double bb = rb - ra * dp2;
-                    const double cq = (rc - ra * ep2) - bb * dp1;
-                    dq = rd - bb * ep1;
-                    yq = (ry[s] - ra * yp2) - bb * yp1;
-                    norm3(cq, dq, eq, yq);
-                }
-                dpL[(int64_t)m * kImprThreads] = dq;
-                epL[(int64_t)m * kImprThreads] = eq;
-                out[(int64_t)m * a.plane] = yq;
-                dp2 = dp1, ep2 = ep1, yp2 = yp1;
-                dp1 = dq, ep1 = eq, yp1 = yq;
-                mC1 = mC0;
-                if (m + kPf < nr) load(s, m + kPf);
-            }
+            const int s0 = (2 + s) % kPf, s1 = (3 + s) % kPf, s2 = (4 + s) % kPf;
+            impr_row<false, ADV>(a, m, in, rA, rrA, rmC[s0], rrh[s0], rkap[s0], ry[s0], rmC[s1], rkap[s1], rw[s1],
+                            rmC[s2], c, dpL, epL, out, lv);
+            load(s0, m + kPf);
+        }
+    }
+    // the remaining rows (at most kPf + 1)
+#pragma unroll
+    for (int s = 0; s < kPf + 2; ++s) {
+        const int m = m0 + s;
+        if (m < nr) {
+            const int s0 = (2 + s) % kPf, s1 = (3 + s) % kPf, s2 = (4 + s) % kPf;
+            impr_row<true, ADV>(a, m, in, rA, rrA, rmC[s0], rrh[s0], rkap[s0], ry[s0], rmC[s1], rkap[s1], rw[s1],
+                           mck(m, s2, s1), c, dpL, epL, out, lv);
+            load(s0, m + kPf);
         }
     }
     // back substitution, reference tracer_implicit.py:150-160: u(nr-1) = y'(nr-1) is in place already
     if (nr < 2) return;
-    double u1 = yp1;                          // u(m+1)
-    double u0 = yp2 - u1 * dpL[(int64_t)(nr - 2) * kImprThreads];
+    double u1 = c.yp1;  // u(m+1)
+    double u0 = c.yp2 - u1 * dpL[(int64_t)(nr - 2) * kImprThreads];
     out[(int64_t)(nr - 2) * a.plane] = u0;
     double u2 = u1;
     u1 = u0;
-    // y'(m) for m = nr-3 .. 0 are re-read from the output column, kPf levels ahead
+    // y'(m) for m = nr-3 .. 0 are re-read from the output column, kPf levels ahead (clamped to level 0)
     double ryp[kPf];
 #pragma unroll
-    for (int s = 0; s < kPf; ++s) ryp[s] = nr - 3 - s >= 0 ? out[(int64_t)(nr - 3 - s) * a.plane] : 0.0;
-    for (int m0 = nr - 3; m0 >= 0; m0 -= kPf) {
+    for (int s = 0; s < kPf; ++s) ryp[s] = out[(int64_t)max(nr - 3 - s, 0) * a.plane];
+    for (int mb = nr - 3; mb >= 0; mb -= kPf) {
 #pragma unroll
         for (int s = 0; s < kPf; ++s) {
-            const int m = m0 - s;
+            const int m = mb - s;
             if (m >= 0) {
                 const double u = (ryp[s] - u1 * dpL[(int64_t)m * kImprThreads]) - u2 * epL[(int64_t)m * kImprThreads];
                 out[(int64_t)m * a.plane] = u;
                 u2 = u1;
                 u1 = u;
-                if (m - kPf >= 0) ryp[s] = out[(int64_t)(m - kPf) * a.plane];
+                ryp[s] = out[(int64_t)max(m - kPf, 0) * a.plane];
             }
         }
     }
@@ -244,14 +292,17 @@ extern "C" int mj_gad_implicit_r(const mj_impr_grid* g, const mj_impr_config* c,
     a.rkSign = g->rkSign;
     a.kap = kappaRX;
     a.rh = recip_hFac;
-    a.w = wFld;
+    a.w = c->implicitAdvection ? wFld : gTracer;  // without advection the w loads read a valid dummy
     a.y = gTracer;
     a.out = out;
-    const size_t lds = (size_t)2 * c->nr * kImprThreads * sizeof(double);
-    if (lds > 65536)
-        MJ_HIP(hipFuncSetAttribute((const void*)impr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const size_t lds = ((size_t)2 * c->nr * kImprThreads + 3 * (size_t)c->nr) * sizeof(double);
+    const void* kern = a.adv ? (const void*)impr_kernel<true> : (const void*)impr_kernel<false>;
+    if (lds > 65536) MJ_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const dim3 grid((unsigned)((cols + kImprThreads - 1) / kImprThreads), (unsigned)c->batch);
-    hipLaunchKernelGGL(impr_kernel, grid, dim3(kImprThreads), lds, st, a);
+    if (a.adv)
+        hipLaunchKernelGGL(impr_kernel<true>, grid, dim3(kImprThreads), lds, st, a);
+    else
+        hipLaunchKernelGGL(impr_kernel<false>, grid, dim3(kImprThreads), lds, st, a);
     MJ_HIP(hipGetLastError());
     return MJ_OK;
 }
