@@ -111,6 +111,38 @@ def test_operator_and_rhs_producers_validate_before_device_work():
     assert "deltaTMom" in _lib.last_error()
 
 
+def test_implicit_r_validates_before_device_work():
+    """mj_gad_implicit_r refuses null arguments, bad sizes and aliasing with a message (MJ_ERR_ARG) before touching
+    the device; the Python op refuses the branches the reference does not port."""
+    import ctypes
+    from types import SimpleNamespace
+    from mit_jax_amd import _lib, tracer_implicit as TI
+    L = _lib.lib()
+    d = [ctypes.c_void_p(16 * (n + 1)) for n in range(8)]
+    grid = _lib.ImprGrid(d[0], d[1], d[2], d[3], d[4], d[5], -1.0)
+    n3 = 13 * 50 * 98 * 98
+
+    def call(cfg, *arrs):
+        return L.mj_gad_implicit_r(ctypes.byref(grid), ctypes.byref(cfg), *arrs, None)
+
+    good = _lib.ImprConfig(2, 13, 90, 90, 4, 50, n3, 0, 1)
+    assert L.mj_gad_implicit_r(None, ctypes.byref(good), d[6], d[7], d[6], d[7], d[6], None) == -1
+    for field, value, word in (("nr", 151, "nr"), ("batch", 0, "batch"), ("member_stride3d", 5, "member strides")):
+        c = _lib.ImprConfig(2, 13, 90, 90, 4, 50, n3, 0, 1)
+        setattr(c, field, value)
+        assert call(c, d[6], d[7], d[5], d[4], d[3]) == -1, field
+        assert word in _lib.last_error(), (field, _lib.last_error())
+    assert call(good, d[6], d[7], None, d[4], d[3]) == -1
+    assert "wFld" in _lib.last_error()
+    assert call(good, d[6], d[7], d[5], d[4], d[6]) == -1
+    assert "alias" in _lib.last_error()
+    p = SimpleNamespace(implicitDiffusion=False, dTtracerLev=[3600.0], rkSign=-1.0)
+    with pytest.raises(NotImplementedError):
+        TI.gad_implicit_r(p, None, True, 3, None, None, None, None)
+    with pytest.raises(NotImplementedError):
+        TI.gad_implicit_r(SimpleNamespace(implicitDiffusion=True), None, True, 4, None, None, None, None)
+
+
 def test_unported_branches_raise():
     """As the reference raises at setup for the branches it does not port (gad.py:159-182, cg2d.py:212-213)."""
     from mit_jax_amd import cg2d, gad
