@@ -49,6 +49,41 @@ class ExchangeMaps:
     def names(self):
         return sorted(self.compact)
 
+    @classmethod
+    def from_full_npz(cls, path):
+        """Maps from a file in the reference's format (reference parallel/exchange.py:72-92, ExchangeMaps.save/load:
+        per map name `<name>.src`, `.comp`, `.sign` over every padded point, and `layout` = sNx, sNy, OLx, OLy,
+        nTiles, Nr), e.g. maps probed from another exch2 configuration. Kept points must be the identity."""
+        z = np.load(path, allow_pickle=False)
+        sNx, sNy, OLx, OLy, nT, Nr = (int(v) for v in z["layout"])
+        L = Layout(sNx, sNy, OLx, OLy, nT, Nr)
+        n = nT * L.ny * L.nx
+        names = sorted({k.rsplit(".", 1)[0] for k in z.files if k != "layout"})
+        comp = {}
+        for name in names:
+            src = np.asarray(z[name + ".src"]).astype(np.int64)
+            cmp_ = np.asarray(z[name + ".comp"]).astype(np.int8)
+            sign = np.asarray(z[name + ".sign"]).astype(np.int8)
+            if src.size != n or cmp_.size != n or sign.size != n:
+                raise ValueError(f"{path}: map {name} does not cover the {n} padded points")
+            keep = cmp_ == 0
+            if not (np.all(src[keep] == np.arange(n)[keep]) and np.all(sign[keep] == 1)):
+                raise ValueError(f"{path}: map {name}: a kept point is not the identity")
+            if not (np.all(np.isin(cmp_, (0, 1, 2))) and np.all(np.isin(sign, (-1, 1))) and
+                    np.all((src >= 0) & (src < n))):
+                raise ValueError(f"{path}: map {name}: comp/sign/src out of range")
+            dst = np.nonzero(~keep)[0]
+            comp[name] = (dst.astype(np.int32), src[dst].astype(np.int32), cmp_[dst].copy(), sign[dst].copy())
+        return cls(L, comp)
+
+    def save_full(self, path):
+        """Write the maps in the reference's format (the inverse of from_full_npz)."""
+        L = self.layout
+        arrs = {"layout": np.array([L.sNx, L.sNy, L.OLx, L.OLy, L.nTiles, L.Nr])}
+        for name in self.names():
+            arrs[name + ".src"], arrs[name + ".comp"], arrs[name + ".sign"] = self.full(name)
+        np.savez_compressed(path, **arrs)
+
 
 @lru_cache(maxsize=4)
 def _load_maps(path):
