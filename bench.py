@@ -292,7 +292,8 @@ def producers(E, dev, reps=5):
     for name, ms, nb, kern in (("update_cg2d", tu, ub, "sfp_opsum_kernel"), ("cg2d_rhs", tr, rb, "sfp_rhs_kernel")):
         gbs = nb / (ms * 1e-3) / 1e9
         res[name] = {"ms": ms, "member_calls_per_s": B / (ms * 1e-3), "algorithmic_bytes": nb, "achieved_GBs": gbs,
-                     "frac": gbs / HBM_PEAK_GBS, "dominant_kernel": kern}
+                     "frac": gbs / HBM_PEAK_GBS, "dominant_kernel": kern,
+                     "dominant_kernel_traffic": measured_traffic(kern, B)}
     res["update_cg2d"]["operators_equal_cpu_builder"] = same
     return res
 
@@ -344,7 +345,8 @@ def implicit_r(E, dev, reps=5):
                  (wt * E["tr"]).sum(dim=2).abs().max())
     del kap, w, out
     return {"ms": t, "member_calls_per_s": B / (t * 1e-3), "algorithmic_bytes": nb, "achieved_GBs": gbs,
-            "frac": gbs / HBM_PEAK_GBS, "kernel": "impr_kernel", "column_integral_max_rel_change": cons}
+            "frac": gbs / HBM_PEAK_GBS, "kernel": "impr_kernel", "traffic": measured_traffic("impr_kernel", B),
+            "column_integral_max_rel_change": cons}
 
 
 def main():
