@@ -316,6 +316,14 @@ typedef struct {
 int mj_gad_implicit_r(const mj_impr_grid* grid, const mj_impr_config* cfg, const double* kappaRX,
                       const double* recip_hFac, const double* wFld, const double* gTracer, double* out, void* stream);
 
+/* Reverse mode of mj_gad_implicit_r (the reference differentiates the literal solve with JAX): for u the forward
+ * result and ubar its cotangent, ybar = A^-T ubar (the same guarded elimination on the transposed diagonals) and the
+ * cotangents of kappaRX and (with implicitAdvection) wFld; halo columns: ybar = ubar, no parameter cotangent. nr >= 2.
+ * Outputs [B][T, nr, ny, nx], none aliasing an input; wFld_bar may be NULL without advection. */
+int mj_gad_implicit_r_adjoint(const mj_impr_grid* grid, const mj_impr_config* cfg, const double* kappaRX,
+                              const double* recip_hFac, const double* wFld, const double* u, const double* ubar,
+                              double* ybar, double* kappaRX_bar, double* wFld_bar, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,6 +110,8 @@ SIGNATURES = {
     "mj_cg2d_rhs": (I32, [ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, P, P, P,
                           P, P, P, P, P, P, P]),
     "mj_gad_implicit_r": (I32, [ctypes.POINTER(ImprGrid), ctypes.POINTER(ImprConfig), P, P, P, P, P, P]),
+    "mj_gad_implicit_r_adjoint": (I32, [ctypes.POINTER(ImprGrid), ctypes.POINTER(ImprConfig), P, P, P, P, P, P, P,
+                                        P, P]),
     "mj_gad_plan_create": (I32, [I32, P, P, I32, I32, I32, I32, ctypes.POINTER(P)]),
     "mj_gad_plan_destroy": (I32, [P]),
     "mj_gad_plan_describe": (I32, [P, P, P]),

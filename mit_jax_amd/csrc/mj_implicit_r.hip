@@ -306,3 +306,233 @@ extern "C" int mj_gad_implicit_r(const mj_impr_grid* g, const mj_impr_config* c,
     MJ_HIP(hipGetLastError());
     return MJ_OK;
 }
+
+// ---------------------------------------------------------------------------------------------------- reverse mode
+// u = A(kappaRX, wFld)^-1 y per column. Given ubar: lam = A^-T ubar by the same guarded elimination on the
+// transposed diagonals (row m of A^T = (e(m-2), d(m-1), c(m), b(m+1), a(m+2))), ybar = lam, and the cotangents of
+// kappaRX and wFld from the derivatives of the rows in those parameters (diffusion: b(k), d(k-1), c in kappaRX(k);
+// U3: rows K-1 and K in the rCenter of interface K). One thread per column; rows are built two ahead of the
+// transposed elimination, from level data read as needed (the reverse mode is not tuned).
+namespace mj {
+namespace {
+
+struct Row {
+    double a, b, c, d, e;
+};
+
+template <bool ADV>
+__device__ Row build_row(const ImprArgs& a, const double* lv, int m, bool in, double rA, double rrA, const double* mCp,
+                         const double* rhp, const double* kp, const double* wp, Iface& up) {
+    const int nr = a.nr;
+    Row r{0.0, 0.0, 1.0, 0.0, 0.0};
+    if (!in) return r;
+    const int64_t P = a.plane;
+    const double dtm = lv[m], rdf = lv[nr + m], rh0 = rhp[m * P];
+    const double* rdrC = lv + 2 * nr;
+    const double mC0 = mCp[m * P], mC1 = m >= 1 ? mCp[(m - 1) * P] : 0.0;
+    const double mCn = m + 1 < nr ? mCp[(m + 1) * P] : 0.0;
+    if (m >= 1) r.b = -(((((dtm * mC1) * rh0) * rdf) * kp[m * P]) * rdrC[m]);
+    if (m + 1 < nr) r.d = -(((((dtm * mCn) * rh0) * rdf) * kp[(m + 1) * P]) * rdrC[m + 1]);
+    r.c = 1.0 - (r.b + r.d);
+    if (ADV) {
+        Iface lo{0.0, 0.0, 0.0, 0.0};
+        if (m + 1 < nr) {
+            const double mCk = m + 2 < nr ? mCp[(m + 2) * P] : mCn;
+            lo = iface(a, m + 1, wp[(m + 1) * P], rA, rrA, mC0, m >= 1 ? mC1 : mC0, mCk);
+            r.b = r.b - ((lo.km * dtm) * rh0) * rdf;
+            r.c = r.c + (((((lo.rC + lo.rUp) + lo.km) * dtm) * rh0) * rdf);
+            r.d = r.d + (((((lo.rC - lo.rUp) + lo.kp) * dtm) * rh0) * rdf);
+            r.e = r.e - ((lo.kp * dtm) * rh0) * rdf;
+        }
+        if (m >= 1) {
+            r.a = r.a + ((up.km * dtm) * rh0) * rdf;
+            r.b = r.b - (((((up.rC + up.rUp) + up.km) * dtm) * rh0) * rdf);
+            r.c = r.c - (((((up.rC - up.rUp) + up.kp) * dtm) * rh0) * rdf);
+            r.d = r.d + ((up.kp * dtm) * rh0) * rdf;
+        }
+        up = lo;
+    }
+    return r;
+}
+
+template <bool ADV>
+__global__ void __launch_bounds__(kImprThreads)
+    impr_adjoint_kernel(ImprArgs a, const double* __restrict__ u, const double* __restrict__ ubar, double* ybar,
+                        double* kbar, double* wbar) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int64_t col = (int64_t)blockIdx.x * kImprThreads + lane;
+    const int mem = blockIdx.y;
+    double* lv = lds + (int64_t)2 * a.nr * kImprThreads;
+    for (int k = lane; k < a.nr; k += kImprThreads) {
+        lv[k] = a.dt[k];
+        lv[a.nr + k] = a.rdrF[k];
+        lv[2 * a.nr + k] = a.rdrC[k];
+    }
+    __syncthreads();
+    if (col >= (int64_t)a.T * a.plane) return;  // no barriers below
+    const int t = (int)(col / a.plane), q = (int)(col - (int64_t)t * a.plane);
+    const int j = q / a.nx, i = q - j * a.nx;
+    const bool in = j >= a.OL && j < a.OL + a.sNy && i >= a.OL && i < a.OL + a.sNx;
+    const int nr = a.nr;
+    const int64_t P = a.plane;
+    const int64_t o3 = mem * a.ms3 + (int64_t)t * nr * P + q;
+    const int64_t om = mem * a.mms + (int64_t)t * nr * P + q;
+    const int64_t o2 = (int64_t)t * P + q;
+    const double rA = ADV && in ? a.rA[o2] : 0.0;
+    const double rrA = ADV && in ? a.recip_rA[o2] : 0.0;
+    const double *mCp = a.maskC + om, *rhp = a.rh + o3, *kp = a.kap + o3, *wp = a.w + o3;
+    const double *uc = u + o3, *ub = ubar + o3;
+    double* yb = ybar + o3;
+    double* dpL = lds + lane;
+    double* epL = lds + (int64_t)nr * kImprThreads + lane;
+    // transposed elimination, rows of A built two ahead
+    Iface up{0.0, 0.0, 0.0, 0.0};
+    const Row zero{0.0, 0.0, 0.0, 0.0, 0.0};
+    Row rM2 = zero, rM1 = zero, r0 = zero, rP1 = zero, rP2 = zero;
+    double dp1 = 0.0, ep1 = 0.0, yp1 = 0.0, dp2 = 0.0, ep2 = 0.0, yp2 = 0.0;
+    for (int r = 0; r < nr + 2; ++r) {
+        rM2 = rM1, rM1 = r0, r0 = rP1, rP1 = rP2;
+        rP2 = r < nr ? build_row<ADV>(a, lv, r, in, rA, rrA, mCp, rhp, kp, wp, up) : zero;
+        const int m = r - 2;
+        if (m < 0) continue;
+        const double ra = rM2.e, rb = rM1.d, rc = r0.c, rd = rP1.b, re = rP2.a;  // row m of A^T
+        double dq, eq = re, yq;
+        const double y0 = ub[m * P];
+        if (m == 0) {
+            dq = rd;
+            yq = y0;
+            norm3(rc, dq, eq, yq);
+        } else if (m == 1) {
+            const double c2 = rc - rb * dp1;
+            dq = rd - rb * ep1;
+            yq = y0 - rb * yp1;
+            norm3(c2, dq, eq, yq);
+        } else {
+            const double bb = rb - ra * dp2;
+            const double cq = (rc - ra * ep2) - bb * dp1;
+            dq = rd - bb * ep1;
+            yq = (y0 - ra * yp2) - bb * yp1;
+            norm3(cq, dq, eq, yq);
+        }
+        dpL[(int64_t)m * kImprThreads] = dq;
+        epL[(int64_t)m * kImprThreads] = eq;
+        yb[m * P] = yq;
+        dp2 = dp1, ep2 = ep1, yp2 = yp1;
+        dp1 = dq, ep1 = eq, yp1 = yq;
+    }
+    if (nr >= 2) {
+        double u1 = yp1;
+        double u0 = yp2 - u1 * dpL[(int64_t)(nr - 2) * kImprThreads];
+        yb[(int64_t)(nr - 2) * P] = u0;
+        double u2 = u1;
+        u1 = u0;
+        for (int m = nr - 3; m >= 0; --m) {
+            const double v = (yb[m * P] - u1 * dpL[(int64_t)m * kImprThreads]) - u2 * epL[(int64_t)m * kImprThreads];
+            yb[m * P] = v;
+            u2 = u1;
+            u1 = v;
+        }
+    }
+    // parameter cotangents (interior columns; level 0 and the surface interface have none)
+    double* kb = kbar + o3;
+    double* wbp = ADV ? wbar + o3 : nullptr;
+    kb[0] = 0.0;
+    if (ADV) wbp[0] = 0.0;
+    const double* rdrC = lv + 2 * nr;
+    for (int k = 1; k < nr; ++k) {
+        double gk = 0.0, gw = 0.0;
+        if (in) {
+            const double lk = yb[k * P], lk1 = yb[(k - 1) * P];
+            const double uk = uc[k * P], uk1 = uc[(k - 1) * P];
+            const double mCk = mCp[k * P], mCk1 = mCp[(k - 1) * P];
+            const double rhk = rhp[k * P], rhk1 = rhp[(k - 1) * P];
+            const double beta = -((((lv[k] * mCk1) * rhk) * lv[nr + k]) * rdrC[k]);
+            const double delta = -((((lv[k - 1] * mCk) * rhk1) * lv[nr + k - 1]) * rdrC[k]);
+            const double du = uk - uk1;
+            gk = -((lk * beta) * (-du) + (lk1 * delta) * du);
+            if (ADV) {
+                const double dw = (((0.5 * rA) * mCk1) * rrA) * a.rkSign;
+                const double rC = ((0.5 * ((wp[k * P] * rA) * mCk1)) * rrA) * a.rkSign;
+                const double sg = (double)((rC > 0.0) - (rC < 0.0));
+                const double mskM = mCp[(k >= 2 ? k - 2 : 0) * P] * (k <= 1 ? 0.0 : 1.0);
+                const double mskP = mCp[(k + 1 < nr ? k + 1 : nr - 1) * P] * (k >= nr - 1 ? 0.0 : 1.0);
+                const double drUp = (2.0 * kSixth) * sg;
+                const double dkm = (kSixth * (1.0 + sg)) * mskM;
+                const double dkp = (kSixth * (1.0 - sg)) * mskP;
+                const double uKm2 = k >= 2 ? uc[(k - 2) * P] : 0.0, uKp1 = k + 1 < nr ? uc[(k + 1) * P] : 0.0;
+                const double phi = (((-dkm) * uKm2 + ((1.0 + drUp) + dkm) * uk1) + ((1.0 - drUp) + dkp) * uk) - dkp * uKp1;
+                const double s1 = (lv[k - 1] * rhk1) * lv[nr + k - 1], s0 = (lv[k] * rhk) * lv[nr + k];
+                const double rCb = -((lk1 * s1) - (lk * s0)) * phi;
+                gw = rCb * dw;
+            }
+        }
+        kb[k * P] = gk;
+        if (ADV) wbp[k * P] = gw;
+    }
+}
+
+}  // namespace
+}  // namespace mj
+
+extern "C" int mj_gad_implicit_r_adjoint(const mj_impr_grid* g, const mj_impr_config* c, const double* kappaRX,
+                                         const double* recip_hFac, const double* wFld, const double* u,
+                                         const double* ubar, double* ybar, double* kappaRX_bar, double* wFld_bar,
+                                         void* stream) {
+    MJ_REQUIRE(g && c, "mj_gad_implicit_r_adjoint: null grid/config");
+    MJ_REQUIRE(c->batch >= 1 && c->batch <= 65535 && c->ntiles >= 1 && c->nr >= 2 && c->nr <= 150,
+               "mj_gad_implicit_r_adjoint: batch in 1..65535, ntiles >= 1, nr in 2..150");
+    MJ_REQUIRE(c->sNx >= 1 && c->sNy >= 1 && c->OL >= 0, "mj_gad_implicit_r_adjoint: bad tile layout");
+    const int nx = c->sNx + 2 * c->OL, ny = c->sNy + 2 * c->OL;
+    const int64_t plane = (int64_t)nx * ny, member = (int64_t)c->ntiles * c->nr * plane;
+    MJ_REQUIRE(c->batch == 1 || (c->member_stride3d >= member && (c->mask_member_stride == 0 ||
+                                                                  c->mask_member_stride >= member)),
+               "mj_gad_implicit_r_adjoint: member strides smaller than one member's arrays");
+    MJ_REQUIRE(kappaRX && recip_hFac && u && ubar && ybar && kappaRX_bar && g->maskC && g->dTtracerLev &&
+                   g->recip_drF && g->recip_drC,
+               "mj_gad_implicit_r_adjoint: null array or grid field");
+    MJ_REQUIRE(!c->implicitAdvection || (wFld && wFld_bar && g->rA && g->recip_rA),
+               "mj_gad_implicit_r_adjoint: implicit advection needs wFld, wFld_bar, rA, recip_rA");
+    const double* ins[] = {kappaRX, recip_hFac, wFld, u, ubar, g->maskC};
+    for (const double* p : ins)
+        MJ_REQUIRE(p != ybar && p != kappaRX_bar && (!wFld_bar || p != wFld_bar),
+                   "mj_gad_implicit_r_adjoint: an output aliases an input");
+    ImprArgs a;
+    a.T = c->ntiles;
+    a.nx = nx;
+    a.ny = ny;
+    a.sNx = c->sNx;
+    a.sNy = c->sNy;
+    a.OL = c->OL;
+    a.nr = c->nr;
+    a.adv = c->implicitAdvection ? 1 : 0;
+    a.plane = plane;
+    a.ms3 = c->member_stride3d;
+    a.mms = c->mask_member_stride;
+    a.maskC = g->maskC;
+    a.rA = g->rA;
+    a.recip_rA = g->recip_rA;
+    a.dt = g->dTtracerLev;
+    a.rdrF = g->recip_drF;
+    a.rdrC = g->recip_drC;
+    a.rkSign = g->rkSign;
+    a.kap = kappaRX;
+    a.rh = recip_hFac;
+    a.w = c->implicitAdvection ? wFld : kappaRX;
+    a.y = nullptr;
+    a.out = nullptr;
+    hipStream_t st = as_stream(stream);
+    const size_t lds = ((size_t)2 * c->nr * kImprThreads + 3 * (size_t)c->nr) * sizeof(double);
+    const void* kern = a.adv ? (const void*)impr_adjoint_kernel<true> : (const void*)impr_adjoint_kernel<false>;
+    if (lds > 65536) MJ_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int64_t cols = (int64_t)c->ntiles * plane;
+    const dim3 grid((unsigned)((cols + kImprThreads - 1) / kImprThreads), (unsigned)c->batch);
+    if (a.adv)
+        hipLaunchKernelGGL(impr_adjoint_kernel<true>, grid, dim3(kImprThreads), lds, st, a, u, ubar, ybar,
+                           kappaRX_bar, wFld_bar);
+    else
+        hipLaunchKernelGGL(impr_adjoint_kernel<false>, grid, dim3(kImprThreads), lds, st, a, u, ubar, ybar,
+                           kappaRX_bar, (double*)nullptr);
+    MJ_HIP(hipGetLastError());
+    return MJ_OK;
+}

@@ -113,3 +113,58 @@ def gad_implicit_r(p, g, implicitAdvection, kappaRX, recip_hFac, wFld, gTracer, 
     if gTracer.shape[1] <= 1:
         return np.array(gTracer, np.float64)
     return solve_pentadiagonal(*matrix(p, g, implicitAdvection, kappaRX, recip_hFac, wFld, sNx, sNy, OL), gTracer)
+
+
+def gad_implicit_r_adjoint(p, g, implicitAdvection, kappaRX, recip_hFac, wFld, u, ubar, sNx=90, sNy=90, OL=4):
+    """Reverse mode of gad_implicit_r for u = A(kappaRX, wFld)^-1 y (the reference differentiates the literal
+    solve with JAX): lam = A^-T ubar (the same elimination on the transposed diagonals), ybar = lam,
+    kappaRX_bar(k) = -(lam(k) beta(k) (u(k-1) - u(k)) + lam(k-1) delta(k-1) (u(k) - u(k-1))) with beta, delta the
+    derivatives of b(k), d(k-1) in kappaRX(k); wFld_bar(K) from the U3 interface K's coefficients (rows K-1 and K).
+    Halo columns: ybar = ubar, no parameter cotangent. Returns (ybar, kappaRX_bar, wFld_bar)."""
+    T, Nr = u.shape[:2]
+    if Nr <= 1:
+        return np.array(ubar, np.float64), np.zeros_like(u), np.zeros_like(u)
+    a, b, c, d, e = matrix(p, g, implicitAdvection, kappaRX, recip_hFac, wFld, sNx, sNy, OL)
+    aT, bT, dT, eT = (np.zeros_like(a) for _ in range(4))
+    aT[:, 2:] = e[:, :-2]
+    bT[:, 1:] = d[:, :-1]
+    dT[:, :-1] = b[:, 1:]
+    eT[:, :-2] = a[:, 2:]
+    lam = solve_pentadiagonal(aT, bT, c, dT, eT, ubar)
+    J, I = slice(OL, OL + sNy), slice(OL, OL + sNx)
+    dt = np.asarray(p.dTtracerLev, np.float64)[None, :, None, None]
+    mC = g.maskC[:, :, J, I]
+    rh = recip_hFac[:, :, J, I]
+    rdrF = np.asarray(g.recip_drF, np.float64)[None, :, None, None]
+    rdrC = np.asarray(g.recip_drC, np.float64)[None, :, None, None]
+    ui = u[:, :, J, I]
+    li = lam[:, :, J, I]
+    kb = np.zeros_like(ui)
+    beta = -((((dt[:, 1:] * mC[:, :-1]) * rh[:, 1:]) * rdrF[:, 1:]) * rdrC[:, 1:])      # d b(k) / d kap(k)
+    delta = -((((dt[:, :-1] * mC[:, 1:]) * rh[:, :-1]) * rdrF[:, :-1]) * rdrC[:, 1:])   # d d(k-1) / d kap(k)
+    du = ui[:, 1:] - ui[:, :-1]                                                       # u(k) - u(k-1)
+    kb[:, 1:] = -((li[:, 1:] * beta) * (-du) + (li[:, :-1] * delta) * du)
+    wb = np.zeros_like(ui)
+    if implicitAdvection:
+        ks = np.arange(1, Nr)                                                         # interfaces K (0-based)
+        rA = g.rA[:, None, J, I]
+        rrA = g.recip_rA[:, None, J, I]
+        dw = ((0.5 * rA) * mC[:, :-1]) * rrA * p.rkSign                               # d rC / d w(K)
+        rC = ((0.5 * ((wFld[:, 1:, J, I] * rA) * mC[:, :-1])) * rrA) * p.rkSign
+        sg = np.sign(rC)
+        mskM = mC[:, np.maximum(0, ks - 2)] * np.where(ks <= 1, 0.0, 1.0)[None, :, None, None]
+        mskP = mC[:, np.minimum(Nr - 1, ks + 1)] * np.where(ks >= Nr - 1, 0.0, 1.0)[None, :, None, None]
+        drUp = (2.0 * ONE_SIXTH) * sg
+        dkm = (ONE_SIXTH * (1.0 + sg)) * mskM
+        dkp = (ONE_SIXTH * (1.0 - sg)) * mskP
+        upad = np.concatenate([np.zeros_like(ui[:, :1]), ui, np.zeros_like(ui[:, :1])], axis=1)  # u(-1) .. u(Nr)
+        uKm2, uKm1, uK, uKp1 = upad[:, ks - 1], upad[:, ks], upad[:, ks + 1], upad[:, ks + 2]
+        phi = ((-dkm * uKm2 + ((1.0 + drUp) + dkm) * uKm1) + ((1.0 - drUp) + dkp) * uK) - dkp * uKp1
+        s = (dt * rh) * rdrF                                                          # row scale of level L
+        rCb = -((li[:, :-1] * s[:, :-1]) - (li[:, 1:] * s[:, 1:])) * phi
+        wb[:, 1:] = rCb * dw
+    def full(x):
+        out = np.zeros_like(u)
+        out[:, :, J, I] = x
+        return out
+    return lam, full(kb), full(wb)

@@ -190,3 +190,52 @@ def test_planted_errors_break_the_fortran_loop_match(old, new):
     got = _planted(old, new).matrix(p, g, True, kap, rh, w, **lay)
     ref = _loop_matrix(p, g, True, kap, rh, w, **lay)
     assert not all(np.array_equal(got[n], ref[n]) for n in range(5))
+
+
+def _adjoint_pins(mod, seed=7):
+    """The reverse mode's pins: the dot-product identity for the tracer cotangent (the solve is linear in y) and
+    central differences of <ubar, u> in kappaRX and wFld at interior points; returns the failed pins."""
+    p, g, kap, rh, w, y, hFac, drF, lay = _case(seed)
+    u = mod.gad_implicit_r(p, g, True, kap, rh, w, y, **lay)
+    rng = np.random.default_rng(seed + 1)
+    ub = rng.normal(size=u.shape)
+    yb, kb, wb = mod.gad_implicit_r_adjoint(p, g, True, kap, rh, w, u, ub, **lay)
+    bad = []
+    L = lambda kap_, w_, y_: float((ub * mod.gad_implicit_r(p, g, True, kap_, rh, w_, y_, **lay)).sum())  # noqa: E731
+    y2 = rng.normal(size=y.shape)
+    if abs(L(kap, w, y2) - float((yb * y2).sum())) > 1e-12 * abs(L(kap, w, y2)):
+        bad.append("dot")
+    pts = [(0, 3, 3, 4), (0, 1, 5, 6), (1, 8, 2, 3), (1, 2, 6, 9), (0, 6, 7, 8)]
+    for name, arr, gb in (("kap", kap, kb), ("w", w, wb)):
+        for pt in pts:
+            if g.maskC[pt] == 0:
+                continue
+            errs = []
+            for rel in (1e-4, 1e-5, 1e-6):
+                h = rel * max(abs(arr[pt]), 1e-8)
+                e = np.zeros_like(arr)
+                e[pt] = h
+                if name == "kap":
+                    fd = (L(kap + e, w, y) - L(kap - e, w, y)) / (2 * h)
+                else:
+                    fd = (L(kap, w + e, y) - L(kap, w - e, y)) / (2 * h)
+                errs.append(abs(fd - gb[pt]) / max(abs(gb[pt]), abs(fd), 1e-30))
+            if min(errs) > 1e-5:
+                bad.append((name, pt, min(errs)))
+    return bad
+
+
+def test_adjoint_dot_product_and_finite_differences():
+    assert _adjoint_pins(OI) == []
+
+
+@pytest.mark.parametrize("old,new", [
+    ("    kb[:, 1:] = -((li[:, 1:] * beta) * (-du) + (li[:, :-1] * delta) * du)",
+     "    kb[:, 1:] = -((li[:, 1:] * beta) * (-du) - (li[:, :-1] * delta) * du)"),
+    ("        dkm = (ONE_SIXTH * (1.0 + sg)) * mskM", "        dkm = (ONE_SIXTH * (1.0 - sg)) * mskM"),
+    ("    bT[:, 1:] = d[:, :-1]", "    bT[:, 1:] = b[:, 1:]"),
+    ("        rCb = -((li[:, :-1] * s[:, :-1]) - (li[:, 1:] * s[:, 1:])) * phi",
+     "        rCb = -((li[:, :-1] * s[:, 1:]) - (li[:, 1:] * s[:, :-1])) * phi"),
+])
+def test_adjoint_planted_errors_fail_the_pins(old, new):
+    assert _adjoint_pins(_planted(old, new)) != []
