@@ -75,14 +75,19 @@ class _GridDev:
 
 
 _grid_cache = {}
+_GRID_CACHE_MAX = 4
 
 
 def _grid(g, device):
+    """Device copies of g's fields, kept for the last few grids used (keyed by identity, so a grid object must not
+    be modified in place between calls)."""
     key = (id(g), str(device))
-    hit = _grid_cache.get(key)
+    hit = _grid_cache.pop(key, None)
     if hit is None or hit[0] is not g:
         hit = (g, _GridDev(g, device))
-        _grid_cache[key] = hit
+    _grid_cache[key] = hit  # most recently used last
+    while len(_grid_cache) > _GRID_CACHE_MAX:
+        _grid_cache.pop(next(iter(_grid_cache)))
     return hit[1]
 
 
