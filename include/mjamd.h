@@ -232,6 +232,17 @@ int mj_cg2d_rhs(const mj_sfp_grid* grid, const mj_fs_params* p, const mj_sfp_con
                 const double* gV, const double* hFacW, const double* hFacS, const double* EmPmR, const double* etaN,
                 const double* etaH, const double* Bo_surf, double* cg2d_b, double* cg2d_x, void* stream);
 
+/* Reverse mode of mj_cg2d_rhs (JAX's transpose of cg2d_rhs in the reference; no counterpart file): from the
+ * cotangents of cg2d_b (its interior) and cg2d_x (NULL: none), the cotangents of gU, gV, hFacW, hFacS ([B][T, nr, ny,
+ * nx], every point written), EmPmR, etaH (interior, 0 elsewhere) and etaN ([B][T, ny, nx]). The face cotangent of
+ * pfu(j, i) is b_bar(j, i-1) - b_bar(j, i) (b_bar taken as 0 off the interior), pfv alike in j. deepFacC and rhoFacC
+ * must be NULL (the V4r4 ones). Outputs must not alias inputs or each other. */
+int mj_cg2d_rhs_adjoint(const mj_sfp_grid* grid, const mj_fs_params* p, const mj_sfp_config* cfg, const double* gU,
+                        const double* gV, const double* hFacW, const double* hFacS, const double* Bo_surf,
+                        const double* cg2d_b_bar, const double* cg2d_x_bar, double* gU_bar, double* gV_bar,
+                        double* hFacW_bar, double* hFacS_bar, double* EmPmR_bar, double* etaN_bar, double* etaH_bar,
+                        void* stream);
+
 /* ======================================================================================= GAD DST3
  * Replaces gad_advection / advect_tiles (pkgs/gad.py:460-560) with dst3_adv_x/y (:409-453): the 3-pass
  * multi-dimensional DST3 on the LLC cubed sphere with per-facet sweep order, overlap/interior update regions and

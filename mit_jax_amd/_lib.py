@@ -109,6 +109,8 @@ SIGNATURES = {
                              P, P, P, P, P, P, P, P, P, P]),
     "mj_cg2d_rhs": (I32, [ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, P, P, P,
                           P, P, P, P, P, P, P]),
+    "mj_cg2d_rhs_adjoint": (I32, [ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, P,
+                                  P, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mj_gad_implicit_r": (I32, [ctypes.POINTER(ImprGrid), ctypes.POINTER(ImprConfig), P, P, P, P, P, P]),
     "mj_gad_implicit_r_adjoint": (I32, [ctypes.POINTER(ImprGrid), ctypes.POINTER(ImprConfig), P, P, P, P, P, P, P,
                                         P, P]),

@@ -244,6 +244,77 @@ __global__ void __launch_bounds__(kPairThreads) sfp_rhs_kernel(SfpGeo g, RhsArgs
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
+struct RhsAdjArgs {
+    const double *dyG, *dxG, *rA, *maskInC, *drF, *d2f;
+    const int32_t* kSurfC;
+    const double *gU, *gV, *hFacW, *hFacS, *Bo_surf, *b_bar, *x_bar;
+    double *gU_bar, *gV_bar, *hW_bar, *hS_bar, *E_bar, *eN_bar, *eH_bar;
+    double tmpFac, fsf, dtm, dtfs;
+    int rfw;
+};
+
+// Reverse mode of sfp_rhs_kernel (V4r4 vertical factors): pfu(j, i) enters b(j, i-1) with + and b(j, i) with -
+// (pfv likewise in j), so the face cotangent is b_bar(left) - b_bar(right) with b_bar taken as 0 off the interior;
+// then gU_bar = (pu ((dyG drF) hFacW)) / deltaTMom, hFacW_bar = (pu ((dyG drF) gU)) / deltaTMom (v alike); the
+// 2-D cotangents of EmPmR, etaH (interior) and etaN (everywhere). One thread per pair of adjacent points.
+__global__ void __launch_bounds__(kPairThreads) sfp_rhs_adjoint_kernel(SfpGeo g, RhsAdjArgs a) {
+    const int p = blockIdx.x * kPairThreads + threadIdx.x, t = blockIdx.y, m = blockIdx.z;
+    const int q0 = 2 * p;
+    if (q0 >= g.plane) return;
+    const int j = q0 / g.nx, i0 = q0 - j * g.nx;
+    const int64_t q = (int64_t)t * g.plane + q0;
+    const int64_t o2 = m * g.ms2 + q;
+    auto inside = [&](int jj, int ii) {
+        return jj >= g.OL && jj < g.OL + g.sNy && ii >= g.OL && ii < g.OL + g.sNx;
+    };
+    const double* bb = a.b_bar + m * g.ms2 + (int64_t)t * g.plane;
+    auto bval = [&](int jj, int ii) { return inside(jj, ii) ? bb[jj * g.nx + ii] : 0.0; };
+    const double b00 = bval(j, i0), b01 = bval(j, i0 + 1);
+    const double pu0 = bval(j, i0 - 1) - b00, pu1 = b00 - b01;
+    const double pv0 = bval(j - 1, i0) - b00, pv1 = bval(j - 1, i0 + 1) - b01;
+    // 2-D cotangents
+    {
+        const double2 rA = ld2(a.rA + q);
+        double e0 = 0.0, e1 = 0.0, h0 = 0.0, h1 = 0.0;
+        const bool in0 = inside(j, i0), in1 = inside(j, i0 + 1);
+        if (a.rfw) {
+            const double2 mk = ld2(a.maskInC + q);
+            if (in0) e0 = b00 * (((a.tmpFac * rA.x) / a.dtm) * mk.x);
+            if (in1) e1 = b01 * (((a.tmpFac * rA.y) / a.dtm) * mk.y);
+        }
+        double f0 = a.fsf * rA.x, f1 = a.fsf * rA.y;
+        if (a.d2f) {
+            int k0 = a.kSurfC[q], k1 = a.kSurfC[q + 1];
+            k0 = k0 < 1 ? 1 : (k0 > g.nr + 1 ? g.nr + 1 : k0);
+            k1 = k1 < 1 ? 1 : (k1 > g.nr + 1 ? g.nr + 1 : k1);
+            f0 = f0 * a.d2f[k0 - 1];
+            f1 = f1 * a.d2f[k1 - 1];
+        }
+        if (in0) h0 = -(b00 * ((f0 / a.dtm) / a.dtfs));
+        if (in1) h1 = -(b01 * ((f1 / a.dtm) / a.dtfs));
+        st2(a.E_bar + o2, make_double2(e0, e1));
+        st2(a.eH_bar + o2, make_double2(h0, h1));
+        double2 xn = make_double2(0.0, 0.0);
+        if (a.x_bar) {
+            const double2 bo = ld2(a.Bo_surf + q), xb = ld2(a.x_bar + o2);
+            xn = make_double2(bo.x * xb.x, bo.y * xb.y);
+        }
+        st2(a.eN_bar + o2, xn);
+    }
+    const double2 cu = ld2(a.dyG + q), cv = ld2(a.dxG + q);
+    const int64_t o3 = m * g.ms3 + (int64_t)t * g.nr * g.plane + q0;
+    for (int k = 0; k < g.nr; ++k) {
+        const int64_t ok = o3 + (int64_t)k * g.plane;
+        const double dz = a.drF[k];
+        const double cu0 = cu.x * dz, cu1 = cu.y * dz, cv0 = cv.x * dz, cv1 = cv.y * dz;
+        const double2 hw = ld2(a.hFacW + ok), gu = ld2(a.gU + ok), hs = ld2(a.hFacS + ok), gv = ld2(a.gV + ok);
+        st2(a.gU_bar + ok, make_double2((pu0 * (cu0 * hw.x)) / a.dtm, (pu1 * (cu1 * hw.y)) / a.dtm));
+        st2(a.hW_bar + ok, make_double2((pu0 * (cu0 * gu.x)) / a.dtm, (pu1 * (cu1 * gu.y)) / a.dtm));
+        st2(a.gV_bar + ok, make_double2((pv0 * (cv0 * hs.x)) / a.dtm, (pv1 * (cv1 * hs.y)) / a.dtm));
+        st2(a.hS_bar + ok, make_double2((pv0 * (cv0 * gv.x)) / a.dtm, (pv1 * (cv1 * gv.y)) / a.dtm));
+    }
+}
+
 int check_sfp(const mj_sfp_grid* gr, const mj_fs_params* p, const mj_sfp_config* c, SfpGeo* g, const char* who) {
     MJ_REQUIRE(gr && p && c, "%s: null grid/params/config", who);
     MJ_REQUIRE(c->batch >= 1 && c->batch <= 65535 && c->ntiles >= 1 && c->ntiles <= 65535 && c->nr >= 1,
@@ -364,6 +435,60 @@ extern "C" int mj_cg2d_rhs(const mj_sfp_grid* grid, const mj_fs_params* p, const
     a.dtfs = p->deltaTFreeSurf;
     a.rfw = p->useRealFreshWaterFlux ? 1 : 0;
     hipLaunchKernelGGL(sfp_rhs_kernel, pair_grid(g), dim3(kPairThreads), 0, as_stream(stream), g, a);
+    MJ_HIP(hipGetLastError());
+    return MJ_OK;
+}
+
+extern "C" int mj_cg2d_rhs_adjoint(const mj_sfp_grid* grid, const mj_fs_params* p, const mj_sfp_config* cfg,
+                                   const double* gU, const double* gV, const double* hFacW, const double* hFacS,
+                                   const double* Bo_surf, const double* cg2d_b_bar, const double* cg2d_x_bar,
+                                   double* gU_bar, double* gV_bar, double* hFacW_bar, double* hFacS_bar,
+                                   double* EmPmR_bar, double* etaN_bar, double* etaH_bar, void* stream) {
+    SfpGeo g;
+    int rc = check_sfp(grid, p, cfg, &g, "mj_cg2d_rhs_adjoint");
+    if (rc) return rc;
+    MJ_REQUIRE(!grid->deepFacC && !grid->rhoFacC, "mj_cg2d_rhs_adjoint: deepFacC / rhoFacC must be NULL (ones)");
+    MJ_REQUIRE(grid->rA && grid->maskInC, "mj_cg2d_rhs_adjoint: null grid field");
+    MJ_REQUIRE(!grid->deepFac2F || grid->kSurfC, "mj_cg2d_rhs_adjoint: deepFac2F needs kSurfC");
+    const double* ins[] = {gU, gV, hFacW, hFacS, Bo_surf, cg2d_b_bar};
+    double* outs[] = {gU_bar, gV_bar, hFacW_bar, hFacS_bar, EmPmR_bar, etaN_bar, etaH_bar};
+    for (const double* in : ins) MJ_REQUIRE(in && aligned16(in), "mj_cg2d_rhs_adjoint: null or unaligned input");
+    MJ_REQUIRE(aligned16(cg2d_x_bar), "mj_cg2d_rhs_adjoint: unaligned cg2d_x_bar");
+    for (double* o : outs) {
+        MJ_REQUIRE(o && aligned16(o), "mj_cg2d_rhs_adjoint: null or unaligned output");
+        for (const double* in : ins) MJ_REQUIRE(o != in, "mj_cg2d_rhs_adjoint: an output aliases an input");
+        MJ_REQUIRE(o != cg2d_x_bar, "mj_cg2d_rhs_adjoint: an output aliases an input");
+    }
+    for (int x = 0; x < 7; ++x)
+        for (int y = x + 1; y < 7; ++y) MJ_REQUIRE(outs[x] != outs[y], "mj_cg2d_rhs_adjoint: two outputs alias");
+    RhsAdjArgs a;
+    a.dyG = grid->dyG;
+    a.dxG = grid->dxG;
+    a.rA = grid->rA;
+    a.maskInC = grid->maskInC;
+    a.drF = grid->drF;
+    a.d2f = grid->deepFac2F;
+    a.kSurfC = grid->kSurfC;
+    a.gU = gU;
+    a.gV = gV;
+    a.hFacW = hFacW;
+    a.hFacS = hFacS;
+    a.Bo_surf = Bo_surf;
+    a.b_bar = cg2d_b_bar;
+    a.x_bar = cg2d_x_bar;
+    a.gU_bar = gU_bar;
+    a.gV_bar = gV_bar;
+    a.hW_bar = hFacW_bar;
+    a.hS_bar = hFacS_bar;
+    a.E_bar = EmPmR_bar;
+    a.eN_bar = etaN_bar;
+    a.eH_bar = etaH_bar;
+    a.tmpFac = (p->freeSurfFac * p->mass2rUnit) * p->implicDiv2DFlow;
+    a.fsf = p->freeSurfFac;
+    a.dtm = p->deltaTMom;
+    a.dtfs = p->deltaTFreeSurf;
+    a.rfw = p->useRealFreshWaterFlux ? 1 : 0;
+    hipLaunchKernelGGL(sfp_rhs_adjoint_kernel, pair_grid(g), dim3(kPairThreads), 0, as_stream(stream), g, a);
     MJ_HIP(hipGetLastError());
     return MJ_OK;
 }

@@ -7,7 +7,8 @@
 arguments with j, i = 1..sNy+1, 1..sNx+1 rewritten, as the reference's `.at[].set`.
 
 `cg2d_rhs(p, g, gU, gV, hFacW, hFacS, EmPmR, etaN, etaH, Bo_surf) -> (cg2d_b, cg2d_x)` (reference :68-106,
-solve_for_pressure.F:124-255 with CALC_DIV_GHAT): through mj_cg2d_rhs.
+solve_for_pressure.F:124-255 with CALC_DIV_GHAT): through mj_cg2d_rhs; `cg2d_rhs_adjoint` is its reverse mode
+(mj_cg2d_rhs_adjoint).
 
 Both are bitwise equal to the literal Fortran-order loops. Field arrays are CUDA float64 [T, K, ny, nx] / [T, ny, nx],
 or with a leading member axis [B, ...] for an ensemble (grid fields, recip_Bo and Bo_surf shared). `g` is a grid
@@ -182,3 +183,29 @@ def cg2d_rhs(p, g, gU, gV, hFacW, hFacS, EmPmR, etaN, etaH, Bo_surf):
     _lib.check(_lib.lib().mj_cg2d_rhs(ctypes.byref(gd.s), ctypes.byref(fs), ctypes.byref(cfg), *(P(a) for a in arrs),
                                       P(em), P(eN), P(eH), P(Bo), P(b), P(x), _lib.stream_ptr()), "mj_cg2d_rhs")
     return (b, x) if batched else (b[0], x[0])
+
+
+def cg2d_rhs_adjoint(p, g, gU, gV, hFacW, hFacS, Bo_surf, cg2d_b_bar, cg2d_x_bar=None):
+    """Reverse mode of cg2d_rhs: from the cotangents of (cg2d_b, cg2d_x), those of (gU, gV, hFacW, hFacS, EmPmR,
+    etaN, etaH), batched like gU (the rhs is linear in gU, gV, EmPmR, etaN, etaH, and in hFacW, hFacS for fixed gU,
+    gV)."""
+    L = _layout(g)
+    _check_layout(L)
+    batched = gU.dim() == 5
+    arrs = [_batch3(a, L, n) for a, n in ((gU, "gU"), (gV, "gV"), (hFacW, "hFacW"), (hFacS, "hFacS"))]
+    if len({tuple(a.shape) for a in arrs}) != 1:
+        raise ValueError("gU, gV, hFacW, hFacS differ in shape")
+    B, K, dev = arrs[0].shape[0], arrs[0].shape[2], arrs[0].device
+    gd = _grid(g, dev)
+    bb = _batch2(cg2d_b_bar, L, B, "cg2d_b_bar")
+    xb = _batch2(cg2d_x_bar, L, B, "cg2d_x_bar") if cg2d_x_bar is not None else None
+    Bo = _shared2(Bo_surf, L, dev, "Bo_surf")
+    o3 = [torch.empty_like(arrs[0]) for _ in range(4)]
+    o2 = [torch.empty((B,) + L.shape2d, dtype=torch.float64, device=dev) for _ in range(3)]
+    cfg, fs = _config(L, B, K), _fs(p)
+    P = _lib.ptr
+    _lib.check(_lib.lib().mj_cg2d_rhs_adjoint(ctypes.byref(gd.s), ctypes.byref(fs), ctypes.byref(cfg),
+                                              *(P(a) for a in arrs), P(Bo), P(bb), P(xb), *(P(o) for o in o3),
+                                              *(P(o) for o in o2), _lib.stream_ptr()), "mj_cg2d_rhs_adjoint")
+    out = tuple(o3) + tuple(o2)
+    return out if batched else tuple(o[0] for o in out)

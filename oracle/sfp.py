@@ -98,3 +98,39 @@ def cg2d_rhs(p, g, gU, gV, hFacW, hFacS, EmPmR, etaN, etaH, Bo_surf, sNx=90, sNy
     b = b - ((e / p.deltaTMom) / p.deltaTFreeSurf) * etaH[:, J, I]               # :103-105
     cg2d_b[:, J, I] = b
     return cg2d_b, cg2d_x
+
+
+def cg2d_rhs_adjoint(p, g, gU, gV, hFacW, hFacS, Bo_surf, b_bar, x_bar=None, sNx=90, sNy=90, OL=4, deepFac2F=None,
+                     maskC=None):
+    """Reverse mode of cg2d_rhs (linear in gU, gV, EmPmR, etaN, etaH; linear in hFacW, hFacS for fixed gU, gV), V4r4
+    vertical factors: from the cotangents of cg2d_b (interior) and cg2d_x, the cotangents (gU, gV, hFacW, hFacS,
+    EmPmR, etaN, etaH). pfu(j, i) enters b(j, i-1) with + and b(j, i) with -, pfv(j, i) enters b(j-1, i) with + and
+    b(j, i) with -."""
+    T, K = gU.shape[:2]
+    ny, nx = gU.shape[2:]
+    J, I = _sl(1, sNy, OL), _sl(1, sNx, OL)
+    bb = np.zeros((T, ny, nx))
+    bb[:, J, I] = b_bar[:, J, I]
+    # face cotangents: u faces (j in J, i = 1..sNx+1), v faces (j = 1..sNy+1, i in I)
+    pu = np.zeros((T, ny, nx))
+    pu[:, J, OL:OL + sNx + 1] = bb[:, J, OL - 1:OL + sNx] - bb[:, J, OL:OL + sNx + 1]
+    pv = np.zeros((T, ny, nx))
+    pv[:, OL:OL + sNy + 1, I] = bb[:, OL - 1:OL + sNy, I] - bb[:, OL:OL + sNy + 1, I]
+    drF = np.asarray(g.drF, np.float64)[None, :, None, None]
+    cu = g.dyG[:, None] * drF                                                       # d xA / d hFacW
+    cv = g.dxG[:, None] * drF
+    gUb = (pu[:, None] * (cu * hFacW)) / p.deltaTMom
+    gVb = (pv[:, None] * (cv * hFacS)) / p.deltaTMom
+    hWb = (pu[:, None] * (cu * gU)) / p.deltaTMom
+    hSb = (pv[:, None] * (cv * gV)) / p.deltaTMom
+    Eb = np.zeros((T, ny, nx))
+    if p.useRealFreshWaterFlux:
+        tmpFac = (p.freeSurfFac * p.mass2rUnit) * p.implicDiv2DFlow
+        Eb[:, J, I] = bb[:, J, I] * (((tmpFac * g.rA[:, J, I]) / p.deltaTMom) * g.maskInC[:, J, I])
+    e = p.freeSurfFac * g.rA[:, J, I]
+    if deepFac2F is not None:
+        e = e * np.asarray(deepFac2F)[k_surf(maskC)[:, J, I] - 1]
+    eHb = np.zeros((T, ny, nx))
+    eHb[:, J, I] = -(bb[:, J, I] * ((e / p.deltaTMom) / p.deltaTFreeSurf))
+    eNb = Bo_surf * x_bar if x_bar is not None else np.zeros((T, ny, nx))
+    return gUb, gVb, hWb, hSb, Eb, eNb, eHb

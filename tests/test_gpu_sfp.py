@@ -166,3 +166,20 @@ def test_producers_feed_the_solve(dev, ex, grid0):
     rx, rd = oracle.cg2d(maps().full("T"), rops, rhs, z, cg2dNorm=norm, tolerance=1e-12, max_iters=2000)
     assert np.array_equal(x.cpu().numpy(), rx)
     assert int(diag["numIters"]) == rd["numIters"]
+
+
+def test_cg2d_rhs_adjoint_bitwise(dev, grid0):
+    """mj_cg2d_rhs_adjoint against the oracle's reverse mode (the same face cotangents and expressions), bitwise,
+    both cotangent inputs given, useRealFreshWaterFlux on."""
+    from mit_jax_amd import solve_for_pressure as SFP
+    g = grid0
+    rng = np.random.default_rng(13)
+    gU, gV, EmPmR, etaN, etaH, Bo = _rhs_fields(rng, g)
+    bb, xb = rng.normal(size=etaN.shape), rng.normal(size=etaN.shape)
+    p = _fs()
+    print("cg2d_rhs_adjoint: seed grid", flush=True)
+    got = SFP.cg2d_rhs_adjoint(p, g, to_dev(gU, dev), to_dev(gV, dev), to_dev(g.hFacW, dev), to_dev(g.hFacS, dev),
+                               to_dev(Bo, dev), to_dev(bb, dev), to_dev(xb, dev))
+    ref = OS.cg2d_rhs_adjoint(p, g, gU, gV, g.hFacW, g.hFacS, Bo, bb, xb)
+    for name, a, b in zip(("gU", "gV", "hFacW", "hFacS", "EmPmR", "etaN", "etaH"), got, ref):
+        assert np.array_equal(a.cpu().numpy(), b), name

@@ -276,7 +276,7 @@ def test_cg2d_rhs_pins_hold():
     ("b = (b + pv[:, 1:, :]) - pv[:, :-1, :]", "b = (b + pv[:, :-1, :]) - pv[:, 1:, :]"),
     ("g.dxG[:, J1, I1] * dfc[k]", "g.dyG[:, J1, I1] * dfc[k]"),
     ("* etaH[:, J, I]", "* etaN[:, J, I]"),
-    ("* g.maskInC[:, J, I]", ""),
+    ("EmPmR[:, J, I]) / p.deltaTMom) * g.maskInC[:, J, I]", "EmPmR[:, J, I]) / p.deltaTMom)"),
     ("cg2d_x = Bo_surf * etaN", "cg2d_x = Bo_surf * etaH"),
 ])
 def test_cg2d_rhs_planted_errors_fail_the_pins(old, new):
@@ -297,3 +297,30 @@ def test_oracle_matches_the_synthetic_operator_builder():
                          z, z)
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
+
+
+def test_cg2d_rhs_adjoint_dot_products():
+    """The rhs is linear in (gU, gV, EmPmR, etaH) jointly and, for fixed gU and gV, in (hFacW, hFacS, EmPmR, etaH);
+    cg2d_x is linear in etaN: <b_bar, b> equals the sum of <cotangent, input> over each such group (Euler's
+    identity), and the same for cg2d_x; a planted sign error in a face cotangent breaks it."""
+    rng = np.random.default_rng(9)
+    g, gU, gV, hW, hS, EmPmR, etaN, etaH, Bo = _rhs_inputs(rng)
+    g.dyG = rng.uniform(1, 2, g.dyG.shape)
+    gU, gV = rng.normal(size=gU.shape), rng.normal(size=gV.shape)
+    bb, xb = rng.normal(size=etaN.shape), rng.normal(size=etaN.shape)
+    b, x = OS.cg2d_rhs(P, g, gU, gV, hW, hS, EmPmR, etaN, etaH, Bo)
+
+    def check(mod):
+        gUb, gVb, hWb, hSb, Eb, eNb, eHb = mod.cg2d_rhs_adjoint(P, g, gU, gV, hW, hS, Bo, bb, xb)
+        lhs = float((bb * b).sum())
+        r1 = float((gUb * gU).sum() + (gVb * gV).sum() + (Eb * EmPmR).sum() + (eHb * etaH).sum())
+        r2 = float((hWb * hW).sum() + (hSb * hS).sum() + (Eb * EmPmR).sum() + (eHb * etaH).sum())
+        rx = float((eNb * etaN).sum())
+        scale = float((np.abs(bb) * np.abs(b)).sum())
+        return (abs(lhs - r1) < 1e-12 * scale and abs(lhs - r2) < 1e-12 * scale and
+                abs(float((xb * x).sum()) - rx) < 1e-12 * float((np.abs(xb) * np.abs(x)).sum()))
+
+    assert check(OS)
+    bad = _planted("pu[:, J, OL:OL + sNx + 1] = bb[:, J, OL - 1:OL + sNx] - bb[:, J, OL:OL + sNx + 1]",
+                   "pu[:, J, OL:OL + sNx + 1] = bb[:, J, OL:OL + sNx + 1] - bb[:, J, OL - 1:OL + sNx]")
+    assert not check(bad)
