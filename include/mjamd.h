@@ -220,6 +220,16 @@ int mj_update_cg2d(const mj_halo_plan* plan_T, const mj_sfp_grid* grid, const mj
                    const double* hFacS, const double* recip_Bo, double* aW2d, double* aS2d, double* aC2d, double* pW,
                    double* pS, double* pC, void* stream);
 
+/* Reverse mode of mj_update_cg2d in hFacW, hFacS (JAX's transpose of update_cg2d in the reference; the operator is
+ * affine in them, and the preconditioner outputs carry no cotangent since CG2D's derivative does not use them,
+ * core/cg2d.py:307-348): aC2d_bar goes back through the transposed T exchange to the interior, then to aW (i, i+1) and
+ * aS (j, j+1), then to every level of hFacW_bar, hFacS_bar ([B][T, nr, ny, nx], every point written). aW2d_bar,
+ * aS2d_bar, aC2d_bar: [B][T, ny, nx]; work: device, B * member_stride2d doubles (T * ny * nx for one member). */
+int mj_update_cg2d_adjoint(const mj_halo_plan* plan_T, const mj_sfp_grid* grid, const mj_fs_params* p,
+                           const mj_sfp_config* cfg, const double* cg2dNorm, const double* aW2d_bar,
+                           const double* aS2d_bar, const double* aC2d_bar, double* hFacW_bar, double* hFacS_bar,
+                           double* work, void* stream);
+
 /* cg2d_rhs (core/solve_for_pressure.py:68-106; solve_for_pressure.F:124-255 with CALC_DIV_GHAT): cg2d_x = Bo_surf *
  * etaN on the whole array; cg2d_b = 0 off the interior and on the interior
  *   b = useRealFreshWaterFlux ? freeSurfFac mass2rUnit implicDiv2DFlow rA EmPmR / deltaTMom maskInC : 0

@@ -107,6 +107,8 @@ SIGNATURES = {
                                   ctypes.POINTER(Cg2dMixedConfig), P, ctypes.c_size_t, P, P]),
     "mj_update_cg2d": (I32, [P, ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, F64,
                              P, P, P, P, P, P, P, P, P, P]),
+    "mj_update_cg2d_adjoint": (I32, [P, ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig),
+                                     P, P, P, P, P, P, P, P]),
     "mj_cg2d_rhs": (I32, [ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, P, P, P,
                           P, P, P, P, P, P, P]),
     "mj_cg2d_rhs_adjoint": (I32, [ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, P,

@@ -315,6 +315,50 @@ __global__ void __launch_bounds__(kPairThreads) sfp_rhs_adjoint_kernel(SfpGeo g,
     }
 }
 
+// Reverse mode of the level sums of UPDATE_CG2D: ct = exch^T(aC2d_bar) (in work), then on j, i = 1..sNy+1,
+// 1..sNx+1: aW_tot = (aW_bar - C(j, i)) - C(j, i-1), aS_tot = (aS_bar - C(j, i)) - C(j-1, i) with C = ct on the
+// interior and 0 elsewhere; hFacW_bar(k) = (aW_tot ((norm isp) idf)) ((dyG drF(k)) recip_dxC) (hFacS_bar alike),
+// 0 off that range. One thread per pair of adjacent points.
+__global__ void __launch_bounds__(kPairThreads)
+    sfp_opsum_adjoint_kernel(SfpGeo g, const double* __restrict__ dyG, const double* __restrict__ dxG,
+                             const double* __restrict__ recip_dxC, const double* __restrict__ recip_dyC,
+                             const double* __restrict__ drF, const double* __restrict__ norm, double isp, double idf,
+                             const double* __restrict__ aWb, const double* __restrict__ aSb,
+                             const double* __restrict__ ct, double* __restrict__ hWb, double* __restrict__ hSb) {
+    const int p = blockIdx.x * kPairThreads + threadIdx.x, t = blockIdx.y, m = blockIdx.z;
+    const int q0 = 2 * p;
+    if (q0 >= g.plane) return;
+    const int j = q0 / g.nx, i0 = q0 - j * g.nx;
+    const int64_t q = (int64_t)t * g.plane + q0;
+    const int64_t o2 = m * g.ms2 + q;
+    const double* c2 = ct + m * g.ms2 + (int64_t)t * g.plane;
+    auto C = [&](int jj, int ii) {
+        const bool in = jj >= g.OL && jj < g.OL + g.sNy && ii >= g.OL && ii < g.OL + g.sNx;
+        return in ? c2[jj * g.nx + ii] : 0.0;
+    };
+    const bool jin = in_range(j, g.OL, g.OL + g.sNy);
+    const bool in0 = jin && in_range(i0, g.OL, g.OL + g.sNx), in1 = jin && in_range(i0 + 1, g.OL, g.OL + g.sNx);
+    double w0 = 0.0, w1 = 0.0, s0 = 0.0, s1 = 0.0;
+    double2 cW = make_double2(0.0, 0.0), cS = cW, rW = cW, rS = cW;
+    if (in0 || in1) {
+        const double2 aw = ld2(aWb + o2), as = ld2(aSb + o2);
+        const double c00 = C(j, i0), c01 = C(j, i0 + 1);
+        const double fac = (norm[m] * isp) * idf;
+        w0 = ((aw.x - c00) - C(j, i0 - 1)) * fac;
+        w1 = ((aw.y - c01) - c00) * fac;
+        s0 = ((as.x - c00) - C(j - 1, i0)) * fac;
+        s1 = ((as.y - c01) - C(j - 1, i0 + 1)) * fac;
+        cW = ld2(dyG + q), cS = ld2(dxG + q), rW = ld2(recip_dxC + q), rS = ld2(recip_dyC + q);
+    }
+    const int64_t o3 = m * g.ms3 + (int64_t)t * g.nr * g.plane + q0;
+    for (int k = 0; k < g.nr; ++k) {
+        const double dz = drF[k];
+        const int64_t ok = o3 + (int64_t)k * g.plane;
+        st2(hWb + ok, make_double2(in0 ? w0 * ((cW.x * dz) * rW.x) : 0.0, in1 ? w1 * ((cW.y * dz) * rW.y) : 0.0));
+        st2(hSb + ok, make_double2(in0 ? s0 * ((cS.x * dz) * rS.x) : 0.0, in1 ? s1 * ((cS.y * dz) * rS.y) : 0.0));
+    }
+}
+
 int check_sfp(const mj_sfp_grid* gr, const mj_fs_params* p, const mj_sfp_config* c, SfpGeo* g, const char* who) {
     MJ_REQUIRE(gr && p && c, "%s: null grid/params/config", who);
     MJ_REQUIRE(c->batch >= 1 && c->batch <= 65535 && c->ntiles >= 1 && c->ntiles <= 65535 && c->nr >= 1,
@@ -489,6 +533,34 @@ extern "C" int mj_cg2d_rhs_adjoint(const mj_sfp_grid* grid, const mj_fs_params* 
     a.dtfs = p->deltaTFreeSurf;
     a.rfw = p->useRealFreshWaterFlux ? 1 : 0;
     hipLaunchKernelGGL(sfp_rhs_adjoint_kernel, pair_grid(g), dim3(kPairThreads), 0, as_stream(stream), g, a);
+    MJ_HIP(hipGetLastError());
+    return MJ_OK;
+}
+
+extern "C" int mj_update_cg2d_adjoint(const mj_halo_plan* plan_T, const mj_sfp_grid* grid, const mj_fs_params* p,
+                                      const mj_sfp_config* cfg, const double* cg2dNorm, const double* aW2d_bar,
+                                      const double* aS2d_bar, const double* aC2d_bar, double* hFacW_bar,
+                                      double* hFacS_bar, double* work, void* stream) {
+    SfpGeo g;
+    int rc = check_sfp(grid, p, cfg, &g, "mj_update_cg2d_adjoint");
+    if (rc) return rc;
+    MJ_REQUIRE(plan_T, "mj_update_cg2d_adjoint: null T plan");
+    MJ_REQUIRE(grid->recip_dxC && grid->recip_dyC, "mj_update_cg2d_adjoint: null grid field");
+    const double* ins[] = {cg2dNorm, aW2d_bar, aS2d_bar, aC2d_bar};
+    double* outs[] = {hFacW_bar, hFacS_bar, work};
+    for (const double* in : ins) MJ_REQUIRE(in, "mj_update_cg2d_adjoint: null input");
+    for (double* o : outs) {
+        MJ_REQUIRE(o && aligned16(o), "mj_update_cg2d_adjoint: null or unaligned output/work");
+        for (const double* in : ins) MJ_REQUIRE(o != in, "mj_update_cg2d_adjoint: an output aliases an input");
+    }
+    MJ_REQUIRE(hFacW_bar != hFacS_bar && work != hFacW_bar && work != hFacS_bar,
+               "mj_update_cg2d_adjoint: outputs alias");
+    MJ_REQUIRE(aligned16(aW2d_bar) && aligned16(aS2d_bar), "mj_update_cg2d_adjoint: unaligned aW2d_bar/aS2d_bar");
+    mj_halo_geom hg{g.B, g.B == 1 ? (int64_t)g.T * g.plane : g.ms2, g.plane, 1, g.plane};
+    if ((rc = mj_halo_exchange_adjoint(plan_T, aC2d_bar, nullptr, work, nullptr, &hg, stream))) return rc;
+    hipLaunchKernelGGL(sfp_opsum_adjoint_kernel, pair_grid(g), dim3(kPairThreads), 0, as_stream(stream), g, grid->dyG,
+                       grid->dxG, grid->recip_dxC, grid->recip_dyC, grid->drF, cg2dNorm, p->implicSurfPress,
+                       p->implicDiv2DFlow, aW2d_bar, aS2d_bar, (const double*)work, hFacW_bar, hFacS_bar);
     MJ_HIP(hipGetLastError());
     return MJ_OK;
 }

@@ -4,7 +4,8 @@
 `update_cg2d(p, cp, g, ex, hFacW, hFacS, recip_Bo, pW, pS, pC) -> (aW2d, aS2d, aC2d, pW, pS, pC)`
 (reference :29-65, update_cg2d.F:59-195): the 5-point operator from the column sums of the r* hFacW/S and the
 5-point preconditioner, through libmjamd's mj_update_cg2d. Returns new arrays; pW, pS, pC are copies of the
-arguments with j, i = 1..sNy+1, 1..sNx+1 rewritten, as the reference's `.at[].set`.
+arguments with j, i = 1..sNy+1, 1..sNx+1 rewritten, as the reference's `.at[].set`. `update_cg2d_adjoint` is its
+reverse mode in hFacW, hFacS (mj_update_cg2d_adjoint).
 
 `cg2d_rhs(p, g, gU, gV, hFacW, hFacS, EmPmR, etaN, etaH, Bo_surf) -> (cg2d_b, cg2d_x)` (reference :68-106,
 solve_for_pressure.F:124-255 with CALC_DIV_GHAT): through mj_cg2d_rhs; `cg2d_rhs_adjoint` is its reverse mode
@@ -209,3 +210,28 @@ def cg2d_rhs_adjoint(p, g, gU, gV, hFacW, hFacS, Bo_surf, cg2d_b_bar, cg2d_x_bar
                                               *(P(o) for o in o2), _lib.stream_ptr()), "mj_cg2d_rhs_adjoint")
     out = tuple(o3) + tuple(o2)
     return out if batched else tuple(o[0] for o in out)
+
+
+def update_cg2d_adjoint(p, cp, g, ex, aW2d_bar, aS2d_bar, aC2d_bar, nr):
+    """Reverse mode of update_cg2d in hFacW, hFacS: from the cotangents of (aW2d, aS2d, aC2d) (the preconditioner's
+    carry none), those of (hFacW, hFacS) with nr levels; batched like aW2d_bar."""
+    L = _layout(g)
+    _check_layout(L)
+    batched = aW2d_bar.dim() == 4
+    aWb = _batch2(aW2d_bar, L, aW2d_bar.shape[0] if batched else 1, "aW2d_bar")
+    B, dev = aWb.shape[0], aWb.device
+    aSb, aCb = _batch2(aS2d_bar, L, B, "aS2d_bar"), _batch2(aC2d_bar, L, B, "aC2d_bar")
+    gd = _grid(g, dev)
+    if gd.keep["drF"].numel() < nr:
+        raise ValueError(f"grid.drF has {gd.keep['drF'].numel()} levels, asked for {nr}")
+    norm = _norm_tensor(cp.cg2dNorm, B, dev)
+    hWb = torch.empty((B, L.nTiles, nr, L.ny, L.nx), dtype=torch.float64, device=dev)
+    hSb = torch.empty_like(hWb)
+    work = torch.empty((B,) + L.shape2d, dtype=torch.float64, device=dev)
+    cfg, fs = _config(L, B, nr), _fs(p)
+    plan = ex.plan("T", dev)
+    P = _lib.ptr
+    _lib.check(_lib.lib().mj_update_cg2d_adjoint(plan.handle, ctypes.byref(gd.s), ctypes.byref(fs), ctypes.byref(cfg),
+                                                 P(norm), P(aWb), P(aSb), P(aCb), P(hWb), P(hSb), P(work),
+                                                 _lib.stream_ptr()), "mj_update_cg2d_adjoint")
+    return (hWb, hSb) if batched else (hWb[0], hSb[0])

@@ -134,3 +134,43 @@ def cg2d_rhs_adjoint(p, g, gU, gV, hFacW, hFacS, Bo_surf, b_bar, x_bar=None, sNx
     eHb[:, J, I] = -(bb[:, J, I] * ((e / p.deltaTMom) / p.deltaTFreeSurf))
     eNb = Bo_surf * x_bar if x_bar is not None else np.zeros((T, ny, nx))
     return gUb, gVb, hWb, hSb, Eb, eNb, eHb
+
+
+def exchange_transpose(Tmap, abar):
+    """Transpose of the scalar exchange on [T, ny, nx]: a written point passes sign * its cotangent to its source and
+    keeps none; other points keep their own (summed with np.add.at)."""
+    src, comp, sign = Tmap
+    a = np.asarray(abar, np.float64).reshape(-1)
+    out = a.copy()
+    w = comp != 0
+    out[w] = 0.0
+    np.add.at(out, src[w], sign[w].astype(np.float64) * a[w])
+    return out.reshape(np.shape(abar))
+
+
+def update_cg2d_adjoint(p, cg2dNorm, g, Tmap, aW_bar, aS_bar, aC_bar, K, sNx=90, sNy=90, OL=4):
+    """Reverse mode of UPDATE_CG2D in hFacW, hFacS (the operator is affine in them; the preconditioner outputs have no
+    cotangent, as CG2D's derivative does not use them): aC2d = exch(aC) gives aC_bar' = exch^T(aC2d_bar) on the
+    interior; aW_tot(j, i) = (aW_bar - aC'(j, i)) - aC'(j, i-1), aS_tot(j, i) = (aS_bar - aC'(j, i)) - aC'(j-1, i)
+    (aC' taken as 0 off the interior); hFacW_bar(k) = (aW_tot ((norm isp) idf)) ((dyG drF(k)) recip_dxC) on
+    j, i = 1..sNy+1, 1..sNx+1, 0 elsewhere (hFacS_bar alike)."""
+    T = aW_bar.shape[0]
+    J, I = _sl(1, sNy, OL), _sl(1, sNx, OL)
+    J1, I1 = _sl(1, sNy + 1, OL), _sl(1, sNx + 1, OL)
+    ct = exchange_transpose(Tmap, aC_bar)
+    C = np.zeros_like(ct)
+    C[:, J, I] = ct[:, J, I]
+    Cw = np.zeros_like(C)
+    Cw[:, :, 1:] = C[:, :, :-1]                                                     # aC'(j, i-1)
+    Cs = np.zeros_like(C)
+    Cs[:, 1:, :] = C[:, :-1, :]                                                     # aC'(j-1, i)
+    aWt = (aW_bar - C) - Cw
+    aSt = (aS_bar - C) - Cs
+    fac = (cg2dNorm * p.implicSurfPress) * p.implicDiv2DFlow
+    hWb = np.zeros((T, K) + aW_bar.shape[1:])
+    hSb = np.zeros_like(hWb)
+    drF = np.asarray(g.drF, np.float64)
+    for k in range(K):
+        hWb[:, k, J1, I1] = (aWt[:, J1, I1] * fac) * ((g.dyG[:, J1, I1] * drF[k]) * g.recip_dxC[:, J1, I1])
+        hSb[:, k, J1, I1] = (aSt[:, J1, I1] * fac) * ((g.dxG[:, J1, I1] * drF[k]) * g.recip_dyC[:, J1, I1])
+    return hWb, hSb

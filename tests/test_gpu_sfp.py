@@ -183,3 +183,20 @@ def test_cg2d_rhs_adjoint_bitwise(dev, grid0):
     ref = OS.cg2d_rhs_adjoint(p, g, gU, gV, g.hFacW, g.hFacS, Bo, bb, xb)
     for name, a, b in zip(("gU", "gV", "hFacW", "hFacS", "EmPmR", "etaN", "etaH"), got, ref):
         assert np.array_equal(a.cpu().numpy(), b), name
+
+
+def test_update_cg2d_adjoint_matches_oracle(dev, ex, grid0):
+    """mj_update_cg2d_adjoint against the oracle's reverse mode (transposed exchange, aC's neighbours, the level
+    factors): equal to 1e-13 relative (the transposed exchange may sum a source's contributions in another order)."""
+    from mit_jax_amd import synthetic as S
+    from mit_jax_amd import solve_for_pressure as SFP
+    g = grid0
+    norm = S.ini_cg2d_norm(g)
+    rng = np.random.default_rng(14)
+    bars = [rng.normal(size=(T, NY, NX)) for _ in range(3)]
+    print("update_cg2d_adjoint: seed grid", flush=True)
+    got = SFP.update_cg2d_adjoint(_fs(), _cp(norm), g, ex, *(to_dev(b, dev) for b in bars), 50)
+    ref = OS.update_cg2d_adjoint(_fs(), norm, g, maps().full("T"), *bars, 50)
+    for name, a, b in zip(("hFacW", "hFacS"), got, ref):
+        a = a.cpu().numpy()
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max(), name

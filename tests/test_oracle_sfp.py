@@ -324,3 +324,27 @@ def test_cg2d_rhs_adjoint_dot_products():
     bad = _planted("pu[:, J, OL:OL + sNx + 1] = bb[:, J, OL - 1:OL + sNx] - bb[:, J, OL:OL + sNx + 1]",
                    "pu[:, J, OL:OL + sNx + 1] = bb[:, J, OL:OL + sNx + 1] - bb[:, J, OL - 1:OL + sNx]")
     assert not check(bad)
+
+
+def test_update_cg2d_adjoint_dot_product():
+    """UPDATE_CG2D is affine in (hFacW, hFacS): <(aW_bar, aS_bar, aC_bar), ops(h) - ops(0)> equals
+    <(hFacW_bar, hFacS_bar), h> (the preconditioner outputs carry no cotangent); a planted index error in the
+    aC cotangent's neighbour breaks it."""
+    rng = np.random.default_rng(11)
+    g, hW, hS, rBo, *_ = _update_case(11)
+    g.dyG = rng.uniform(1, 2, g.dyG.shape)
+    hW, hS = rng.uniform(0, 1, hW.shape), rng.uniform(0, 1, hS.shape)
+    Tm = maps().full("T")
+    z = np.zeros(rBo.shape)
+    ops = OS.update_cg2d(P, 0.25, 0.51, g, Tm, hW, hS, rBo, z, z, z)
+    ops0 = OS.update_cg2d(P, 0.25, 0.51, g, Tm, np.zeros_like(hW), np.zeros_like(hS), rBo, z, z, z)
+    bars = [rng.normal(size=rBo.shape) for _ in range(3)]
+
+    def check(mod):
+        hWb, hSb = mod.update_cg2d_adjoint(P, 0.25, g, Tm, *bars, K)
+        lhs = sum(float((b * (o - o0)).sum()) for b, o, o0 in zip(bars, ops[:3], ops0[:3]))
+        rhs = float((hWb * hW).sum() + (hSb * hS).sum())
+        return abs(lhs - rhs) < 1e-11 * max(abs(lhs), 1.0)
+
+    assert check(OS)
+    assert not check(_planted("    Cw[:, :, 1:] = C[:, :, :-1]", "    Cw[:, :, :-1] = C[:, :, 1:]"))
