@@ -471,6 +471,9 @@ int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
         } else {
             const int ts = s / plane, rs = s % plane, js = rs / nx - OL, is = rs % nx - OL;
             csrc[k] = ts * TS + js * sNx + is;
+            // a source in the west or east column: its contiguous copy after the tile's interior
+            if (MJ_MX_EDGES && is == 0) csrc[k] = ts * TS + sNx * sNy + js;
+            if (MJ_MX_EDGES && is == sNx - 1) csrc[k] = ts * TS + sNx * sNy + sNy + js;
         }
     }
     if ((rc = up(clds.data(), clds.size() * 4, (void**)&P->cg_cring_lds)) ||

@@ -484,8 +484,15 @@ extern __shared__ __align__(16) double mj_lds[];
 __host__ __device__ constexpr size_t lds_tile_bytes(int sNx, int sNy, int elem) {
     return (size_t)(sNy + 2) * (sNx + 4) * elem;
 }
-// Tile stride of the compact work vectors (interior only, row-major sNy x sNx): rounded up to 128 bytes of FP32.
-__host__ __device__ constexpr int compact_tile_stride(int sNx, int sNy) { return (sNx * sNy + 31) / 32 * 32; }
+#ifndef MJ_MX_EDGES
+#define MJ_MX_EDGES 1
+#endif
+// Tile stride of the compact work vectors (interior only, row-major sNy x sNx; with MJ_MX_EDGES followed by copies
+// of the west and east columns, sNy values each, at sNx*sNy and sNx*sNy + sNy, so that a neighbour's ring read of
+// a column is contiguous): rounded up to 128 bytes of FP32.
+__host__ __device__ constexpr int compact_tile_stride(int sNx, int sNy) {
+    return (sNx * sNy + (MJ_MX_EDGES ? 2 * sNy : 0) + 31) / 32 * 32;
+}
 struct LdsGeo {
     int W;
     __device__ __forceinline__ int at(int j, int i) const { return (j + 1) * W + i + 2; }

@@ -102,6 +102,25 @@ __device__ __forceinline__ void lds_put4(float* tile, const LdsGeo& G, int e, fl
     st2(tile + G.at(j1, e + 2 - j1 * sNx), make_float2(v.z, v.w));
 }
 
+// The west/east column copies of a tile's compact vector (base: the tile's compact start) for the four elements
+// e .. e+3 (two column pairs, each within one row; column 0 is the first of a pair, column sNx-1 the second).
+__device__ __forceinline__ void edge_put4(float* base, int e, float4 v, int sNx, int sNy, float inv_sNx) {
+    if (!MJ_MX_EDGES) return;
+    const int j0 = crow(e, inv_sNx), j1 = crow(e + 2, inv_sNx);
+    const int c0 = e - j0 * sNx, c1 = e + 2 - j1 * sNx;
+    float* ed = base + sNx * sNy;
+    if (c0 == 0) ed[j0] = v.x;
+    if (c0 + 1 == sNx - 1) ed[sNy + j0] = v.y;
+    if (c1 == 0) ed[j1] = v.z;
+    if (c1 + 1 == sNx - 1) ed[sNy + j1] = v.w;
+}
+__device__ __forceinline__ void edge_put2(float* base, int j, int c, float2 v, int sNx, int sNy) {
+    if (!MJ_MX_EDGES) return;
+    float* ed = base + sNx * sNy;
+    if (c == 0) ed[j] = v.x;
+    if (c + 1 == sNx - 1) ed[sNy + j] = v.y;
+}
+
 // Stencil over the LDS tile: IS_A: A32 = (W, S, C) = (aW, aS, aC); else M32 = (pW, pS, pC). Rows go in groups of
 // kRowGroup, group g to wave g % nwaves; the result of each row goes to out (compact, row j at j * sNx) and
 // prods(Xc, v, pv) gives the row's NR per-point products, reduced in the inner dot-product order; tile sum r of
@@ -141,7 +160,10 @@ __device__ __forceinline__ void mx_stencil(const MxDev& a, int t, const float* _
             const float xl = tile[q0 - 1], xr = tile[q0 + 2];
             const float2 v = IS_A ? pair_A(Wc, wr, S0, S1, C, Xm, Xc, Xp, xl, xr)
                                   : pair_M(C, Wc, wr, S0, S1, Xm, Xc, Xp, xl, xr);
-            if (L.act) st2(out + j * sNx + L.c, v);
+            if (L.act) {
+                st2(out + j * sNx + L.c, v);
+                edge_put2(out, j, L.c, v, sNx, sNy);
+            }
             double2 pv[NR];
             prods(Xc, v, pv);
 #pragma unroll
@@ -231,6 +253,7 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
         const float2 v = make_float2((float)(R.x / sigma), (float)(R.y / sigma));
         if (L.act) {
             st2(r32 + j * sNx + L.c, v);
+            edge_put2(r32, j, L.c, v, sNx, sNy);
             st2(tile + G.at(j, L.c), v);
         }
     }
@@ -349,6 +372,7 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
             v.w = v.w + beta * O.w;
         }
         st4(sn + e, v);
+        edge_put4(sn, e, v, sNx, sNy, inv);
         lds_put4(tile, G, e, v, sNx, inv);
     }
     for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
@@ -389,6 +413,7 @@ __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, doubl
         rn.z = R.z - alpha * Q.z;
         rn.w = R.w - alpha * Q.w;
         st4(rout + e, rn);
+        edge_put4(rout, e, rn, sNx, sNy, inv);
         lds_put4(tile, G, e, rn, sNx, inv);
     }
     for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {

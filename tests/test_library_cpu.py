@@ -44,7 +44,8 @@ def test_workspace_query_and_config_validation():
 
 
 def test_mixed_workspace_and_config_validation():
-    """The mixed solve's workspace query: compact FP32 work vectors (7 x members x tiles x 8,128 values) besides the
+    """The mixed solve's workspace query: compact FP32 work vectors (7 x members x tiles x 8,288 values: the 8,100
+    interior values and copies of the two edge columns, 128-byte aligned) besides the
     FP64 residual; invalid configurations are refused with a message, before any device work."""
     import ctypes
     from mit_jax_amd import _lib, cg2d
@@ -54,7 +55,7 @@ def test_mixed_workspace_and_config_validation():
     cfg = cg2d.config(LLC90, B, 1e-12, 100, "tile", launch="phases")
     mcfg = _lib.Cg2dMixedConfig(1, 1e-5, 400, 40)
     n = L.mj_cg2d_mixed_workspace_bytes(ctypes.byref(cfg), ctypes.byref(mcfg), 0)
-    assert n >= 7 * B * LLC90.nTiles * 8128 * 4 + B * LLC90.npad * 8
+    assert n >= 7 * B * LLC90.nTiles * 8288 * 4 + B * LLC90.npad * 8
     for field, value, word in (("sNy", 89, "sNy"), ("OL", 0, "OL"), ("sum_order", 0, "MJ_SUM_TREE"),
                                ("tolerance", 0.0, "tolerance"), ("block_threads", 2048, "block_threads")):
         bad = cg2d.config(LLC90, B, 1e-12, 100, "tile", launch="phases")
