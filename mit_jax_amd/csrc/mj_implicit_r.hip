@@ -16,6 +16,9 @@
 namespace mj {
 namespace {
 
+#ifndef MJ_IMPR_REG50
+#define MJ_IMPR_REG50 1
+#endif
 constexpr int kImprThreads = 64;  // one wave per workgroup (LDS: 2 * nr * 64 + 3 * nr doubles)
 #ifndef MJ_IMPR_PF
 #define MJ_IMPR_PF 16
@@ -73,12 +76,14 @@ struct ColState {
 // 2 <= m <= nr-3, branch-free, so that the loads of the ring stay in flight across rows. Level data: mC0, rh0, kap0,
 // y0 of level m; mCn, kapn, wn of level m+1; mCk = maskC(min(nr-1, m+2)). Halo columns (in = false) get identity
 // rows; their level data are loaded like the others' and ignored.
-template <bool GEN, bool ADV>
+// STORE: write d', e' to LDS and y' to the output column (else the caller keeps them, from c); NRC: the level count
+// when known at compile time (0: a.nr).
+template <bool GEN, bool ADV, bool STORE = true, int NRC = 0>
 __device__ __forceinline__ void impr_row(const ImprArgs& a, int m, bool in, double rA, double rrA, double mC0,
                                          double rh0, double kap0, double y0, double mCn, double kapn, double wn,
                                          double mCk, ColState& c, double* dpL, double* epL, double* out,
                                          const double* lv) {
-    const int nr = a.nr;
+    const int nr = NRC ? NRC : a.nr;
     const double dtm = lv[m], rdf = lv[nr + m];
     const double* rdrC = lv + 2 * nr;
     const bool hasU = !GEN || m >= 1, hasL = !GEN || m + 1 < nr;
@@ -125,9 +130,11 @@ __device__ __forceinline__ void impr_row(const ImprArgs& a, int m, bool in, doub
         yq = (y0 - ra * c.yp2) - bb * c.yp1;
         norm3(cq, dq, eq, yq);
     }
-    dpL[(int64_t)m * kImprThreads] = dq;
-    epL[(int64_t)m * kImprThreads] = eq;
-    out[(int64_t)m * a.plane] = yq;
+    if (STORE) {
+        dpL[(int64_t)m * kImprThreads] = dq;
+        epL[(int64_t)m * kImprThreads] = eq;
+        out[(int64_t)m * a.plane] = yq;
+    }
     c.dp2 = c.dp1, c.ep2 = c.ep1, c.yp2 = c.yp1;
     c.dp1 = dq, c.ep1 = eq, c.yp1 = yq;
     c.mC1 = mC0;
@@ -238,6 +245,82 @@ __global__ void __launch_bounds__(kImprThreads) impr_kernel(ImprArgs a) {
     }
 }
 
+// The same sweep for a level count known at compile time: the rows are unrolled, so y' of every level stays in
+// registers (d', e' in LDS as before) and makes no round trip through the output column: each column is read once
+// and written once. Bitwise the generic kernel's arithmetic.
+constexpr int kPfR = 8;
+template <bool ADV, int NR>
+__global__ void __launch_bounds__(kImprThreads) impr_reg_kernel(ImprArgs a) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int64_t col = (int64_t)blockIdx.x * kImprThreads + lane;
+    const int mem = blockIdx.y;
+    double* lv = lds;
+    for (int k = lane; k < NR; k += kImprThreads) {
+        lv[k] = a.dt[k];
+        lv[NR + k] = a.rdrF[k];
+        lv[2 * NR + k] = a.rdrC[k];
+    }
+    __syncthreads();
+    if (col >= (int64_t)a.T * a.plane) return;  // no barriers below
+    const int t = (int)(col / a.plane), q = (int)(col - (int64_t)t * a.plane);
+    const int j = q / a.nx, i = q - j * a.nx;
+    const bool in = j >= a.OL && j < a.OL + a.sNy && i >= a.OL && i < a.OL + a.sNx;
+    const int64_t o3 = mem * a.ms3 + (int64_t)t * NR * a.plane + q;
+    const int64_t om = mem * a.mms + (int64_t)t * NR * a.plane + q;
+    const double* y = a.y + o3;
+    double* out = a.out + o3;
+    const int64_t o2 = (int64_t)t * a.plane + q;
+    const double rA = ADV ? a.rA[o2] : 0.0;
+    const double rrA = ADV ? a.recip_rA[o2] : 0.0;
+    const double* mCp = a.maskC + om;
+    const double *rhp = a.rh + o3, *kp = a.kap + o3, *wp = a.w + o3;
+    double rmC[kPfR], rrh[kPfR], rkap[kPfR], rw[kPfR], ry[kPfR];
+    auto load = [&](int s, int lev) {
+        const int64_t ok = (int64_t)lev * a.plane;
+        ry[s] = y[ok];
+        rmC[s] = mCp[ok];
+        rrh[s] = rhp[ok];
+        rkap[s] = kp[ok];
+        rw[s] = wp[ok];
+    };
+#pragma unroll
+    for (int s = 0; s < kPfR; ++s) load(s, s < NR ? s : NR - 1);
+    ColState c{0.0, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double* dpL = lds + 3 * NR + lane;
+    double* epL = dpL + NR * kImprThreads;
+    double ypR[NR];
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        const int s = m % kPfR, s1 = (m + 1) % kPfR, s2 = (m + 2) % kPfR;
+        const double mCn = m + 1 < NR ? rmC[s1] : 0.0;
+        const double mCk = m + 2 < NR ? rmC[s2] : rmC[s1];
+        if (m <= 1 || m >= NR - 2)
+            impr_row<true, ADV, false, NR>(a, m, in, rA, rrA, rmC[s], rrh[s], rkap[s], ry[s], mCn, rkap[s1], rw[s1],
+                                           mCk, c, nullptr, nullptr, nullptr, lv);
+        else
+            impr_row<false, ADV, false, NR>(a, m, in, rA, rrA, rmC[s], rrh[s], rkap[s], ry[s], mCn, rkap[s1], rw[s1],
+                                            mCk, c, nullptr, nullptr, nullptr, lv);
+        dpL[m * kImprThreads] = c.dp1;
+        epL[m * kImprThreads] = c.ep1;
+        ypR[m] = c.yp1;
+        if (m + kPfR < NR) load(s, m + kPfR);
+    }
+    double u1 = ypR[NR - 1];
+    out[(int64_t)(NR - 1) * a.plane] = u1;
+    double u0 = ypR[NR - 2] - u1 * dpL[(NR - 2) * kImprThreads];
+    out[(int64_t)(NR - 2) * a.plane] = u0;
+    double u2 = u1;
+    u1 = u0;
+#pragma unroll
+    for (int m = NR - 3; m >= 0; --m) {
+        const double u = (ypR[m] - u1 * dpL[m * kImprThreads]) - u2 * epL[m * kImprThreads];
+        out[(int64_t)m * a.plane] = u;
+        u2 = u1;
+        u1 = u;
+    }
+}
+
 }  // namespace
 }  // namespace mj
 
@@ -296,9 +379,20 @@ extern "C" int mj_gad_implicit_r(const mj_impr_grid* g, const mj_impr_config* c,
     a.y = gTracer;
     a.out = out;
     const size_t lds = ((size_t)2 * c->nr * kImprThreads + 3 * (size_t)c->nr) * sizeof(double);
+    const dim3 grid((unsigned)((cols + kImprThreads - 1) / kImprThreads), (unsigned)c->batch);
+#if MJ_IMPR_REG50
+    if (c->nr == 50) {  // the LLC90 level count: the register-resident sweep
+        const size_t lds50 = (3 * 50 + 2 * 50 * kImprThreads) * sizeof(double);
+        if (a.adv)
+            hipLaunchKernelGGL((impr_reg_kernel<true, 50>), grid, dim3(kImprThreads), lds50, st, a);
+        else
+            hipLaunchKernelGGL((impr_reg_kernel<false, 50>), grid, dim3(kImprThreads), lds50, st, a);
+        MJ_HIP(hipGetLastError());
+        return MJ_OK;
+    }
+#endif
     const void* kern = a.adv ? (const void*)impr_kernel<true> : (const void*)impr_kernel<false>;
     if (lds > 65536) MJ_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const dim3 grid((unsigned)((cols + kImprThreads - 1) / kImprThreads), (unsigned)c->batch);
     if (a.adv)
         hipLaunchKernelGGL(impr_kernel<true>, grid, dim3(kImprThreads), lds, st, a);
     else

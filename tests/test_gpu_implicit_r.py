@@ -106,3 +106,19 @@ def test_gad_implicit_r_autograd(dev):
         a = t.grad.cpu().numpy()
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
     assert torch.isfinite(k_t.grad).all()
+
+
+@pytest.mark.parametrize("K", [2, 17, 30])
+def test_gad_implicit_r_other_level_counts(dev, K):
+    """Level counts other than 50 take the generic sweep (y' through the output column): bitwise the oracle too."""
+    from mit_jax_amd import tracer_implicit as TI
+    g, p, kap, w, tr = _inputs()
+    sl = lambda a: np.ascontiguousarray(a[:, :K])  # noqa: E731
+    gk = SimpleNamespace(layout=g.L, maskC=sl(g.maskC), rA=g.rA, recip_rA=g.recip_rA,
+                         recip_drF=np.asarray(g.v["recip_drF"])[:K], recip_drC=np.asarray(g.v["recip_drC"])[:K])
+    pk = SimpleNamespace(dTtracerLev=np.asarray(p.dTtracerLev)[:K], implicitDiffusion=True, rkSign=-1.0)
+    print(f"gad_implicit_r: {K} levels", flush=True)
+    got = TI.gad_implicit_r(pk, gk, True, 3, to_dev(sl(kap), dev), to_dev(sl(g.recip_hFacC), dev), to_dev(sl(w), dev),
+                            to_dev(sl(tr), dev))
+    ref = OI.gad_implicit_r(pk, gk, True, sl(kap), sl(g.recip_hFacC), sl(w), sl(tr))
+    assert np.array_equal(got.cpu().numpy(), ref)
