@@ -8,6 +8,7 @@ Kinds: T, 3D, Z, SMs (scalar) and UVs, UVn, UV3s, As, An, Bs, Bn, Ds (vector).
 """
 
 import ctypes
+import sys
 import threading
 from dataclasses import dataclass
 from functools import lru_cache
@@ -115,12 +116,15 @@ class HaloPlan:
             _lib.check(_lib.lib().mj_halo_plan_create(nm, npts, *[ctypes.cast(p, ctypes.c_void_p) for p in ptrs],
                                                       L.nTiles, L.ny, L.nx, ctypes.byref(h)), "mj_halo_plan_create")
         self.handle = h
+        self._destroy = _lib.lib().mj_halo_plan_destroy
         self.inplace_ok = bool(_lib.lib().mj_halo_plan_inplace_ok(h))
 
-    def __del__(self):
-        h = getattr(self, "handle", None)
-        if h and _lib._lib is not None:
-            _lib.lib().mj_halo_plan_destroy(h)
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # the destroy function is bound at creation: module globals may already be gone when this runs late, and at
+        # interpreter exit the process end releases the device memory anyway
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h and destroy is not None and not _finalizing():
+            destroy(h)
 
 
 _plans = {}

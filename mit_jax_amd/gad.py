@@ -15,6 +15,7 @@ kernel.
 """
 
 import ctypes
+import sys
 from dataclasses import dataclass, replace
 
 import numpy as np
@@ -96,6 +97,7 @@ class GadPlan:
             _lib.check(_lib.lib().mj_gad_plan_create(len(face), _lib.ptr(face), _lib.ptr(edges), L.sNx, L.sNy, L.OLx,
                                                      L.OLy, ctypes.byref(h)), "mj_gad_plan_create")
         self.handle = h
+        self._destroy = _lib.lib().mj_gad_plan_destroy
         self.ntiles = len(face)
         self.L = L
 
@@ -108,10 +110,12 @@ class GadPlan:
                    "mj_gad_plan_describe")
         return masks, fills
 
-    def __del__(self):
-        h = getattr(self, "handle", None)
-        if h and _lib._lib is not None:
-            _lib.lib().mj_gad_plan_destroy(h)
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # the destroy function is bound at creation: module globals may already be gone when this runs late, and at
+        # interpreter exit the process end releases the device memory anyway
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h and destroy is not None and not _finalizing():
+            destroy(h)
 
 
 def plan_tables_host(topo: TileTopology, L: Layout = LLC90):
