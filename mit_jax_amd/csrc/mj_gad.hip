@@ -3,9 +3,10 @@
 // Horizontal part: the reference's GAD_ADVECTION with implicitAdvection (pkgs/gad.py:460-539, gad_advection.F):
 // three passes, each an X and a Y block whose action per tile comes from the facet pass table (which direction
 // sweeps, overlap-only / interior-only update regions, FILL_CS_CORNER_TR_RL corner fills before/after the flux).
-// Everything is tile-local, so one workgroup owns one (member, tile, level): the 98x98 FP64 tracer (localTij) and
-// the flux array live in LDS (2 x 76.8 KB, gfx950's 160 KB LDS), every input point is read from HBM once per sweep
-// that needs it, and only the tendency is written. The reference evaluates all six flux arrays on every tile and
+// Everything is tile-local, so one workgroup owns one (member, tile, level): the 98x98 FP64 working tracer
+// (localTij) and the input tracer live in LDS (2 x 76.8 KB, gfx950's 160 KB LDS); each point evaluates the
+// interfaces on both its faces, velocities and metrics are read once per sweep that needs them, and only the
+// tendency is written. The reference evaluates all six flux arrays on every tile and
 // masks; here each tile evaluates only the sweeps its table runs (2-3 of 6).
 //
 // Vertical part (R, only when wFld is given): a DST3 flux through the level interfaces applied after the
@@ -267,16 +268,21 @@ __device__ __forceinline__ int box_point(const MjBox& B, int bw, float inv_bw, i
     return (B.j0 + r) * GNX + B.i0 + (idx - r * bw);
 }
 
-#ifndef MJ_GAD_BATCH
-#define MJ_GAD_BATCH 2
-#endif  // points per thread whose global loads are issued together (the kernel is bound by load latency)
+// One workgroup of 1024 threads per (member, tile, level). Every box point of a sweep evaluates the interface
+// values of both its faces (q and q + d) itself, with no flux plane: from the same loads and in the same order of
+// operations, so the two points beside a face get the same bits, and the update is the same IEEE expression as
+// with one shared evaluation. The point keeps its increment in a register until every thread has read the tracer
+// plane, and only then updates it. LDS: the working plane, the input tracer (read by every update and by the
+// output) and the mask bits, 156 KB.
+constexpr int kHzThreads = 1024;
+constexpr int kHzPts = (GPLANE + kHzThreads - 1) / kHzThreads;  // box points per thread (disjoint boxes in the plane)
 
-__global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
+__global__ void __launch_bounds__(kHzThreads, 4) gad_hz_kernel(GadDev a) {
+    constexpr int NT = kHzThreads;
     __shared__ double loc[GPLANE];
-    __shared__ double af[GPLANE];
+    __shared__ double trc[GPLANE];
     __shared__ unsigned long long mlw[kMaskWords], mls[kMaskWords];
-    constexpr int NB = MJ_GAD_BATCH;
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int tid = threadIdx.x;
     int t, m, k;
     if (!gad_unit(a, t, m, k)) return;
     if (a.nlev && k >= a.nlev[m]) return;
@@ -295,109 +301,75 @@ __global__ void __launch_bounds__(1024) gad_hz_kernel(GadDev a) {
     const double oneSixth = a.oneSixth;
 
     load_tile_and_masks(a, tp, tr, mW, mS, loc, mlw, mls);
+    for (int q = tid; q < GPLANE; q += NT) trc[q] = loc[q];
+    __syncthreads();
 
     for (int pass = 0; pass < 3; ++pass) {
         for (int dir = 0; dir < 2; ++dir) {
             const MjSweep& S = tp.sweep[pass][dir];
             if (!S.run) continue;
-            // X: uFld, hFacW, dyG, recip_dxC along i; Y: vFld, hFacS, dxG, recip_dyC along j
             const double* __restrict__ vel = (dir == 0 ? a.u : a.v) + b3;
             const double* __restrict__ hf = (dir == 0 ? a.hFacW : a.hFacS) + b3;
             const double* __restrict__ gl = (dir == 0 ? a.dyG : a.dxG) + b2;
             const double* __restrict__ rdc = (dir == 0 ? a.recip_dxC : a.recip_dyC) + b2;
             const unsigned long long* mb = dir == 0 ? mlw : mls;
-            const int d = dir == 0 ? 1 : GNX;
-            // uTrans / vTrans = vel * (gl*deepFacC*drF*hf) * rhoFacC (gad.py:482-485), left to right
+            const int d = dir == 0 ? 1 : GNX, cmax = dir == 0 ? GNX - 2 : GNY - 2;
             auto trans = [&](double vv, double g, double h) { return vv * (g * deepFacC * drF * h) * rhoFacC; };
+            // interface value of face q (gad.py:482-485 transport; 0 outside the Fortran flux loops
+            // i = 1-OLx+2 .. sNx+OLx-1, gad_dst3_adv_x.F:80, and j = 1-OLy+2 .. sNy+OLy-1, gad_dst3_adv_y.F:78)
+            auto face = [&](int q, double V, double Tq, double R) {
+                const int j = q / GNX, c = dir == 0 ? q - j * GNX : j;
+                if (c < 2 || c > cmax) return 0.0;
+                const double cfl = fabs(V * dt * R * rdeepC);
+                return dst3_flux(Tq, cfl, oneSixth, loc[q - 2 * d], loc[q - d], loc[q], loc[q + d], mbit(mb, q - d),
+                                 mbit(mb, q), mbit(mb, q + d));
+            };
             if (S.fill_before) corner_fill(loc, a, tp, S.fill_before, tid);
-            // interface values over the faces the update boxes read (faces q and q + d of every box point)
-            for (int bx = 0; bx < S.nbox; ++bx) {
-                const MjBox B = S.box[bx];
-                const int i1 = B.i1 + (dir == 0 ? 1 : 0), j1 = B.j1 + (dir == 1 ? 1 : 0);
-                const int bw = i1 - B.i0, n = bw * (j1 - B.j0);
-                const float inv = 1.0f / (float)bw;
-                for (int i0 = tid; i0 < n; i0 += NB * nthr) {
-                    int q[NB];
-                    double V[NB], G[NB], H[NB], R[NB];
+            // the boxes' points as one index range (the boxes are disjoint, so at most GPLANE points)
+            const MjBox B0 = S.box[0], B1 = S.box[1];
+            const int bw0 = B0.i1 - B0.i0, n0 = S.nbox > 0 ? bw0 * (B0.j1 - B0.j0) : 0;
+            const int bw1 = B1.i1 - B1.i0, n1 = S.nbox > 1 ? bw1 * (B1.j1 - B1.j0) : 0;
+            const float inv0 = 1.0f / (float)max(bw0, 1), inv1 = 1.0f / (float)max(bw1, 1);
+            auto point = [&](int idx) {
+                return idx < n0 ? box_point(B0, bw0, inv0, idx) : box_point(B1, bw1, inv1, idx - n0);
+            };
+            double inc[kHzPts];
 #pragma unroll
-                    for (int b = 0; b < NB; ++b) {
-                        const int idx = i0 + b * nthr;
-                        q[b] = box_point(B, bw, inv, idx < n ? idx : i0);
-                        V[b] = vel[q[b]];
-                        G[b] = gl[q[b]];
-                        H[b] = hf[q[b]];
-                        R[b] = rdc[q[b]];
-                    }
-#pragma unroll
-                    for (int b = 0; b < NB; ++b) {
-                        if (i0 + b * nthr >= n) continue;
-                        const int qq = q[b], j = qq / GNX, i = qq - j * GNX;
-                        double f = 0.0;
-                        // 0 outside the Fortran flux loops: i = 1-OLx+2 .. sNx+OLx-1 (gad_dst3_adv_x.F:80),
-                        // j = 1-OLy+2 .. sNy+OLy-1 (gad_dst3_adv_y.F:78)
-                        const int c = dir == 0 ? i : j, cmax = dir == 0 ? GNX - 2 : GNY - 2;
-                        if (c >= 2 && c <= cmax) {
-                            const double cfl = fabs(V[b] * dt * R[b] * rdeepC);
-                            f = dst3_flux(trans(V[b], G[b], H[b]), cfl, oneSixth, loc[qq - 2 * d], loc[qq - d], loc[qq],
-                                          loc[qq + d], mbit(mb, qq - d), mbit(mb, qq), mbit(mb, qq + d));
-                        }
-                        af[qq] = f;
-                    }
+            for (int r = 0; r < kHzPts; ++r) {
+                const int idx = tid + r * NT;
+                inc[r] = 0.0;
+                if (idx < n0 + n1) {
+                    const int qq = point(idx), qd = qq + d;
+                    const double V0 = vel[qq], G0 = gl[qq], H0 = hf[qq], R0 = rdc[qq];
+                    const double V1 = vel[qd], G1 = gl[qd], H1 = hf[qd], R1 = rdc[qd];
+                    const double TR = trc[qq], RH = rhC[qq], RA = rrA[qq], MI = mInC[qq];
+                    const double T0 = trans(V0, G0, H0), T1 = trans(V1, G1, H1);
+                    const double f0 = face(qq, V0, T0, R0), f1 = face(qd, V1, T1, R1);
+                    // local - coef*(d(af) - tracer*d(trans))*maskInC (gad.py:511-514), coefficient gad.py:494
+                    const double daf = f1 - f0, dtr = T1 - T0;
+                    const double coef = dt * rrhoFacC * RH * rdrF * RA * rdeep2C;
+                    inc[r] = coef * (daf - TR * dtr) * MI;
                 }
             }
-            if (S.nbox) __syncthreads();
+            __syncthreads();  // every read of the plane precedes the updates
             if (S.fill_after) corner_fill(loc, a, tp, S.fill_after, tid);
-            // local - coef*(d(af) - tracer*d(trans))*maskInC (gad.py:511-514); update coefficient
-            // dt*recip_rhoFacC*recip_hFacC*recip_drF*recip_rA*recip_deepFac2C (gad.py:494)
-            for (int bx = 0; bx < S.nbox; ++bx) {
-                const MjBox B = S.box[bx];
-                const int bw = B.i1 - B.i0, n = bw * (B.j1 - B.j0);
-                const float inv = 1.0f / (float)bw;
-                for (int i0 = tid; i0 < n; i0 += NB * nthr) {
-                    int q[NB];
-                    double T0[NB], T1[NB], TR[NB], RH[NB], RA[NB], MI[NB];
 #pragma unroll
-                    for (int b = 0; b < NB; ++b) {
-                        const int idx = i0 + b * nthr;
-                        const int qq = box_point(B, bw, inv, idx < n ? idx : i0);
-                        q[b] = qq;
-                        T0[b] = trans(vel[qq], gl[qq], hf[qq]);
-                        T1[b] = trans(vel[qq + d], gl[qq + d], hf[qq + d]);
-                        TR[b] = tr[qq];
-                        RH[b] = rhC[qq];
-                        RA[b] = rrA[qq];
-                        MI[b] = mInC[qq];
-                    }
-#pragma unroll
-                    for (int b = 0; b < NB; ++b) {
-                        if (i0 + b * nthr >= n) continue;
-                        const int qq = q[b];
-                        const double daf = af[qq + d] - af[qq], dtr = T1[b] - T0[b];
-                        const double coef = dt * rrhoFacC * RH[b] * rdrF * RA[b] * rdeep2C;
-                        loc[qq] = loc[qq] - coef * (daf - TR[b] * dtr) * MI[b];
-                    }
+            for (int r = 0; r < kHzPts; ++r) {
+                const int idx = tid + r * NT;
+                if (idx < n0 + n1) {
+                    const int qq = point(idx);
+                    loc[qq] = loc[qq] - inc[r];
                 }
             }
-            if (S.nbox) __syncthreads();
+            __syncthreads();
         }
     }
     double* out = a.out + b3;
-    for (int q0 = tid; q0 < GPLANE; q0 += NB * nthr) {
-        double v[NB];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int q = q0 + b * nthr;
-            v[b] = q < GPLANE ? tr[q] : 0.0;
-        }
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int q = q0 + b * nthr;
-            if (q >= GPLANE) continue;
-            if (a.w)
-                out[q] = loc[q];
-            else
-                out[q] = (loc[q] - v[b]) / dt;  // gad.py:539
-        }
+    for (int q = tid; q < GPLANE; q += NT) {
+        if (a.w)
+            out[q] = loc[q];
+        else
+            out[q] = (loc[q] - trc[q]) / dt;  // gad.py:539
     }
 }
 
@@ -912,7 +884,7 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
     a.out = wFld ? work : gTracer;
     hipStream_t st = as_stream(stream);
     const unsigned nblk = (unsigned)((c->batch * P->ntiles * c->nr + 7) / 8 * 8);
-    hipLaunchKernelGGL(gad_hz_kernel, dim3(nblk), dim3(1024), 0, st, a);
+    hipLaunchKernelGGL(gad_hz_kernel, dim3(nblk), dim3(kHzThreads), 0, st, a);
     MJ_HIP(hipGetLastError());
     if (wFld) {
         a.out = gTracer;
