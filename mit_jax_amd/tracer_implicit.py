@@ -26,7 +26,7 @@ def _f64(a, device, name):
     if isinstance(a, torch.Tensor):
         if a.device.type != "cuda":
             raise TypeError(f"{name}: expected a CUDA tensor")
-        return a.to(dtype=torch.float64).contiguous()
+        return a.to(device=device, dtype=torch.float64).contiguous()
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
 
 
@@ -38,6 +38,17 @@ def _batch(a, name):
     if a.dim() != 5:
         raise ValueError(f"{name}: expected [T, Nr, ny, nx] or [B, T, Nr, ny, nx], got {tuple(a.shape)}")
     return a.contiguous()
+
+
+def _check_shapes(L, y, name, others):
+    """The tile layout of y ([B, T, Nr, ny, nx]) and the same shape for every (tensor, name) of others: the kernels
+    index all arrays with y's strides."""
+    B, T, K, ny, nx = y.shape
+    if (T, ny, nx) != (L.nTiles, L.ny, L.nx):
+        raise ValueError(f"{name}: tile layout {(T, ny, nx)} is not {(L.nTiles, L.ny, L.nx)}")
+    for t, n in others:
+        if t is not None and t.shape != y.shape:
+            raise ValueError(f"{n}: shape {tuple(t.shape)} differs from {name}'s {tuple(y.shape)}")
 
 
 def _check_branch(p, implicitAdvection, advectionScheme):
@@ -77,6 +88,7 @@ def gad_implicit_r_adjoint(p, g, implicitAdvection, advectionScheme, kappaRX, re
     L = getattr(g, "layout", None) or g.L
     batched = u.dim() == 5
     uu, ub = _batch(u, "u"), _batch(ubar, "ubar")
+    _check_shapes(L, uu, "u", [(ub, "ubar")])
     B, T, K, ny, nx = uu.shape
     if K <= 1:
         z = torch.zeros_like(uu)
@@ -84,6 +96,7 @@ def gad_implicit_r_adjoint(p, g, implicitAdvection, advectionScheme, kappaRX, re
         return r if batched else tuple(x[0] if x is not None else None for x in r)
     kap, rh = _batch(kappaRX, "kappaRX"), _batch(recip_hFac, "recip_hFac")
     w = _batch(wFld, "wFld") if implicitAdvection else None
+    _check_shapes(L, uu, "u", [(kap, "kappaRX"), (rh, "recip_hFac"), (w, "wFld")])
     gs, cfg, keep = _structs(p, g, L, B, T, K, ny, nx, implicitAdvection, uu.device)
     yb, kb = torch.empty_like(uu), torch.empty_like(uu)
     wb = torch.empty_like(uu) if implicitAdvection else None
@@ -123,18 +136,15 @@ def gad_implicit_r(p, g, implicitAdvection, advectionScheme, kappaRX, recip_hFac
     L = getattr(g, "layout", None) or g.L
     batched = gTracer.dim() == 5
     y = _batch(gTracer, "gTracer")
+    _check_shapes(L, y, "gTracer", [])
     B, T, K, ny, nx = y.shape
-    if (T, ny, nx) != (L.nTiles, L.ny, L.nx):
-        raise ValueError(f"gTracer: tile layout {(T, ny, nx)} is not {(L.nTiles, L.ny, L.nx)}")
     out = torch.empty_like(y)
     if K <= 1:
         out.copy_(y)
         return out if batched else out[0]
     kap, rh = _batch(kappaRX, "kappaRX"), _batch(recip_hFac, "recip_hFac")
     w = _batch(wFld, "wFld") if implicitAdvection else None
-    for t, n in ((kap, "kappaRX"), (rh, "recip_hFac")) + (((w, "wFld"),) if w is not None else ()):
-        if t.shape != y.shape:
-            raise ValueError(f"{n}: shape {tuple(t.shape)} differs from gTracer's {tuple(y.shape)}")
+    _check_shapes(L, y, "gTracer", [(kap, "kappaRX"), (rh, "recip_hFac"), (w, "wFld")])
     gs, cfg, keep = _structs(p, g, L, B, T, K, ny, nx, implicitAdvection, y.device)
     P = _lib.ptr
     _lib.check(_lib.lib().mj_gad_implicit_r(ctypes.byref(gs), ctypes.byref(cfg), P(kap), P(rh), P(w), P(y), P(out),

@@ -122,3 +122,21 @@ def test_gad_implicit_r_other_level_counts(dev, K):
                             to_dev(sl(tr), dev))
     ref = OI.gad_implicit_r(pk, gk, True, sl(kap), sl(g.recip_hFacC), sl(w), sl(tr))
     assert np.array_equal(got.cpu().numpy(), ref)
+
+
+def test_gad_implicit_r_refuses_mismatched_shapes(dev):
+    """Forward and reverse mode refuse arrays whose shape differs from the tracer's (the kernels index every array
+    with the tracer's strides) before any launch."""
+    from mit_jax_amd import tracer_implicit as TI
+    g, p, kap, w, tr = _inputs()
+    kd, rh, wd, y = to_dev(kap, dev), to_dev(g.recip_hFacC, dev), to_dev(w, dev), to_dev(tr, dev)
+    short = kd[:, :30].contiguous()
+    print("gad_implicit_r: mismatched shapes", flush=True)
+    with pytest.raises(ValueError, match="kappaRX"):
+        TI.gad_implicit_r(p, g, True, 3, short, rh, wd, y)
+    with pytest.raises(ValueError, match="wFld"):
+        TI.gad_implicit_r_adjoint(p, g, True, 3, kd, rh, wd[:, :30].contiguous(), y, y)
+    with pytest.raises(ValueError, match="ubar"):
+        TI.gad_implicit_r_adjoint(p, g, True, 3, kd, rh, wd, y, y[:, :30].contiguous())
+    with pytest.raises(ValueError, match="tile layout"):
+        TI.gad_implicit_r_adjoint(p, g, True, 3, kd, rh, wd, y[:12].contiguous(), y[:12].contiguous())
