@@ -747,6 +747,32 @@ static int build_plan_host(mj_gad_plan* P, int ntiles, const int32_t* face, cons
     return MJ_OK;
 }
 
+// What gad_hz_kernel relies on: the update boxes of a sweep are disjoint and lie in the plane together with the
+// face q + d of each of their points (so a sweep has at most GPLANE points, kHzPts per thread).
+static int check_boxes(const mj_gad_plan& P) {
+    for (int t = 0; t < P.ntiles; ++t)
+        for (int pass = 0; pass < 3; ++pass)
+            for (int dir = 0; dir < 2; ++dir) {
+                const MjSweep& S = P.tiles[t].sweep[pass][dir];
+                bool ok = S.nbox >= 0 && S.nbox <= 2;
+                for (int b = 0; ok && b < S.nbox; ++b) {
+                    const MjBox& B = S.box[b];
+                    ok = B.i0 >= 0 && B.j0 >= 0 && B.i0 < B.i1 && B.j0 < B.j1 && B.i1 + (dir == 0) <= GNX &&
+                         B.j1 + (dir == 1) <= GNY;
+                }
+                if (ok && S.nbox == 2) {
+                    const MjBox &A = S.box[0], &B = S.box[1];
+                    ok = A.i1 <= B.i0 || B.i1 <= A.i0 || A.j1 <= B.j0 || B.j1 <= A.j0;
+                }
+                if (!ok) {
+                    set_error("gad plan: tile %d pass %d dir %d: update boxes overlap or leave the plane", t, pass + 1,
+                              dir);
+                    return MJ_ERR_ARG;
+                }
+            }
+    return MJ_OK;
+}
+
 extern "C" int mj_gad_plan_create(int ntiles, const int32_t* face, const uint8_t* edges, int sNx, int sNy, int OLx,
                                   int OLy, mj_gad_plan** out) {
     MJ_REQUIRE(out && face && edges && ntiles > 0, "mj_gad_plan_create: bad argument");
@@ -755,6 +781,10 @@ extern "C" int mj_gad_plan_create(int ntiles, const int32_t* face, const uint8_t
                GNY, GNX, sNx, sNy, OLx, OLy);
     mj_gad_plan* P = new mj_gad_plan();
     build_plan_host(P, ntiles, face, edges, sNx, sNy, OLx, OLy);
+    if (check_boxes(*P) != MJ_OK) {
+        delete P;
+        return MJ_ERR_ARG;
+    }
     hipError_t e = hipMalloc((void**)&P->d_tiles, sizeof(MjTilePlan) * ntiles);
     if (e == hipSuccess)
         e = hipMemcpy(P->d_tiles, P->tiles.data(), sizeof(MjTilePlan) * ntiles, hipMemcpyHostToDevice);
@@ -790,6 +820,7 @@ extern "C" int mj_gad_tables_host(int ntiles, const int32_t* face, const uint8_t
     P.OLx = OLx;
     P.OLy = OLy;
     build_plan_host(&P, ntiles, face, edges, sNx, sNy, OLx, OLy);
+    if (check_boxes(P) != MJ_OK) return MJ_ERR_ARG;
     return describe(&P, masks, fills);
 }
 
