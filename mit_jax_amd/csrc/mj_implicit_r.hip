@@ -168,9 +168,9 @@ __global__ void __launch_bounds__(kImprThreads) impr_kernel(ImprArgs a) {
     const int64_t o2 = (int64_t)t * a.plane + q;
     const double rA = ADV ? a.rA[o2] : 0.0;
     const double rrA = ADV ? a.recip_rA[o2] : 0.0;
-    const bool any_in = __any(in);  // as in impr_reg_kernel: halo-only waves re-read the tracer's lines
-    const double* mCp = any_in ? a.maskC + om : y;
-    const double *rhp = any_in ? a.rh + o3 : y, *kp = any_in ? a.kap + o3 : y, *wp = any_in ? a.w + o3 : y;
+    // as in impr_reg_kernel: halo columns re-read the tracer's lines
+    const double* mCp = in ? a.maskC + om : y;
+    const double *rhp = in ? a.rh + o3 : y, *kp = in ? a.kap + o3 : y, *wp = in ? a.w + o3 : y;
 
     // ring of level data: slot s holds a level m with m % kPf == s; loads are unconditional (levels clamped to
     // nr-1) so that no branch separates a load from its use
@@ -274,10 +274,10 @@ __global__ void __launch_bounds__(kImprThreads) impr_reg_kernel(ImprArgs a) {
     const int64_t o2 = (int64_t)t * a.plane + q;
     const double rA = ADV ? a.rA[o2] : 0.0;
     const double rrA = ADV ? a.recip_rA[o2] : 0.0;
-    // a wave of halo columns only needs the tracer: its other loads read the tracer's lines again (no HBM traffic)
-    const bool any_in = __any(in);
-    const double* mCp = any_in ? a.maskC + om : y;
-    const double *rhp = any_in ? a.rh + o3 : y, *kp = any_in ? a.kap + o3 : y, *wp = any_in ? a.w + o3 : y;
+    // a halo column only needs the tracer (identity rows): its other loads read the tracer's lines again, which
+    // the wave loads anyway (no HBM traffic)
+    const double* mCp = in ? a.maskC + om : y;
+    const double *rhp = in ? a.rh + o3 : y, *kp = in ? a.kap + o3 : y, *wp = in ? a.w + o3 : y;
     double rmC[kPfR], rrh[kPfR], rkap[kPfR], rw[kPfR], ry[kPfR];
     auto load = [&](int s, int lev) {
         const int64_t ok = (int64_t)lev * a.plane;
