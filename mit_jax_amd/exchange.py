@@ -102,6 +102,7 @@ class HaloPlan:
 
     def __init__(self, maps: ExchangeMaps, kind, device):
         L = maps.layout
+        self.maps = maps  # held, so that the id keying the plan cache stays this object's
         names = [kind] if kind in SCALAR_KINDS else [kind + "_u", kind + "_v"]
         arrs = [maps.compact[n] for n in names]
         nm = len(arrs)
@@ -138,7 +139,7 @@ def halo_plan(maps: ExchangeMaps, kind, device):
     key = (id(maps), kind, idx)
     with _plans_lock:
         p = _plans.get(key)
-        if p is None:
+        if p is None or p.maps is not maps:
             p = _plans[key] = HaloPlan(maps, kind, idx)
         return p
 
