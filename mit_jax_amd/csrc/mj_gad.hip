@@ -204,10 +204,11 @@ __device__ __forceinline__ void build_mask_bits(const GadDev& a, const MjTilePla
     __syncthreads();
     fix_mask_corners(a, tp, mW, mS, bw, bs);
 }
-// The tracer plane into LDS and the mask bit planes, with the tracer's and the plain masks' loads in flight
-// together (one memory latency for the three planes of a 1024-thread block), then the corner-square fix-up.
+// The tracer plane into LDS (into loc and loc2) and the mask bit planes, with the tracer's and the plain masks'
+// loads in flight together (one memory latency for the three planes of a 1024-thread block), then the
+// corner-square fix-up.
 __device__ __forceinline__ void load_tile_and_masks(const GadDev& a, const MjTilePlan& tp, const double* tr,
-                                                    const double* mW, const double* mS, double* loc,
+                                                    const double* mW, const double* mS, double* loc, double* loc2,
                                                     unsigned long long* bw, unsigned long long* bs) {
     const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
     constexpr int KP = (GPLANE + 1023) / 1024, KW = (kMaskWords + 15) / 16;  // per thread / per wave at 1024
@@ -228,7 +229,7 @@ __device__ __forceinline__ void load_tile_and_masks(const GadDev& a, const MjTil
 #pragma unroll
         for (int b = 0; b < KP; ++b) {
             const int q = (it * KP + b) * nthr + tid;
-            if (q < GPLANE) loc[q] = v[b];
+            if (q < GPLANE) loc[q] = loc2[q] = v[b];
         }
 #pragma unroll
         for (int r = 0; r < KW; ++r) {
@@ -300,9 +301,7 @@ __global__ void __launch_bounds__(kHzThreads, 4) gad_hz_kernel(GadDev a) {
     const double rhoFacC = a.rhoFacC[k], rrhoFacC = a.recip_rhoFacC[k], drF = a.drF[k], rdrF = a.recip_drF[k];
     const double oneSixth = a.oneSixth;
 
-    load_tile_and_masks(a, tp, tr, mW, mS, loc, mlw, mls);
-    for (int q = tid; q < GPLANE; q += NT) trc[q] = loc[q];
-    __syncthreads();
+    load_tile_and_masks(a, tp, tr, mW, mS, loc, trc, mlw, mls);
 
     for (int pass = 0; pass < 3; ++pass) {
         for (int dir = 0; dir < 2; ++dir) {
