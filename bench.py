@@ -40,8 +40,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 N2 = 13 * 90 * 90
-LEVEL_CHOICES = (15, 30, 50)
 TOL = 1e-12
+LEVEL_CHOICES = (15, 30, 50)  # the member level counts of the ensemble (mit_jax_amd.workload)
 
 
 def parse():
@@ -69,66 +69,6 @@ def parse():
                     help="only run one FP64 and one mixed solve of --members right-hand sides on the shared 5-point "
                          "operator (the workload of the hardware-counter profiles) and exit")
     return ap.parse_args()
-
-
-def member_levels(B, seed=4000):
-    return np.random.default_rng(seed).choice(LEVEL_CHOICES, size=B)
-
-
-def build_ensemble(B, dev):
-    """Per-member inputs on the device: level counts, operators, rhs, flow and tracer (config 4)."""
-    import torch
-    from mit_jax_amd import synthetic as S
-    from mit_jax_amd.exchange import default_exchanger
-
-    ex = default_exchanger()
-    nlev = member_levels(B)
-    cls = {}
-    for n in LEVEL_CHOICES:
-        g, ops, norm = S.cg2d_case(0, "5point", nlev=int(n))
-        u, v, w = S.flow(g, 2)
-        tr = S.tracer(g, 3)
-        cls[int(n)] = (g, ops, norm, u, v, tr, w)
-    T = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
-    idx = torch.as_tensor([LEVEL_CHOICES.index(int(n)) for n in nlev], device=dev)
-
-    def per_member(get):
-        base = torch.stack([T(get(cls[n])) for n in LEVEL_CHOICES])
-        return base.index_select(0, idx).contiguous()
-
-    # members with the same level count share one operator: 3 slots, op_index picks each member's
-    ops = [torch.stack([T(cls[n][1][k]) for n in LEVEL_CHOICES]) for k in range(6)]
-    op_index = idx.to(torch.int32)
-    norm = torch.as_tensor([cls[int(n)][2] for n in nlev], device=dev, dtype=torch.float64)
-    rhs = torch.stack([T(S.rhs(cls[int(n)][0], 100 + m)) for m, n in enumerate(nlev)])
-    x0 = torch.zeros_like(rhs)
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(4000)
-    u = per_member(lambda c: c[3])
-    v = per_member(lambda c: c[4])
-    tr = per_member(lambda c: c[5])
-    mW = per_member(lambda c: c[0].maskW)
-    mS = per_member(lambda c: c[0].maskS)
-    mC = per_member(lambda c: c[0].maskC)
-    hW = per_member(lambda c: c[0].hFacW)
-    hS = per_member(lambda c: c[0].hFacS)
-    rhC = per_member(lambda c: c[0].recip_hFacC)
-    # perturbations: tracer noise 0.05 degC on wet cells, velocity x (1 + 0.05 N(0,1)); halos re-exchanged
-    tr += torch.randn(tr.shape, generator=gen, device=dev, dtype=torch.float64) * 0.05 * mC
-    u *= 1.0 + 0.05 * torch.randn(u.shape, generator=gen, device=dev, dtype=torch.float64)
-    v *= 1.0 + 0.05 * torch.randn(v.shape, generator=gen, device=dev, dtype=torch.float64)
-    for m in range(B):
-        ex.fill_(tr[m], "T")
-        uu, vv = ex.vector(u[m], v[m], "UVs")
-        u[m].copy_(uu)
-        v[m].copy_(vv)
-    g0 = cls[50][0]
-    g2d = {k: T(g0.f[k]) for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "rA", "maskInC")}
-    g2d.update(maskW=mW, maskS=mS)
-    vg = {k: g0.v[k] for k in ("drF", "recip_drF", "rhoFacC", "recip_drC")}
-    return dict(ex=ex, nlev=nlev, nlev_t=torch.as_tensor(nlev.astype(np.int32), device=dev), ops=ops,
-                op_index=op_index, norm=norm, mC=mC, per_member=per_member,
-                rhs=rhs, x0=x0, u=u, v=v, tr=tr, hW=hW, hS=hS, rhC=rhC, g2d=g2d, vg=vg, cls=cls)
 
 
 def cpu_baseline(E, seconds):
@@ -365,6 +305,7 @@ def main():
     dev = torch.device("cuda:0")
     B = args.members
     print(f"bench: building {B} members", file=sys.stderr, flush=True)
+    from mit_jax_amd.workload import build_ensemble
     E = build_ensemble(B, dev)
     params = gad.GADParams.llc90()
     log = lambda m: print("bench: autotune " + m, file=sys.stderr, flush=True)  # noqa: E731

@@ -208,6 +208,17 @@ typedef struct {
     int64_t member_stride2d; /* elements between members' 2-D arrays ([T, ny, nx]) */
 } mj_sfp_config;
 
+/* INI_CG2D's normalisation (core/cg2d.py:108-134 ini_cg2d_norm / _ini_cg2d_norm; ini_cg2d.F:90-135). Per member m,
+ * on the interior points (i, j = 1..sNx, 1..sNy): aW = sum over k = 1..nr in order of
+ * (fac ((dyG drF(k)) hFacW)) recip_dxC and aS likewise with dxG, hFacS, recip_dyC, fac = implicSurfPress *
+ * implicDiv2DFlow; cg2dNorm[m] = 1 / max(|aW|, |aS|) over the member's tiles (1 when that max is 0). The maximum is
+ * order-free, so the result is deterministic. hFacW, hFacS: [B][T, nr, ny, nx] (member_stride3d apart; the
+ * reference passes INITIALISE_FIXED's h0FacW/S); grid: dyG, dxG, recip_dxC, recip_dyC, drF; cg2dNorm: device [B],
+ * also used as scratch during the call. Only cfg's batch/layout/nr fields are read. */
+int mj_ini_cg2d_norm(const mj_sfp_grid* grid, const mj_sfp_config* cfg, double implicSurfPress,
+                     double implicDiv2DFlow, const double* hFacW, const double* hFacS, double* cg2dNorm,
+                     void* stream);
+
 /* UPDATE_CG2D (core/solve_for_pressure.py:29-65). Per member m: aW2d = ((sum_k dyG drF(k) hFacW recip_dxC) *
  * cg2dNorm[m]) * implicSurfPress * implicDiv2DFlow on j, i = 1..sNy+1, 1..sNx+1 and 0 elsewhere (aS2d likewise);
  * aC2d = -(aW(i) + aW(i+1) + aS(j) + aS(j+1) + freeSurfFac cg2dNorm[m] recip_Bo rA / deltaTMom / deltaTFreeSurf) on
@@ -300,6 +311,30 @@ typedef struct {
 int mj_gad_dst3(const mj_gad_plan* plan, const mj_gad_grid* grid, const mj_gad_config* cfg, const double* uFld,
                 const double* vFld, const double* wFld, const double* tracer, const double* hFacW,
                 const double* hFacS, const double* recip_hFacC, double* gTracer, double* work, void* stream);
+
+/* ---- the DST3 interface fluxes alone, for every (member, tile, level) plane (the reference's flux kernels; the
+ * fused mj_gad_dst3 evaluates the same values inside its sweeps). */
+typedef struct {
+    int batch, ntiles, nr;   /* members, tiles, levels of the 3-D arrays */
+    int sNx, sNy, OLx, OLy;  /* tile interior and halos (OLx, OLy >= 2): ny = sNy + 2 OLy, nx = sNx + 2 OLx (even) */
+    int64_t member_stride3d; /* elements between members' 3-D arrays (even, >= ntiles * nr * ny * nx) */
+} mj_dst3_flux_config;
+
+/* Replaces dst3_adv_x (pkgs/gad.py:409-430; gad_dst3_adv_x.F:74-121 with calcCFL): on every row and at array
+ * columns i = 2 .. nx-2, with t = tracer, m = maskLocW, U = uTrans,
+ *   Rjp = (t(i+1) - t(i)) m(i+1), Rj = (t(i) - t(i-1)) m(i), Rjm = (t(i-1) - t(i-2)) m(i-1),
+ *   c = |uFld deltaTloc(k) recip_dxC recip_deepFacC(k)|, d0 = (2 - c)(1 - c) oneSixth, d1 = (1 - c c) oneSixth,
+ *   uT = 0.5 (U + |U|)(t(i-1) + (d0 Rj + d1 Rjm)) + 0.5 (U - |U|)(t(i) - (d0 Rj + d1 Rjp)),
+ * and uT = 0 at every other point. uTrans, uFld, maskLocW, tracer, uT: [B][T, nr, ny, nx] (member_stride3d apart);
+ * recip_dxC: [T, ny, nx] shared; deltaTloc, recip_deepFacC: [nr]. All 16-byte aligned; uT aliases no input. */
+int mj_dst3_adv_x(const mj_dst3_flux_config* cfg, double oneSixth, const double* deltaTloc, const double* uTrans,
+                  const double* uFld, const double* maskLocW, const double* tracer, const double* recip_dxC,
+                  const double* recip_deepFacC, double* uT, void* stream);
+/* Replaces dst3_adv_y (pkgs/gad.py:433-453; gad_dst3_adv_y.F:73-120): the same in j (rows j = 2 .. ny-2, every
+ * column) with vTrans, vFld, maskLocS, recip_dyC. */
+int mj_dst3_adv_y(const mj_dst3_flux_config* cfg, double oneSixth, const double* deltaTloc, const double* vTrans,
+                  const double* vFld, const double* maskLocS, const double* tracer, const double* recip_dyC,
+                  const double* recip_deepFacC, double* vT, void* stream);
 
 /* Reverse mode of mj_gad_dst3 without the R flux (wFld NULL; the reference's advect_tiles, pkgs/gad.py:460-539,
  * differentiated as test_gad.py:297-346 does): for the cotangent gTracer_bar of gTracer, writes the cotangents of

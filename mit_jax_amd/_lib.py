@@ -61,6 +61,11 @@ class GadConfig(ctypes.Structure):
     _fields_ = [("batch", I32), ("nr", I32), ("member_stride3d", I64), ("mask_member_stride", I64), ("nlev", P)]
 
 
+class Dst3FluxConfig(ctypes.Structure):
+    _fields_ = [("batch", I32), ("ntiles", I32), ("nr", I32), ("sNx", I32), ("sNy", I32), ("OLx", I32), ("OLy", I32),
+                ("member_stride3d", I64)]
+
+
 class FsParams(ctypes.Structure):
     _fields_ = [(k, F64) for k in ("deltaTMom", "deltaTFreeSurf", "freeSurfFac", "implicSurfPress", "implicDiv2DFlow",
                                    "mass2rUnit")] + [("useRealFreshWaterFlux", I32)]
@@ -105,6 +110,7 @@ SIGNATURES = {
                                                         I64]),
     "mj_cg2d_solve_mixed": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, P, ctypes.POINTER(Cg2dConfig),
                                   ctypes.POINTER(Cg2dMixedConfig), P, ctypes.c_size_t, P, P]),
+    "mj_ini_cg2d_norm": (I32, [ctypes.POINTER(SfpGrid), ctypes.POINTER(SfpConfig), F64, F64, P, P, P, P]),
     "mj_update_cg2d": (I32, [P, ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig), P, F64,
                              P, P, P, P, P, P, P, P, P, P]),
     "mj_update_cg2d_adjoint": (I32, [P, ctypes.POINTER(SfpGrid), ctypes.POINTER(FsParams), ctypes.POINTER(SfpConfig),
@@ -123,6 +129,8 @@ SIGNATURES = {
     "mj_gad_dst3_adjoint_workspace_bytes": (ctypes.c_size_t, []),
     "mj_gad_dst3_adjoint": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P,
                                   P, P, ctypes.c_size_t, P]),
+    "mj_dst3_adv_x": (I32, [ctypes.POINTER(Dst3FluxConfig), F64, P, P, P, P, P, P, P, P, P]),
+    "mj_dst3_adv_y": (I32, [ctypes.POINTER(Dst3FluxConfig), F64, P, P, P, P, P, P, P, P, P]),
     "mj_gad_dst3": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P, P]),
 }
 

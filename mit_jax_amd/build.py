@@ -13,7 +13,8 @@ import tempfile
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SOURCES = ["mj_halo.hip", "mj_cg2d.hip", "mj_gad.hip", "mj_adjoint.hip", "mj_mixed.hip", "mj_sfp.hip", "mj_implicit_r.hip"]
+SOURCES = ["mj_halo.hip", "mj_cg2d.hip", "mj_gad.hip", "mj_adjoint.hip", "mj_mixed.hip", "mj_sfp.hip", "mj_implicit_r.hip",
+           "mj_flux.hip"]
 OUT = HERE / "libmjamd.so"
 ARCH = os.environ.get("MJ_OFFLOAD_ARCH", "gfx950")
 COMPILE_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",

@@ -189,6 +189,12 @@ def matvec(ex, A, v, cg2dNorm, op_index=None):
     return y if batched else y[0]
 
 
+def ini_cg2d_norm(g, hFacW, hFacS, implicSurfPress, implicDiv2DFlow):
+    """cg2dNorm of INI_CG2D (reference core/cg2d.py:108-134) on the GPU; see solve_for_pressure.ini_cg2d_norm."""
+    from .solve_for_pressure import ini_cg2d_norm as _norm
+    return _norm(g, hFacW, hFacS, implicSurfPress, implicDiv2DFlow)
+
+
 def cg2d_solve(p: Cg2dParams, ex, ops, cg2d_b, cg2d_x):
     """Solve A x = b with the literal CG2D forward; differentiable in cg2d_b and aW2d/aS2d/aC2d through the
     implicit derivative (reference core/cg2d.py:351-375). Returns (x with 0 halos, diag)."""

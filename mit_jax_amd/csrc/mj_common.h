@@ -34,6 +34,20 @@ int fail_arg(const char* fmt, ...);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// ---------------------------------------------------------------------------------------------------- DST3
+// DST3 interface value (gad_dst3_adv_x.F:81-117 via gad.py:420-428): t_m2, t_m1 upstream side for positive
+// transport, t_0, t_p1 downstream; m_p1, m_0, m_m1 the face masks of Rjp, Rj, Rjm.
+__device__ __forceinline__ double dst3_flux(double tr_, double vel_cfl_abs, double oneSixth, double t_m2, double t_m1,
+                                            double t_0, double t_p1, double m_m1, double m_0, double m_p1) {
+    const double Rjp = (t_p1 - t_0) * m_p1;
+    const double Rj = (t_0 - t_m1) * m_0;
+    const double Rjm = (t_m1 - t_m2) * m_m1;
+    const double cfl = vel_cfl_abs;
+    const double d0 = (2. - cfl) * (1. - cfl) * oneSixth;
+    const double d1 = (1. - cfl * cfl) * oneSixth;
+    return 0.5 * (tr_ + fabs(tr_)) * (t_m1 + (d0 * Rj + d1 * Rjm)) + 0.5 * (tr_ - fabs(tr_)) * (t_0 - (d0 * Rj + d1 * Rjp));
+}
+
 // ---------------------------------------------------------------------------------------------------- reductions
 // The fixed reduction tree of the "tile" sum order (documented in mjamd.h): n values (64 <= n <= 128) are folded
 // onto the 64 lanes of a wavefront, lane l holding v[l] + v[l+64] (v[l] alone when l+64 >= n), then a 6-step

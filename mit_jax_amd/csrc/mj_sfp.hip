@@ -359,6 +359,52 @@ __global__ void __launch_bounds__(kPairThreads)
     }
 }
 
+// INI_CG2D's cg2dNorm (reference core/cg2d.py:108-134, ini_cg2d.F:90-135). Per interior point of a member:
+// aW = sum over k = 1..nr in order of (fac ((dyG drF(k)) hFacW)) recip_dxC (aS likewise), fac = implicSurfPress
+// implicDiv2DFlow; the member's max of |aW|, |aS| is kept as the bits of a nonnegative double in `bits` (the
+// unsigned-integer order of nonnegative doubles is their numeric order, so the atomic maxima are order-free and
+// the result is deterministic); ini_norm_finish turns it into 1 / max (1 when the max is 0), in place.
+__global__ void __launch_bounds__(256) ini_norm_zero_kernel(unsigned long long* bits, int B) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < B) bits[m] = 0ull;
+}
+
+__global__ void __launch_bounds__(256) ini_norm_max_kernel(SfpGeo g, const double* __restrict__ dyG,
+                                                           const double* __restrict__ dxG,
+                                                           const double* __restrict__ recip_dxC,
+                                                           const double* __restrict__ recip_dyC,
+                                                           const double* __restrict__ drF,
+                                                           const double* __restrict__ hFacW,
+                                                           const double* __restrict__ hFacS, double fac,
+                                                           unsigned long long* bits) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y, m = blockIdx.z;
+    double mx = 0.0;
+    if (e < g.sNx * g.sNy) {
+        const int j = e / g.sNx + g.OL, i = e % g.sNx + g.OL;
+        const int64_t q = (int64_t)t * g.plane + (int64_t)j * g.nx + i;
+        const double cW = dyG[q], cS = dxG[q], rW = recip_dxC[q], rS = recip_dyC[q];
+        const int64_t o3 = m * g.ms3 + (int64_t)t * g.nr * g.plane + (int64_t)j * g.nx + i;
+        double aW = 0.0, aS = 0.0;
+        for (int k = 0; k < g.nr; ++k) {
+            const double dz = drF[k];
+            const int64_t ok = o3 + (int64_t)k * g.plane;
+            aW = aW + (fac * ((cW * dz) * hFacW[ok])) * rW;
+            aS = aS + (fac * ((cS * dz) * hFacS[ok])) * rS;
+        }
+        mx = fmax(fabs(aW), fabs(aS));
+    }
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&bits[m], (unsigned long long)__double_as_longlong(mx));
+}
+
+__global__ void __launch_bounds__(256) ini_norm_finish_kernel(unsigned long long* bits, int B) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= B) return;
+    const double v = __longlong_as_double((long long)bits[m]);
+    reinterpret_cast<double*>(bits)[m] = v != 0.0 ? 1.0 / v : 1.0;
+}
+
 int check_sfp(const mj_sfp_grid* gr, const mj_fs_params* p, const mj_sfp_config* c, SfpGeo* g, const char* who) {
     MJ_REQUIRE(gr && p && c, "%s: null grid/params/config", who);
     MJ_REQUIRE(c->batch >= 1 && c->batch <= 65535 && c->ntiles >= 1 && c->ntiles <= 65535 && c->nr >= 1,
@@ -398,6 +444,31 @@ dim3 pair_grid(const SfpGeo& g) {
 }  // namespace mj
 
 using namespace mj;
+
+extern "C" int mj_ini_cg2d_norm(const mj_sfp_grid* grid, const mj_sfp_config* cfg, double implicSurfPress,
+                                double implicDiv2DFlow, const double* hFacW, const double* hFacS, double* cg2dNorm,
+                                void* stream) {
+    SfpGeo g;
+    const mj_fs_params unit{1.0, 1.0, 1.0, implicSurfPress, implicDiv2DFlow, 1.0, 0};  // only the checks read it
+    int rc = check_sfp(grid, &unit, cfg, &g, "mj_ini_cg2d_norm");
+    if (rc) return rc;
+    MJ_REQUIRE(grid->recip_dxC && grid->recip_dyC, "mj_ini_cg2d_norm: null grid field");
+    MJ_REQUIRE(hFacW && hFacS && cg2dNorm, "mj_ini_cg2d_norm: null array");
+    MJ_REQUIRE(cg2dNorm != hFacW && cg2dNorm != hFacS, "mj_ini_cg2d_norm: cg2dNorm aliases an input");
+    MJ_REQUIRE(((uintptr_t)cg2dNorm & 7) == 0, "mj_ini_cg2d_norm: cg2dNorm must be 8-byte aligned");
+    hipStream_t st = as_stream(stream);
+    unsigned long long* bits = reinterpret_cast<unsigned long long*>(cg2dNorm);
+    const unsigned nb = (unsigned)((g.B + 255) / 256);
+    hipLaunchKernelGGL(ini_norm_zero_kernel, dim3(nb), dim3(256), 0, st, bits, g.B);
+    MJ_HIP(hipGetLastError());
+    const dim3 grd((unsigned)((g.sNx * g.sNy + 255) / 256), (unsigned)g.T, (unsigned)g.B);
+    hipLaunchKernelGGL(ini_norm_max_kernel, grd, dim3(256), 0, st, g, grid->dyG, grid->dxG, grid->recip_dxC,
+                       grid->recip_dyC, grid->drF, hFacW, hFacS, implicSurfPress * implicDiv2DFlow, bits);
+    MJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ini_norm_finish_kernel, dim3(nb), dim3(256), 0, st, bits, g.B);
+    MJ_HIP(hipGetLastError());
+    return MJ_OK;
+}
 
 extern "C" int mj_update_cg2d(const mj_halo_plan* plan_T, const mj_sfp_grid* grid, const mj_fs_params* p,
                               const mj_sfp_config* cfg, const double* cg2dNorm, double cg2dpcOffDFac,

@@ -7,8 +7,8 @@ Python-side allocation, validation or host read-back besides the solver's own ac
 records one step into a `torch.cuda.CUDAGraph` (a hipGraph on ROCm) with the solve enqueued for a fixed number of
 iterations (`MJ_CG_LAUNCH_PHASES_FIXED`: no read-back; members that finish early exit their blocks at once), so a
 replay is one graph launch per part (solve, DST3) from the host. The capture cap comes from an eager step
-(`cap_from`); a member that hits it finishes with converged = 0 in `diag()`, and the caller can fall back to an
-eager step.
+(`cap_from`); after a replay `step()` reads the convergence flags and runs the step again eagerly when a member
+stopped at the cap unconverged.
 
 `autotune()` times candidate configurations (launch shape and block size of the solve, inner tolerance of the
 mixed solve, polling chunk, eager vs graph) on the actual inputs and returns the fastest; results are deterministic
@@ -74,8 +74,10 @@ class EnsembleStep:
         plan, grid_s, gcfg, trc, fields, _, keep = gad._prepare(params, None, g2d, vg, u, v, tr, hFacW, hFacS,
                                                                  recip_hFacC, None, nlev)
         self._gad = (plan, grid_s, gcfg, trc, fields, keep)
-        self.gT = torch.empty_like(trc)
+        # levels at or beyond a member's level count are never written by the DST3 launch: they stay 0
+        self.gT = torch.zeros_like(trc)
         self.graph = None
+        self.fallbacks = 0  # graph replays re-run eagerly because a member hit the captured iteration cap
         self.graph_cfg = None
         self.set_config(cfg)
 
@@ -146,9 +148,14 @@ class EnsembleStep:
                 e1.record()
             self._advect()
 
-    def step(self, events=None):
+    def step(self, events=None, verify=True):
         """One step: the graph if one is captured and selected, else eager launches. events: optional list that
-        receives (start, after-solve, end) HIP events (with overlap, after-solve marks the solve's own stream)."""
+        receives (start, after-solve, end) HIP events (with overlap, after-solve marks the solve's own stream).
+
+        A captured solve runs for a fixed number of iterations (`capture`). With verify (the default) a replay is
+        followed by a read of the members' convergence flags, and when a member stopped at that cap unconverged
+        the step runs again eagerly (same inputs, so the eager step's result; counted in `fallbacks`). Without
+        verify the caller reads `diag()` itself."""
         e1 = None
         if events is not None:
             e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
@@ -163,6 +170,9 @@ class EnsembleStep:
                 if e1 is not None:
                     e1.record()
                 self.graph[1].replay()
+            if verify and not bool((self.diag()["converged"] == 1).all()):
+                self.fallbacks += 1
+                self._both()
         else:
             self._both(e1)
         if events is not None:

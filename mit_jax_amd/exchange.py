@@ -176,7 +176,16 @@ class Exchanger:
         p = self.plan(kind, a.device)
         if p.vector:
             raise ValueError(f"{kind} is a vector kind")
-        out = a.clone() if out is None else out
+        if out is None:
+            out = a.clone()
+        elif out is not a:
+            # the exchange writes only the halo points it fills: the rest of out must hold a's values
+            if not isinstance(out, torch.Tensor) or out.device != a.device or out.dtype != a.dtype or \
+                    tuple(out.shape) != tuple(a.shape) or not out.is_contiguous():
+                raise ValueError(f"scalar({kind}): out must be a contiguous tensor like a")
+            out.copy_(a)
+        elif not p.inplace_ok:
+            raise ValueError(f"scalar({kind}): in place (out=a) is not exact for this kind")
         g = _geom(a, self.L)
         _lib.check(_lib.lib().mj_halo_exchange(p.handle, _lib.ptr(a), _lib.ptr(out), ctypes.byref(g),
                                                _lib.stream_ptr()), "mj_halo_exchange")

@@ -10,8 +10,8 @@ gTracer = (localTij - tracer)/deltaTLev(k) on the whole tile array. Fields are C
 `implicitAdvection=True` is the reference branch (vertical advection left to the implicit solver, wFld not read).
 `implicitAdvection=False` adds the vertical DST3 flux (R) of wFld after the horizontal passes, a formulation this
 library defines (the reference raises NotImplementedError for it, gad.py:164-166); see mj_gad.hip.
-`dst3_adv_x` / `dst3_adv_y` (gad.py:409-453) are not exposed separately: the fluxes exist only inside the fused
-kernel.
+`dst3_adv_x` / `dst3_adv_y` (gad.py:409-453) are the interface fluxes alone (libmjamd mj_dst3_adv_x / _y), for
+callers that assemble their own update; the fused kernel evaluates the same values inside its sweeps.
 """
 
 import ctypes
@@ -231,6 +231,57 @@ def _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, re
     return plan, grid_s, cfg, tr, fields, w, keep
 
 
+def _dst3_adv(direction, L, oneSixth, deltaTloc, trans, vel, maskLoc, tracer, rd, recip_deepFacC):
+    if not isinstance(tracer, torch.Tensor) or tracer.device.type != "cuda":
+        raise TypeError("tracer: expected a CUDA tensor (the flux kernels run on the GPU only)")
+    dev = tracer.device
+    tr = tracer.to(torch.float64).contiguous()
+    tr5 = tr if tr.dim() == 5 else tr.unsqueeze(0)
+    if tr5.dim() != 5 or tuple(tr5.shape[1:2]) + tuple(tr5.shape[3:]) != L.shape2d:
+        raise ValueError(f"tracer: expected [T, K, ny, nx] or [B, T, K, ny, nx] of the layout, got {tuple(tr.shape)}")
+    B, T, K, ny, nx = tr5.shape
+    f = [_dev(a, dev, n) for a, n in ((trans, "trans"), (vel, "vel"), (maskLoc, "mask"))]
+    for a, n in zip(f, ("trans", "vel", "mask")):
+        if tuple(a.shape) != tuple(tr.shape):
+            raise ValueError(f"{n}: shape {tuple(a.shape)} != tracer {tuple(tr.shape)}")
+    r = _dev(rd, dev, "recip_dxC/recip_dyC").reshape(-1)
+    if r.numel() != T * ny * nx:
+        raise ValueError("recip_dxC/recip_dyC: expected [T, ny, nx] (or [T, 1, ny, nx])")
+    dt = _dev(np.broadcast_to(np.asarray(deltaTloc, np.float64).reshape(-1), (K,)).copy()
+              if not isinstance(deltaTloc, torch.Tensor) else deltaTloc.reshape(-1).expand(K), dev, "deltaTloc")
+    rdp = _dev(np.broadcast_to(np.asarray(recip_deepFacC, np.float64).reshape(-1), (K,)).copy()
+               if not isinstance(recip_deepFacC, torch.Tensor) else recip_deepFacC.reshape(-1).expand(K), dev,
+               "recip_deepFacC")
+    out = torch.empty_like(tr)
+    cfg = _lib.Dst3FluxConfig(B, T, K, L.sNx, L.sNy, L.OLx, L.OLy, T * K * ny * nx)
+    fn = _lib.lib().mj_dst3_adv_x if direction == 0 else _lib.lib().mj_dst3_adv_y
+    _lib.check(fn(ctypes.byref(cfg), float(oneSixth), _lib.ptr(dt), *[_lib.ptr(a) for a in f], _lib.ptr(tr),
+                  _lib.ptr(r), _lib.ptr(rdp), _lib.ptr(out), _lib.stream_ptr()),
+               "mj_dst3_adv_x" if direction == 0 else "mj_dst3_adv_y")
+    return out
+
+
+def dst3_adv_x(L, oneSixth, deltaTloc, uTrans, uFld, maskLocW, tracer, recip_dxC, recip_deepFacC):
+    """GAD_DST3_ADV_X with calcCFL for all tiles and levels (reference gad.py:409-430) -> the interface flux uT,
+    0 outside i = 2 .. nx-2. deltaTloc, recip_deepFacC: K values ([1, K, 1, 1] as the reference passes them, or
+    [K]); recip_dxC: [T, ny, nx] or [T, 1, ny, nx]; the rest [T, K, ny, nx] (or [B, T, K, ny, nx]), CUDA float64."""
+    return _dst3_adv(0, L, oneSixth, deltaTloc, uTrans, uFld, maskLocW, tracer, recip_dxC, recip_deepFacC)
+
+
+def dst3_adv_y(L, oneSixth, deltaTloc, vTrans, vFld, maskLocS, tracer, recip_dyC, recip_deepFacC):
+    """GAD_DST3_ADV_Y (reference gad.py:433-453) -> vT, 0 outside j = 2 .. ny-2; arguments as dst3_adv_x."""
+    return _dst3_adv(1, L, oneSixth, deltaTloc, vTrans, vFld, maskLocS, tracer, recip_dyC, recip_deepFacC)
+
+
+def _check_out(out, like, name):
+    """A caller-supplied output: a contiguous CUDA float64 tensor of the tracer's shape on its device."""
+    if not isinstance(out, torch.Tensor) or out.device != like.device or out.dtype != torch.float64:
+        raise TypeError(f"{name}: expected a CUDA float64 tensor on {like.device}")
+    if tuple(out.shape) != tuple(like.shape) or not out.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous tensor of shape {tuple(like.shape)}")
+    return out
+
+
 def advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, wFld=None,
                  nlev=None, out=None):
     """GAD_ADVECTION on all tiles (reference gad.py:460-539) -> gTracer, same shape as tracer.
@@ -240,7 +291,11 @@ def advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS
     vgrid: drF, recip_drF, rhoFacC [K] (+ recip_drC with wFld). nlev: per-member level counts (ensemble)."""
     plan, grid_s, cfg, tr, fields, w, keep = _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW,
                                                       hFacS, recip_hFacC, wFld, nlev)
-    gT = torch.empty_like(tr) if out is None else out
+    if out is None:
+        # with level counts the levels at or beyond a member's count are not written: they read 0
+        gT = torch.zeros_like(tr) if nlev is not None else torch.empty_like(tr)
+    else:
+        gT = _check_out(out, tr, "out")
     work = torch.empty_like(tr) if w is not None else None
     u, v, hW, hS, rhC = fields
     _lib.check(_lib.lib().mj_gad_dst3(plan.handle, ctypes.byref(grid_s), ctypes.byref(cfg), _lib.ptr(u), _lib.ptr(v),

@@ -133,6 +133,28 @@ def _check_layout(L):
         raise ValueError("the library's tile layout has OLx == OLy")
 
 
+def ini_cg2d_norm(g, hFacW, hFacS, implicSurfPress, implicDiv2DFlow):
+    """cg2dNorm = 1/max|aW2d, aS2d| of the initial operator (reference core/cg2d.py:108-134) on the GPU
+    (libmjamd mj_ini_cg2d_norm). hFacW, hFacS: CUDA float64 [T, K, ny, nx] (a float is returned, as the reference
+    does) or [B, T, K, ny, nx] (a CUDA tensor of B norms)."""
+    L = _layout(g)
+    _check_layout(L)
+    batched = hFacW.dim() == 5
+    hW, hS = _batch3(hFacW, L, "hFacW"), _batch3(hFacS, L, "hFacS")
+    if hW.shape != hS.shape:
+        raise ValueError("hFacW and hFacS differ in shape")
+    B, K, dev = hW.shape[0], hW.shape[2], hW.device
+    gd = _grid(g, dev)
+    if gd.keep["drF"].numel() < K:
+        raise ValueError(f"grid.drF has {gd.keep['drF'].numel()} levels, the hFac arrays {K}")
+    norm = torch.empty(B, dtype=torch.float64, device=dev)
+    cfg = _config(L, B, K)
+    _lib.check(_lib.lib().mj_ini_cg2d_norm(ctypes.byref(gd.s), ctypes.byref(cfg), float(implicSurfPress),
+                                           float(implicDiv2DFlow), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(norm),
+                                           _lib.stream_ptr()), "mj_ini_cg2d_norm")
+    return norm if batched else float(norm[0])
+
+
 def update_cg2d(p, cp, g, ex, hFacW, hFacS, recip_Bo, pW, pS, pC):
     """UPDATE_CG2D (reference core/solve_for_pressure.py:29-65). cp: Cg2dParams (cg2dNorm: a scalar or one value
     per member; cg2dpcOffDFac). Returns (aW2d, aS2d, aC2d, pW, pS, pC), batched like hFacW."""

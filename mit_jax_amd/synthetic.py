@@ -191,15 +191,18 @@ def make_grid(seed=0, nlev=None, L=LLC90, land_frac=0.30, hFacMin=0.2, hFacMinDr
 
 
 # ------------------------------------------------------------------------------------------------ CG2D inputs
-def ini_cg2d_norm(g):
-    """cg2dNorm = 1/max|aW, aS| of the interior operator (core/cg2d.py:108-134), sums over k in order."""
+def ini_cg2d_norm(g, implicSurfPress=1.0, implicDiv2DFlow=1.0):
+    """cg2dNorm = 1/max|aW, aS| of the interior operator (core/cg2d.py:108-134) for the synthetic inputs: the
+    per-level terms (fac ((dyG drF) hFacW)) recip_dxC summed over k in order, fac = implicSurfPress implicDiv2DFlow.
+    (The GPU op is cg2d.ini_cg2d_norm.)"""
     L = g.L
     J, I = L.js(1, L.sNy), L.is_(1, L.sNx)
+    fac = implicSurfPress * implicDiv2DFlow
     aW = np.zeros((L.nTiles, L.sNy, L.sNx))
     aS = np.zeros_like(aW)
     for k in range(L.Nr):
-        aW = aW + (g.dyG[:, J, I] * g.drF[k] * g.hFacW[:, k, J, I]) * g.recip_dxC[:, J, I]
-        aS = aS + (g.dxG[:, J, I] * g.drF[k] * g.hFacS[:, k, J, I]) * g.recip_dyC[:, J, I]
+        aW = aW + (fac * ((g.dyG[:, J, I] * g.drF[k]) * g.hFacW[:, k, J, I])) * g.recip_dxC[:, J, I]
+        aS = aS + (fac * ((g.dxG[:, J, I] * g.drF[k]) * g.hFacS[:, k, J, I])) * g.recip_dyC[:, J, I]
     m = max(np.abs(aW).max(), np.abs(aS).max())
     return 1.0 / m if m != 0.0 else 1.0
 

@@ -45,6 +45,7 @@ def lib():
         L.mjo_cg2d.restype = I
         L.mjo_cg2d_matvec.argtypes = [P, P, P, I, I, I, I, P, P, P, P, P, D]
         L.mjo_advect_tiles.argtypes = [I, I, I, I, P, D] + [P] * 8 + [P, P, P, P] + [P] * 6 + [P, I]
+        L.mjo_dst3_adv.argtypes = [I, I, I, I, I, D] + [P] * 8
         L.mjo_vert_dst3.argtypes = [I, I, I, I, P, P, D, P, P, P, P, P, P, P, P]
         _lib = L
     return _lib
@@ -165,6 +166,19 @@ def advect_tiles(tab, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFa
                            ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "maskInC", "maskW", "maskS")],
                            _p(masks), ctypes.cast(farr, ctypes.c_void_p), _p(uvu), _p(uvv),
                            *[_p(a) for a in fields], _p(out), int(bool(want_local)))
+    return out
+
+
+def dst3_adv(direction, trans, vel, mask, tracer, rd, deltaTloc, rdeep, oneSixth=1.0 / 6.0):
+    """dst3_adv_x (direction "x") / dst3_adv_y ("y") (pkgs/gad.py:409-453) over [T, K, ny, nx] arrays; rd = recip_dxC /
+    recip_dyC [T, ny, nx]; deltaTloc, rdeep = recip_deepFacC [K]. Returns the flux array."""
+    tracer = _f64(tracer)
+    T, K, ny, nx = tracer.shape
+    args = [_f64(a) for a in (trans, vel, mask)] + [tracer, _f64(rd)]
+    dt, rdp = _f64(np.broadcast_to(deltaTloc, (K,))), _f64(np.broadcast_to(rdeep, (K,)))
+    out = np.empty_like(tracer)
+    lib().mjo_dst3_adv(0 if direction == "x" else 1, T, K, ny, nx, float(oneSixth), _p(dt), *[_p(a) for a in args],
+                       _p(rdp), _p(out))
     return out
 
 

@@ -274,6 +274,25 @@ static void dst3_y(int ny, int nx, double oneSixth, double dt, double rdeep, con
         }
 }
 
+/* dst3_adv_x / dst3_adv_y (pkgs/gad.py:409-453) on [T, K, ny, nx] arrays: per (tile, level) plane the interface
+ * fluxes of dst3_x / dst3_y with deltaTloc(k) and recip_deepFacC(k); rd = recip_dxC / recip_dyC [T, ny, nx]. */
+EXPORT void mjo_dst3_adv(int dir, int T, int K, int ny, int nx, double oneSixth, const double* deltaTloc,
+                         const double* trans, const double* vel, const double* mask, const double* tracer,
+                         const double* rd, const double* rdeep, double* out) {
+    const int64_t plane = (int64_t)ny * nx;
+#pragma omp parallel for schedule(static)
+    for (int tk = 0; tk < T * K; ++tk) {
+        const int t = tk / K, k = tk % K;
+        const int64_t o = (int64_t)tk * plane;
+        if (dir == 0)
+            dst3_x(ny, nx, oneSixth, deltaTloc[k], rdeep[k], trans + o, vel + o, mask + o, tracer + o,
+                   rd + (int64_t)t * plane, out + o);
+        else
+            dst3_y(ny, nx, oneSixth, deltaTloc[k], rdeep[k], trans + o, vel + o, mask + o, tracer + o,
+                   rd + (int64_t)t * plane, out + o);
+    }
+}
+
 static void gather_plane(double* a, const int32_t* src, int plane, double* tmp) {
     for (int q = 0; q < plane; ++q) tmp[q] = a[src[q]];
     memcpy(a, tmp, sizeof(double) * plane);

@@ -18,6 +18,22 @@ def _sl(lo, hi, OL):
     return slice(lo - 1 + OL, hi + OL)
 
 
+def ini_cg2d_norm(g, hFacW, hFacS, implicSurfPress, implicDiv2DFlow, sNx=90, sNy=90, OL=4):
+    """INI_CG2D's cg2dNorm (reference core/cg2d.py:108-134): on the interior, aW = sum_k (fac ((dyG drF) hFacW))
+    recip_dxC, k = 1..Nr in order from 0 (:123-128), aS likewise, fac = implicSurfPress implicDiv2DFlow (:124);
+    1 / max(|aW|, |aS|), 1 when the max is 0 (:129-134)."""
+    K = hFacW.shape[1]
+    J, I = _sl(1, sNy, OL), _sl(1, sNx, OL)
+    fac = implicSurfPress * implicDiv2DFlow
+    aW = np.zeros_like(hFacW[:, 0, J, I])
+    aS = np.zeros_like(aW)
+    for k in range(K):
+        aW = aW + (fac * ((g.dyG[:, J, I] * g.drF[k]) * hFacW[:, k, J, I])) * g.recip_dxC[:, J, I]
+        aS = aS + (fac * ((g.dxG[:, J, I] * g.drF[k]) * hFacS[:, k, J, I])) * g.recip_dyC[:, J, I]
+    m = max(float(np.abs(aW).max()), float(np.abs(aS).max()))
+    return 1.0 / m if m != 0.0 else 1.0
+
+
 def update_cg2d(p, cg2dNorm, cg2dpcOffDFac, g, Tmap, hFacW, hFacS, recip_Bo, pW, pS, pC, sNx=90, sNy=90, OL=4):
     """UPDATE_CG2D (reference core/solve_for_pressure.py:29-65). p: deltaTMom, deltaTFreeSurf, freeSurfFac,
     implicSurfPress, implicDiv2DFlow. Returns (aW2d, aS2d, aC2d, pW, pS, pC), new arrays."""
