@@ -56,9 +56,9 @@ def _write_dump(d, fields, its=(0, 72)):
     """fields: {(stage, field): array [T, (nz,) ny, nx]} written for every iteration (values + it)."""
     for it in its:
         for t in range(LLC90.nTiles):
-            with open(D.dump_name(d, it, t + 1), "wb") as fh:
+            with open(D.file_name(d, it, t + 1), "wb") as fh:
                 for seq, ((stage, field), a) in enumerate(fields.items()):
-                    D.write_record(fh, it, seq, stage, field, "RL", t + 1, FACE[t], 90 * (t % 3), 0, a[t] + it)
+                    D.append_record(fh, it, seq, stage, field, "RL", t + 1, FACE[t], 90 * (t % 3), 0, a[t] + it)
 
 
 def test_dump_round_trip_and_index(tmp_path):
@@ -67,33 +67,57 @@ def test_dump_round_trip_and_index(tmp_path):
     fields = {("S05_dynamics", "gU"): rng.normal(size=(T, K, ny, nx)), ("S05_dynamics", "etaN"):
               rng.normal(size=(T, ny, nx)), ("C01_cg2d_inputs", "cg2d_b"): rng.normal(size=(T, ny, nx))}
     _write_dump(tmp_path, fields)
-    ds = D.DumpSet(tmp_path, threads=4)
+    ds = D.DumpIndex(tmp_path)
     assert ds.keys() == [(it, st, f) for it in (0, 72) for (st, f) in fields]
+    assert len(ds.table) == 2 * T * len(fields)
     for it in (0, 72):
         for (st, f), a in fields.items():
             got = ds.padded(it, st, f, LLC90)
             assert np.array_equal(got, a + it), (it, st, f)
-            r = ds.tiles(it, st, f)[8]  # exch2 tile 8 = array tile 7, facet 4
-            assert (r.face, r.tbx, r.olx, r.oly, r.kind) == (4, 90, 4, 4, "RL")
-            assert np.array_equal(r.interior, (a[7] + it).reshape(r.shape)[:, 4:94, 4:94])
-    # lazy records read their values on first use; drop forgets them
-    r = ds.tiles(0, "S05_dynamics", "gU")[1]
-    r.drop()
-    assert r._data is None and np.array_equal(r.data, fields[("S05_dynamics", "gU")][0])
+            assert ds.tiles(it, st, f) == list(range(1, T + 1))
+            h = ds.header(it, st, f, 8)  # exch2 tile 8 = array tile 7, facet 4
+            assert (int(h["face"]), int(h["tbx"]), int(h["olx"]), int(h["oly"]), str(h["kind"])) == (4, 90, 4, 4, "RL")
+            v = ds.values(it, st, f, 8)
+            assert v.dtype == np.float64 and np.array_equal(v, (a[7] + it).reshape(v.shape))
+    # the call order is the header's seq, not the file order of the records
+    d2 = tmp_path / "reordered"
+    d2.mkdir()
+    with open(D.file_name(d2, 5, 1), "wb") as fh:
+        D.append_record(fh, 5, 2, "B", "x", "RL", 1, 1, 0, 0, np.zeros((ny, nx)))
+        D.append_record(fh, 5, 1, "A", "y", "RL", 1, 1, 0, 0, np.ones((ny, nx)))
+    assert D.DumpIndex(d2).keys() == [(5, "A", "y"), (5, "B", "x")]
+
+
+def test_dump_vertical_record(tmp_path):
+    """The packed vertical record of the geometry stage: each array at its row, from column OLx on."""
+    rng = np.random.default_rng(1)
+    Nr = LLC90.Nr
+    arrs = {"drF": rng.random(Nr), "recip_drF": rng.random(Nr), "rhoFacC": rng.random(Nr),
+            "drC": rng.random(Nr + 1), "recip_drC": rng.random(Nr + 1), "rC": rng.random(Nr), "rF": rng.random(Nr + 1)}
+    v = D.pack_vertical(LLC90, arrs)
+    assert v[LLC90.OLy + 10, LLC90.OLx + 3] == arrs["rhoFacC"][3]   # row 11 (1-based)
+    assert v[LLC90.OLy, LLC90.OLx + Nr - 1] == arrs["drF"][Nr - 1]   # row 1
+    stack = np.broadcast_to(v, (LLC90.nTiles,) + v.shape)
+    _write_dump(tmp_path, {("G00_geometry", "vertical"): stack}, its=(0,))
+    got = D.DumpIndex(tmp_path).vertical(0, LLC90)
+    for k, a in arrs.items():
+        assert np.array_equal(got[k], a), k
+    with pytest.raises(ValueError, match="holds"):
+        D.pack_vertical(LLC90, {"drF": np.ones(Nr + 1)})
 
 
 def test_dump_errors(tmp_path):
     a = np.zeros((LLC90.nTiles, LLC90.ny, LLC90.nx))
     _write_dump(tmp_path, {("G00_geometry", "rA"): a}, its=(0,))
-    f = D.dump_name(tmp_path, 0, 1)
+    f = D.file_name(tmp_path, 0, 1)
     raw = bytearray(f.read_bytes())
     raw[0] ^= 0xFF
     f.write_bytes(bytes(raw))
     with pytest.raises(ValueError, match="magic"):
-        D.read_file(f)
-    g = D.dump_name(tmp_path, 0, 2)
+        D.scan_file(f)
+    g = D.file_name(tmp_path, 0, 2)
     g.write_bytes(g.read_bytes()[:-8])
     with pytest.raises(ValueError, match="truncated"):
-        D.read_file(g)
+        D.scan_file(g)
     with pytest.raises(FileNotFoundError):
-        D.DumpSet(tmp_path / "none")
+        D.DumpIndex(tmp_path / "none")

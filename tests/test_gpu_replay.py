@@ -1,6 +1,7 @@
-"""The reference's dump-replay gates (tests/test_cg2d.py:116-160, tests/test_free_surface.py:78-125) through the
-GPU path, on a synthetic jaxdump directory written from the CPU oracle's results: S07 (UPDATE_CG2D), C01 (cg2d_b,
-cg2d_x) and C02 (the solve, halos included, and its iteration count), each bitwise. With dumps of the Fortran model
+"""The reference's dump-replay gates (tests/test_cg2d.py:116-160, tests/test_free_surface.py:78-125,
+tests/test_gad.py:119-130) through the GPU path, on synthetic jaxdump directories written from the CPU oracle's
+results: S07 (UPDATE_CG2D), C01 (cg2d_b, cg2d_x), C02 (the solve, halos included, and its iteration count) and
+T10 / T20 (GAD_ADVECTION of theta and salt on the whole tile arrays), each bitwise. With dumps of the Fortran model
 the same functions (`mit_jax_amd.replay`) run the reference's gates."""
 
 from types import SimpleNamespace
@@ -52,10 +53,10 @@ def case(tmp_path_factory):
     d_dir = tmp_path_factory.mktemp("jaxdump")
     it = 8
     for t in range(T):
-        with open(D.dump_name(d_dir, it, t + 1), "wb") as fh:
+        with open(D.file_name(d_dir, it, t + 1), "wb") as fh:
             for seq, ((stage, field), a) in enumerate(fields.items()):
-                D.write_record(fh, it, seq, stage, field, "RL", t + 1, FACE[t], 90 * (t % 3), 0, a[t])
-    return SimpleNamespace(ds=D.DumpSet(d_dir), it=it, g=g, p=p, cp=cp, iters=d["numIters"])
+                D.append_record(fh, it, seq, stage, field, "RL", t + 1, FACE[t], 90 * (t % 3), 0, a[t])
+    return SimpleNamespace(ds=D.DumpIndex(d_dir), it=it, g=g, p=p, cp=cp, iters=d["numIters"])
 
 
 @pytest.fixture(scope="module")
@@ -86,3 +87,41 @@ def test_replay_cg2d_solution(dev, case, ex):
     x, d, rx = replay.replay_cg2d(case.ds, case.it, ex, case.cp)
     assert np.array_equal(x, rx)
     assert int(d["numIters"]) == case.iters
+
+
+def test_replay_gad_t10_t20(dev, tmp_path):
+    """T01 residual flow + S04 tracers + S11 hFac + G00 geometry -> T10 gT_loc / T20 gS_loc, bitwise (the
+    reference's gate, tests/test_gad.py:119-130); a planted error in the dumped output is caught."""
+    from mit_jax_amd import dump as D, gad, replay, synthetic as S
+    from tests.conftest import dst3_case, oracle_tables
+    g, u, v, w, theta = dst3_case(0)
+    salt = 34.0 + 0.3 * np.random.default_rng(8).normal(size=theta.shape) * g.maskC
+    salt = oracle.scalar(maps().full("T"), salt)
+    g2, vg = S.gad_inputs(g)
+    tab = oracle_tables()
+    refs = {n: oracle.advect_tiles(tab, g2, vg, u, v, tr, g.hFacW, g.hFacS, g.recip_hFacC)
+            for n, tr in (("temp", theta), ("salt", salt))}
+    L = g.L
+    vert = D.pack_vertical(L, {"drF": g.drF, "recip_drF": g.recip_drF, "rhoFacC": g.rhoFacC,
+                               "recip_drC": np.append(g.recip_drC, g.recip_drC[-1])})
+    fields = {("G00_geometry", n): g.f[n] for n in replay.GAD_GRID2D + replay.GAD_GRID3D}
+    fields[("G00_geometry", "vertical")] = np.broadcast_to(vert, (13,) + vert.shape)
+    fields.update({("T01_residual_flow", "uFld"): u, ("T01_residual_flow", "vFld"): v, ("T01_residual_flow", "wFld"): w,
+                   ("S04_oceanic_phys", "theta"): theta, ("S04_oceanic_phys", "salt"): salt,
+                   ("S11_calc_rstar", "hFacW"): g.hFacW, ("S11_calc_rstar", "hFacS"): g.hFacS,
+                   ("S11_calc_rstar", "recip_hFacC"): g.recip_hFacC,
+                   ("T10_temp_adv", "gT_loc"): refs["temp"], ("T20_salt_adv", "gS_loc"): refs["salt"]})
+    it = 3
+    for t in range(13):
+        with open(D.file_name(tmp_path, it, t + 1), "wb") as fh:
+            for seq, ((stage, field), a) in enumerate(fields.items()):
+                D.append_record(fh, it, seq, stage, field, "RL", t + 1, FACE[t], 90 * (t % 3), 0, a[t])
+    ds = D.DumpIndex(tmp_path)
+    print("replay T10/T20 GAD_ADVECTION", flush=True)
+    out = replay.replay_gad(ds, it, gad.GADParams.llc90())
+    for n, (got, dumped) in out.items():
+        assert np.array_equal(got, dumped), n
+        assert np.abs(dumped).max() > 0
+    # negative control: oneSixth off by 1e-6 relative fails the gate
+    bad = replay.replay_gad(ds, it, gad.GADParams.llc90(oneSixth=(1.0 / 6.0) * (1 + 1e-6)), tracers=("temp",))
+    assert not np.array_equal(bad["temp"][0], bad["temp"][1])
