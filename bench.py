@@ -6,24 +6,30 @@ solve (per-member operators through operator slots, per-member convergence) and 
 level counts). Inputs are seeded synthetic fields resident in HBM before timing starts.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--members B] [--precision mixed|fp64]
+    python bench.py --kernel-profile --block-threads N   (the per-kernel timings alone, the workload for rocprofv3)
 
 The headline (default) is the mixed-precision solve (FP32 inner PCG + FP64 refinement, mj_cg2d_solve_mixed); the
 FP64 solve (mj_cg2d_solve) is timed on the same step for `mixed_over_fp64`, and the two solutions are compared
-member by member (`error_vs_fp64`). Prints one JSON line. `roofline` is the solve (the dominant kernel), timed with
-HIP events on its stream: achieved = algorithmic bytes of the solve / solve time. Algorithmic bytes count the
-vector streams the fused recurrence must move per member (N2 = 105,300 interior points): mixed, per inner
-iteration 10 FP32 streams (B: z, s_{n-1}, d read, d, s_n, q written; C: q, r_{n-1} read, r_n, z written), per
-refinement 68 B per point (true residual x, b read and r written in FP64; restart r read, r32, d, z written;
-solution update x read and written, d, s read); FP64, per iteration 11 FP64 streams (B: z, s_{n-1} read, s_n, q
-written; C: s, q, r_{n-1}, x read, x, r_n, z written); both, once per member 48 B per point (first guess, initial
-residual, un-normalisation). The operator and preconditioner coefficients are shared by the members of an operator
-slot and served from cache: not counted. DST3: per (member, tile, level) 9 FP64 planes of 98 x 98 (tracer, u, v,
-hFacW, hFacS, recip_hFacC, maskW, maskS read, the tendency written), and the 2-D metrics (6 planes per tile)
-once. `traffic` = measured HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
+member by member (`error_vs_fp64`). Prints one JSON line.
+
+`roofline` is the solve's step kernel (the dominant kernel), from the solve alone on its stream (eager, not
+overlapped), timed with HIP events: achieved = algorithmic bytes per launch / average launch duration (= the solve's
+algorithmic bytes / its time; the average includes the few setup kernels and the launch gaps, so it is an upper bound
+of the profiler's per-kernel average). Algorithmic bytes are what the recurrence itself must move, per interior
+point (N2 = 105,300 per member): per member-iteration the vectors r, s, x read and written (6 streams of s_v bytes);
+per operator slot and iteration in which any member of the slot iterates, the six operator and preconditioner arrays
+(6 x s_c); per refinement of the mixed solve 64 B (the FP64 true residual and solution update); s_v = s_c = 4 for
+the mixed inner solve, 8 for the FP64 solve. `implementation_bytes` counts what this build's kernels are laid out to
+stream instead (mixed: 10 FP32 vector streams per inner iteration, 68 B per point per refinement, 48 B per point
+once; FP64: 11 FP64 streams per iteration, 48 B per point once; coefficients served from cache not counted).
+DST3 (`dst3`): per (member, tile, level) the padded 3-D planes the step reads (tracer, u, v, hFacW, hFacS,
+recip_hFacC, maskW, maskS; with the R flux also w) and the tendency it writes, 8 B each, plus the 2-D metrics
+(6 planes per tile) once. `traffic` = measured HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
 profile of this workload. `cpu_baseline` is the FP64 C oracle (OpenMP, all host cores) on a bounded sample of the
 same member-steps. `producers` times the operator builder (UPDATE_CG2D) and the rhs builder (cg2d_rhs) on the same
 ensemble, each alone (not part of the headline step, whose operators and rhs are inputs); `implicit_r` likewise
-times GAD_IMPLICIT_R (the implicit vertical advection/diffusion solve) on the ensemble's tracer.
+times GAD_IMPLICIT_R (the implicit vertical advection/diffusion solve) on the ensemble's tracer; `dst3_xyr` the DST3
+step with the vertical (R) flux.
 """
 
 import argparse
@@ -55,6 +61,9 @@ def parse():
                     help="run the launch configuration of --block-threads / --inner-tolerance / --overlap, no graph")
     ap.add_argument("--block-threads", type=int, default=0, help="with --no-autotune (0: the library default)")
     ap.add_argument("--inner-tolerance", type=float, default=1e-5, help="with --no-autotune")
+    ap.add_argument("--poll-iters", type=int, default=64,
+                    help="with --no-autotune: iterations between the solve's active-member polls (the autotuner's "
+                         "candidates use 64)")
     ap.add_argument("--overlap", action="store_true", help="with --no-autotune: DST3 concurrent with the solve")
     ap.add_argument("--no-fp64-reference", action="store_true",
                     help="with --precision mixed: skip timing the FP64 solve (the ratio and the error vs FP64)")
@@ -62,9 +71,13 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-producers", action="store_true",
-                    help="skip the UPDATE_CG2D / cg2d_rhs / GAD_IMPLICIT_R measurements")
+                    help="skip the UPDATE_CG2D / cg2d_rhs / GAD_IMPLICIT_R / DST3-with-R measurements")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (the reference kernel for HBM counter scaling)")
+    ap.add_argument("--kernel-profile", action="store_true",
+                    help="only the per-kernel timings (solve alone, DST3 alone, DST3 with R) of the launch "
+                         "configuration given by --block-threads / --inner-tolerance, three times each (no autotune): "
+                         "the workload of the committed kernel-trace profiles")
     ap.add_argument("--profile-solves", action="store_true",
                     help="only run one FP64 and one mixed solve of --members right-hand sides on the shared 5-point "
                          "operator (the workload of the hardware-counter profiles) and exit")
@@ -290,6 +303,149 @@ def implicit_r(E, dev, reps=5):
             "column_integral_max_rel_change": cons}
 
 
+def solve_bytes(precision, d, op_index):
+    """(algorithmic bytes, implementation bytes, operator-slot iterations) of one batched solve from its
+    diagnostics (the counts of the module docstring)."""
+    iters = d["numIters"].cpu().numpy().astype(np.int64)
+    outer = d["outerIters"].cpu().numpy().astype(np.int64)
+    B = len(iters)
+    slot = op_index.cpu().numpy() if op_index is not None else np.arange(B)
+    slot_iters = sum(int(iters[slot == s].max()) for s in np.unique(slot))  # iterations with a member of the slot active
+    sv = 8 if precision == "fp64" else 4
+    alg = float(iters.sum()) * N2 * 6 * sv + slot_iters * N2 * 6 * sv
+    if precision == "fp64":
+        impl = float(iters.sum()) * N2 * 11 * 8 + B * N2 * 48
+    else:
+        alg += float(outer.sum()) * N2 * 64
+        impl = float(iters.sum()) * N2 * 10 * 4 + float(outer.sum()) * N2 * 68 + B * N2 * 48
+    return alg, impl, slot_iters
+
+
+def dst3_bytes(nlev, with_w):
+    """Algorithmic bytes of one batched DST3 launch: per (member, tile, level) 9 padded planes (10 with w) of 8 B,
+    plus the 2-D metrics (6 planes per tile) once."""
+    planes = 10 if with_w else 9
+    return float(sum(planes * 8 * 13 * 98 * 98 * int(n) for n in nlev)) + 6 * 8 * 13 * 98 * 98
+
+
+def dst3_xyr(E, dev, params, reps=3):
+    """The DST3 step with the vertical (R) flux of each member's w (the headline step's DST3 launch plus the
+    vertical sweep), timed alone with HIP events on the current stream (median of reps)."""
+    import ctypes
+    import torch
+    from dataclasses import replace as _replace
+    import mit_jax_amd._lib as L
+    from mit_jax_amd import gad
+    w = E["per_member"](lambda c: c[6])
+    g2d = dict(E["g2d"], maskC=E["mC"])
+    pr = _replace(params, implicitAdvection=False)
+    plan, grid_s, gcfg, trc, fields, wd, keep = gad._prepare(pr, None, g2d, dict(E["vg"]), E["u"], E["v"], E["tr"],
+                                                             E["hW"], E["hS"], E["rhC"], w, E["nlev_t"])
+    u, v, hW, hS, rhC = fields
+    out = torch.zeros_like(trc)
+    work = torch.empty_like(trc)
+    lib = L.lib()
+
+    def run():
+        L.check(lib.mj_gad_dst3(plan.handle, ctypes.byref(grid_s), ctypes.byref(gcfg), L.ptr(u), L.ptr(v), L.ptr(wd),
+                                L.ptr(trc), L.ptr(hW), L.ptr(hS), L.ptr(rhC), L.ptr(out), L.ptr(work),
+                                L.stream_ptr()), "mj_gad_dst3")
+
+    print("bench: DST3 with the R flux", file=sys.stderr, flush=True)
+    run()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    t = float(np.median(ms))
+    nb = dst3_bytes(E["nlev"], True)
+    del w, out, work, keep
+    gbs = nb / (t * 1e-3) / 1e9
+    return {"ms": t, "member_steps_per_s": len(E["nlev"]) / (t * 1e-3), "algorithmic_bytes": nb,
+            "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+
+
+def kernel_times(st, reps=3):
+    """The solve alone and the DST3 step alone (eager, polled, not overlapped, no graph), HIP events on the current
+    stream (both launch on it): the per-kernel times behind `roofline` and `dst3` (the timed steps may overlap the
+    two). Returns (solve ms, DST3 ms, the solve's diagnostics), medians of reps."""
+    import torch
+    from dataclasses import replace as _replace
+    st.drop_graph()
+    st.set_config(_replace(st.cfg, graph=False, overlap=False))
+    ts, ta = [], []
+    for _ in range(reps):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        st._solve()
+        e[1].record()
+        st._advect()
+        e[2].record()
+        torch.cuda.synchronize()
+        ts.append(e[0].elapsed_time(e[1]))
+        ta.append(e[1].elapsed_time(e[2]))
+    return float(np.median(ts)), float(np.median(ta)), st.diag().check()
+
+
+def roofline_of(precision, cg_ms, d, op_index, members, launch):
+    """The `roofline` object of a solve timed alone (cg_ms) with diagnostics d."""
+    alg, impl, slot_iters = solve_bytes(precision, d, op_index)
+    launches = int(d["launches"].max())
+    if precision == "fp64":
+        kernel = "cg2d_persistent_kernel" if launch == "persistent" else "cg2d_phase_kernel"
+    else:
+        kernel = "mx_step_kernel"
+    achieved = alg / (cg_ms * 1e-3) / 1e9
+    tr = measured_traffic(kernel, members)
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": tr, "traffic_source": TRAFFIC_FILE.name if tr else None,
+            "launches": launches, "avg_launch_us": cg_ms * 1e3 / launches if launches else None,
+            "algorithmic_bytes_per_launch": alg / launches if launches else None, "algorithmic_bytes": alg,
+            "operator_slot_iterations": slot_iters, "implementation_bytes": impl,
+            "implementation_GBs": impl / (cg_ms * 1e-3) / 1e9, "solve_ms": cg_ms}
+
+
+def make_step(E, params, precision, args):
+    from mit_jax_amd.ensemble import EnsembleStep, StepConfig
+    return EnsembleStep(E["ex"], E["ops"], E["op_index"], E["norm"], E["rhs"], E["x0"], params, E["g2d"], E["vg"],
+                        E["u"], E["v"], E["tr"], E["hW"], E["hS"], E["rhC"], nlev=E["nlev_t"], tolerance=TOL,
+                        max_iters=2000, cfg=StepConfig(precision=precision, sum_order=args.sum_order,
+                                                       block_threads=args.block_threads, poll_iters=args.poll_iters,
+                                                       inner_tolerance=args.inner_tolerance, overlap=args.overlap))
+
+
+def kernel_profile(args):
+    """The per-kernel timings of one launch configuration alone: the workload of the kernel-trace profiles."""
+    import torch
+    import mit_jax_amd
+    from mit_jax_amd import gad
+    from mit_jax_amd.workload import build_ensemble
+    dev = torch.device("cuda:0")
+    B = args.members
+    print(f"bench: building {B} members", file=sys.stderr, flush=True)
+    E = build_ensemble(B, dev)
+    params = gad.GADParams.llc90()
+    st = make_step(E, params, args.precision, args)
+    print("bench: kernel timings (solve alone, DST3 alone)", file=sys.stderr, flush=True)
+    cg_ms, gad_ms, d = kernel_times(st)
+    gb = dst3_bytes(E["nlev"], False)
+    out = {"kernel_profile": B, "precision": args.precision,
+           "launch": {k: v for k, v in st.cfg.__dict__.items() if k != "precision"},
+           "cg2d_iterations_mean": float(d["numIters"].float().mean()),
+           "roofline": roofline_of(args.precision, cg_ms, d, E["op_index"], B, st.cfg.launch),
+           "dst3": {"launch_ms": gad_ms, "achieved_GBs": gb / (gad_ms * 1e-3) / 1e9, "algorithmic_bytes": gb,
+                    "frac": gb / (gad_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "dst3_xyr": dst3_xyr(E, dev, params)}
+    print(json.dumps(out), flush=True)
+    del st, E
+    mit_jax_amd.release()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -298,9 +454,11 @@ def main():
         sys.exit(2)
     if args.profile_solves:
         return profile_solves(args.members)
+    if args.kernel_profile:
+        return kernel_profile(args)
     import torch
     from mit_jax_amd import cg2d, gad
-    from mit_jax_amd.ensemble import EnsembleStep, StepConfig, autotune
+    from mit_jax_amd.ensemble import autotune
 
     dev = torch.device("cuda:0")
     B = args.members
@@ -311,12 +469,7 @@ def main():
     log = lambda m: print("bench: autotune " + m, file=sys.stderr, flush=True)  # noqa: E731
 
     def make(precision):
-        st = EnsembleStep(E["ex"], E["ops"], E["op_index"], E["norm"], E["rhs"], E["x0"], params, E["g2d"],
-                          E["vg"], E["u"], E["v"], E["tr"], E["hW"], E["hS"], E["rhC"], nlev=E["nlev_t"], tolerance=TOL,
-                          max_iters=2000, cfg=StepConfig(precision=precision, sum_order=args.sum_order,
-                                                         block_threads=args.block_threads,
-                                                         inner_tolerance=args.inner_tolerance,
-                                                         overlap=args.overlap))
+        st = make_step(E, params, precision, args)
         tuned = None
         if not args.no_autotune:
             best, results = autotune(st, precision, steps=2, warmup=1, log=log)
@@ -347,40 +500,15 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    def kernel_times(st, reps=3):
-        """The solve alone and the DST3 step alone (eager, polled, not overlapped), HIP events on the current
-        stream: the per-kernel times behind `roofline` and `dst3` (the timed steps may overlap the two)."""
-        from dataclasses import replace as _replace
-        st.set_config(_replace(st.cfg, graph=False))
-        ts, ta = [], []
-        for _ in range(reps):
-            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-            e[0].record()
-            st._solve()
-            e[1].record()
-            st._advect()
-            e[2].record()
-            torch.cuda.synchronize()
-            ts.append(e[0].elapsed_time(e[1]))
-            ta.append(e[1].elapsed_time(e[2]))
-        return float(np.median(ts)), float(np.median(ta))
-
     main_step, tuned = make(args.precision)
     run = timed(main_step)
-    run["cg_ms"], run["gad_ms"] = kernel_times(main_step)
+    run["cg_ms"], run["gad_ms"], dk = kernel_times(main_step)
     d = run["d"]
     iters = d["numIters"].cpu().numpy().astype(np.int64)
     outer = d["outerIters"].cpu().numpy().astype(np.int64)
     conv = d["converged"].cpu().numpy()
     sel = main_step.cfg
-    if args.precision == "fp64":
-        cg_bytes = float(iters.sum()) * N2 * 11 * 8 + B * N2 * 48
-        kernel = "cg2d_phase_kernel" if sel.launch != "persistent" else "cg2d_persistent_kernel"
-    else:
-        cg_bytes = float(iters.sum()) * N2 * 10 * 4 + float(outer.sum()) * N2 * 68 + B * N2 * 48
-        kernel = "mx_step_kernel"
-    gad_bytes = float(sum(9 * 8 * 13 * 98 * 98 * int(n) for n in E["nlev"])) + 6 * 8 * 13 * 98 * 98
-    achieved = cg_bytes / (run["cg_ms"] * 1e-3) / 1e9
+    gad_bytes = dst3_bytes(E["nlev"], False)
     ms = run["el"] / args.steps * 1e3
     cfg = {"workload": f"LLC90 ensemble: {B} members x (CG2D to 1e-12 normalised residual + one DST3 "
                        "step), 13x90x90 tiles, 50 levels, member level counts 15/30/50",
@@ -402,18 +530,14 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": cfg,
-        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(kernel, B),
-                     "traffic_source": TRAFFIC_FILE.name if measured_traffic(kernel, B) else None,
-                     "solve_ms": run["cg_ms"], "algorithmic_bytes": cg_bytes},
+        "roofline": roofline_of(args.precision, run["cg_ms"], dk, E["op_index"], B, sel.launch),
         "dst3": {"launch_ms": run["gad_ms"], "achieved_GBs": gad_bytes / (run["gad_ms"] * 1e-3) / 1e9,
-                 "algorithmic_bytes": gad_bytes},
+                 "algorithmic_bytes": gad_bytes, "frac": gad_bytes / (run["gad_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     if args.precision == "mixed" and not args.no_fp64_reference:
         ref_step, ref_tuned = make("fp64")
         ref = timed(ref_step)
-        ref["cg_ms"], _ = kernel_times(ref_step)
+        ref["cg_ms"], _, dref = kernel_times(ref_step)
         inner = cg2d.interior(E["ex"].L, dev)
         x, xr = main_step.x, ref_step.x
         diff = torch.where(inner, (x - xr).abs(), 0.0).amax(dim=(1, 2, 3))
@@ -421,6 +545,7 @@ def main():
         rel = (diff / scale).cpu().numpy()
         fp64_value = B * args.steps / ref["el"]
         out["fp64"] = {"value": fp64_value, "ms_per_step": ref["el"] / args.steps * 1e3, "solve_ms": ref["cg_ms"],
+                       "roofline": roofline_of("fp64", ref["cg_ms"], dref, E["op_index"], B, ref_step.cfg.launch),
                        "cg2d_iterations_mean": float(ref["d"]["numIters"].float().mean()),
                        "launch": {k: v for k, v in ref_step.cfg.__dict__.items() if k != "precision"},
                        "autotune": ref_tuned}
@@ -439,10 +564,14 @@ def main():
     if not args.no_producers:
         out["producers"] = producers(E, dev)
         out["implicit_r"] = implicit_r(E, dev)
+        out["dst3_xyr"] = dst3_xyr(E, dev, params)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(E, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+    del main_step, E
+    import mit_jax_amd
+    mit_jax_amd.release()
 
 
 if __name__ == "__main__":

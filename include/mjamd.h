@@ -111,7 +111,8 @@ typedef struct {
     int32_t numIters;      /* iterations done */
     int32_t converged;     /* 1 if err_sq < tol^2 at exit */
     int32_t outerIters;    /* refinement steps (mixed precision), 0 for the plain solve */
-    int32_t pad_;
+    int32_t launches;      /* kernel launches of the iteration loop (PHASES modes; the same value in every member's
+                              record, written when the loop ends); 0 for the persistent launch */
 } mj_cg2d_diag;
 
 typedef struct {

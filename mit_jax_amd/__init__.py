@@ -7,4 +7,25 @@ exposed as PyTorch-ROCm ops with the reference's argument signatures.
 
 from .layout import LLC90, Layout, interior_mask  # noqa: F401
 
-__version__ = "0.1.0"
+__version__ = "0.2.0"
+
+
+def release():
+    """Destroys the library objects this package caches (halo plans, DST3 plans, device grid fields) after the
+    device has finished with them. Call before the process exits when something else (a profiler, a HIP runtime
+    torn down early) runs at exit: the library then holds no device objects there. Later calls rebuild the caches
+    on use."""
+    import sys
+    import torch
+    if torch.cuda.is_available() and torch.cuda.is_initialized():
+        torch.cuda.synchronize()
+    for name in ("exchange", "gad"):
+        mod = sys.modules.get(f"{__name__}.{name}")
+        if mod is not None:
+            plans = list(mod._plans.values())
+            mod._plans.clear()
+            for p in plans:
+                p.close()
+    sfp = sys.modules.get(f"{__name__}.solve_for_pressure")
+    if sfp is not None:
+        sfp._grid_cache.clear()

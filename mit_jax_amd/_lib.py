@@ -33,7 +33,7 @@ class HaloGeom(ctypes.Structure):
 class Cg2dDiag(ctypes.Structure):
     _fields_ = [("firstResidual", F64), ("lastResidual", F64), ("sumRHS", F64), ("rhsMax", F64),
                 ("numIters", ctypes.c_int32), ("converged", ctypes.c_int32), ("outerIters", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("launches", ctypes.c_int32)]
 
 
 class Cg2dOps(ctypes.Structure):

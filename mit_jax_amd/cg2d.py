@@ -47,7 +47,8 @@ class Cg2dParams:
 class Cg2dDiag(dict):
     """Per-member diagnostics of a solve (tensors on the device; the reference returns firstResidual,
     lastResidual, numIters, sumRHS, rhsMax, core/cg2d.py:267-268), plus `converged` (1 / 0; -2 when a device
-    barrier timed out) and `outerIters` (refinement steps of the mixed-precision solve)."""
+    barrier timed out), `outerIters` (refinement steps of the mixed-precision solve) and `launches` (kernel launches
+    of the solve's iteration loop in the per-phase launch modes, the same in every member's record)."""
 
     def __init__(self, raw, batched):
         f = raw.view(torch.float64).view(-1, 6)
@@ -55,7 +56,7 @@ class Cg2dDiag(dict):
         sel = (lambda t: t) if batched else (lambda t: t[0])
         super().__init__(firstResidual=sel(f[:, 0]), lastResidual=sel(f[:, 1]), sumRHS=sel(f[:, 2]),
                          rhsMax=sel(f[:, 3]), numIters=sel(i[:, 8]), converged=sel(i[:, 9]),
-                         outerIters=sel(i[:, 10]))
+                         outerIters=sel(i[:, 10]), launches=sel(i[:, 11]))
         self.raw = raw
 
     def check(self):

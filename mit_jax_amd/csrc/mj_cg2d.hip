@@ -157,7 +157,7 @@ __device__ __forceinline__ void unit_I2(const CgDev& a, int m, int t) {
         a.diag[m].sumRHS = tile_order_sum(a.psum + m * a.T, a.T);
         a.diag[m].rhsMax = member_rhs_max(a, m);
         a.diag[m].outerIters = 0;
-        a.diag[m].pad_ = 0;
+        a.diag[m].launches = 0;
         a.mdone[m] = -1;
     }
 }
@@ -352,6 +352,11 @@ cg2d_phase_kernel(CgDev a, int n) {
     if (PH == PH_I0) unit_I0(a, m, t);
     if (PH == PH_I1) unit_I1<MODE>(a, m, t);
     if (PH == PH_I2) unit_I2<MODE>(a, m, t);
+}
+
+// The launch count of the iteration loop into every member's record (written after the loop).
+__global__ void diag_launches_kernel(mj_cg2d_diag* diag, int B, int launches) {
+    for (int m = threadIdx.x; m < B; m += blockDim.x) diag[m].launches = launches;
 }
 
 __global__ void cg2d_init_kernel(unsigned int* counter, int* error, int* nactive, int B) {
@@ -588,12 +593,16 @@ int launch_phases(const CgDev& a, const mj_cg2d_config* cfg, int threads, int mo
     // B(n) for n = 1 .. max_iters + 1 (the last one only finishes members), C(n) for n <= max_iters
     const int nlast = cfg->max_iters + 1;
     const int chunk = cfg->poll_iters > 0 ? cfg->poll_iters : 32;
-    int n = 1;
+    int n = 1, launches = 0;
     while (n <= nlast) {
         const int nend = mode == MJ_CG_LAUNCH_PHASES_FIXED ? nlast + 1 : std::min(nlast + 1, n + chunk);
         for (; n < nend; ++n) {
             if ((rc = go(PH_B, n))) return rc;
-            if (n <= cfg->max_iters && (rc = go(PH_C, n))) return rc;
+            ++launches;
+            if (n <= cfg->max_iters) {
+                if ((rc = go(PH_C, n))) return rc;
+                ++launches;
+            }
         }
         if (n > nlast) break;
         int h = 0;
@@ -601,6 +610,8 @@ int launch_phases(const CgDev& a, const mj_cg2d_config* cfg, int threads, int mo
         MJ_HIP(hipStreamSynchronize(st));
         if (h == 0) break;
     }
+    hipLaunchKernelGGL(diag_launches_kernel, dim3(1), dim3(256), 0, st, a.diag, a.B, launches);
+    MJ_HIP(hipGetLastError());
     return MJ_OK;
 }
 

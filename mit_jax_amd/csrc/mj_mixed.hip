@@ -542,7 +542,7 @@ mx_step_kernel(MxDev a, int k) {
                         a.diag[m].firstResidual = sqrt(e);
                         a.diag[m].sumRHS = sr;
                         a.diag[m].rhsMax = member_rhs_max(a, m);
-                        a.diag[m].pad_ = 0;
+                        a.diag[m].launches = 0;
                     }
                 }
                 if (e < a.tol_sq || r.iters >= a.max_iters || r.outer >= a.max_outer) {
@@ -637,6 +637,11 @@ __global__ void mx_convert_kernel(const double* aW, const double* aS, const doub
 
 __global__ void mx_reset_kernel(int* nactive, int B) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *nactive = B;
+}
+
+// The launch count of the step loop into every member's record (written after the loop).
+__global__ void mx_launches_kernel(mj_cg2d_diag* diag, int B, int launches) {
+    for (int m = threadIdx.x; m < B; m += blockDim.x) diag[m].launches = launches;
 }
 
 }  // namespace mj
@@ -845,5 +850,7 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
         MJ_HIP(hipStreamSynchronize(st));
         if (h == 0) break;
     }
+    hipLaunchKernelGGL(mx_launches_kernel, dim3(1), dim3(256), 0, st, a.diag, a.B, (int)(k - 1));
+    MJ_HIP(hipGetLastError());
     return MJ_OK;
 }

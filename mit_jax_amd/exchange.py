@@ -120,12 +120,18 @@ class HaloPlan:
         self._destroy = _lib.lib().mj_halo_plan_destroy
         self.inplace_ok = bool(_lib.lib().mj_halo_plan_inplace_ok(h))
 
+    def close(self):
+        """Destroys the device plan now (idempotent); the object is unusable afterwards."""
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        self.handle = None
+        if h and destroy is not None:
+            destroy(h)
+
     def __del__(self, _finalizing=sys.is_finalizing):
         # the destroy function is bound at creation: module globals may already be gone when this runs late, and at
         # interpreter exit the process end releases the device memory anyway
-        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
-        if h and destroy is not None and not _finalizing():
-            destroy(h)
+        if not _finalizing():
+            self.close()
 
 
 _plans = {}

@@ -110,12 +110,18 @@ class GadPlan:
                    "mj_gad_plan_describe")
         return masks, fills
 
+    def close(self):
+        """Destroys the device plan now (idempotent); the object is unusable afterwards."""
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        self.handle = None
+        if h and destroy is not None:
+            destroy(h)
+
     def __del__(self, _finalizing=sys.is_finalizing):
         # the destroy function is bound at creation: module globals may already be gone when this runs late, and at
         # interpreter exit the process end releases the device memory anyway
-        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
-        if h and destroy is not None and not _finalizing():
-            destroy(h)
+        if not _finalizing():
+            self.close()
 
 
 def plan_tables_host(topo: TileTopology, L: Layout = LLC90):
