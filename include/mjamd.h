@@ -144,6 +144,20 @@ int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const doub
                   const double* cg2dNorm, const mj_cg2d_config* cfg, void* workspace, size_t workspace_bytes,
                   mj_cg2d_diag* diag, void* stream);
 
+/* Allocation-free variant of mj_cg2d_solve. It has no counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * arguments and results are those of mj_cg2d_solve without the workspace: the solve's scratch is a device array the
+ * library holds per device (MJ_CG2D_NOALLOC_SCRATCH_BYTES, kept until the process ends), and the call makes no
+ * allocation. A configuration whose mj_cg2d_workspace_bytes exceeds that size is rejected (MJ_ERR_ARG; at 13 tiles
+ * of 90 x 90 with OL 4 that is above 11 members in the MJ_SUM_TREE order, above 8 in MJ_SUM_FORTRAN). The T plan
+ * must already hold its CG tables, which the plan's first mj_cg2d_solve or mj_cg2d_solve_mixed call builds (with
+ * allocations); otherwise MJ_ERR_ARG. Every call on one device uses the same array and leaves its contents there:
+ * calls that overlap on one device (other streams, other threads) race and must be serialised by the caller. */
+#define MJ_CG2D_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mj_cg2d_solve_noalloc(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* b, const double* x0,
+                          double* x, const double* cg2dNorm, const mj_cg2d_config* cfg, mj_cg2d_diag* diag,
+                          void* stream);
+
 /* The linear operator of the implicit derivative (core/cg2d.py:292-298 _matvec): y = (A / cg2dNorm) v on the
  * interior, 0 on halos, with v exchanged after zeroing its halos. Batched like mj_cg2d_solve. work: device
  * scratch of B*member_stride doubles. */

@@ -104,6 +104,7 @@ SIGNATURES = {
     "mj_cg2d_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Cg2dConfig)]),
     "mj_cg2d_solve": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, P, ctypes.POINTER(Cg2dConfig), P, ctypes.c_size_t,
                             P, P]),
+    "mj_cg2d_solve_noalloc": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, P, ctypes.POINTER(Cg2dConfig), P, P]),
     "mj_cg2d_matvec": (I32, [P, ctypes.POINTER(Cg2dOps), P, P, P, ctypes.POINTER(Cg2dConfig), P, P]),
     "mj_cg2d_coeff_grad": (I32, [P, P, P, P, P, P, P, ctypes.POINTER(Cg2dConfig), P, P]),
     "mj_cg2d_mixed_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Cg2dConfig), ctypes.POINTER(Cg2dMixedConfig),

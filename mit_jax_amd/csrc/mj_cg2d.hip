@@ -622,9 +622,13 @@ extern "C" size_t mj_cg2d_workspace_bytes(const mj_cg2d_config* cfg) {
     return ws_layout(cfg).total;
 }
 
-extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* b, const double* x0,
-                             double* x, const double* cg2dNorm, const mj_cg2d_config* cfg, void* workspace,
-                             size_t workspace_bytes, mj_cg2d_diag* diag, void* stream) {
+namespace {
+
+// The solve behind mj_cg2d_solve and mj_cg2d_solve_noalloc. With may_alloc false it allocates nothing: the T plan's
+// CG tables must already exist (built by an earlier call that may allocate).
+int solve_impl(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* b, const double* x0, double* x,
+               const double* cg2dNorm, const mj_cg2d_config* cfg, void* workspace, size_t workspace_bytes,
+               mj_cg2d_diag* diag, void* stream, bool may_alloc) {
     int rc = check_cfg(cfg);
     if (rc) return rc;
     MJ_REQUIRE(plan_T && ops && b && x0 && x && cg2dNorm && workspace && diag, "mj_cg2d_solve: null argument");
@@ -639,7 +643,14 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     MJ_REQUIRE(workspace_bytes >= L.total, "mj_cg2d_solve: workspace %zu < %zu bytes", workspace_bytes, L.total);
     mj_halo_plan* P = const_cast<mj_halo_plan*>(plan_T);
     MJ_REQUIRE(P->ntiles == cfg->ntiles, "mj_cg2d_solve: plan has %d tiles, config %d", P->ntiles, cfg->ntiles);
-    if ((rc = ensure_cg_tables(P, cfg->sNx, cfg->sNy, cfg->OL))) return rc;
+    if (may_alloc) {
+        if ((rc = ensure_cg_tables(P, cfg->sNx, cfg->sNy, cfg->OL))) return rc;
+    } else {
+        std::lock_guard<std::mutex> lk(P->cg_mu);
+        MJ_REQUIRE(P->cg_full_off && P->cg_sNx == cfg->sNx && P->cg_sNy == cfg->sNy && P->cg_OL == cfg->OL,
+                   "mj_cg2d_solve_noalloc: the T plan has no CG tables for this layout yet (they are built, with "
+                   "device allocations, by the plan's first mj_cg2d_solve / mj_cg2d_solve_mixed call)");
+    }
     const int mode = launch_mode_of(cfg);
     const int threads = block_threads_of(cfg, mode);
     if (threads < 0) return threads;
@@ -725,6 +736,57 @@ extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops,
     void* args[] = {(void*)&a, (void*)&gs};
     MJ_HIP(hipLaunchCooperativeKernel(kfn, dim3(grid), dim3(threads), args, 0u, st));
     return MJ_OK;
+}
+
+}  // namespace
+
+// Scratch of mj_cg2d_solve_noalloc: one copy per device, held until the process ends. Host code never uses this
+// name as an address: it passes the device address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[MJ_CG2D_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = (size_t)MJ_CG2D_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch` (looked up once per device, with that device current).
+int scratch_of_current_device(void** out) {
+    int dev = 0;
+    MJ_HIP(hipGetDevice(&dev));
+    MJ_REQUIRE(dev >= 0 && dev < kMaxDevices, "mj_cg2d_solve_noalloc: device index %d >= %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        MJ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        MJ_REQUIRE(p != nullptr, "mj_cg2d_solve_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MJ_OK;
+}
+
+}  // namespace
+
+extern "C" int mj_cg2d_solve(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* b, const double* x0,
+                             double* x, const double* cg2dNorm, const mj_cg2d_config* cfg, void* workspace,
+                             size_t workspace_bytes, mj_cg2d_diag* diag, void* stream) {
+    return solve_impl(plan_T, ops, b, x0, x, cg2dNorm, cfg, workspace, workspace_bytes, diag, stream, true);
+}
+
+extern "C" int mj_cg2d_solve_noalloc(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* b,
+                                     const double* x0, double* x, const double* cg2dNorm, const mj_cg2d_config* cfg,
+                                     mj_cg2d_diag* diag, void* stream) {
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    const size_t need = ws_layout(cfg).total;
+    MJ_REQUIRE(need <= kNoAllocScratchBytes,
+               "mj_cg2d_solve_noalloc: this configuration needs %zu bytes of scratch, more than the %zu the library "
+               "holds per device (use mj_cg2d_solve with a workspace)", need, kNoAllocScratchBytes);
+    void* ws = nullptr;
+    if ((rc = scratch_of_current_device(&ws))) return rc;
+    return solve_impl(plan_T, ops, b, x0, x, cg2dNorm, cfg, ws, kNoAllocScratchBytes, diag, stream, false);
 }
 
 extern "C" int mj_cg2d_matvec(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* v, double* y,
