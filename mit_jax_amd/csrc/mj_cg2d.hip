@@ -464,21 +464,21 @@ int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
         (rc = up(rsrc.data(), rsrc.size() * 4, (void**)&P->cg_ring_src)) ||
         (rc = up(rsg.data(), rsg.size(), (void**)&P->cg_ring_sign)))
         return rc;
-    // compact form: LDS offset of each ring point, compact source index
-    const int TS = compact_tile_stride(sNx, sNy);
+    // the mixed solve's compact form: LDS offset of each ring point, compact source index (mx_geo)
+    const MxGeo G = mx_geo(sNx, sNy);
     std::vector<int32_t> clds(rdst.size()), csrc(rdst.size());
     for (size_t k = 0; k < rdst.size(); ++k) {
         const int r = rdst[k] % plane, jh = r / nx, ih = r % nx;
-        clds[k] = (jh - OL + 1) * (sNx + 4) + (ih - OL + 2);
+        clds[k] = G.lds(jh - OL, ih - OL);
         const int s = rsrc[k];
         if (s < 0) {
             csrc[k] = -1;
         } else {
             const int ts = s / plane, rs = s % plane, js = rs / nx - OL, is = rs % nx - OL;
-            csrc[k] = ts * TS + js * sNx + is;
-            // a source in the west or east column: its contiguous copy after the tile's interior
-            if (MJ_MX_EDGES && is == 0) csrc[k] = ts * TS + sNx * sNy + js;
-            if (MJ_MX_EDGES && is == sNx - 1) csrc[k] = ts * TS + sNx * sNy + sNy + js;
+            csrc[k] = ts * G.TS + js * G.CP + is;
+            // a source in the west or east column: its contiguous copy after the tile's rows
+            if (is == 0) csrc[k] = ts * G.TS + sNy * G.CP + js;
+            if (is == sNx - 1) csrc[k] = ts * G.TS + sNy * G.CP + sNy + js;
         }
     }
     if ((rc = up(clds.data(), clds.size() * 4, (void**)&P->cg_cring_lds)) ||

@@ -484,15 +484,30 @@ extern __shared__ __align__(16) double mj_lds[];
 __host__ __device__ constexpr size_t lds_tile_bytes(int sNx, int sNy, int elem) {
     return (size_t)(sNy + 2) * (sNx + 4) * elem;
 }
-#ifndef MJ_MX_EDGES
-#define MJ_MX_EDGES 1
-#endif
-// Tile stride of the compact work vectors (interior only, row-major sNy x sNx; with MJ_MX_EDGES followed by copies
-// of the west and east columns, sNy values each, at sNx*sNy and sNx*sNy + sNy, so that a neighbour's ring read of
-// a column is contiguous): rounded up to 128 bytes of FP32.
-__host__ __device__ constexpr int compact_tile_stride(int sNx, int sNy) {
-    return (sNx * sNy + (MJ_MX_EDGES ? 2 * sNy : 0) + 31) / 32 * 32;
+// Geometry of the mixed solve's FP32 work vectors (interior only) and FP32 coefficient copies. A work vector's
+// tile holds sNy rows of CP floats (sNx rounded up to a multiple of 4, so every row starts 16-byte aligned and a
+// row is NC float4 chunks; the padding columns always hold 0), then contiguous copies of its west and east columns
+// (sNy values each, so that a neighbour's ring read of a column is one contiguous run), rounded up to 128 bytes.
+// A coefficient copy's tile holds rows 0..sNy of CPc floats (columns 0..sNx, rounded up to 4): column sNx carries
+// the east faces (aW, pW at i = sNx) and row sNy the north faces (aS, pS at j = sNy). The LDS tile of a stencil
+// operand holds rows -1..sNy with column i at offset 4 + i of its row (column -1 at 3, the east ring column sNx at
+// 4 + sNx) and a row pitch W = 4 + CPc, a multiple of 4.
+struct MxGeo {
+    int CP, NC, NS, TS, CPc, TSc, W;
+    __host__ __device__ int lds(int j, int i) const { return (j + 1) * W + 4 + i; }
+};
+__host__ __device__ inline MxGeo mx_geo(int sNx, int sNy) {
+    MxGeo g;
+    g.CP = (sNx + 3) / 4 * 4;
+    g.NC = g.CP / 4;
+    g.NS = sNy * g.NC;
+    g.TS = (sNy * g.CP + 2 * sNy + 31) / 32 * 32;
+    g.CPc = (sNx + 1 + 3) / 4 * 4;
+    g.TSc = (sNy + 1) * g.CPc;
+    g.W = 4 + g.CPc;
+    return g;
 }
+__host__ __device__ inline size_t mx_lds_bytes(int sNx, int sNy) { return (size_t)(sNy + 2) * mx_geo(sNx, sNy).W * 4; }
 struct LdsGeo {
     int W;
     __device__ __forceinline__ int at(int j, int i) const { return (j + 1) * W + i + 2; }

@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const do
 // with t = localTij after the horizontal passes. Update (the horizontal form, net outflow wT(k) - wT(k+1)):
 //   local(k) <- local(k) - coef(k) ((wT(k) - wT(k+1)) - tracer(k) (rTrans(k) - rTrans(k+1))) maskC(k),
 // gTracer = (local - tracer)/dt(k) on the whole tile array.
-__global__ void __launch_bounds__(256) gad_vert_kernel(GadDev a, const double* loc_h) {
+__global__ void __launch_bounds__(256) gad_vert_kernel(GadDev a, const double* __restrict__ loc_h) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int mt = blockIdx.y;
     if (q >= GPLANE) return;
@@ -605,28 +605,50 @@ __global__ void __launch_bounds__(256) gad_vert_kernel(GadDev a, const double* l
     const int kmax = a.nlev ? a.nlev[m] : K;
     const int64_t b2 = (int64_t)t * GPLANE + q;
     const double rA = a.rA[b2], rrA = a.recip_rA[b2];
-    auto off3 = [&](int k) { return m * a.ms3 + ((int64_t)t * K + k) * GPLANE + q; };
-    auto offm = [&](int k) { return m * a.mms + ((int64_t)t * K + k) * GPLANE + q; };
-    auto T_ = [&](int k) { return (k >= 0 && k < kmax) ? loc_h[off3(k)] : 0.0; };
-    auto mC = [&](int k) { return (k >= 0 && k < kmax) ? a.maskC[offm(k)] : 0.0; };
-    auto mk = [&](int k) { return (k >= 1 && k <= kmax - 1) ? mC(k - 1) * mC(k) : 0.0; };
-    auto rTr = [&](int k) { return (k >= 1 && k <= kmax - 1) ? a.w[off3(k)] * rA * mk(k) : 0.0; };
-    auto wT = [&](int k) {
-        if (k < 1 || k > kmax - 1) return 0.0;
-        const double cfl = fabs(a.w[off3(k)] * a.dTlev[k] * a.recip_drC[k]);
-        // upstream for positive (upward) transport: cell k, then k+1; downstream: k-1, then k-2
-        return dst3_flux(rTr(k), cfl, a.oneSixth, T_(k + 1), T_(k), T_(k - 1), T_(k - 2), mk(k + 1), mk(k), mk(k - 1));
+    const double* __restrict__ w = a.w + m * a.ms3 + (int64_t)t * K * GPLANE + q;
+    const double* __restrict__ tr = a.tr + m * a.ms3 + (int64_t)t * K * GPLANE + q;
+    const double* __restrict__ rhC = a.recip_hFacC + m * a.ms3 + (int64_t)t * K * GPLANE + q;
+    const double* __restrict__ lh = loc_h + m * a.ms3 + (int64_t)t * K * GPLANE + q;
+    const double* __restrict__ mc = a.maskC + m * a.mms + (int64_t)t * K * GPLANE + q;
+    double* __restrict__ out = a.out + m * a.ms3 + (int64_t)t * K * GPLANE + q;
+    // the column as a window over k: T(k-1) .. T(k+2), maskC(k-1) .. maskC(k+2) (0 outside 0 .. kmax-1); each level
+    // loads one new localTij, maskC and w value (the expressions are those of the formulas above)
+    auto Tk = [&](int k) { return (k >= 0 && k < kmax) ? lh[(int64_t)k * GPLANE] : 0.0; };
+    auto Mk = [&](int k) { return (k >= 0 && k < kmax) ? mc[(int64_t)k * GPLANE] : 0.0; };
+    double Tm1 = 0.0, T0 = Tk(0), Tp1 = Tk(1), Tp2 = Tk(2);
+    double Mm1 = 0.0, M0 = Mk(0), Mp1 = Mk(1), Mp2 = Mk(2);
+    // interface k + 1 from the window centred on level k: mk(k+1) = M0 Mp1, mk(k+2) = Mp1 Mp2, mk(k) = Mm1 M0
+    auto iface = [&](int k, double& wTk, double& rTk) {
+        wTk = 0.0;
+        rTk = 0.0;
+        const int ki = k + 1;
+        if (ki < 1 || ki > kmax - 1) return;
+        const double wk = w[(int64_t)ki * GPLANE];
+        const double mk1 = M0 * Mp1;
+        rTk = wk * rA * mk1;
+        const double cfl = fabs(wk * a.dTlev[ki] * a.recip_drC[ki]);
+        // upstream for positive (upward) transport: cell k+1, then k+2; downstream: k, then k-1
+        wTk = dst3_flux(rTk, cfl, a.oneSixth, Tp2, Tp1, T0, Tm1, Mp1 * Mp2, mk1, Mm1 * M0);
     };
-    double wTop = wT(0), rTop = rTr(0);
+    double wTop = 0.0, rTop = 0.0;  // interface 0: the surface
     for (int k = 0; k < kmax; ++k) {
-        const double wBot = wT(k + 1), rBot = rTr(k + 1);
-        const int64_t o = off3(k);
-        const double tr = a.tr[o];
-        const double cf = a.dTlev[k] * a.recip_rhoFacC[k] * a.recip_hFacC[o] * a.recip_drF[k] * rrA * a.recip_deepFac2C[k];
-        const double loc = loc_h[o] - cf * ((wTop - wBot) - tr * (rTop - rBot)) * mC(k);
-        a.out[o] = (loc - tr) / a.dTlev[k];
+        double wBot, rBot;
+        iface(k, wBot, rBot);
+        const double trk = tr[(int64_t)k * GPLANE];
+        const double cf = a.dTlev[k] * a.recip_rhoFacC[k] * rhC[(int64_t)k * GPLANE] * a.recip_drF[k] * rrA *
+                          a.recip_deepFac2C[k];
+        const double loc = T0 - cf * ((wTop - wBot) - trk * (rTop - rBot)) * M0;
+        out[(int64_t)k * GPLANE] = (loc - trk) / a.dTlev[k];
         wTop = wBot;
         rTop = rBot;
+        Tm1 = T0;
+        T0 = Tp1;
+        Tp1 = Tp2;
+        Tp2 = Tk(k + 3);
+        Mm1 = M0;
+        M0 = Mp1;
+        Mp1 = Mp2;
+        Mp2 = Mk(k + 3);
     }
 }
 

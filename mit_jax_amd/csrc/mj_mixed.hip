@@ -15,9 +15,11 @@
 //
 // Work decomposition as the FP64 solve: one block per (member, tile), rings computed from their interior sources,
 // per-tile partials summed in tile order. The FP32 work vectors (r32, s, d, z, q) are the solver's own and live in a
-// compact layout: per member and tile the interior only, row-major sNy x sNx, tiles 128-byte aligned. The pointwise
-// updates stream them as flat 16-byte accesses; the stencil operand is staged in an LDS tile whose ring is computed
-// from the sources' compact values; the stencils stream the FP32 coefficient rows down each wave's strip.
+// compact layout (mx_geo: per member and tile the interior rows padded to a multiple of 4 floats, then copies of the
+// west and east columns). A tile is NS float4 "slots" (row j, columns 4c..4c+3); every phase works on slots: the
+// pointwise updates stream them flat, the stencil operand goes to an LDS tile (ring computed from the sources'
+// compact values), and the stencil gives each lane one slot per 64-slot block, reading its neighbours from LDS and
+// its FP32 coefficients (own 4-aligned copies, mx_geo) as float4 loads issued before the pointwise phase.
 //
 // Members move through their own state machine, one state per launch ("step"): every block of a member derives the
 // member's action for the step from the member's record of the previous step and the partials, so all tiles of a
@@ -30,10 +32,10 @@
 // (z and q are kept as FP32 vectors, so each inner phase applies one stencil and reads one coefficient set.)
 //   TRUE  x's ring from its sources; r = bn - A x; err partials
 //
-// Inner dot products (FP64): per tile, the rows in fixed groups of kRowGroup consecutive rows; each column pair is
-// accumulated down the group's rows in row order, the group's column sums are reduced by the row tree of the
-// "tile" order (pair_tree), and the group sums are added in group order. The order depends on the tile shape only,
-// not on the launch geometry. The true-residual sums (FP64 refinement) use the "tile" order itself.
+// Inner dot products (FP64): per tile, the slots in blocks of 64 consecutive slots (row-major); each slot's four
+// products are added in column order, a block's 64 slot sums are reduced by the wave tree of the "tile" order
+// (wave_tree, slot k of the block in lane k), and the block sums are added in block order. The order depends on the
+// tile shape only, not on the launch geometry. The true-residual sums (FP64 refinement) use the "tile" order itself.
 #include <algorithm>
 
 #include "mj_cg_sweep.h"
@@ -51,7 +53,7 @@ enum { AX_FINISH = 1, AX_RESTART, AX_UPD, AX_IB, AX_IC, AX_TRUE };
 
 struct MxDev {
     int B, T, sNx, sNy, OL, nx, ny, plane;
-    int64_t ms, oms, cms;  // member stride; FP64 operator slot stride; FP32 coefficient slot stride
+    int64_t ms, oms, cms;  // member stride; FP64 operator slot stride; FP32 coefficient slot stride (T * G.TSc)
     const double *aW, *aS, *aC;
     const float *aW32, *aS32, *aC32, *pW32, *pS32, *pC32;
     const double *b, *x0;
@@ -65,8 +67,8 @@ struct MxDev {
     const int8_t* ring_sign;
     const int32_t *cring_lds, *cring_src;  // ring entries in compact terms (LDS offset, compact source or -1)
     double* r64;
-    int TS;      // compact tile stride (elements)
-    int64_t cs;  // compact member stride = T * TS
+    MxGeo G;     // compact geometry of the work vectors and coefficient copies
+    int64_t cs;  // compact member stride = T * G.TS
     float *r0, *r1, *s0, *s1, *d, *z, *q;  // compact
     double *pmax, *psum, *perr64, *perr, *peta, *psig;  // [B*T] x3, [2][B*T] x3
     const int32_t* op_index;
@@ -87,110 +89,177 @@ __device__ __forceinline__ int slot_of(const MxDev& a, int m) { return a.op_inde
 __device__ __forceinline__ float* R32(const MxDev& a, int n) { return n & 1 ? a.r1 : a.r0; }
 __device__ __forceinline__ float* S32(const MxDev& a, int n) { return n & 1 ? a.s1 : a.s0; }
 // compact offset of member m, tile t
-__device__ __forceinline__ int64_t cbase(const MxDev& a, int m, int t) { return m * a.cs + (int64_t)t * a.TS; }
-
+__device__ __forceinline__ int64_t cbase(const MxDev& a, int m, int t) { return m * a.cs + (int64_t)t * a.G.TS; }
+// FP32 coefficient copies of member m's operator slot at tile t
+__device__ __forceinline__ int64_t coef_base(const MxDev& a, int m, int t) {
+    return (int64_t)slot_of(a, m) * a.cms + (int64_t)t * a.G.TSc;
+}
 __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
 __device__ __forceinline__ void st4(float* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
-// Row and column of compact element e of a tile (exact: e < 2^14, the fraction of (e + 0.5) / sNx stays >= 1/256
-// away from an integer).
-__device__ __forceinline__ int crow(int e, float inv_sNx) { return (int)((float(e) + 0.5f) * inv_sNx); }
-// The four compact elements e .. e+3 (e a multiple of 4, sNx even) into the LDS tile: two column pairs, each within
-// one row.
-__device__ __forceinline__ void lds_put4(float* tile, const LdsGeo& G, int e, float4 v, int sNx, float inv_sNx) {
-    const int j0 = crow(e, inv_sNx), j1 = crow(e + 2, inv_sNx);
-    st2(tile + G.at(j0, e - j0 * sNx), make_float2(v.x, v.y));
-    st2(tile + G.at(j1, e + 2 - j1 * sNx), make_float2(v.z, v.w));
+// Row of slot f of a tile (exact: f < 2^14, and the fraction of (f + 0.5) / NC stays >= 1/64 away from an integer).
+__device__ __forceinline__ int srow(int f, float inv_nc) { return (int)((float(f) + 0.5f) * inv_nc); }
+// The components of a slot at columns >= sNx (padding) set to 0.
+__device__ __forceinline__ float4 mask_pad(float4 v, int c4, int sNx) {
+    if (c4 + 4 <= sNx) return v;
+    v.w = 0.0f;
+    if (c4 + 2 >= sNx) v.z = 0.0f;
+    if (c4 + 1 >= sNx) v.y = 0.0f;
+    return v;
+}
+// A slot's values into the LDS tile (its columns < sNx only: column sNx belongs to the ring).
+__device__ __forceinline__ void lds_put(float* tile, const MxGeo& G, int j, int c4, float4 v, int sNx) {
+    float* p = tile + G.lds(j, c4);
+    if (c4 + 4 <= sNx) {
+        st4(p, v);
+        return;
+    }
+    p[0] = v.x;
+    if (c4 + 1 < sNx) p[1] = v.y;
+    if (c4 + 2 < sNx) p[2] = v.z;
+}
+// The west/east column copies of a tile's compact vector (base: the tile's compact start) for one slot.
+__device__ __forceinline__ void edge_put(float* base, const MxGeo& G, int j, int c4, float4 v, int sNx, int sNy) {
+    float* ed = base + sNy * G.CP;
+    if (c4 == 0) ed[j] = v.x;
+    const int k = sNx - 1 - c4;
+    if (k >= 0 && k < 4) ed[sNy + j] = k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
 
-// The west/east column copies of a tile's compact vector (base: the tile's compact start) for the four elements
-// e .. e+3 (two column pairs, each within one row; column 0 is the first of a pair, column sNx-1 the second).
-__device__ __forceinline__ void edge_put4(float* base, int e, float4 v, int sNx, int sNy, float inv_sNx) {
-    if (!MJ_MX_EDGES) return;
-    const int j0 = crow(e, inv_sNx), j1 = crow(e + 2, inv_sNx);
-    const int c0 = e - j0 * sNx, c1 = e + 2 - j1 * sNx;
-    float* ed = base + sNx * sNy;
-    if (c0 == 0) ed[j0] = v.x;
-    if (c0 + 1 == sNx - 1) ed[sNy + j0] = v.y;
-    if (c1 == 0) ed[j1] = v.z;
-    if (c1 + 1 == sNx - 1) ed[sNy + j1] = v.w;
+// The FP32 coefficients one stencil slot reads: W (aW or pW at columns c4..c4+3), S0 / S1 (aS or pS of rows j and
+// j + 1), C (aC or pC), and wr = W at column c4 + 4 for the lanes that cannot take it from the next lane (lane 63,
+// and the last chunk of a row when the row has no padding).
+struct MxCoef {
+    float4 W, S0, S1, C;
+    float wr;
+};
+__device__ __forceinline__ bool own_wr(int lane, int c4, const MxGeo& G, int sNx) {
+    return (lane == 63 || c4 + 4 == G.CP) && c4 + 4 <= sNx;
 }
-__device__ __forceinline__ void edge_put2(float* base, int j, int c, float2 v, int sNx, int sNy) {
-    if (!MJ_MX_EDGES) return;
-    float* ed = base + sNx * sNy;
-    if (c == 0) ed[j] = v.x;
-    if (c + 1 == sNx - 1) ed[sNy + j] = v.y;
+__device__ __forceinline__ MxCoef coef_load(const MxGeo& G, int sNx, const float* __restrict__ cW,
+                                            const float* __restrict__ cS, const float* __restrict__ cC, int f,
+                                            float inv) {
+    const int lane = threadIdx.x & 63;
+    const int fi = f < G.NS ? f : G.NS - 1;
+    const int j = srow(fi, inv), c4 = 4 * (fi - j * G.NC);
+    const int qc = j * G.CPc + c4;
+    MxCoef k;
+    k.W = ld4(cW + qc);
+    k.S0 = ld4(cS + qc);
+    k.S1 = ld4(cS + qc + G.CPc);
+    k.C = ld4(cC + qc);
+    k.wr = own_wr(lane, c4, G, sNx) ? cW[qc + 4] : 0.0f;
+    return k;
 }
 
-// Stencil over the LDS tile: IS_A: A32 = (W, S, C) = (aW, aS, aC); else M32 = (pW, pS, pC). Rows go in groups of
-// kRowGroup, group g to wave g % nwaves; the result of each row goes to out (compact, row j at j * sNx) and
-// prods(Xc, v, pv) gives the row's NR per-point products, reduced in the inner dot-product order; tile sum r of
-// the block is written to *dst[r].
-#ifndef MJ_ROW_GROUP
-#define MJ_ROW_GROUP 6
-#endif
-constexpr int kRowGroup = MJ_ROW_GROUP;
+// IS_A: apply_operator (core/cg2d.py:181-188) term order; else apply_preconditioner (:191-198), on four columns.
+template <bool IS_A>
+__device__ __forceinline__ float4 stencil4(const MxCoef& k, float wr, float4 Xm, float4 Xc, float4 Xp, float xl,
+                                           float xr) {
+    float4 o;
+    if (IS_A) {
+        o.x = k.W.x * xl + k.W.y * Xc.y + k.S0.x * Xm.x + k.S1.x * Xp.x + k.C.x * Xc.x;
+        o.y = k.W.y * Xc.x + k.W.z * Xc.z + k.S0.y * Xm.y + k.S1.y * Xp.y + k.C.y * Xc.y;
+        o.z = k.W.z * Xc.y + k.W.w * Xc.w + k.S0.z * Xm.z + k.S1.z * Xp.z + k.C.z * Xc.z;
+        o.w = k.W.w * Xc.z + wr * xr + k.S0.w * Xm.w + k.S1.w * Xp.w + k.C.w * Xc.w;
+    } else {
+        o.x = k.C.x * Xc.x + k.W.x * xl + k.W.y * Xc.y + k.S0.x * Xm.x + k.S1.x * Xp.x;
+        o.y = k.C.y * Xc.y + k.W.y * Xc.x + k.W.z * Xc.z + k.S0.y * Xm.y + k.S1.y * Xp.y;
+        o.z = k.C.z * Xc.z + k.W.z * Xc.y + k.W.w * Xc.w + k.S0.z * Xm.z + k.S1.z * Xp.z;
+        o.w = k.C.w * Xc.w + k.W.w * Xc.z + wr * xr + k.S0.w * Xm.w + k.S1.w * Xp.w;
+    }
+    return o;
+}
+// FP64 dot-product term of one slot: the four products in column order.
+__device__ __forceinline__ double dot4(float4 u, float4 v) {
+    const double p0 = (double)u.x * (double)v.x, p1 = (double)u.y * (double)v.y;
+    const double p2 = (double)u.z * (double)v.z, p3 = (double)u.w * (double)v.w;
+    return ((p0 + p1) + p2) + p3;
+}
+
+// Stencil over the LDS tile (IS_A: A32 = (W, S, C) = (aW, aS, aC); else M32 = (pW, pS, pC); coefficient pointers
+// at the slot's tile). Wave w takes the 64-slot blocks w, w + nwaves, ...; the coefficients of the next block are
+// loaded while the current one is computed (the first block's, `pre`, before the caller's pointwise phase). The
+// result of each slot goes to out (compact tile) with its column copies; prods(X, o, pv) gives the slot's NR
+// dot-product terms (X the operand with padding zeroed, o the result), reduced in the inner dot-product order; the
+// tile sum r is written to *dst[r].
 template <bool IS_A, int NR, class F>
-__device__ __forceinline__ void mx_stencil(const MxDev& a, int t, const float* __restrict__ cW,
-                                           const float* __restrict__ cS, const float* __restrict__ cC,
-                                           const float* tile, float* __restrict__ out, F&& prods,
-                                           double* const (&dst)[NR]) {
-    MJ_UNIT_IDS
-    const Lanes L = lanes_of(lane, sNx);
-    const LdsGeo G{sNx + 4};
-    const int ce = east_col(L, sNx);
-    const int ng = (sNy + kRowGroup - 1) / kRowGroup;
-    for (int g = wave; g < ng; g += nwaves) {
-        const int j0 = g * kRowGroup, j1 = min(j0 + kRowGroup, sNy);
-        double2 acc[NR];
-        int pr = MJ_ROW(j0);
-        // the coefficient rows of row j + 1 are loaded while row j is computed
-        float2 S0 = ld2(cS + pr + L.c), S1n = ld2(cS + pr + nx + L.c), Cn = ld2(cC + pr + L.c), Wn = ld2(cW + pr + ce);
-        for (int j = j0; j < j1; ++j, pr += nx) {
-            const int p = pr + L.c;
-            const float2 S1 = S1n, C = Cn, Wc = Wn;
-            if (j + 1 < j1) {
-                S1n = ld2(cS + p + 2 * nx);
-                Cn = ld2(cC + p + nx);
-                Wn = ld2(cW + pr + nx + ce);
-            }
-            float wr = __shfl_down(Wc.x, 1, 64);
-            if (L.last == 63 && lane == 63) wr = cW[pr + sNx];
-            const int q0 = G.at(j, L.c);
-            const float2 Xc = ld2(tile + q0), Xm = ld2(tile + q0 - G.W), Xp = ld2(tile + q0 + G.W);
-            const float xl = tile[q0 - 1], xr = tile[q0 + 2];
-            const float2 v = IS_A ? pair_A(Wc, wr, S0, S1, C, Xm, Xc, Xp, xl, xr)
-                                  : pair_M(C, Wc, wr, S0, S1, Xm, Xc, Xp, xl, xr);
-            if (L.act) {
-                st2(out + j * sNx + L.c, v);
-                edge_put2(out, j, L.c, v, sNx, sNy);
-            }
-            double2 pv[NR];
-            prods(Xc, v, pv);
-#pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                if (!L.act) pv[k] = make_double2(0.0, 0.0);
-                if (j == j0) {
-                    acc[k] = pv[k];
-                } else {
-                    acc[k].x = acc[k].x + pv[k].x;
-                    acc[k].y = acc[k].y + pv[k].y;
-                }
-            }
-            S0 = S1;
+__device__ __forceinline__ void mx_stencil(const MxDev& a, const float* __restrict__ cW, const float* __restrict__ cS,
+                                           const float* __restrict__ cC, const float* tile, float* __restrict__ out,
+                                           MxCoef pre, F&& prods, double* const (&dst)[NR]) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const MxGeo G = a.G;
+    const int sNx = a.sNx, sNy = a.sNy;
+    const float inv = 1.0f / (float)G.NC;
+    const int nblk = (G.NS + 63) / 64;
+    MxCoef cur = pre;
+    for (int blk = wave; blk < nblk; blk += nwaves) {
+        const int bn = blk + nwaves;
+        MxCoef nxt = cur;
+        if (bn < nblk) nxt = coef_load(G, sNx, cW, cS, cC, bn * 64 + lane, inv);
+        const int f = blk * 64 + lane;
+        const bool valid = f < G.NS;
+        const int fi = valid ? f : G.NS - 1;
+        const int j = srow(fi, inv), c4 = 4 * (fi - j * G.NC);
+        float wr = __shfl_down(cur.W.x, 1, 64);
+        if (own_wr(lane, c4, G, sNx)) wr = cur.wr;
+        const int q0 = G.lds(j, c4);
+        const float4 Xc = ld4(tile + q0), Xm = ld4(tile + q0 - G.W), Xp = ld4(tile + q0 + G.W);
+        const float xl = tile[q0 - 1], xr = tile[q0 + 4];
+        const float4 o = mask_pad(stencil4<IS_A>(cur, wr, Xm, Xc, Xp, xl, xr), c4, sNx);
+        if (valid) {
+            st4(out + j * G.CP + c4, o);
+            edge_put(out, G, j, c4, o, sNx, sNy);
         }
+        double pv[NR];
+        prods(mask_pad(Xc, c4, sNx), o, pv);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            const double w = pair_tree(acc[k], sNx, lane);
-            if (lane == 0) g_rows[k * 128 + g] = w;
+            const double w = wave_tree(valid ? pv[k] : 0.0, 0.0, false);
+            if (lane == 0) g_rows[k * 128 + blk] = w;
         }
+        cur = nxt;
     }
     __syncthreads();
     if (tid < NR) {
         double sum = g_rows[tid * 128];
-        for (int g = 1; g < ng; ++g) sum = sum + g_rows[tid * 128 + g];
+        for (int b = 1; b < nblk; ++b) sum = sum + g_rows[tid * 128 + b];
         *dst[tid] = sum;
     }
     __syncthreads();
+}
+
+// This thread's first ring entry of tile t (LDS offset, compact or padded source, sign), loaded ahead of the
+// pointwise phase; the rest of the ring (blocks smaller than the ring) is read as it is filled.
+struct MxRing {
+    int k, src, lds;
+    bool neg;
+};
+__device__ __forceinline__ MxRing ring_first(const MxDev& a, int t, const int32_t* src_table) {
+    MxRing r;
+    r.k = a.ring_off[t] + (int)threadIdx.x;
+    r.src = -1;
+    r.lds = 0;
+    r.neg = false;
+    if (r.k < a.ring_off[t + 1]) {
+        r.src = src_table[r.k];
+        r.lds = a.cring_lds[r.k];
+        r.neg = a.ring_sign[r.k] < 0;
+    }
+    return r;
+}
+// Fills the ring of the LDS tile: value(s) of each entry's source (0 without one), signed.
+template <class V>
+__device__ __forceinline__ void ring_fill(const MxDev& a, int t, const MxRing& r0, const int32_t* src_table,
+                                          float* tile, V&& value) {
+    if (r0.k < a.ring_off[t + 1]) {
+        const float v = r0.src < 0 ? 0.0f : value(r0.src);
+        tile[r0.lds] = r0.neg ? -v : v;
+    }
+    for (int k = r0.k + blockDim.x; k < a.ring_off[t + 1]; k += blockDim.x) {
+        const int s = src_table[k];
+        const float v = s < 0 ? 0.0f : value(s);
+        tile[a.cring_lds[k]] = a.ring_sign[k] < 0 ? -v : v;
+    }
 }
 
 // I1: x = normalised first guess with exchanged halos; r = bn - A x (FP64); partials of err and sumRHS; the
@@ -237,64 +306,68 @@ __device__ __forceinline__ void mx_finish(const MxDev& a, int m, int t) {
     exchanged_first_guess(a, m, t, rhsNorm);
 }
 
-// Restart: r32 = r / sigma on the interior (compact) and the ring (LDS, from the sources' r), d = 0, z = M32 r32,
-// partial of eta(1) = z.r32.
+// Restart: r32 = r / sigma on the interior (compact, padding 0) and the ring (LDS, from the sources' r), d = 0,
+// z = M32 r32, partial of eta(1) = z.r32.
 __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double sigma) {
-    MJ_UNIT_IDS
-    MJ_STRIP
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int sNx = a.sNx, sNy = a.sNy, nx = a.nx, OL = a.OL, tbase = t * a.plane;
     float* tile = reinterpret_cast<float*>(mj_lds);
-    const LdsGeo G{sNx + 4};
+    const MxGeo G = a.G;
+    const float inv = 1.0f / (float)G.NC;
     const double* r = a.r64 + m * a.ms;
-    const int64_t cb = cbase(a, m, t);
+    const int64_t cb = cbase(a, m, t), oc = coef_base(a, m, t);
+    const float *cW = a.pW32 + oc, *cS = a.pS32 + oc, *cC = a.pC32 + oc;
+    const MxCoef pre = coef_load(G, sNx, cW, cS, cC, wave * 64 + lane, inv);
+    const MxRing rr = ring_first(a, t, a.ring_src);
     float* r32 = R32(a, 0) + cb;
     float* d = a.d + cb;
-    for (int j = j0; j < j1; ++j) {
-        const double2 R = ld2(r + MJ_ROW(j) + L.c);
-        const float2 v = make_float2((float)(R.x / sigma), (float)(R.y / sigma));
-        if (L.act) {
-            st2(r32 + j * sNx + L.c, v);
-            edge_put2(r32, j, L.c, v, sNx, sNy);
-            st2(tile + G.at(j, L.c), v);
-        }
+    for (int f = tid; f < G.NS; f += blockDim.x) {
+        const int e = 4 * f, j = srow(f, inv), c4 = e - j * G.CP;
+        const int p = MJ_ROW(j) + c4;
+        const double2 R0 = ld2(r + p), R1 = ld2(r + p + 2);
+        const float4 v = mask_pad(make_float4((float)(R0.x / sigma), (float)(R0.y / sigma), (float)(R1.x / sigma),
+                                              (float)(R1.y / sigma)), c4, sNx);
+        st4(r32 + e, v);
+        edge_put(r32, G, j, c4, v, sNx, sNy);
+        lds_put(tile, G, j, c4, v, sNx);
+        st4(d + e, make_float4(0.f, 0.f, 0.f, 0.f));
     }
-    const int nf4 = sNx * sNy / 4;
-    for (int f = tid; f < nf4; f += blockDim.x) st4(d + 4 * f, make_float4(0.f, 0.f, 0.f, 0.f));
-    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const int s = a.ring_src[k];
-        const float v = s < 0 ? 0.0f : (float)(r[s] / sigma);
-        tile[a.cring_lds[k]] = a.ring_sign[k] < 0 ? -v : v;
-    }
+    ring_fill(a, t, rr, a.ring_src, tile, [&](int s) { return (float)(r[s] / sigma); });
     __syncthreads();
-    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
     double* const dst[1] = {a.peta + a.B * a.T + m * a.T + t};  // eta(1)
     mx_stencil<false, 1>(
-        a, t, a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, tile, a.z + cb,
-        [](float2 Rc, float2 q, double2(&pv)[1]) {
-            pv[0] = make_double2((double)q.x * (double)Rc.x, (double)q.y * (double)Rc.y);
-        },
-        dst);
+        a, cW, cS, cC, tile, a.z + cb, pre, [](float4 R, float4 zz, double(&pv)[1]) { pv[0] = dot4(zz, R); }, dst);
 }
 
 // Solution update of a finished inner solve at B(n): the pending d += alpha_{n-1} s_{n-1} (n > 1), then
 // x = x + sigma * d on the interior (FP64).
 __device__ __forceinline__ void mx_update(const MxDev& a, int m, int t, int n, double sigma, double alpha64) {
-    MJ_UNIT_IDS
-    MJ_STRIP
+    const int tid = threadIdx.x;
+    const int sNx = a.sNx, nx = a.nx, OL = a.OL, tbase = t * a.plane;
+    const MxGeo G = a.G;
+    const float inv = 1.0f / (float)G.NC;
     double* x = a.x + m * a.ms;
     const int64_t cb = cbase(a, m, t);
     const float* d = a.d + cb;
     const float* so = S32(a, n - 1) + cb;
     const float alpha = (float)alpha64;
-    for (int j = j0; j < j1; ++j) {
-        const int p = MJ_ROW(j) + L.c;
-        const double2 X = ld2(x + p);
-        float2 D = ld2(d + j * sNx + L.c);
+    for (int f = tid; f < G.NS; f += blockDim.x) {
+        const int e = 4 * f, j = srow(f, inv), c4 = e - j * G.CP;
+        float4 D = ld4(d + e);
         if (n > 1) {
-            const float2 S = ld2(so + j * sNx + L.c);
+            const float4 S = ld4(so + e);
             D.x = D.x + alpha * S.x;
             D.y = D.y + alpha * S.y;
+            D.z = D.z + alpha * S.z;
+            D.w = D.w + alpha * S.w;
         }
-        if (L.act) st2(x + p, make_double2(X.x + sigma * (double)D.x, X.y + sigma * (double)D.y));
+        const int p = MJ_ROW(j) + c4;
+        const double2 X0 = ld2(x + p);
+        st2(x + p, make_double2(X0.x + sigma * (double)D.x, X0.y + sigma * (double)D.y));
+        if (c4 + 2 < sNx) {  // sNx is even: a slot holds 2 or 4 interior columns
+            const double2 X1 = ld2(x + p + 2);
+            st2(x + p + 2, make_double2(X1.x + sigma * (double)D.z, X1.y + sigma * (double)D.w));
+        }
     }
 }
 
@@ -345,17 +418,22 @@ __device__ __forceinline__ void mx_true(const MxDev& a, int m, int t) {
 // Inner phase B(n): d += alpha_{n-1} s_{n-1} and s_n = z + beta s_{n-1} (s_1 = z) on the interior (compact, flat),
 // s_n on the ring (LDS), q = A32 s_n, sigma_n partial (s_n, q).
 __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64, double alpha64) {
-    MJ_UNIT_IDS
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int sNx = a.sNx, sNy = a.sNy;
     float* tile = reinterpret_cast<float*>(mj_lds);
-    const LdsGeo G{sNx + 4};
-    const float inv = 1.0f / (float)sNx;
-    const int64_t cm = m * a.cs, cb = cbase(a, m, t), ot = (int64_t)t * a.TS;
-    const float *z = a.z + cm, *so = S32(a, n - 1) + cm;
-    float *sn = S32(a, n) + cb, *d = a.d + cb;
+    const MxGeo G = a.G;
+    const float inv = 1.0f / (float)G.NC;
+    const int64_t cm = m * a.cs, cb = cbase(a, m, t), ot = (int64_t)t * G.TS, oc = coef_base(a, m, t);
+    const float *cW = a.aW32 + oc, *cS = a.aS32 + oc, *cC = a.aC32 + oc;
+    const MxCoef pre = coef_load(G, sNx, cW, cS, cC, wave * 64 + lane, inv);
+    const MxRing rr = ring_first(a, t, a.cring_src);
+    const float* __restrict__ z = a.z + cm;
+    const float* __restrict__ so = S32(a, n - 1) + cm;
+    float* __restrict__ sn = S32(a, n) + cb;
+    float* __restrict__ d = a.d + cb;
     const float beta = (float)beta64, alpha = (float)alpha64;
     const bool first = n == 1;
-    const int nf4 = sNx * sNy / 4;
-    for (int f = tid; f < nf4; f += blockDim.x) {
+    for (int f = tid; f < G.NS; f += blockDim.x) {
         const int e = 4 * f;
         float4 v = ld4(z + ot + e);
         if (!first) {
@@ -372,39 +450,34 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
             v.w = v.w + beta * O.w;
         }
         st4(sn + e, v);
-        edge_put4(sn, e, v, sNx, sNy, inv);
-        lds_put4(tile, G, e, v, sNx, inv);
+        const int j = srow(f, inv), c4 = e - j * G.CP;
+        edge_put(sn, G, j, c4, v, sNx, sNy);
+        lds_put(tile, G, j, c4, v, sNx);
     }
-    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const int s = a.cring_src[k];
-        float v = 0.0f;
-        if (s >= 0) v = first ? z[s] : z[s] + beta * so[s];
-        tile[a.cring_lds[k]] = a.ring_sign[k] < 0 ? -v : v;
-    }
+    ring_fill(a, t, rr, a.cring_src, tile, [&](int s) { return first ? z[s] : z[s] + beta * so[s]; });
     __syncthreads();
-    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
     double* const dst[1] = {a.psig + (n & 1) * a.B * a.T + m * a.T + t};
     mx_stencil<true, 1>(
-        a, t, a.aW32 + oc, a.aS32 + oc, a.aC32 + oc, tile, a.q + cb,
-        [](float2 Xc, float2 q, double2(&pv)[1]) {
-            pv[0] = make_double2((double)Xc.x * (double)q.x, (double)Xc.y * (double)q.y);
-        },
-        dst);
+        a, cW, cS, cC, tile, a.q + cb, pre, [](float4 X, float4 q, double(&pv)[1]) { pv[0] = dot4(X, q); }, dst);
 }
 
 // Inner phase C(n): r32_n = r32_{n-1} - alpha q on the interior (compact, flat) and the ring (LDS); z = M32 r32_n;
 // partials of err(n) = (r32_n, r32_n) and eta(n+1) = (z, r32_n).
 __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, double alpha64) {
-    MJ_UNIT_IDS
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int sNx = a.sNx, sNy = a.sNy;
     float* tile = reinterpret_cast<float*>(mj_lds);
-    const LdsGeo G{sNx + 4};
-    const float inv = 1.0f / (float)sNx;
-    const int64_t cm = m * a.cs, cb = cbase(a, m, t), ot = (int64_t)t * a.TS;
-    const float *qv = a.q + cm, *rin = R32(a, n - 1) + cm;
-    float* rout = R32(a, n) + cb;
+    const MxGeo G = a.G;
+    const float inv = 1.0f / (float)G.NC;
+    const int64_t cm = m * a.cs, cb = cbase(a, m, t), ot = (int64_t)t * G.TS, oc = coef_base(a, m, t);
+    const float *cW = a.pW32 + oc, *cS = a.pS32 + oc, *cC = a.pC32 + oc;
+    const MxCoef pre = coef_load(G, sNx, cW, cS, cC, wave * 64 + lane, inv);
+    const MxRing rr = ring_first(a, t, a.cring_src);
+    const float* __restrict__ qv = a.q + cm;
+    const float* __restrict__ rin = R32(a, n - 1) + cm;
+    float* __restrict__ rout = R32(a, n) + cb;
     const float alpha = (float)alpha64;
-    const int nf4 = sNx * sNy / 4;
-    for (int f = tid; f < nf4; f += blockDim.x) {
+    for (int f = tid; f < G.NS; f += blockDim.x) {
         const int e = 4 * f;
         const float4 Q = ld4(qv + ot + e), R = ld4(rin + ot + e);
         float4 rn;
@@ -413,23 +486,19 @@ __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, doubl
         rn.z = R.z - alpha * Q.z;
         rn.w = R.w - alpha * Q.w;
         st4(rout + e, rn);
-        edge_put4(rout, e, rn, sNx, sNy, inv);
-        lds_put4(tile, G, e, rn, sNx, inv);
+        const int j = srow(f, inv), c4 = e - j * G.CP;
+        edge_put(rout, G, j, c4, rn, sNx, sNy);
+        lds_put(tile, G, j, c4, rn, sNx);
     }
-    for (int k = a.ring_off[t] + tid; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const int s = a.cring_src[k];
-        const float v = s < 0 ? 0.0f : rin[s] - alpha * qv[s];
-        tile[a.cring_lds[k]] = a.ring_sign[k] < 0 ? -v : v;
-    }
+    ring_fill(a, t, rr, a.cring_src, tile, [&](int s) { return rin[s] - alpha * qv[s]; });
     __syncthreads();
-    const int64_t oc = (int64_t)slot_of(a, m) * a.cms;
     const int BT = a.B * a.T;
     double* const dst[2] = {a.perr + (n & 1) * BT + m * a.T + t, a.peta + ((n + 1) & 1) * BT + m * a.T + t};
     mx_stencil<false, 2>(
-        a, t, a.pW32 + oc, a.pS32 + oc, a.pC32 + oc, tile, a.z + cb,
-        [](float2 Rc, float2 zz, double2(&pv)[2]) {
-            pv[0] = make_double2((double)Rc.x * (double)Rc.x, (double)Rc.y * (double)Rc.y);
-            pv[1] = make_double2((double)zz.x * (double)Rc.x, (double)zz.y * (double)Rc.y);
+        a, cW, cS, cC, tile, a.z + cb, pre,
+        [](float4 R, float4 zz, double(&pv)[2]) {
+            pv[0] = dot4(R, R);
+            pv[1] = dot4(zz, R);
         },
         dst);
 }
@@ -454,8 +523,8 @@ __device__ __forceinline__ double lane_tile_sum(double v, int T) {
 
 // One step of every member's state machine.
 #ifndef MJ_PH_WAVES_PER_EU
-#define MJ_PH_WAVES_PER_EU 8
-#endif
+#define MJ_PH_WAVES_PER_EU 7
+#endif  // 72 VGPRs, no spills; 4 resident blocks of up to 448 threads per CU (the LDS tile allows 4)
 __global__ void __launch_bounds__(MJ_MX_THREADS) __attribute__((amdgpu_waves_per_eu(MJ_PH_WAVES_PER_EU)))
 mx_step_kernel(MxDev a, int k) {
     int m, t;
@@ -623,16 +692,21 @@ mx_step_kernel(MxDev a, int k) {
     }
 }
 
-// FP32 copies of the six coefficient arrays of every slot: dst[k][slot][q].
+// FP32 copies of the six coefficient arrays of every slot in the coefficient layout of mx_geo:
+// dst[k][slot][t][j * CPc + i] = src_k[slot][t][OL + j][OL + i] for j <= sNy, i <= sNx, 0 in the padding columns.
 __global__ void mx_convert_kernel(const double* aW, const double* aS, const double* aC, const double* pW,
-                                  const double* pS, const double* pC, int64_t oms, int nslots, int64_t tp,
-                                  float* dst) {
+                                  const double* pS, const double* pC, int64_t oms, int nslots, int T, int OL, int nx,
+                                  int plane, int sNx, MxGeo G, float* dst) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
-    if (q >= tp) return;
-    const double* src[6] = {aW, aS, aC, pW, pS, pC};
+    const int64_t per_slot = (int64_t)T * G.TSc;
+    if (q >= per_slot) return;
+    const int t = (int)(q / G.TSc), r = (int)(q - (int64_t)t * G.TSc), j = r / G.CPc, i = r - j * G.CPc;
+    const bool inside = i <= sNx;
+    const int64_t src = s * oms + (int64_t)t * plane + (int64_t)(OL + j) * nx + OL + (inside ? i : 0);
+    const double* from[6] = {aW, aS, aC, pW, pS, pC};
 #pragma unroll
-    for (int k = 0; k < 6; ++k) dst[((int64_t)k * nslots + s) * tp + q] = (float)src[k][s * oms + q];
+    for (int k = 0; k < 6; ++k) dst[((int64_t)k * nslots + s) * per_slot + q] = inside ? (float)from[k][src] : 0.0f;
 }
 
 __global__ void mx_reset_kernel(int* nactive, int B) {
@@ -663,8 +737,9 @@ MxLayout mx_layout(const mj_cg2d_config* c, int nslots) {
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     MxLayout L{};
     const size_t n = (size_t)c->batch * c->member_stride;
-    const size_t nc = (size_t)c->batch * c->ntiles * compact_tile_stride(c->sNx, c->sNy);  // compact vectors
-    const size_t tp = (size_t)c->ntiles * (c->sNx + 2 * c->OL) * (c->sNy + 2 * c->OL);
+    const MxGeo G = mx_geo(c->sNx, c->sNy);
+    const size_t nc = (size_t)c->batch * c->ntiles * G.TS;  // compact vectors
+    const size_t tp = (size_t)c->ntiles * G.TSc;             // one coefficient copy of one slot
     const size_t bt = al((size_t)c->batch * c->ntiles * 8);
     size_t o = 0;
     L.r64 = o; o += al(n * 8);
@@ -763,7 +838,8 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     a.plane = a.nx * a.ny;
     a.ms = cfg->member_stride;
     a.oms = ops->member_stride;
-    const int64_t tp = (int64_t)a.T * a.plane;
+    a.G = mx_geo(a.sNx, a.sNy);
+    const int64_t tp = (int64_t)a.T * a.G.TSc;  // one FP32 coefficient copy of one slot
     a.cms = ops->member_stride == 0 ? 0 : tp;
     a.aW = ops->aW2d;
     a.aS = ops->aS2d;
@@ -794,8 +870,7 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     a.ring_sign = P->cg_ring_sign;
     a.cring_lds = P->cg_cring_lds;
     a.cring_src = P->cg_cring_src;
-    a.TS = compact_tile_stride(a.sNx, a.sNy);
-    a.cs = (int64_t)a.T * a.TS;
+    a.cs = (int64_t)a.T * a.G.TS;
     a.r64 = (double*)(w + L.r64);
     a.r0 = (float*)(w + L.r0);
     a.r1 = (float*)(w + L.r1);
@@ -821,14 +896,14 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     const int nunits = a.B * a.T;
     const dim3 grid((unsigned)((nunits + kXcds - 1) / kXcds * kXcds)), block((unsigned)threads);
     // the step kernel's LDS tile of the stencil operand (FP32)
-    const size_t lds = lds_tile_bytes(cfg->sNx, cfg->sNy, (int)sizeof(float));
+    const size_t lds = mx_lds_bytes(cfg->sNx, cfg->sNy);
     if (lds > 65536)
         MJ_HIP(hipFuncSetAttribute((const void*)mx_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(mx_reset_kernel, dim3(1), dim3(64), 0, st, a.nactive, a.B);
     MJ_HIP(hipGetLastError());
     hipLaunchKernelGGL(mx_convert_kernel, dim3((unsigned)((tp + 255) / 256), (unsigned)nslots), dim3(256), 0, st,
-                       ops->aW2d, ops->aS2d, ops->aC2d, ops->pW, ops->pS, ops->pC, ops->member_stride, nslots, tp,
-                       coef);
+                       ops->aW2d, ops->aS2d, ops->aC2d, ops->pW, ops->pS, ops->pC, ops->member_stride, nslots, a.T,
+                       a.OL, a.nx, a.plane, a.sNx, a.G, coef);
     MJ_HIP(hipGetLastError());
     for (int ph = 0; ph < 2; ++ph) {
         hipLaunchKernelGGL(mx_init_kernel, grid, block, 0, st, a, ph);

@@ -239,7 +239,7 @@ def candidates(precision, B, T=13):
     for ov in (False, True):
         if precision == "mixed":
             for itol in (1e-4, 1e-5):
-                for bt in (128, 256, 512, 1024):
+                for bt in (256, 448, 512, 1024):
                     out.append(StepConfig("mixed", "phases", bt, 64, itol, overlap=ov))
         else:
             for bt in (128, 256, 512, 1024):
