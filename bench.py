@@ -303,6 +303,16 @@ def implicit_r(E, dev, reps=5):
             "column_integral_max_rel_change": cons}
 
 
+def copy_probe(dev):
+    """A 1 GiB device-to-device copy: the kernel of known traffic that scales the HBM byte counters."""
+    import torch
+    a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
 def solve_bytes(precision, d, op_index):
     """(algorithmic bytes, implementation bytes, operator-slot iterations) of one batched solve from its
     diagnostics (the counts of the module docstring)."""
@@ -431,6 +441,8 @@ def kernel_profile(args):
     E = build_ensemble(B, dev)
     params = gad.GADParams.llc90()
     st = make_step(E, params, args.precision, args)
+    if args.copy_probe:
+        copy_probe(dev)
     print("bench: kernel timings (solve alone, DST3 alone)", file=sys.stderr, flush=True)
     cg_ms, gad_ms, d = kernel_times(st)
     gb = dst3_bytes(E["nlev"], False)
@@ -495,11 +507,7 @@ def main():
                     gad_ms=float(np.mean([b.elapsed_time(c) for _, b, c in ev])))
 
     if args.copy_probe:
-        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+        copy_probe(dev)
     main_step, tuned = make(args.precision)
     run = timed(main_step)
     run["cg_ms"], run["gad_ms"], dk = kernel_times(main_step)

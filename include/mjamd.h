@@ -351,16 +351,20 @@ int mj_dst3_adv_y(const mj_dst3_flux_config* cfg, double oneSixth, const double*
                   const double* vFld, const double* maskLocS, const double* tracer, const double* recip_dyC,
                   const double* recip_deepFacC, double* vT, void* stream);
 
-/* Reverse mode of mj_gad_dst3 without the R flux (wFld NULL; the reference's advect_tiles, pkgs/gad.py:460-539,
- * differentiated as test_gad.py:297-346 does): for the cotangent gTracer_bar of gTracer, writes the cotangents of
- * tracer, uFld and vFld (every padded point of [B][T, nr, ny, nx]; levels at or beyond a member's nlev get 0).
- * The grid, hFac and mask inputs are constants. work: device, >= mj_gad_dst3_adjoint_workspace_bytes(). */
-size_t mj_gad_dst3_adjoint_workspace_bytes(void);
+/* Reverse mode of mj_gad_dst3 (the reference's advect_tiles, pkgs/gad.py:460-539, differentiated as
+ * test_gad.py:297-346 does; the R flux, which the reference does not have, differentiated the same way): for the
+ * cotangent gTracer_bar of gTracer, writes the cotangents of tracer, uFld and vFld, and with wFld that of wFld
+ * (every padded point of [B][T, nr, ny, nx]; levels at or beyond a member's nlev get 0; w at the surface
+ * interface, which the R flux does not read, gets 0). The grid, hFac and mask inputs are constants. With wFld the
+ * horizontal passes are replayed forward once (localTij into work), the vertical sweep is reversed per column,
+ * then the horizontal sweeps per (member, tile, level). work: device, >= mj_gad_dst3_adjoint_workspace_bytes(cfg,
+ * ntiles, wFld != NULL). Outputs must not alias inputs or each other. */
+size_t mj_gad_dst3_adjoint_workspace_bytes(const mj_gad_config* cfg, int ntiles, int with_w);
 int mj_gad_dst3_adjoint(const mj_gad_plan* plan, const mj_gad_grid* grid, const mj_gad_config* cfg,
-                        const double* uFld, const double* vFld, const double* tracer, const double* hFacW,
-                        const double* hFacS, const double* recip_hFacC, const double* gTracer_bar,
-                        double* tracer_bar, double* uFld_bar, double* vFld_bar, double* work, size_t work_bytes,
-                        void* stream);
+                        const double* uFld, const double* vFld, const double* wFld, const double* tracer,
+                        const double* hFacW, const double* hFacS, const double* recip_hFacC,
+                        const double* gTracer_bar, double* tracer_bar, double* uFld_bar, double* vFld_bar,
+                        double* wFld_bar, double* work, size_t work_bytes, void* stream);
 
 /* ======================================================================================= GAD_IMPLICIT_R
  * Replaces gad_implicit_r (core/tracer_implicit.py:165-173) with gad_implicit_r_matrix (:34-114) and

@@ -127,9 +127,9 @@ SIGNATURES = {
     "mj_gad_plan_destroy": (I32, [P]),
     "mj_gad_plan_describe": (I32, [P, P, P]),
     "mj_gad_tables_host": (I32, [I32, P, P, I32, I32, I32, I32, P, P]),
-    "mj_gad_dst3_adjoint_workspace_bytes": (ctypes.c_size_t, []),
+    "mj_gad_dst3_adjoint_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(GadConfig), I32, I32]),
     "mj_gad_dst3_adjoint": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P,
-                                  P, P, ctypes.c_size_t, P]),
+                                  P, P, P, P, ctypes.c_size_t, P]),
     "mj_dst3_adv_x": (I32, [ctypes.POINTER(Dst3FluxConfig), F64, P, P, P, P, P, P, P, P, P]),
     "mj_dst3_adv_y": (I32, [ctypes.POINTER(Dst3FluxConfig), F64, P, P, P, P, P, P, P, P, P]),
     "mj_gad_dst3": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P, P]),

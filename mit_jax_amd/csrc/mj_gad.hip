@@ -396,8 +396,11 @@ __device__ __forceinline__ FluxPartials dst3_partials(double tr_, double cfl, do
 // before each sweep (work: 7 planes per block, the first holds the running cotangent of localTij); the sweeps
 // are then reversed: the box update (af, tracer and transport terms), the corner fill after the flux, the flux
 // (weights of the four tracer values, transport and CFL terms), the corner fill before it.
+//
+// With lbar (the R flux's reverse mode) the sweeps start from lbar, the cotangent of localTij after the horizontal
+// passes, instead of from gbar, and the tracer cotangent already holds the vertical part (added to, not set).
 __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const double* gbar, double* trb, double* ub,
-                                                              double* vb, double* work, int blk0) {
+                                                              double* vb, double* work, int blk0, const double* lbar) {
     __shared__ double loc[GPLANE];
     __shared__ double fb[GPLANE];
     __shared__ unsigned long long mlw[kMaskWords], mls[kMaskWords];
@@ -514,11 +517,15 @@ __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const do
         }
     }
 
-    // ---- cotangent of the output (localTij - tracer) / dt
+    // ---- cotangent of the output (localTij - tracer) / dt, or of localTij itself (lbar)
     for (int q = tid; q < GPLANE; q += nthr) {
-        const double g = gbar[b3 + q];
-        Lb[q] = g / dt;
-        TB[q] = -(g / dt);
+        if (lbar) {
+            Lb[q] = lbar[b3 + q];
+        } else {
+            const double g = gbar[b3 + q];
+            Lb[q] = g / dt;
+            TB[q] = -(g / dt);
+        }
         UB[q] = 0.0;
         VB[q] = 0.0;
     }
@@ -649,6 +656,74 @@ __global__ void __launch_bounds__(256) gad_vert_kernel(GadDev a, const double* _
         M0 = Mp1;
         Mp1 = Mp2;
         Mp2 = Mk(k + 3);
+    }
+}
+
+// Reverse mode of gad_vert_kernel for one column (member, tile, q): from gbar, the cotangent of gTracer, and loc_h,
+// the forward's localTij after the horizontal passes, it writes every level of the column of
+//   locbar = the cotangent of loc_h (the identity term gbar/dt plus the interface values' tracer weights),
+//   trb    = the vertical part of the tracer cotangent: -gbar/dt + e(k) (rTrans(k) - rTrans(k+1)),
+//   wb     = the cotangent of w at interfaces 1 .. nlev-1 (0 at the surface and below the member's levels),
+// with e(k) = coef(k) maskC(k) gbar(k)/dt(k). Through the update of level k the interface value wT(k) has the
+// cotangent e(k-1) - e(k) and rTrans(k) e(k) tracer(k) - e(k-1) tracer(k-1); the interface value's partials
+// (dst3_partials: the four tracer weights, d/dtransport, d/dcfl) carry them on to loc_h and to w through
+// rTrans = w rA mk and cfl = |w dt recip_drC| (d|x|/dx = sign x, 0 at 0).
+__global__ void __launch_bounds__(256) gad_vert_adjoint_kernel(GadDev a, const double* __restrict__ loc_h,
+                                                               const double* __restrict__ gbar,
+                                                               double* __restrict__ locbar, double* __restrict__ trb,
+                                                               double* __restrict__ wb) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mt = blockIdx.y;
+    if (q >= GPLANE) return;
+    const int t = mt % a.T, m = mt / a.T;
+    const int K = a.K;
+    const int kmax = a.nlev ? a.nlev[m] : K;
+    const int64_t b2 = (int64_t)t * GPLANE + q;
+    const double rA = a.rA[b2], rrA = a.recip_rA[b2];
+    const int64_t c3 = m * a.ms3 + (int64_t)t * K * GPLANE + q;
+    const double* __restrict__ w = a.w + c3;
+    const double* __restrict__ tr = a.tr + c3;
+    const double* __restrict__ rhC = a.recip_hFacC + c3;
+    const double* __restrict__ lh = loc_h + c3;
+    const double* __restrict__ gb = gbar + c3;
+    const double* __restrict__ mc = a.maskC + m * a.mms + (int64_t)t * K * GPLANE + q;
+    double* __restrict__ lb = locbar + c3;
+    double* __restrict__ tb = trb + c3;
+    double* __restrict__ wbar = wb + c3;
+    auto L = [](int k) { return (int64_t)k * GPLANE; };
+    auto Tk = [&](int k) { return (k >= 0 && k < kmax) ? lh[L(k)] : 0.0; };
+    auto Mk = [&](int k) { return (k >= 0 && k < kmax) ? mc[L(k)] : 0.0; };
+    auto mk = [&](int k) { return (k >= 1 && k <= kmax - 1) ? Mk(k - 1) * Mk(k) : 0.0; };
+    auto rT = [&](int k) { return (k >= 1 && k <= kmax - 1) ? w[L(k)] * rA * mk(k) : 0.0; };
+    auto e = [&](int k) {
+        if (k < 0 || k >= kmax) return 0.0;
+        const double cf = a.dTlev[k] * a.recip_rhoFacC[k] * rhC[L(k)] * a.recip_drF[k] * rrA * a.recip_deepFac2C[k];
+        return cf * Mk(k) * (gb[L(k)] / a.dTlev[k]);
+    };
+    for (int k = 0; k < K; ++k) {
+        double l = 0.0, tv = 0.0;
+        if (k < kmax) {
+            const double g = gb[L(k)];
+            l = g / a.dTlev[k];
+            tv = -(g / a.dTlev[k]) + e(k) * (rT(k) - rT(k + 1));
+        }
+        lb[L(k)] = l;
+        tb[L(k)] = tv;
+        wbar[L(k)] = 0.0;
+    }
+    for (int ki = 1; ki <= kmax - 1; ++ki) {
+        const double wk = w[L(ki)];
+        const double wTb = e(ki - 1) - e(ki);
+        const double arg = wk * a.dTlev[ki] * a.recip_drC[ki];
+        const FluxPartials P = dst3_partials(rT(ki), fabs(arg), a.oneSixth, Tk(ki + 1), Tk(ki), Tk(ki - 1),
+                                             Tk(ki - 2), mk(ki + 1), mk(ki), mk(ki - 1));
+        if (ki + 1 < kmax) lb[L(ki + 1)] = lb[L(ki + 1)] + P.w[0] * wTb;
+        lb[L(ki)] = lb[L(ki)] + P.w[1] * wTb;
+        lb[L(ki - 1)] = lb[L(ki - 1)] + P.w[2] * wTb;
+        if (ki - 2 >= 0) lb[L(ki - 2)] = lb[L(ki - 2)] + P.w[3] * wTb;
+        const double rTb = e(ki) * tr[L(ki)] - e(ki - 1) * tr[L(ki - 1)] + P.dtr * wTb;
+        const double sgn = arg > 0.0 ? 1.0 : (arg < 0.0 ? -1.0 : 0.0);
+        wbar[L(ki)] = rTb * (rA * mk(ki)) + (P.dc * wTb) * (sgn * a.dTlev[ki] * a.recip_drC[ki]);
     }
 }
 
@@ -934,13 +1009,17 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
     return MJ_OK;
 }
 
-extern "C" size_t mj_gad_dst3_adjoint_workspace_bytes(void) { return (size_t)kAdjChunk * 7 * GPLANE * sizeof(double); }
+extern "C" size_t mj_gad_dst3_adjoint_workspace_bytes(const mj_gad_config* c, int ntiles, int with_w) {
+    size_t n = (size_t)kAdjChunk * 7 * GPLANE * sizeof(double);  // the replay scratch of the horizontal reverse
+    if (with_w && c) n += 2 * (size_t)c->batch * (size_t)std::max<int64_t>(c->member_stride3d, (int64_t)ntiles * c->nr * GPLANE) * sizeof(double);
+    return n;
+}
 
 extern "C" int mj_gad_dst3_adjoint(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c,
-                                   const double* uFld, const double* vFld, const double* tracer, const double* hFacW,
-                                   const double* hFacS, const double* recip_hFacC, const double* gTracer_bar,
-                                   double* tracer_bar, double* uFld_bar, double* vFld_bar, double* work,
-                                   size_t work_bytes, void* stream) {
+                                   const double* uFld, const double* vFld, const double* wFld, const double* tracer,
+                                   const double* hFacW, const double* hFacS, const double* recip_hFacC,
+                                   const double* gTracer_bar, double* tracer_bar, double* uFld_bar, double* vFld_bar,
+                                   double* wFld_bar, double* work, size_t work_bytes, void* stream) {
     MJ_REQUIRE(P && g && c, "mj_gad_dst3_adjoint: null plan/grid/config");
     MJ_REQUIRE(uFld && vFld && tracer && hFacW && hFacS && recip_hFacC && gTracer_bar && tracer_bar && uFld_bar &&
                    vFld_bar && work,
@@ -951,15 +1030,36 @@ extern "C" int mj_gad_dst3_adjoint(const mj_gad_plan* P, const mj_gad_grid* g, c
     MJ_REQUIRE(c->member_stride3d >= (int64_t)P->ntiles * c->nr * GPLANE || c->batch == 1,
                "mj_gad_dst3_adjoint: member_stride3d too small");
     MJ_REQUIRE((int64_t)c->batch * P->ntiles * c->nr < (1ll << 31), "mj_gad_dst3_adjoint: too many blocks");
-    MJ_REQUIRE(work_bytes >= mj_gad_dst3_adjoint_workspace_bytes(), "mj_gad_dst3_adjoint: work %zu < %zu bytes",
-               work_bytes, mj_gad_dst3_adjoint_workspace_bytes());
-    GadDev a = gad_dev(P, g, c, uFld, vFld, nullptr, tracer, hFacW, hFacS, recip_hFacC);
+    if (wFld) {
+        MJ_REQUIRE(wFld_bar && g->maskC && g->rA && g->recip_drC,
+                   "mj_gad_dst3_adjoint: the R flux needs wFld_bar, maskC, rA, recip_drC");
+        for (const double* o : {(const double*)tracer_bar, (const double*)uFld_bar, (const double*)vFld_bar})
+            MJ_REQUIRE(o != wFld_bar, "mj_gad_dst3_adjoint: wFld_bar aliases another output");
+    }
+    const size_t need = mj_gad_dst3_adjoint_workspace_bytes(c, P->ntiles, wFld != nullptr);
+    MJ_REQUIRE(work_bytes >= need, "mj_gad_dst3_adjoint: work %zu < %zu bytes", work_bytes, need);
     hipStream_t st = as_stream(stream);
     const int nblk = c->batch * P->ntiles * c->nr;
+    double* lbar = nullptr;
+    if (wFld) {
+        // forward replay of the horizontal passes (localTij), then the vertical reverse per column
+        const int64_t ms = std::max<int64_t>(c->member_stride3d, (int64_t)P->ntiles * c->nr * GPLANE);
+        double* loc_h = work + (size_t)kAdjChunk * 7 * GPLANE;
+        lbar = loc_h + (size_t)c->batch * ms;
+        GadDev f = gad_dev(P, g, c, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC);
+        f.out = loc_h;
+        const unsigned nb8 = (unsigned)((nblk + 7) / 8 * 8);
+        hipLaunchKernelGGL(gad_hz_kernel, dim3(nb8), dim3(kHzThreads), 0, st, f);
+        MJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(gad_vert_adjoint_kernel, dim3((GPLANE + 255) / 256, (unsigned)(c->batch * P->ntiles)),
+                           dim3(256), 0, st, f, (const double*)loc_h, gTracer_bar, lbar, tracer_bar, wFld_bar);
+        MJ_HIP(hipGetLastError());
+    }
+    GadDev a = gad_dev(P, g, c, uFld, vFld, nullptr, tracer, hFacW, hFacS, recip_hFacC);
     for (int b0 = 0; b0 < nblk; b0 += kAdjChunk) {
         const int n = std::min(kAdjChunk, nblk - b0);
         hipLaunchKernelGGL(gad_hz_adjoint_kernel, dim3((unsigned)n), dim3(1024), 0, st, a, gTracer_bar, tracer_bar,
-                           uFld_bar, vFld_bar, work, b0);
+                           uFld_bar, vFld_bar, work, b0, (const double*)lbar);
         MJ_HIP(hipGetLastError());
     }
     return MJ_OK;

@@ -312,48 +312,56 @@ def advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS
 
 
 def advect_tiles_adjoint(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, gT_bar,
-                         nlev=None):
-    """Reverse mode of advect_tiles without the R flux (libmjamd mj_gad_dst3_adjoint): cotangents
-    (tracer_bar, uFld_bar, vFld_bar) of the inputs for the cotangent gT_bar of gTracer (the reference's
-    jax.grad of advect_tiles in the tracer and the velocities, test_gad.py:297-346)."""
+                         nlev=None, wFld=None):
+    """Reverse mode of advect_tiles (libmjamd mj_gad_dst3_adjoint): cotangents (tracer_bar, uFld_bar, vFld_bar) of
+    the inputs for the cotangent gT_bar of gTracer (the reference's jax.grad of advect_tiles in the tracer and the
+    velocities, test_gad.py:297-346); with wFld (and params.implicitAdvection False: the R flux) also wFld_bar, as
+    a fourth element."""
     plan, grid_s, cfg, tr, fields, w, keep = _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW,
-                                                      hFacS, recip_hFacC, None, nlev)
+                                                      hFacS, recip_hFacC, wFld, nlev)
     gb = gT_bar.to(torch.float64).contiguous()
     if tuple(gb.shape) != tuple(tr.shape):
         raise ValueError("gT_bar shape differs from tracer")
-    bars = [torch.empty_like(tr) for _ in range(3)]
+    bars = [torch.empty_like(tr) for _ in range(4 if w is not None else 3)]
     lib = _lib.lib()
-    wsb = lib.mj_gad_dst3_adjoint_workspace_bytes()
+    wsb = lib.mj_gad_dst3_adjoint_workspace_bytes(ctypes.byref(cfg), params.layout.nTiles, int(w is not None))
     work = torch.empty(wsb, dtype=torch.uint8, device=tr.device)
     u, v, hW, hS, rhC = fields
     _lib.check(lib.mj_gad_dst3_adjoint(plan.handle, ctypes.byref(grid_s), ctypes.byref(cfg), _lib.ptr(u), _lib.ptr(v),
-                                       _lib.ptr(tr), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(rhC), _lib.ptr(gb),
-                                       *[_lib.ptr(b) for b in bars], _lib.ptr(work), ctypes.c_size_t(wsb),
-                                       _lib.stream_ptr()), "mj_gad_dst3_adjoint")
+                                       _lib.ptr(w), _lib.ptr(tr), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(rhC),
+                                       _lib.ptr(gb), *[_lib.ptr(b) for b in bars[:3]],
+                                       _lib.ptr(bars[3]) if w is not None else None, _lib.ptr(work),
+                                       ctypes.c_size_t(wsb), _lib.stream_ptr()), "mj_gad_dst3_adjoint")
     del keep, work
     return tuple(bars)
 
 
 class _AdvectTiles(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, tracer, uFld, vFld, params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC, nlev):
-        ctx.save_for_backward(tracer, uFld, vFld)
+    def forward(ctx, tracer, uFld, vFld, wFld, params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC, nlev):
+        ctx.save_for_backward(tracer, uFld, vFld, wFld)
         ctx.rest = (params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC, nlev)
-        return advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, nlev=nlev)
+        return advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, wFld=wFld,
+                            nlev=nlev)
 
     @staticmethod
     def backward(ctx, gT_bar):
-        tracer, uFld, vFld = ctx.saved_tensors
+        tracer, uFld, vFld, wFld = ctx.saved_tensors
         params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC, nlev = ctx.rest
-        tb, ub, vb = advect_tiles_adjoint(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS,
-                                          recip_hFacC, gT_bar, nlev=nlev)
-        return tb, ub, vb, None, None, None, None, None, None, None, None
+        bars = advect_tiles_adjoint(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC,
+                                    gT_bar, nlev=nlev, wFld=wFld)
+        wb = bars[3] if len(bars) == 4 else None
+        return bars[0], bars[1], bars[2], wb, None, None, None, None, None, None, None, None
 
 
-def advect_tiles_ad(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, nlev=None):
-    """advect_tiles (no R flux) differentiable in tracer, uFld and vFld through torch.autograd, with the
-    hand-written reverse mode (advect_tiles_adjoint)."""
-    return _AdvectTiles.apply(tracer, uFld, vFld, params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC, nlev)
+def advect_tiles_ad(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, nlev=None,
+                    wFld=None):
+    """advect_tiles differentiable in tracer, uFld and vFld (and wFld with the R flux, params.implicitAdvection
+    False) through torch.autograd, with the hand-written reverse mode (advect_tiles_adjoint)."""
+    if wFld is not None and params.implicitAdvection:
+        wFld = None  # the reference branch: w is not read
+    return _AdvectTiles.apply(tracer, uFld, vFld, wFld, params, tables, grid2d, vgrid, hFacW, hFacS, recip_hFacC,
+                              nlev)
 
 
 def gad_advection(params, g, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC):
