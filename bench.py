@@ -130,7 +130,9 @@ def plain_fp64(E, params, members):
         ops = [o[li] for o in E["ops"]]
         x, _ = cg2d.cg2d_fortran(ex, ops, E["rhs"][m], E["x0"][m], cg2dNorm=E["norm"][m:m + 1], tolerance=TOL,
                                  max_iters=2000, sum_order="tile")
-        g2d = dict(E["g2d"], maskW=E["g2d"]["maskW"][m:m + 1], maskS=E["g2d"]["maskS"][m:m + 1])
+        c = int(E["g2d"]["mask_index"][m])
+        g2d = {k: a for k, a in E["g2d"].items() if k != "mask_index"}
+        g2d.update(maskW=E["g2d"]["maskW"][c:c + 1], maskS=E["g2d"]["maskS"][c:c + 1])
         gad.advect_tiles(params, plan, g2d, E["vg"], E["u"][m:m + 1], E["v"][m:m + 1], E["tr"][m:m + 1],
                          E["hW"][m:m + 1], E["hS"][m:m + 1], E["rhC"][m:m + 1], nlev=E["nlev_t"][m:m + 1])
         return x
@@ -347,7 +349,7 @@ def dst3_xyr(E, dev, params, reps=3):
     import mit_jax_amd._lib as L
     from mit_jax_amd import gad
     w = E["per_member"](lambda c: c[6])
-    g2d = dict(E["g2d"], maskC=E["mC"])
+    g2d = dict(E["g2d"], maskC=E["mC_slots"])
     pr = _replace(params, implicitAdvection=False)
     plan, grid_s, gcfg, trc, fields, wd, keep = gad._prepare(pr, None, g2d, dict(E["vg"]), E["u"], E["v"], E["tr"],
                                                              E["hW"], E["hS"], E["rhC"], w, E["nlev_t"])

@@ -317,8 +317,10 @@ typedef struct {
     int batch;               /* members */
     int nr;                  /* K levels per member array */
     int64_t member_stride3d; /* elements between members' 3-D arrays (u, v, w, tracer, hFac*, gTracer) */
-    int64_t mask_member_stride; /* elements between members' maskW/maskS/maskC; 0 = shared */
+    int64_t mask_member_stride; /* elements between mask slots of maskW/maskS/maskC; 0 = one shared slot */
     const int32_t* nlev;     /* device [batch] active levels per member (levels >= nlev untouched), or NULL */
+    const int32_t* mask_index; /* device [batch]: mask slot of each member (members of one level class share
+                                  their grid masks), or NULL: slot = member. Ignored when mask_member_stride is 0 */
 } mj_gad_config;
 
 /* uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, gTracer: [B][T, K, ny, nx]; wFld: same or NULL (no R flux).

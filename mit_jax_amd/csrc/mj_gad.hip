@@ -26,6 +26,7 @@ constexpr int GNY = 98, GNX = 98, GPLANE = GNY * GNX;  // LLC90 tiles with OL = 
 struct GadDev {
     int B, T, K, OLx, OLy, sNx, sNy;
     int64_t ms3, mms;
+    const int32_t* mask_index;  // device [B]: mask slot of each member, or NULL (slot = member)
     const MjTilePlan* tiles;
     const double *dyG, *dxG, *recip_dxC, *recip_dyC, *recip_rA, *rA, *maskInC;
     const double *maskW, *maskS, *maskC;
@@ -36,6 +37,9 @@ struct GadDev {
     double* out;  // gTracer (no R flux) or localTij after the horizontal passes (R flux follows)
     const int32_t* nlev;
 };
+
+// Mask slot of member m: members of one level class share their grid masks.
+__device__ __forceinline__ int64_t mask_slot(const GadDev& a, int m) { return a.mask_index ? a.mask_index[m] : m; }
 
 // maskLocW / maskLocS: maskW / maskS with FILL_CS_CORNER_UV_RS(.FALSE.) applied (gad.py:263-281, :489-490).
 // (ia, ja) array indices; returns the corner-filled mask value at that point.
@@ -276,7 +280,7 @@ __global__ void __launch_bounds__(kHzThreads, 4) gad_hz_kernel(GadDev a) {
     if (a.nlev && k >= a.nlev[m]) return;
     const MjTilePlan& tp = a.tiles[t];
     const int64_t b3 = m * a.ms3 + ((int64_t)t * a.K + k) * GPLANE;
-    const int64_t bm = m * a.mms + ((int64_t)t * a.K + k) * GPLANE;
+    const int64_t bm = mask_slot(a, m) * a.mms + ((int64_t)t * a.K + k) * GPLANE;
     const int64_t b2 = (int64_t)t * GPLANE;
     const double* __restrict__ tr = a.tr + b3;
     const double* mW = a.maskW + bm;
@@ -422,7 +426,7 @@ __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const do
         return;
     }
     const MjTilePlan& tp = a.tiles[t];
-    const int64_t bm = m * a.mms + ((int64_t)t * a.K + k) * GPLANE;
+    const int64_t bm = mask_slot(a, m) * a.mms + ((int64_t)t * a.K + k) * GPLANE;
     const int64_t b2 = (int64_t)t * GPLANE;
     const double* tr = a.tr + b3;
     const double* u = a.u + b3;
@@ -616,7 +620,7 @@ __global__ void __launch_bounds__(256) gad_vert_kernel(GadDev a, const double* _
     const double* __restrict__ tr = a.tr + m * a.ms3 + (int64_t)t * K * GPLANE + q;
     const double* __restrict__ rhC = a.recip_hFacC + m * a.ms3 + (int64_t)t * K * GPLANE + q;
     const double* __restrict__ lh = loc_h + m * a.ms3 + (int64_t)t * K * GPLANE + q;
-    const double* __restrict__ mc = a.maskC + m * a.mms + (int64_t)t * K * GPLANE + q;
+    const double* __restrict__ mc = a.maskC + mask_slot(a, m) * a.mms + (int64_t)t * K * GPLANE + q;
     double* __restrict__ out = a.out + m * a.ms3 + (int64_t)t * K * GPLANE + q;
     // the column as a window over k: T(k-1) .. T(k+2), maskC(k-1) .. maskC(k+2) (0 outside 0 .. kmax-1); each level
     // loads one new localTij, maskC and w value (the expressions are those of the formulas above)
@@ -686,7 +690,7 @@ __global__ void __launch_bounds__(256) gad_vert_adjoint_kernel(GadDev a, const d
     const double* __restrict__ rhC = a.recip_hFacC + c3;
     const double* __restrict__ lh = loc_h + c3;
     const double* __restrict__ gb = gbar + c3;
-    const double* __restrict__ mc = a.maskC + m * a.mms + (int64_t)t * K * GPLANE + q;
+    const double* __restrict__ mc = a.maskC + mask_slot(a, m) * a.mms + (int64_t)t * K * GPLANE + q;
     double* __restrict__ lb = locbar + c3;
     double* __restrict__ tb = trb + c3;
     double* __restrict__ wbar = wb + c3;
@@ -942,6 +946,7 @@ GadDev gad_dev(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* 
     a.sNy = P->sNy;
     a.ms3 = c->member_stride3d;
     a.mms = c->mask_member_stride;
+    a.mask_index = c->mask_member_stride ? c->mask_index : nullptr;
     a.tiles = P->d_tiles;
     a.dyG = g->dyG;
     a.dxG = g->dxG;

@@ -189,17 +189,31 @@ def _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, re
         w = _dev(wFld, dev, "wFld")
         if tuple(w.shape) != tuple(tr.shape):
             raise ValueError("wFld shape differs from tracer")
-    g = {k: _dev(grid2d[k], dev, k) for k in grid2d if grid2d[k] is not None}
+    g = {k: _dev(grid2d[k], dev, k) for k in grid2d if grid2d[k] is not None and k != "mask_index"}
     for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "maskInC"):
         if tuple(g[k].shape) != L.shape2d:
             raise ValueError(f"{k}: expected {L.shape2d}")
+    # masks: one shared slot [T, K, ny, nx], one per member [B, T, K, ny, nx], or C slots [C, T, K, ny, nx] picked
+    # per member by grid2d["mask_index"] (members of one level class share their grid masks)
     mshape = tuple(g["maskW"].shape)
+    midx = None
     if mshape == (T, K, ny, nx):
         mms = 0
-    elif mshape == (B, T, K, ny, nx) and batched:
+    elif len(mshape) == 5 and mshape[1:] == (T, K, ny, nx) and batched:
         mms = T * K * ny * nx
+        mi = grid2d.get("mask_index")
+        if mi is not None:
+            midx = torch.as_tensor(np.asarray(mi, np.int32).reshape(-1), device=dev) \
+                if not isinstance(mi, torch.Tensor) else mi.to(device=dev, dtype=torch.int32).contiguous()
+            if midx.numel() != B or int(midx.min()) < 0 or int(midx.max()) >= mshape[0]:
+                raise ValueError(f"mask_index: {B} slot numbers in 0..{mshape[0] - 1} expected")
+        elif mshape[0] != B:
+            raise ValueError(f"maskW: {mshape[0]} mask slots for {B} members need grid2d['mask_index']")
     else:
-        raise ValueError(f"maskW: shape {mshape} is neither [T, K, ny, nx] nor per member")
+        raise ValueError(f"maskW: shape {mshape} is neither [T, K, ny, nx], per member nor per slot")
+    for k in ("maskS", "maskC"):
+        if k in g and tuple(g[k].shape) != mshape:
+            raise ValueError(f"{k}: shape {tuple(g[k].shape)} differs from maskW {mshape}")
     vert = {
         "dTlev": params.dTtracerLev, "deepFacC": params.deepFacC, "recip_deepFacC": params.recip_deepFacC,
         "recip_deepFac2C": params.recip_deepFac2C, "recip_rhoFacC": params.recip_rhoFacC,
@@ -231,9 +245,9 @@ def _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, re
             else nlev.to(device=dev, dtype=torch.int32).contiguous()
         if nl.numel() != B or int(nl.max()) > K or int(nl.min()) < 0:
             raise ValueError("nlev: one value in 0..K per member")
-    cfg = _lib.GadConfig(B, K, T * K * ny * nx, mms, _lib.ptr(nl))
+    cfg = _lib.GadConfig(B, K, T * K * ny * nx, mms, _lib.ptr(nl), _lib.ptr(midx))
     plan = tables if isinstance(tables, GadPlan) else gad_plan(params.topo, L, dev)
-    keep = (g, vt, nl)
+    keep = (g, vt, nl, midx)
     return plan, grid_s, cfg, tr, fields, w, keep
 
 

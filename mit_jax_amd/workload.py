@@ -1,7 +1,8 @@
 """The seeded LLC90 ensemble workload of the batched CG2D + DST3 step (13 tiles of 90x90, halo 4, 50 levels).
 
 B members, each with a level count drawn from {15, 30, 50}; members with the same level count share one operator
-(three operator slots, `op_index` picks a member's); per member a rhs (seed 100 + m), the level class's
+and one set of grid masks (three operator slots and three mask slots, `op_index` / `mask_index` pick a member's); per
+member a rhs (seed 100 + m), the level class's
 streamfunction flow and tracer perturbed by tracer noise 0.05 degC on wet cells and velocity x (1 + 0.05 N(0, 1))
 (torch generator seeded 4000 on the device), halos re-exchanged. Built on the device once; `bench.py` times steps
 over it and the parity tests check every member against the oracle.
@@ -55,12 +56,17 @@ def build_ensemble(B, dev):
     x0 = torch.zeros_like(rhs)
     gen = torch.Generator(device=dev)
     gen.manual_seed(4000)
+    def per_class(get):
+        return torch.stack([T(get(cls[n])) for n in LEVEL_CHOICES]).contiguous()
+
     u = per_member(lambda c: c[3])
     v = per_member(lambda c: c[4])
     tr = per_member(lambda c: c[5])
-    mW = per_member(lambda c: c[0].maskW)
-    mS = per_member(lambda c: c[0].maskS)
-    mC = per_member(lambda c: c[0].maskC)
+    # grid masks: one slot per level class, member m reads slot idx[m]
+    mW = per_class(lambda c: c[0].maskW)
+    mS = per_class(lambda c: c[0].maskS)
+    mC_slots = per_class(lambda c: c[0].maskC)
+    mC = mC_slots.index_select(0, idx).contiguous()  # per member: GAD_IMPLICIT_R's maskC, the tracer noise mask
     hW = per_member(lambda c: c[0].hFacW)
     hS = per_member(lambda c: c[0].hFacS)
     rhC = per_member(lambda c: c[0].recip_hFacC)
@@ -75,10 +81,10 @@ def build_ensemble(B, dev):
         v[m].copy_(vv)
     g0 = cls[50][0]
     g2d = {k: T(g0.f[k]) for k in ("dyG", "dxG", "recip_dxC", "recip_dyC", "recip_rA", "rA", "maskInC")}
-    g2d.update(maskW=mW, maskS=mS)
+    g2d.update(maskW=mW, maskS=mS, mask_index=idx.to(torch.int32))
     vg = {k: g0.v[k] for k in ("drF", "recip_drF", "rhoFacC", "recip_drC")}
     return dict(ex=ex, nlev=nlev, nlev_t=torch.as_tensor(nlev.astype(np.int32), device=dev), ops=ops,
-                op_index=op_index, norm=norm, mC=mC, per_member=per_member,
+                op_index=op_index, norm=norm, mC=mC, mC_slots=mC_slots, per_member=per_member,
                 rhs=rhs, x0=x0, u=u, v=v, tr=tr, hW=hW, hS=hS, rhC=rhC, g2d=g2d, vg=vg, cls=cls)
 
 
@@ -90,6 +96,7 @@ def member_host(E, m, active_levels=True):
     K = n if active_levels else E["tr"].shape[2]
     f = {k: np.ascontiguousarray(E[k][m, :, :K].cpu().numpy())
          for k in ("u", "v", "tr", "hW", "hS", "rhC")}
-    f["maskW"] = np.ascontiguousarray(E["g2d"]["maskW"][m, :, :K].cpu().numpy())
-    f["maskS"] = np.ascontiguousarray(E["g2d"]["maskS"][m, :, :K].cpu().numpy())
+    c = int(E["g2d"]["mask_index"][m])
+    f["maskW"] = np.ascontiguousarray(E["g2d"]["maskW"][c, :, :K].cpu().numpy())
+    f["maskS"] = np.ascontiguousarray(E["g2d"]["maskS"][c, :, :K].cpu().numpy())
     return dict(nlev=n, levels=K, grid=g, ops=ops, norm=norm, rhs=E["rhs"][m].cpu().numpy(), **f)
