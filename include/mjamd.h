@@ -321,6 +321,8 @@ typedef struct {
     const int32_t* nlev;     /* device [batch] active levels per member (levels >= nlev untouched), or NULL */
     const int32_t* mask_index; /* device [batch]: mask slot of each member (members of one level class share
                                   their grid masks), or NULL: slot = member. Ignored when mask_member_stride is 0 */
+    int levels_per_block;    /* consecutive levels of one (member, tile) per workgroup of the horizontal passes
+                                (0 = 1); a launch shape: the results are the same for every value */
 } mj_gad_config;
 
 /* uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, gTracer: [B][T, K, ny, nx]; wFld: same or NULL (no R flux).

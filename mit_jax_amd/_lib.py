@@ -59,7 +59,7 @@ class GadGrid(ctypes.Structure):
 
 class GadConfig(ctypes.Structure):
     _fields_ = [("batch", I32), ("nr", I32), ("member_stride3d", I64), ("mask_member_stride", I64), ("nlev", P),
-                ("mask_index", P)]
+                ("mask_index", P), ("levels_per_block", I32)]
 
 
 class Dst3FluxConfig(ctypes.Structure):

@@ -27,6 +27,7 @@ struct GadDev {
     int B, T, K, OLx, OLy, sNx, sNy;
     int64_t ms3, mms;
     const int32_t* mask_index;  // device [B]: mask slot of each member, or NULL (slot = member)
+    int lpb;                    // levels per block of the horizontal launch (consecutive levels of one tile)
     const MjTilePlan* tiles;
     const double *dyG, *dxG, *recip_dxC, *recip_dyC, *recip_rA, *rA, *maskInC;
     const double *maskW, *maskS, *maskC;
@@ -124,17 +125,19 @@ __device__ __forceinline__ void corner_fill_T(double* bar, const GadDev& a, cons
     __syncthreads();
 }
 
-// Unit (tile, member, level) of a DST3 launch of gridDim.x >= B*T*K blocks (a multiple of 8): units ordered tile-major
-// (tile, member, level) and cut into 8 contiguous ranges, one per XCD (block b runs on XCD b % 8), so each XCD
-// works through about T/8 tiles and their 2-D metrics stay in its L2 across members and levels.
+// Unit (tile, member, level block) of a DST3 launch of gridDim.x >= B*T*ceil(K/L) blocks (a multiple of 8, L =
+// a.lpb levels per block): units ordered tile-major (tile, member, level block) and cut into 8 contiguous ranges, one
+// per XCD (block b runs on XCD b % 8), so each XCD works through about T/8 tiles and their 2-D metrics stay in its L2
+// across members and levels. k is the block's first level.
 __device__ __forceinline__ bool gad_unit(const GadDev& a, int& t, int& m, int& k) {
-    const int n = a.B * a.T * a.K;
+    const int nkb = (a.K + a.lpb - 1) / a.lpb;
+    const int n = a.B * a.T * nkb;
     const int chunk = (n + 7) / 8;
     const int pos = blockIdx.x / 8;
     const int u = (blockIdx.x % 8) * chunk + pos;
     if (pos >= chunk || u >= n) return false;
-    k = u % a.K;
-    const int tm = u / a.K;
+    k = (u % nkb) * a.lpb;
+    const int tm = u / nkb;
     m = tm % a.B;
     t = tm / a.B;
     return true;
@@ -269,15 +272,11 @@ __device__ __forceinline__ int box_point(const MjBox& B, int bw, float inv_bw, i
 constexpr int kHzThreads = 1024;
 constexpr int kHzPts = (GPLANE + kHzThreads - 1) / kHzThreads;  // box points per thread (disjoint boxes in the plane)
 
-__global__ void __launch_bounds__(kHzThreads, 4) gad_hz_kernel(GadDev a) {
+// One (member, tile, level) plane of the horizontal passes, LDS planes given.
+__device__ __forceinline__ void hz_plane(const GadDev& a, int t, int m, int k, double* loc, double* trc,
+                                         unsigned long long* mlw, unsigned long long* mls) {
     constexpr int NT = kHzThreads;
-    __shared__ double loc[GPLANE];
-    __shared__ double trc[GPLANE];
-    __shared__ unsigned long long mlw[kMaskWords], mls[kMaskWords];
     const int tid = threadIdx.x;
-    int t, m, k;
-    if (!gad_unit(a, t, m, k)) return;
-    if (a.nlev && k >= a.nlev[m]) return;
     const MjTilePlan& tp = a.tiles[t];
     const int64_t b3 = m * a.ms3 + ((int64_t)t * a.K + k) * GPLANE;
     const int64_t bm = mask_slot(a, m) * a.mms + ((int64_t)t * a.K + k) * GPLANE;
@@ -360,6 +359,33 @@ __global__ void __launch_bounds__(kHzThreads, 4) gad_hz_kernel(GadDev a) {
             out[q] = loc[q];
         else
             out[q] = (loc[q] - trc[q]) / dt;  // gad.py:539
+    }
+}
+
+// The plane as a call of its own (the multi-level blocks: inlined into the level loop it spills registers).
+__device__ __noinline__ void hz_plane_call(const GadDev& a, int t, int m, int k, double* loc, double* trc,
+                                           unsigned long long* mlw, unsigned long long* mls) {
+    hz_plane(a, t, m, k, loc, trc, mlw, mls);
+}
+
+// One workgroup per (member, tile, level block): LEVELS false, one level per block (a.lpb == 1); true, a.lpb
+// consecutive levels, one after the other in the same LDS planes.
+template <bool LEVELS>
+__global__ void __launch_bounds__(kHzThreads, 4) gad_hz_kernel(GadDev a) {
+    __shared__ double loc[GPLANE];
+    __shared__ double trc[GPLANE];
+    __shared__ unsigned long long mlw[kMaskWords], mls[kMaskWords];
+    int t, m, k0;
+    if (!gad_unit(a, t, m, k0)) return;
+    if (!LEVELS) {
+        if (a.nlev && k0 >= a.nlev[m]) return;
+        hz_plane(a, t, m, k0, loc, trc, mlw, mls);
+        return;
+    }
+    const int kend = min(k0 + a.lpb, a.nlev ? min(a.nlev[m], a.K) : a.K);
+    for (int k = k0; k < kend; ++k) {
+        if (k > k0) __syncthreads();  // the previous level's output has read the LDS planes
+        hz_plane_call(a, t, m, k, loc, trc, mlw, mls);
     }
 }
 
@@ -947,6 +973,7 @@ GadDev gad_dev(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* 
     a.ms3 = c->member_stride3d;
     a.mms = c->mask_member_stride;
     a.mask_index = c->mask_member_stride ? c->mask_index : nullptr;
+    a.lpb = c->levels_per_block > 0 ? c->levels_per_block : 1;
     a.tiles = P->d_tiles;
     a.dyG = g->dyG;
     a.dxG = g->dxG;
@@ -1002,8 +1029,12 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
     GadDev a = gad_dev(P, g, c, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC);
     a.out = wFld ? work : gTracer;
     hipStream_t st = as_stream(stream);
-    const unsigned nblk = (unsigned)((c->batch * P->ntiles * c->nr + 7) / 8 * 8);
-    hipLaunchKernelGGL(gad_hz_kernel, dim3(nblk), dim3(kHzThreads), 0, st, a);
+    const int nkb = (c->nr + a.lpb - 1) / a.lpb;
+    const unsigned nblk = (unsigned)((c->batch * P->ntiles * nkb + 7) / 8 * 8);
+    if (a.lpb > 1)
+        hipLaunchKernelGGL(gad_hz_kernel<true>, dim3(nblk), dim3(kHzThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL(gad_hz_kernel<false>, dim3(nblk), dim3(kHzThreads), 0, st, a);
     MJ_HIP(hipGetLastError());
     if (wFld) {
         a.out = gTracer;
@@ -1053,8 +1084,9 @@ extern "C" int mj_gad_dst3_adjoint(const mj_gad_plan* P, const mj_gad_grid* g, c
         lbar = loc_h + (size_t)c->batch * ms;
         GadDev f = gad_dev(P, g, c, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC);
         f.out = loc_h;
+        f.lpb = 1;
         const unsigned nb8 = (unsigned)((nblk + 7) / 8 * 8);
-        hipLaunchKernelGGL(gad_hz_kernel, dim3(nb8), dim3(kHzThreads), 0, st, f);
+        hipLaunchKernelGGL(gad_hz_kernel<false>, dim3(nb8), dim3(kHzThreads), 0, st, f);
         MJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(gad_vert_adjoint_kernel, dim3((GPLANE + 255) / 256, (unsigned)(c->batch * P->ntiles)),
                            dim3(256), 0, st, f, (const double*)loc_h, gTracer_bar, lbar, tracer_bar, wFld_bar);

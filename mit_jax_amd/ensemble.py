@@ -39,6 +39,7 @@ class StepConfig:
     sum_order: str = "tile"       # fp64 only
     graph: bool = False           # replay a captured graph
     overlap: bool = False         # run the DST3 step on a second stream, concurrently with the solve
+    dst3_levels: int = 1          # DST3 tile shape: consecutive levels of one (member, tile) per workgroup
 
 
 @dataclass
@@ -84,6 +85,7 @@ class EnsembleStep:
     # ------------------------------------------------------------------------------------------------ config
     def set_config(self, cfg: StepConfig, max_iters=None, max_outer=None, fixed=False):
         self.cfg = cfg
+        self._gad[2].levels_per_block = int(cfg.dst3_levels)
         lib = _lib.lib()
         # the mixed solve caps total inner iterations (about 1.1x the FP64 count to the same residual)
         mi = (self.max_iters if cfg.precision == "fp64" else 2 * self.max_iters) if max_iters is None else int(max_iters)
@@ -250,11 +252,16 @@ def candidates(precision, B, T=13):
     return out
 
 
+DST3_LEVELS = (1, 2, 5, 10)  # levels per workgroup tried for the DST3 launch
+
+
 def autotune(step: EnsembleStep, precision="mixed", steps=3, warmup=1, try_graph=True, log=None):
-    """Time every candidate (and its graph replay) on the step's own inputs; leave the fastest selected and
-    return (best Timing, all Timings)."""
+    """Time every candidate on the step's own inputs: the solve's launch shapes (block size, inner tolerance,
+    overlap), then the DST3 tile shape (levels per workgroup) with the best of those, then the graph replay of the
+    best; leave the fastest selected and return (best Timing, all Timings)."""
     results = []
-    for c in candidates(precision, step.B):
+
+    def trial(c):
         step.drop_graph()
         step.set_config(c)
         ms = _time(step, steps, warmup)
@@ -263,8 +270,17 @@ def autotune(step: EnsembleStep, precision="mixed", steps=3, warmup=1, try_graph
         results.append(Timing(c, ms, conv))
         if log:
             log(json.dumps({"config": c.__dict__, "ms": ms, "converged": conv}))
+        return results[-1]
+
+    for c in candidates(precision, step.B):
+        trial(c)
     ok = [r for r in results if r.all_converged] or results
     best = min(ok, key=lambda r: r.ms)
+    for lv in DST3_LEVELS:
+        if lv != best.config.dst3_levels:
+            r = trial(replace(best.config, dst3_levels=lv))
+            if r.all_converged and r.ms < best.ms:
+                best = r
     step.drop_graph()
     step.set_config(best.config)
     if try_graph:

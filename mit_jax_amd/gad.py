@@ -303,14 +303,16 @@ def _check_out(out, like, name):
 
 
 def advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, wFld=None,
-                 nlev=None, out=None):
+                 nlev=None, out=None, levels_per_block=1):
     """GAD_ADVECTION on all tiles (reference gad.py:460-539) -> gTracer, same shape as tracer.
 
     tables: gad_tables(params) (or None). grid2d: dyG, dxG, recip_dxC, recip_dyC, recip_rA, maskInC [T, ny, nx]
     and maskW, maskS [T, K, ny, nx] (or [B, T, K, ny, nx] per member); with wFld also rA [T, ny, nx] and maskC.
-    vgrid: drF, recip_drF, rhoFacC [K] (+ recip_drC with wFld). nlev: per-member level counts (ensemble)."""
+    vgrid: drF, recip_drF, rhoFacC [K] (+ recip_drC with wFld). nlev: per-member level counts (ensemble).
+    levels_per_block: the launch's tile shape (consecutive levels per workgroup); the result does not depend on it."""
     plan, grid_s, cfg, tr, fields, w, keep = _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW,
                                                       hFacS, recip_hFacC, wFld, nlev)
+    cfg.levels_per_block = int(levels_per_block)
     if out is None:
         # with level counts the levels at or beyond a member's count are not written: they read 0
         gT = torch.zeros_like(tr) if nlev is not None else torch.empty_like(tr)

@@ -211,6 +211,26 @@ def test_dst3_batched_members(dev):
     assert np.all(gT[1][:, 30:] == 7.0)
 
 
+def test_dst3_levels_per_block_same_bits(dev):
+    """The DST3 tile shape (levels per workgroup) changes the launch only: two members with 50 and 30 active
+    levels, with and without the R flux, give the same bits for every shape."""
+    import torch
+    from mit_jax_amd import gad
+    g, u, v, w, tr = dst3_case(0)
+    g2d, vg = _gad_dev(dev, g, u, v, w, tr)
+    tr2 = tr + np.where(g.maskC > 0, np.random.default_rng(5).normal(0, 0.05, tr.shape), 0.0)
+    st = lambda a: to_dev(np.stack([a] * 2), dev)  # noqa: E731
+    args = (st(u), st(v), to_dev(np.stack([tr, tr2]), dev), st(g.hFacW), st(g.hFacS), st(g.recip_hFacC))
+    for implicit in (True, False):
+        params = gad.GADParams.llc90(implicitAdvection=implicit)
+        wf = None if implicit else st(w)
+        ref = gad.advect_tiles(params, None, g2d, vg, *args, wFld=wf, nlev=[50, 30])
+        for lpb in (2, 3, 7, 50):
+            print(f"levels per block {lpb}, implicitAdvection {implicit}", flush=True)
+            got = gad.advect_tiles(params, None, g2d, vg, *args, wFld=wf, nlev=[50, 30], levels_per_block=lpb)
+            assert torch.equal(got, ref), (lpb, implicit)
+
+
 def test_runs_are_deterministic(dev, ex):
     from mit_jax_amd import cg2d, gad
     g, ops, norm, b = _cg_inputs("5point")
