@@ -148,17 +148,21 @@ def plain_fp64(E, params, members):
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "mx_step_kernel_ensemble128_traffic.md"
+TRAFFIC_FILES = (TRAFFIC_FILE, ROOT / "profiles" / "sfp_impr_kernels_ensemble128_traffic.md")
 
 
 def measured_traffic(kernel, members):
-    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE profile of the 128-member
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE profiles of the 128-member
     workload (None for another workload or kernel)."""
-    if members != 128 or not TRAFFIC_FILE.exists():
+    if members != 128:
         return None
-    for line in TRAFFIC_FILE.read_text().splitlines():
-        f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
-            return float(f[2])
+    for path in TRAFFIC_FILES:
+        if not path.exists():
+            continue
+        for line in path.read_text().splitlines():
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                return float(f[2])
     return None
 
 
@@ -544,7 +548,8 @@ def main():
         "config": cfg,
         "roofline": roofline_of(args.precision, run["cg_ms"], dk, E["op_index"], B, sel.launch),
         "dst3": {"launch_ms": run["gad_ms"], "achieved_GBs": gad_bytes / (run["gad_ms"] * 1e-3) / 1e9,
-                 "algorithmic_bytes": gad_bytes, "frac": gad_bytes / (run["gad_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                 "algorithmic_bytes": gad_bytes, "frac": gad_bytes / (run["gad_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                 "kernel": "gad_hz_kernel", "traffic": measured_traffic("gad_hz_kernel", B)},
     }
     if args.precision == "mixed" and not args.no_fp64_reference:
         ref_step, ref_tuned = make("fp64")

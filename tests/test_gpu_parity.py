@@ -231,6 +231,36 @@ def test_dst3_levels_per_block_same_bits(dev):
             assert torch.equal(got, ref), (lpb, implicit)
 
 
+def test_dst3_mask_slots_same_bits(dev):
+    """Mask slots picked per member by grid2d["mask_index"] give the same bits as the same masks stored once per
+    member, with and without the R flux; all members on slot 0 give the shared-mask result."""
+    import torch
+    from mit_jax_amd import gad
+    g, u, v, w, tr = dst3_case(0)
+    g2d, vg = _gad_dev(dev, g, u, v, w, tr)
+    keep = (torch.arange(g2d["maskW"].shape[1], device=dev) < 20).to(torch.float64).view(1, -1, 1, 1)
+    slots = {k: torch.stack([g2d[k], g2d[k] * keep]) for k in ("maskW", "maskS", "maskC")}
+    idx = [1, 0, 1]
+    B = len(idx)
+    rng = np.random.default_rng(11)
+    trs = np.stack([tr + np.where(g.maskC > 0, rng.normal(0, 0.05, tr.shape), 0.0) for _ in range(B)])
+    st = lambda a: to_dev(np.stack([a] * B), dev)  # noqa: E731
+    args = (st(u), st(v), to_dev(trs, dev), st(g.hFacW), st(g.hFacS), st(g.recip_hFacC))
+    per_member = dict(g2d, **{k: s[idx].contiguous() for k, s in slots.items()})
+    by_slot = dict(g2d, **slots, mask_index=idx)
+    all_zero = dict(g2d, **slots, mask_index=[0] * B)
+    for implicit in (True, False):
+        print(f"mask slots, implicitAdvection {implicit}", flush=True)
+        params = gad.GADParams.llc90(implicitAdvection=implicit)
+        wf = None if implicit else st(w)
+        ref = gad.advect_tiles(params, None, per_member, vg, *args, wFld=wf)
+        got = gad.advect_tiles(params, None, by_slot, vg, *args, wFld=wf)
+        assert torch.equal(got, ref), implicit
+        shared = gad.advect_tiles(params, None, g2d, vg, *args, wFld=wf)
+        assert torch.equal(gad.advect_tiles(params, None, all_zero, vg, *args, wFld=wf), shared), implicit
+        assert not torch.equal(got, shared)
+
+
 def test_runs_are_deterministic(dev, ex):
     from mit_jax_amd import cg2d, gad
     g, ops, norm, b = _cg_inputs("5point")
