@@ -110,7 +110,7 @@ typedef struct {
     double rhsMax;         /* max |cg2d_b * cg2dNorm| */
     int32_t numIters;      /* iterations done */
     int32_t converged;     /* 1 if err_sq < tol^2 at exit */
-    int32_t outerIters;    /* refinement steps (mixed precision), 0 for the plain solve */
+    int32_t outerIters;    /* FP64 residual replacements (mixed precision), 0 for the plain solve */
     int32_t launches;      /* kernel launches of the iteration loop (PHASES modes; the same value in every member's
                               record, written when the loop ends); 0 for the persistent launch */
 } mj_cg2d_diag;
@@ -173,15 +173,19 @@ int mj_cg2d_coeff_grad(const mj_halo_plan* plan_T, const double* x, const double
                        double* gaW, double* gaS, double* gaC, const mj_cg2d_config* cfg, double* work, void* stream);
 
 /* ---- mixed-precision CG2D (no single counterpart in the reference: it solves the system of cg2d_fortran,
- * core/cg2d.py:205-269, to the same normalised stopping rule err < tolerance^2, by FP32 inner PCG with FP64
- * iterative refinement). True residuals, dot products (MJ_SUM_TREE order), alpha/beta and the solution update are
- * FP64; the inner operator/preconditioner copies, inner vectors and stencils are FP32. diag: numIters = total inner
- * iterations, outerIters = refinements, lastResidual = the last FP64 true residual (normalised). */
+ * core/cg2d.py:205-269, to the same normalised stopping rule err < tolerance^2, by FP32 PCG whose recursive
+ * residual is replaced by the FP64 true residual from time to time: at each replacement the FP32 solution
+ * increment goes into the FP64 solution, the true residual is formed in FP64 and the PCG goes on from the same
+ * search direction). True residuals, dot products (MJ_SUM_TREE order), alpha/beta and the solution update are
+ * FP64; the operator/preconditioner copies, inner vectors and stencils are FP32. diag: numIters = total inner
+ * iterations, outerIters = true residuals that replaced the recursive one (the first included), lastResidual = the
+ * last FP64 true residual (normalised). */
 typedef struct {
     int op_slots;            /* number of operator slots when ops.op_index is set (else ignored) */
-    double inner_tolerance;  /* inner solve stops when its residual norm falls below this fraction (e.g. 1e-5) */
-    int inner_max_iters;     /* inner iterations per refinement */
-    int max_outer;           /* refinements */
+    double inner_tolerance;  /* replace when the recursive residual norm has fallen by this factor since the last
+                                replacement (e.g. 1e-4), or below the target */
+    int inner_max_iters;     /* or after this many iterations since the last replacement */
+    int max_outer;           /* replacements */
 } mj_cg2d_mixed_config;
 
 /* Workspace of mj_cg2d_solve_mixed; ops_member_stride as in mj_cg2d_ops (0 = one shared operator). */

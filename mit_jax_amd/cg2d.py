@@ -47,7 +47,7 @@ class Cg2dParams:
 class Cg2dDiag(dict):
     """Per-member diagnostics of a solve (tensors on the device; the reference returns firstResidual,
     lastResidual, numIters, sumRHS, rhsMax, core/cg2d.py:267-268), plus `converged` (1 / 0; -2 when a device
-    barrier timed out), `outerIters` (refinement steps of the mixed-precision solve) and `launches` (kernel launches
+    barrier timed out), `outerIters` (FP64 residual replacements of the mixed-precision solve) and `launches` (kernel launches
     of the solve's iteration loop in the per-phase launch modes, the same in every member's record)."""
 
     def __init__(self, raw, batched):

@@ -1,17 +1,20 @@
-// Mixed-precision CG2D on gfx950: FP32 inner PCG with FP64 iterative refinement, batched over members.
+// Mixed-precision CG2D on gfx950: FP32 PCG with FP64 residual replacement, batched over members.
 //
 // The system is the one cg2d_fortran solves (reference core/cg2d.py:205-269): the same normalisation (b * cg2dNorm,
 // rhsNorm = 1/max|b|, normalised first guess with exchanged halos), the same 5-point operator and preconditioner,
 // the same stopping rule err < tol^2 on the normalised residual. Per member:
-//   r = bn - A x                                     (FP64: true residual, FP64 operator)
-//   err = sum r^2                                    (FP64); stop when err < tol^2
-//   sigma = sqrt(err); r32 = r / sigma               (inner right-hand side of unit norm)
-//   PCG on A32 d = r32 from d = 0 in FP32 with FP32 operator/preconditioner copies; every dot product is formed
-//   and summed in FP64 in a fixed order (below) and alpha/beta are FP64 ratios; the inner solve stops when its
-//   recursive residual falls below inner_tolerance (or after inner_max_iters)
-//   x = x + sigma * d                                (FP64 solution update)
-// and repeat. max_iters caps the total inner iterations and max_outer the refinements; a member that reaches a
-// cap finishes with converged = 0. Un-normalisation and the halos of x are as cg2d_fortran leaves them.
+//   r = bn - A x0, err = sum r^2                     (FP64: true residual, FP64 operator); stop when err < tol^2
+//   sigma = sqrt(err); r32 = r / sigma; d = 0        (FP32 residual in units of sigma, kept for the whole solve)
+//   PCG on A32 d = r32 in FP32 with FP32 operator/preconditioner copies; every dot product is formed and summed
+//   in FP64 in a fixed order (below) and alpha/beta are FP64 ratios. When the recursive residual has fallen by
+//   inner_tolerance since the last replacement (or below tol^2 / sigma^2, or after inner_max_iters iterations):
+//     x = x + sigma * d (FP64); r = bn - A x (FP64); stop when err < tol^2; else the recursive residual is
+//     replaced: r32 = r / sigma, d = 0, z = M32 r32, and PCG goes on from the same search direction with
+//     beta = (z, r32) / eta of the previous iteration ("reliable updates": the Krylov space is kept, so the
+//     iteration count stays that of the FP64 PCG instead of growing with every restart).
+// max_iters caps the total inner iterations and max_outer the replacements (the first residual counted); a member
+// that reaches a cap finishes with converged = 0. Un-normalisation and the halos of x are as cg2d_fortran leaves
+// them.
 //
 // Work decomposition as the FP64 solve: one block per (member, tile), rings computed from their interior sources,
 // per-tile partials summed in tile order. The FP32 work vectors (r32, s, d, z, q) are the solver's own and live in a
@@ -24,10 +27,11 @@
 // Members move through their own state machine, one state per launch ("step"): every block of a member derives the
 // member's action for the step from the member's record of the previous step and the partials, so all tiles of a
 // member act alike; tile 0 writes the record of the step (records are double-buffered by step parity). States:
-//   RST   err from the true-residual partials; finish, or restart the inner solve (r32, d = 0, eta(1))
+//   RST   err from the true-residual partials; finish, or (re)place the recursive residual (r32 = r / sigma,
+//         d = 0, z, eta(n)); the first one starts the solve (sigma = sqrt(err), n = 1)
 //   IB    inner phase B(n): d += alpha_{n-1} s_{n-1} (the solution update of iteration n-1, moved here where
-//         s_{n-1} is read anyway); inner stop -> x += sigma d (next: TRUE); else s = z + beta s, q = A32 s,
-//         sigma_n = s.q
+//         s_{n-1} is read anyway; not right after a replacement, which absorbed it into x); replacement due ->
+//         x += sigma d (next: TRUE); else s = z + beta s (s_1 = z), q = A32 s, sigma_n = s.q
 //   IC    inner phase C(n): r32 -= alpha q; z = M32 r32; err(n) = r32.r32, eta(n+1) = z.r32
 // (z and q are kept as FP32 vectors, so each inner phase applies one stencil and reads one coefficient set.)
 //   TRUE  x's ring from its sources; r = bn - A x; err partials
@@ -45,11 +49,13 @@
 namespace mj {
 
 struct MxRec {
-    int mode, n, iters, outer;  // state, inner iteration, inner iterations done, refinements started
-    double sigma;               // norm of the residual the current inner solve started from
+    int mode, n, iters, outer;  // state, inner iteration, inner iterations done, residuals placed
+    int n0, fresh;              // n of the last replacement; 1 until the IB step after a replacement has run
+    double sigma;               // units of r32: the norm of the first true residual
+    double elast;               // err of the last replacement in units of sigma^2
 };
 enum { MX_RST = 0, MX_IB = 1, MX_IC = 2, MX_TRUE = 3 };
-enum { AX_FINISH = 1, AX_RESTART, AX_UPD, AX_IB, AX_IC, AX_TRUE };
+enum { AX_FINISH = 1, AX_RESTART, AX_UPD, AX_IB, AX_IBF, AX_IC, AX_TRUE };  // AX_IBF: B(n) with no update pending
 
 struct MxDev {
     int B, T, sNx, sNy, OL, nx, ny, plane;
@@ -288,7 +294,7 @@ __device__ __forceinline__ void mx_I1(const MxDev& a, int m, int t) {
     })
     store_tile_sums<MJ_SUM_TREE>(a.perr64, a.psum, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
     if (t == 0 && tid == 0) {
-        a.rec[m] = MxRec{MX_RST, 0, 0, 0, 0.0};
+        a.rec[m] = MxRec{MX_RST, 0, 0, 0, 0, 0, 0.0, 0.0};
         a.mdone[m] = -1;
     }
 }
@@ -306,9 +312,9 @@ __device__ __forceinline__ void mx_finish(const MxDev& a, int m, int t) {
     exchanged_first_guess(a, m, t, rhsNorm);
 }
 
-// Restart: r32 = r / sigma on the interior (compact, padding 0) and the ring (LDS, from the sources' r), d = 0,
-// z = M32 r32, partial of eta(1) = z.r32.
-__device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double sigma) {
+// Residual replacement before inner phase B(n): r32_{n-1} = r / sigma on the interior (compact, padding 0) and the
+// ring (LDS, from the sources' r), d = 0, z = M32 r32, partial of eta(n) = z.r32.
+__device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double sigma, int n) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sNx = a.sNx, sNy = a.sNy, nx = a.nx, OL = a.OL, tbase = t * a.plane;
     float* tile = reinterpret_cast<float*>(mj_lds);
@@ -319,7 +325,7 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
     const float *cW = a.pW32 + oc, *cS = a.pS32 + oc, *cC = a.pC32 + oc;
     const MxCoef pre = coef_load(G, sNx, cW, cS, cC, wave * 64 + lane, inv);
     const MxRing rr = ring_first(a, t, a.ring_src);
-    float* r32 = R32(a, 0) + cb;
+    float* r32 = R32(a, n - 1) + cb;
     float* d = a.d + cb;
     for (int f = tid; f < G.NS; f += blockDim.x) {
         const int e = 4 * f, j = srow(f, inv), c4 = e - j * G.CP;
@@ -334,7 +340,7 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
     }
     ring_fill(a, t, rr, a.ring_src, tile, [&](int s) { return (float)(r[s] / sigma); });
     __syncthreads();
-    double* const dst[1] = {a.peta + a.B * a.T + m * a.T + t};  // eta(1)
+    double* const dst[1] = {a.peta + (n & 1) * a.B * a.T + m * a.T + t};  // eta(n)
     mx_stencil<false, 1>(
         a, cW, cS, cC, tile, a.z + cb, pre, [](float4 R, float4 zz, double(&pv)[1]) { pv[0] = dot4(zz, R); }, dst);
 }
@@ -415,9 +421,9 @@ __device__ __forceinline__ void mx_true(const MxDev& a, int m, int t) {
     store_tile_sums<MJ_SUM_TREE>(a.perr64, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
 }
 
-// Inner phase B(n): d += alpha_{n-1} s_{n-1} and s_n = z + beta s_{n-1} (s_1 = z) on the interior (compact, flat),
-// s_n on the ring (LDS), q = A32 s_n, sigma_n partial (s_n, q).
-__device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64, double alpha64) {
+// Inner phase B(n): d += alpha_{n-1} s_{n-1} (pend) and s_n = z + beta s_{n-1} (s_1 = z) on the interior (compact,
+// flat), s_n on the ring (LDS), q = A32 s_n, sigma_n partial (s_n, q).
+__device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64, double alpha64, bool pend) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sNx = a.sNx, sNy = a.sNy;
     float* tile = reinterpret_cast<float*>(mj_lds);
@@ -437,13 +443,16 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
         const int e = 4 * f;
         float4 v = ld4(z + ot + e);
         if (!first) {
-            const float4 O = ld4(so + ot + e), D = ld4(d + e);
-            float4 dn;
-            dn.x = D.x + alpha * O.x;
-            dn.y = D.y + alpha * O.y;
-            dn.z = D.z + alpha * O.z;
-            dn.w = D.w + alpha * O.w;
-            st4(d + e, dn);
+            const float4 O = ld4(so + ot + e);
+            if (pend) {
+                const float4 D = ld4(d + e);
+                float4 dn;
+                dn.x = D.x + alpha * O.x;
+                dn.y = D.y + alpha * O.y;
+                dn.z = D.z + alpha * O.z;
+                dn.w = D.w + alpha * O.w;
+                st4(d + e, dn);
+            }
             v.x = v.x + beta * O.x;
             v.y = v.y + beta * O.y;
             v.z = v.z + beta * O.z;
@@ -512,6 +521,22 @@ __global__ void __launch_bounds__(MJ_MX_THREADS) mx_init_kernel(MxDev a, int ph)
         mx_I1(a, m, t);
 }
 
+// The record of inner phase B(n + 1) after C(n).
+__device__ __forceinline__ MxRec mx_next_ib(const MxRec& r) {
+    MxRec nr = r;
+    nr.mode = MX_IB;
+    nr.n = r.n + 1;
+    nr.iters = r.iters + 1;
+    nr.fresh = 0;
+    return nr;
+}
+// Whether B(n) replaces the recursive residual instead (err(n-1) the recursive err in units of sigma^2): a cap
+// reached, the recursive residual fallen by inner_tolerance since the last replacement, or below the target.
+__device__ __forceinline__ bool mx_replace_due(const MxDev& a, const MxRec& r, double errp) {
+    if (r.iters >= a.max_iters || r.n - r.n0 >= a.inner_max) return true;
+    return errp < fmax(a.itol_sq * r.elast, a.tol_sq / (r.sigma * r.sigma));
+}
+
 // Tile-order sum of T <= 64 partials held by the lanes of one wave (lane t: tile t); every lane gets it.
 __device__ __forceinline__ double lane_tile_sum(double v, int T) {
     const int lo = __double2loint(v), hi = __double2hiint(v);
@@ -541,32 +566,36 @@ mx_step_kernel(MxDev a, int k) {
         if (r0.mode == MX_IC) {
             const double E = tl ? a.peta[cp * BT + o + lane] : 0.0, S = tl ? a.psig[cp * BT + o + lane] : 0.0;
             const double alpha = lane_tile_sum(E, a.T) / lane_tile_sum(S, a.T);
-            if (t == 0 && threadIdx.x == 0) a.rec[(k & 1) * a.B + m] = MxRec{MX_IB, n + 1, r0.iters + 1, r0.outer, r0.sigma};
+            if (t == 0 && threadIdx.x == 0) a.rec[(k & 1) * a.B + m] = mx_next_ib(r0);
             mx_IC(a, m, t, n, alpha);
             return;
         }
-        bool stop = r0.iters >= a.max_iters || n - 1 >= a.inner_max;
+        const bool pend = !r0.fresh;  // n > 1 whenever an update is pending
+        bool stop = false;
         double beta = 0.0, alpha_prev = 0.0;
         if (n > 1) {
-            // err(n-1) (inner stopping rule), eta(n), eta(n-1) (beta), sigma(n-1) (alpha_{n-1} of the pending
-            // solution update: the value C(n-1) used)
+            // eta(n), eta(n-1) (beta); with an update pending also err(n-1) (replacement rule) and sigma(n-1)
+            // (alpha_{n-1} of the pending solution update: the value C(n-1) used)
             const double ER = tl ? a.perr[pp * BT + o + lane] : 0.0, EN = tl ? a.peta[cp * BT + o + lane] : 0.0;
             const double EP = tl ? a.peta[pp * BT + o + lane] : 0.0, SP = tl ? a.psig[pp * BT + o + lane] : 0.0;
-            const double errp = lane_tile_sum(ER, a.T), etan = lane_tile_sum(EN, a.T);
-            const double etap = lane_tile_sum(EP, a.T), sigp = lane_tile_sum(SP, a.T);
-            if (!stop) stop = errp < a.itol_sq;
-            alpha_prev = etap / sigp;
+            const double etan = lane_tile_sum(EN, a.T), etap = lane_tile_sum(EP, a.T);
             beta = etan / etap;
+            if (pend) {
+                const double errp = lane_tile_sum(ER, a.T), sigp = lane_tile_sum(SP, a.T);
+                stop = mx_replace_due(a, r0, errp);
+                alpha_prev = etap / sigp;
+            }
         }
         if (t == 0 && threadIdx.x == 0) {
             MxRec nr = r0;
             nr.mode = stop ? MX_TRUE : MX_IC;
+            nr.fresh = 0;
             a.rec[(k & 1) * a.B + m] = nr;
         }
         if (stop)
             mx_update(a, m, t, n, r0.sigma, alpha_prev);
         else
-            mx_IB(a, m, t, n, beta, alpha_prev);
+            mx_IB(a, m, t, n, beta, alpha_prev, pend);
         return;
     }
     // wave 0 derives the member's action (every lane alike), lane 0 publishes it. The member's record and every
@@ -617,33 +646,38 @@ mx_step_kernel(MxDev a, int k) {
                 if (e < a.tol_sq || r.iters >= a.max_iters || r.outer >= a.max_outer) {
                     act = AX_FINISH;
                 } else {
+                    // the first residual sets the units of r32 and starts at n = 1; a replacement keeps both
                     act = AX_RESTART;
-                    nr = MxRec{MX_IB, 1, r.iters, r.outer + 1, sqrt(e)};
+                    const double sg = r.outer == 0 ? sqrt(e) : r.sigma;
+                    const int n1 = r.outer == 0 ? 1 : r.n;
+                    nr = MxRec{MX_IB, n1, r.iters, r.outer + 1, n1, 1, sg, e / (sg * sg)};
                 }
                 break;
             }
             case MX_IB: {
-                bool stop = r.iters >= a.max_iters || r.n - 1 >= a.inner_max;
+                bool stop = false;
                 if (r.n > 1) {
-                    // err(n-1) (inner stopping rule), eta(n), eta(n-1) (beta), sigma(n-1) (alpha_{n-1} of the
-                    // pending solution update: the value C(n-1) used)
+                    // eta(n), eta(n-1) (beta); with an update pending err(n-1) (replacement rule) and sigma(n-1)
+                    // (alpha_{n-1} of the pending solution update: the value C(n-1) used)
                     const int cp = r.n & 1, pp = (r.n - 1) & 1;
-                    const double errp = tsum(pp ? P[3] : P[2], a.perr + pp * BT + o);
                     const double etan = tsum(cp ? P[5] : P[4], a.peta + cp * BT + o);
                     const double etap = tsum(pp ? P[5] : P[4], a.peta + pp * BT + o);
-                    const double sigp = tsum(pp ? P[7] : P[6], a.psig + pp * BT + o);
-                    if (!stop) stop = errp < a.itol_sq;
-                    alpha_prev = etap / sigp;
                     c = etan / etap;
+                    if (!r.fresh) {
+                        const double errp = tsum(pp ? P[3] : P[2], a.perr + pp * BT + o);
+                        const double sigp = tsum(pp ? P[7] : P[6], a.psig + pp * BT + o);
+                        stop = mx_replace_due(a, r, errp);
+                        alpha_prev = etap / sigp;
+                    }
                 }
                 if (stop) {
                     act = AX_UPD;
                     nr.mode = MX_TRUE;
                 } else {
-                    act = AX_IB;
+                    act = r.fresh ? AX_IBF : AX_IB;
                     nr.mode = MX_IC;
                 }
-                if (r.n == 1) c = 0.0;
+                nr.fresh = 0;
                 break;
             }
             case MX_IC: {
@@ -652,7 +686,7 @@ mx_step_kernel(MxDev a, int k) {
                 const double sign = tsum(cp ? P[7] : P[6], a.psig + cp * BT + o);
                 act = AX_IC;
                 c = etan / sign;
-                nr = MxRec{MX_IB, r.n + 1, r.iters + 1, r.outer, r.sigma};
+                nr = mx_next_ib(r);
                 break;
             }
             default:
@@ -664,8 +698,8 @@ mx_step_kernel(MxDev a, int k) {
             g_bcast[3] = alpha_prev;
             g_ibcast = act;
             g_bcast[0] = c;
-            g_bcast[1] = act == AX_RESTART ? nr.sigma : r.sigma;
-            g_bcast[2] = (double)r.n;
+            g_bcast[1] = nr.sigma;
+            g_bcast[2] = (double)(act == AX_RESTART ? nr.n : r.n);  // the step's n
             if (t == 0) {
                 a.rec[(k & 1) * a.B + m] = nr;
                 if (act == AX_FINISH) {
@@ -680,13 +714,15 @@ mx_step_kernel(MxDev a, int k) {
         }
     }
     __syncthreads();
-    const int act = g_ibcast, n = (int)g_bcast[2];
+    const int act = g_ibcast;
     const double c = g_bcast[0], sigma = g_bcast[1], alpha_prev = g_bcast[3];
+    const int n = (int)g_bcast[2];
     switch (act) {
         case AX_FINISH: mx_finish(a, m, t); break;
-        case AX_RESTART: mx_restart(a, m, t, sigma); break;
+        case AX_RESTART: mx_restart(a, m, t, sigma, n); break;
         case AX_UPD: mx_update(a, m, t, n, sigma, alpha_prev); break;
-        case AX_IB: mx_IB(a, m, t, n, c, alpha_prev); break;
+        case AX_IB: mx_IB(a, m, t, n, c, alpha_prev, true); break;
+        case AX_IBF: mx_IB(a, m, t, n, c, alpha_prev, false); break;
         case AX_IC: mx_IC(a, m, t, n, c); break;
         default: mx_true(a, m, t); break;
     }

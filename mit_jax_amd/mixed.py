@@ -1,12 +1,16 @@
-"""Mixed-precision CG2D: FP32 inner PCG with FP64 iterative refinement (libmjamd mj_cg2d_solve_mixed).
+"""Mixed-precision CG2D: FP32 PCG with FP64 residual replacement (libmjamd mj_cg2d_solve_mixed).
 
 `cg2d_mixed(ex, ops, cg2d_b, cg2d_x, *, cg2dNorm, tolerance, max_iters, ...) -> (x, diag)` solves the system of
 `cg2d.cg2d_fortran` (reference core/cg2d.py:205-269) with the same normalisation, operator, preconditioner and
 stopping rule `err < tolerance**2` on the normalised residual, but stops on the FP64 *true* residual
-`b - A x` instead of the recursive one. FP64: true residuals, dot products, alpha/beta, the solution update.
-FP32: the inner operator/preconditioner copies, inner vectors and stencil arithmetic. Arguments, batching, operator
-slots and the returned x (un-normalised interior, Fortran halos) are as `cg2d.cg2d_fortran`; diag numIters counts
-inner iterations and outerIters the refinements.
+`b - A x` instead of the recursive one. The PCG runs in FP32; whenever its recursive residual has fallen by
+`inner_tolerance` since the last replacement (or below the target, or after `inner_max_iters` iterations) the FP32
+solution increment goes into the FP64 solution, the true residual is formed in FP64 and replaces the recursive one,
+and the PCG goes on from the same search direction (the Krylov space is kept: the iteration count is that of the
+FP64 PCG). FP64: true residuals, dot products, alpha/beta, the solution update. FP32: the operator/preconditioner
+copies, inner vectors and stencil arithmetic. Arguments, batching, operator slots and the returned x
+(un-normalised interior, Fortran halos) are as `cg2d.cg2d_fortran`; diag numIters counts inner iterations and
+outerIters the replacements (the first residual included).
 """
 
 import ctypes
@@ -16,7 +20,7 @@ import torch
 from . import _lib
 from .cg2d import Cg2dDiag, _as_batch, _norm_tensor, _ops_struct, config
 
-INNER_TOLERANCE = 1e-5
+INNER_TOLERANCE = 1e-4
 INNER_MAX_ITERS = 400
 MAX_OUTER = 40
 

@@ -66,7 +66,7 @@ def test_config2_64_rhs_one_mixed_launch(dev):
         res = _true_residual(ops, norm, bs[m], xs[m])
         err = _rel_err(xs[m], xr)
         errs.append(err)
-        print(f"rhs {100 + m}: inner {int(d['numIters'][m])} (fp64 {dr['numIters']}), refinements "
+        print(f"rhs {100 + m}: inner {int(d['numIters'][m])} (fp64 {dr['numIters']}), replacements "
               f"{int(d['outerIters'][m])}, true residual {res:.2e}, error vs FP64 oracle {err:.2e}", flush=True)
         assert res < RES_BOUND, (m, res)
         assert err < X_REL, (m, err)

@@ -1,4 +1,4 @@
-"""GPU: the mixed-precision CG2D (FP32 inner PCG + FP64 refinement) against the FP64 oracle.
+"""GPU: the mixed-precision CG2D (FP32 PCG with FP64 residual replacement) against the FP64 oracle.
 
 The mixed solve is not bitwise the FP64 solve; what it must meet is the FP64 stopping rule on the *true* normalised
 residual (checked here independently with the oracle's operator), and a solution that agrees with the FP64
@@ -58,10 +58,14 @@ def test_mixed_reaches_fp64_residual(dev, ex, precond):
                              sum_order="tile")
         res = _true_residual(ops, norm, bs[m], xs[m])
         last = float(d["lastResidual"][m])
-        print(f"member {m}: inner iters {int(d['numIters'][m])} (fp64 {dr['numIters']}), refinements "
+        print(f"member {m}: inner iters {int(d['numIters'][m])} (fp64 {dr['numIters']}), replacements "
               f"{int(d['outerIters'][m])}, true residual {res:.3e} (diag {last:.3e})", flush=True)
         assert last < TOL
         assert res < 1.5 * TOL  # the independent FP64 evaluation agrees with the kernel's
+        # residual replacement keeps the Krylov space: the FP32 PCG needs the FP64 PCG's iterations (a restart per
+        # replacement would cost 10-12 % more here)
+        assert int(d["numIters"][m]) <= dr["numIters"] + 3
+        assert 2 <= int(d["outerIters"][m]) <= 8
         assert float(d["firstResidual"][m]) == dr["firstResidual"]
         assert float(d["sumRHS"][m]) == dr["sumRHS"] and float(d["rhsMax"][m]) == dr["rhsMax"]
         err = np.abs(xs[m] - xr)[4:94, 4:94].max() / np.abs(xr[4:94, 4:94]).max()
