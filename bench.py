@@ -29,7 +29,8 @@ profile of this workload. `cpu_baseline` is the FP64 C oracle (OpenMP, all host 
 same member-steps. `producers` times the operator builder (UPDATE_CG2D) and the rhs builder (cg2d_rhs) on the same
 ensemble, each alone (not part of the headline step, whose operators and rhs are inputs); `implicit_r` likewise
 times GAD_IMPLICIT_R (the implicit vertical advection/diffusion solve) on the ensemble's tracer; `dst3_xyr` the DST3
-step with the vertical (R) flux.
+step with the vertical (R) flux. `single_member` gives B = 1, which is latency-bound and so has no roofline
+fraction, as microseconds per iteration of the FP64 and the mixed solve for one member of each level class.
 """
 
 import argparse
@@ -145,6 +146,33 @@ def plain_fp64(E, params, members):
         xs.append(one(m))
     torch.cuda.synchronize()
     return members / (time.perf_counter() - t0), xs
+
+
+def single_member(E, reps=5):
+    """B = 1 (latency-bound, no roofline fraction): for the first member of each level class, the FP64 solve
+    (cg2d_fortran, launch shape picked for one member) and the mixed solve alone, microseconds per iteration
+    (wall time of `reps` solves / their iterations)."""
+    import torch
+    from mit_jax_amd import cg2d, mixed
+    ex = E["ex"]
+    out = {}
+    for li, lv in enumerate(LEVEL_CHOICES):
+        m = int(np.nonzero(np.asarray(E["nlev"]) == lv)[0][0])
+        ops = [o[li] for o in E["ops"]]
+        args = (ex, ops, E["rhs"][m], E["x0"][m])
+        kw = dict(cg2dNorm=E["norm"][m:m + 1], tolerance=TOL, max_iters=2000)
+        row = {"member": m}
+        for name, fn, extra in (("fp64", cg2d.cg2d_fortran, {"sum_order": "tile"}), ("mixed", mixed.cg2d_mixed, {})):
+            _, d = fn(*args, **kw, **extra)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _, d = fn(*args, **kw, **extra)
+            torch.cuda.synchronize()
+            it = int(d["numIters"])
+            row[name] = {"iterations": it, "us_per_iteration": (time.perf_counter() - t0) / reps / it * 1e6}
+        out[str(lv)] = row
+    return out
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "mx_step_kernel_ensemble128_traffic.md"
@@ -579,6 +607,7 @@ def main():
                                                   "advect_tiles (member's levels)",
                              "solutions_equal_batched_fp64": same}
         out["mixed_over_plain_fp64"] = out["value"] / pv
+        out["single_member"] = single_member(E)
     if not args.no_producers:
         out["producers"] = producers(E, dev)
         out["implicit_r"] = implicit_r(E, dev)
