@@ -388,13 +388,14 @@ def dst3_xyr(E, dev, params, reps=3):
                                                              E["hW"], E["hS"], E["rhC"], w, E["nlev_t"])
     u, v, hW, hS, rhC = fields
     out = torch.zeros_like(trc)
-    work = torch.empty_like(trc)
     lib = L.lib()
+    wsb = lib.mj_gad_dst3_workspace_bytes(ctypes.byref(gcfg), trc.shape[1], 1)
+    work = torch.empty(wsb, dtype=torch.uint8, device=dev)
 
     def run():
         L.check(lib.mj_gad_dst3(plan.handle, ctypes.byref(grid_s), ctypes.byref(gcfg), L.ptr(u), L.ptr(v), L.ptr(wd),
                                 L.ptr(trc), L.ptr(hW), L.ptr(hS), L.ptr(rhC), L.ptr(out), L.ptr(work),
-                                L.stream_ptr()), "mj_gad_dst3")
+                                ctypes.c_size_t(wsb), L.stream_ptr()), "mj_gad_dst3")
 
     print("bench: DST3 with the R flux", file=sys.stderr, flush=True)
     run()

@@ -327,13 +327,19 @@ typedef struct {
                                   their grid masks), or NULL: slot = member. Ignored when mask_member_stride is 0 */
     int levels_per_block;    /* consecutive levels of one (member, tile) per workgroup of the horizontal passes
                                 (0 = 1); a launch shape: the results are the same for every value */
+    int mask_slots;          /* number of mask slots when mask_index is set (else ignored) */
 } mj_gad_config;
 
+/* Workspace of mj_gad_dst3 (with_w: the R flux runs): the maskLocW / maskLocS bit planes of every (mask slot, tile,
+ * level), built once per call, and with the R flux the localTij of the batch. */
+size_t mj_gad_dst3_workspace_bytes(const mj_gad_config* cfg, int ntiles, int with_w);
+
 /* uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, gTracer: [B][T, K, ny, nx]; wFld: same or NULL (no R flux).
- * work: device scratch of B*member_stride3d doubles when wFld != NULL (else may be NULL). */
+ * work: device scratch (256-byte aligned, not overlapping gTracer) of mj_gad_dst3_workspace_bytes. */
 int mj_gad_dst3(const mj_gad_plan* plan, const mj_gad_grid* grid, const mj_gad_config* cfg, const double* uFld,
                 const double* vFld, const double* wFld, const double* tracer, const double* hFacW,
-                const double* hFacS, const double* recip_hFacC, double* gTracer, double* work, void* stream);
+                const double* hFacS, const double* recip_hFacC, double* gTracer, void* work, size_t work_bytes,
+                void* stream);
 
 /* ---- the DST3 interface fluxes alone, for every (member, tile, level) plane (the reference's flux kernels; the
  * fused mj_gad_dst3 evaluates the same values inside its sweeps). */
@@ -365,8 +371,9 @@ int mj_dst3_adv_y(const mj_dst3_flux_config* cfg, double oneSixth, const double*
  * (every padded point of [B][T, nr, ny, nx]; levels at or beyond a member's nlev get 0; w at the surface
  * interface, which the R flux does not read, gets 0). The grid, hFac and mask inputs are constants. With wFld the
  * horizontal passes are replayed forward once (localTij into work), the vertical sweep is reversed per column,
- * then the horizontal sweeps per (member, tile, level). work: device, >= mj_gad_dst3_adjoint_workspace_bytes(cfg,
- * ntiles, wFld != NULL). Outputs must not alias inputs or each other. */
+ * then the horizontal sweeps per (member, tile, level). work: device, 256-byte aligned, of
+ * mj_gad_dst3_adjoint_workspace_bytes(cfg, ntiles, wFld != NULL) bytes (the mask bit planes as in mj_gad_dst3, and the
+ * replay scratch). Outputs must not alias inputs or each other. */
 size_t mj_gad_dst3_adjoint_workspace_bytes(const mj_gad_config* cfg, int ntiles, int with_w);
 int mj_gad_dst3_adjoint(const mj_gad_plan* plan, const mj_gad_grid* grid, const mj_gad_config* cfg,
                         const double* uFld, const double* vFld, const double* wFld, const double* tracer,

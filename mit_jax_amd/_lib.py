@@ -59,7 +59,7 @@ class GadGrid(ctypes.Structure):
 
 class GadConfig(ctypes.Structure):
     _fields_ = [("batch", I32), ("nr", I32), ("member_stride3d", I64), ("mask_member_stride", I64), ("nlev", P),
-                ("mask_index", P), ("levels_per_block", I32)]
+                ("mask_index", P), ("levels_per_block", I32), ("mask_slots", I32)]
 
 
 class Dst3FluxConfig(ctypes.Structure):
@@ -133,7 +133,9 @@ SIGNATURES = {
                                   P, P, P, P, ctypes.c_size_t, P]),
     "mj_dst3_adv_x": (I32, [ctypes.POINTER(Dst3FluxConfig), F64, P, P, P, P, P, P, P, P, P]),
     "mj_dst3_adv_y": (I32, [ctypes.POINTER(Dst3FluxConfig), F64, P, P, P, P, P, P, P, P, P]),
-    "mj_gad_dst3": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P, P]),
+    "mj_gad_dst3_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(GadConfig), I32, I32]),
+    "mj_gad_dst3": (I32, [P, ctypes.POINTER(GadGrid), ctypes.POINTER(GadConfig), P, P, P, P, P, P, P, P, P,
+                          ctypes.c_size_t, P]),
 }
 
 _lib = None

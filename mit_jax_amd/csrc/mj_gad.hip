@@ -37,10 +37,17 @@ struct GadDev {
     const double *u, *v, *w, *tr, *hFacW, *hFacS, *recip_hFacC;
     double* out;  // gTracer (no R flux) or localTij after the horizontal passes (R flux follows)
     const int32_t* nlev;
+    unsigned long long* mbits;  // maskLocW / maskLocS bit planes per (mask slot, tile, level), gad_maskbits_kernel
 };
 
 // Mask slot of member m: members of one level class share their grid masks.
 __device__ __forceinline__ int64_t mask_slot(const GadDev& a, int m) { return a.mask_index ? a.mask_index[m] : m; }
+constexpr int kMaskWords = (GPLANE + 63) / 64;
+// The bit planes of member m's masks at (tile t, level k): maskLocW's kMaskWords words, then maskLocS's.
+__device__ __forceinline__ const unsigned long long* unit_bits(const GadDev& a, int m, int t, int k) {
+    const int64_t s = a.mms ? mask_slot(a, m) : 0;
+    return a.mbits + ((s * a.T + t) * a.K + k) * 2 * kMaskWords;
+}
 
 // maskLocW / maskLocS: maskW / maskS with FILL_CS_CORNER_UV_RS(.FALSE.) applied (gad.py:263-281, :489-490).
 // (ia, ja) array indices; returns the corner-filled mask value at that point.
@@ -172,7 +179,6 @@ __device__ __forceinline__ void fix_mask_corners(const GadDev& a, const MjTilePl
 // so a bit and its double 0.0/1.0 carry the same value). Off the four corner halo squares mask_loc is the plain
 // mask, so every wave ballots the plain maskW / maskS of its words (all its loads in flight at once); then the
 // points of the corner squares of the corners the tile has are set from mask_loc.
-constexpr int kMaskWords = (GPLANE + 63) / 64;
 __device__ __forceinline__ void build_mask_bits(const GadDev& a, const MjTilePlan& tp, const double* mW,
                                                 const double* mS, unsigned long long* bw, unsigned long long* bs) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
@@ -198,45 +204,48 @@ __device__ __forceinline__ void build_mask_bits(const GadDev& a, const MjTilePla
     __syncthreads();
     fix_mask_corners(a, tp, mW, mS, bw, bs);
 }
-// The tracer plane into LDS (into loc and loc2) and the mask bit planes, with the tracer's and the plain masks'
-// loads in flight together (one memory latency for the three planes of a 1024-thread block), then the
-// corner-square fix-up.
-__device__ __forceinline__ void load_tile_and_masks(const GadDev& a, const MjTilePlan& tp, const double* tr,
-                                                    const double* mW, const double* mS, double* loc, double* loc2,
-                                                    unsigned long long* bw, unsigned long long* bs) {
-    const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
-    constexpr int KP = (GPLANE + 1023) / 1024, KW = (kMaskWords + 15) / 16;  // per thread / per wave at 1024
-    for (int it = 0; it * KP * nthr < GPLANE || it * KW * nwaves < kMaskWords; ++it) {
-        double v[KP], vw[KW], vs[KW];
+// maskLocW / maskLocS of every (mask slot, tile, level) as bit planes in global memory, once per call: the DST3
+// blocks of the members that share a slot load its 2 x 151 words instead of the two mask planes (2 x 77 KB).
+__global__ void __launch_bounds__(256) gad_maskbits_kernel(GadDev a, int nslots) {
+    __shared__ unsigned long long bw[kMaskWords], bs[kMaskWords];
+    const int u = blockIdx.x;
+    const int k = u % a.K, st = u / a.K, t = st % a.T, sl = st / a.T;
+    if (sl >= nslots) return;
+    const int64_t bm = sl * a.mms + ((int64_t)t * a.K + k) * GPLANE;
+    build_mask_bits(a, a.tiles[t], a.maskW + bm, a.maskS + bm, bw, bs);
+    unsigned long long* o = a.mbits + (int64_t)u * 2 * kMaskWords;
+    for (int i = threadIdx.x; i < kMaskWords; i += blockDim.x) {
+        o[i] = bw[i];
+        o[kMaskWords + i] = bs[i];
+    }
+}
+
+// The tracer plane into LDS (into loc and loc2) and the unit's mask bit planes, all loads in flight together (one
+// memory latency for a 1024-thread block).
+__device__ __forceinline__ void load_tile_and_bits(const double* tr, const unsigned long long* gb, double* loc,
+                                                   double* loc2, unsigned long long* bw, unsigned long long* bs) {
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    constexpr int KP = (GPLANE + 1023) / 1024;  // per thread at 1024
+    for (int it = 0; it * KP * nthr < GPLANE || it * nthr < 2 * kMaskWords; ++it) {
+        double v[KP];
 #pragma unroll
         for (int b = 0; b < KP; ++b) {
             const int q = (it * KP + b) * nthr + tid;
             v[b] = q < GPLANE ? tr[q] : 0.0;
         }
-#pragma unroll
-        for (int r = 0; r < KW; ++r) {
-            const int wd = (it * KW + r) * nwaves + wave, q = wd * 64 + lane;
-            const bool in = wd < kMaskWords && q < GPLANE;
-            vw[r] = in ? mW[q] : 0.0;
-            vs[r] = in ? mS[q] : 0.0;
-        }
+        const int wd = it * nthr + tid;
+        const unsigned long long word = wd < 2 * kMaskWords ? gb[wd] : 0ull;
 #pragma unroll
         for (int b = 0; b < KP; ++b) {
             const int q = (it * KP + b) * nthr + tid;
             if (q < GPLANE) loc[q] = loc2[q] = v[b];
         }
-#pragma unroll
-        for (int r = 0; r < KW; ++r) {
-            const int wd = (it * KW + r) * nwaves + wave;
-            const unsigned long long bwv = __ballot(vw[r] != 0.0), bsv = __ballot(vs[r] != 0.0);
-            if (lane == 0 && wd < kMaskWords) {
-                bw[wd] = bwv;
-                bs[wd] = bsv;
-            }
-        }
+        if (wd < kMaskWords)
+            bw[wd] = word;
+        else if (wd < 2 * kMaskWords)
+            bs[wd - kMaskWords] = word;
     }
     __syncthreads();
-    fix_mask_corners(a, tp, mW, mS, bw, bs);
 }
 
 __device__ __forceinline__ double mbit(const unsigned long long* b, int q) {
@@ -279,11 +288,8 @@ __device__ __forceinline__ void hz_plane(const GadDev& a, int t, int m, int k, d
     const int tid = threadIdx.x;
     const MjTilePlan& tp = a.tiles[t];
     const int64_t b3 = m * a.ms3 + ((int64_t)t * a.K + k) * GPLANE;
-    const int64_t bm = mask_slot(a, m) * a.mms + ((int64_t)t * a.K + k) * GPLANE;
     const int64_t b2 = (int64_t)t * GPLANE;
     const double* __restrict__ tr = a.tr + b3;
-    const double* mW = a.maskW + bm;
-    const double* mS = a.maskS + bm;
     const double* __restrict__ rhC = a.recip_hFacC + b3;
     const double *__restrict__ rrA = a.recip_rA + b2, *__restrict__ mInC = a.maskInC + b2;
     const double dt = a.dTlev[k];
@@ -291,7 +297,7 @@ __device__ __forceinline__ void hz_plane(const GadDev& a, int t, int m, int k, d
     const double rhoFacC = a.rhoFacC[k], rrhoFacC = a.recip_rhoFacC[k], drF = a.drF[k], rdrF = a.recip_drF[k];
     const double oneSixth = a.oneSixth;
 
-    load_tile_and_masks(a, tp, tr, mW, mS, loc, trc, mlw, mls);
+    load_tile_and_bits(tr, unit_bits(a, m, t, k), loc, trc, mlw, mls);
 
     for (int pass = 0; pass < 3; ++pass) {
         for (int dir = 0; dir < 2; ++dir) {
@@ -452,7 +458,6 @@ __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const do
         return;
     }
     const MjTilePlan& tp = a.tiles[t];
-    const int64_t bm = mask_slot(a, m) * a.mms + ((int64_t)t * a.K + k) * GPLANE;
     const int64_t b2 = (int64_t)t * GPLANE;
     const double* tr = a.tr + b3;
     const double* u = a.u + b3;
@@ -460,8 +465,6 @@ __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const do
     const double* hW = a.hFacW + b3;
     const double* hS = a.hFacS + b3;
     const double* rhC = a.recip_hFacC + b3;
-    const double* mW = a.maskW + bm;
-    const double* mS = a.maskS + bm;
     const double *dyG = a.dyG + b2, *dxG = a.dxG + b2, *rdxC = a.recip_dxC + b2, *rdyC = a.recip_dyC + b2;
     const double *rrA = a.recip_rA + b2, *mInC = a.maskInC + b2;
     const double dt = a.dTlev[k];
@@ -513,7 +516,16 @@ __global__ void __launch_bounds__(1024) gad_hz_adjoint_kernel(GadDev a, const do
     // ---- forward replay, snapshot before each sweep
     int order[6], ns = 0;
     for (int q = tid; q < GPLANE; q += nthr) loc[q] = tr[q];
-    build_mask_bits(a, tp, mW, mS, mlw, mls);
+    {
+        const unsigned long long* gb = unit_bits(a, m, t, k);
+        for (int wd = tid; wd < 2 * kMaskWords; wd += nthr) {
+            if (wd < kMaskWords)
+                mlw[wd] = gb[wd];
+            else
+                mls[wd - kMaskWords] = gb[wd];
+        }
+    }
+    __syncthreads();
     for (int pass = 0; pass < 3; ++pass) {
         for (int dir = 0; dir < 2; ++dir) {
             const MjSweep& S = tp.sweep[pass][dir];
@@ -958,6 +970,40 @@ static int describe(const mj_gad_plan* P, uint8_t* masks, int32_t* fills) {
 }
 
 namespace {
+size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+// Mask slots: one shared set (mask_member_stride 0), one per member, or mask_slots sets picked by mask_index.
+int mask_slots_of(const mj_gad_config* c) {
+    return c->mask_member_stride == 0 ? 1 : (c->mask_index ? c->mask_slots : c->batch);
+}
+// The bit planes of every (mask slot, tile, level).
+size_t mask_bits_bytes(const mj_gad_config* c, int ntiles) {
+    return (size_t)mask_slots_of(c) * ntiles * c->nr * 2 * kMaskWords * sizeof(unsigned long long);
+}
+// One member-strided FP64 3-D array of the batch (the R flux's localTij, the reverse mode's replay planes).
+size_t batch3_bytes(const mj_gad_config* c, int ntiles) {
+    return (size_t)c->batch * (size_t)std::max<int64_t>(c->member_stride3d, (int64_t)ntiles * c->nr * GPLANE) *
+           sizeof(double);
+}
+// The reverse mode's part of its workspace (before the bit planes).
+size_t adjoint_scratch_bytes(const mj_gad_config* c, int ntiles, int with_w) {
+    size_t n = (size_t)kAdjChunk * 7 * GPLANE * sizeof(double);  // the replay scratch of the horizontal reverse
+    if (with_w) n += 2 * batch3_bytes(c, ntiles);
+    return al256(n);
+}
+// The bit planes of maskLocW / maskLocS for every (mask slot, tile, level) into a.mbits.
+int launch_mask_bits(const GadDev& a, const mj_gad_config* c, hipStream_t st) {
+    const int nslots = mask_slots_of(c);
+    hipLaunchKernelGGL(gad_maskbits_kernel, dim3((unsigned)(nslots * a.T * a.K)), dim3(256), 0, st, a, nslots);
+    MJ_HIP(hipGetLastError());
+    return MJ_OK;
+}
+int check_mask_slots(const mj_gad_config* c, int ntiles) {
+    MJ_REQUIRE(c->mask_member_stride == 0 || c->mask_index == nullptr || c->mask_slots >= 1,
+               "mj_gad: mask_slots (the number of mask slots) is required with mask_index");
+    MJ_REQUIRE((int64_t)mask_slots_of(c) * ntiles * c->nr < (1ll << 31), "mj_gad: too many mask slots");
+    return MJ_OK;
+}
+
 // Device arguments of the DST3 kernels shared by the forward and reverse entry points.
 GadDev gad_dev(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c, const double* uFld,
                const double* vFld, const double* wFld, const double* tracer, const double* hFacW, const double* hFacS,
@@ -1007,10 +1053,15 @@ GadDev gad_dev(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* 
 }
 }  // namespace
 
+extern "C" size_t mj_gad_dst3_workspace_bytes(const mj_gad_config* c, int ntiles, int with_w) {
+    if (!c || c->batch < 1 || c->nr < 1 || ntiles < 1) return 0;
+    return (with_w ? al256(batch3_bytes(c, ntiles)) : 0) + al256(mask_bits_bytes(c, ntiles));
+}
+
 extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c, const double* uFld,
                            const double* vFld, const double* wFld, const double* tracer, const double* hFacW,
-                           const double* hFacS, const double* recip_hFacC, double* gTracer, double* work,
-                           void* stream) {
+                           const double* hFacS, const double* recip_hFacC, double* gTracer, void* work,
+                           size_t work_bytes, void* stream) {
     MJ_REQUIRE(P && g && c, "mj_gad_dst3: null plan/grid/config");
     MJ_REQUIRE(uFld && vFld && tracer && hFacW && hFacS && recip_hFacC && gTracer, "mj_gad_dst3: null field");
     MJ_REQUIRE(g->dyG && g->dxG && g->recip_dxC && g->recip_dyC && g->recip_rA && g->maskInC && g->maskW && g->maskS,
@@ -1022,13 +1073,23 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
     MJ_REQUIRE(c->member_stride3d >= (int64_t)P->ntiles * c->nr * GPLANE || c->batch == 1,
                "mj_gad_dst3: member_stride3d too small");
     MJ_REQUIRE((int64_t)c->batch * P->ntiles * c->nr < (1ll << 31), "mj_gad_dst3: too many (member, tile, level)s");
-    if (wFld) {
-        MJ_REQUIRE(work && g->maskC && g->rA && g->recip_drC, "mj_gad_dst3: the R flux needs work, maskC, rA, recip_drC");
-        MJ_REQUIRE(work != gTracer, "mj_gad_dst3: work must not alias gTracer");
+    int rc = check_mask_slots(c, P->ntiles);
+    if (rc) return rc;
+    if (wFld) MJ_REQUIRE(g->maskC && g->rA && g->recip_drC, "mj_gad_dst3: the R flux needs maskC, rA, recip_drC");
+    const size_t need = mj_gad_dst3_workspace_bytes(c, P->ntiles, wFld != nullptr);
+    MJ_REQUIRE(work && work_bytes >= need, "mj_gad_dst3: work %zu < %zu bytes", work ? work_bytes : 0, need);
+    MJ_REQUIRE(((uintptr_t)work & 255) == 0, "mj_gad_dst3: work must be 256-byte aligned");
+    {
+        const char *w0 = (const char*)work, *w1 = w0 + need;
+        const char *g0 = (const char*)gTracer, *g1 = g0 + batch3_bytes(c, P->ntiles);
+        MJ_REQUIRE(w1 <= g0 || g1 <= w0, "mj_gad_dst3: work must not overlap gTracer");
     }
+    double* loc_h = (double*)work;
     GadDev a = gad_dev(P, g, c, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC);
-    a.out = wFld ? work : gTracer;
+    a.mbits = (unsigned long long*)((char*)work + (wFld ? al256(batch3_bytes(c, P->ntiles)) : 0));
+    a.out = wFld ? loc_h : gTracer;
     hipStream_t st = as_stream(stream);
+    if ((rc = launch_mask_bits(a, c, st))) return rc;
     const int nkb = (c->nr + a.lpb - 1) / a.lpb;
     const unsigned nblk = (unsigned)((c->batch * P->ntiles * nkb + 7) / 8 * 8);
     if (a.lpb > 1)
@@ -1039,16 +1100,15 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
     if (wFld) {
         a.out = gTracer;
         hipLaunchKernelGGL(gad_vert_kernel, dim3((GPLANE + 255) / 256, (unsigned)(c->batch * P->ntiles)), dim3(256), 0,
-                           st, a, (const double*)work);
+                           st, a, (const double*)loc_h);
         MJ_HIP(hipGetLastError());
     }
     return MJ_OK;
 }
 
 extern "C" size_t mj_gad_dst3_adjoint_workspace_bytes(const mj_gad_config* c, int ntiles, int with_w) {
-    size_t n = (size_t)kAdjChunk * 7 * GPLANE * sizeof(double);  // the replay scratch of the horizontal reverse
-    if (with_w && c) n += 2 * (size_t)c->batch * (size_t)std::max<int64_t>(c->member_stride3d, (int64_t)ntiles * c->nr * GPLANE) * sizeof(double);
-    return n;
+    if (!c || c->batch < 1 || c->nr < 1 || ntiles < 1) return 0;
+    return adjoint_scratch_bytes(c, ntiles, with_w) + al256(mask_bits_bytes(c, ntiles));
 }
 
 extern "C" int mj_gad_dst3_adjoint(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c,
@@ -1072,17 +1132,25 @@ extern "C" int mj_gad_dst3_adjoint(const mj_gad_plan* P, const mj_gad_grid* g, c
         for (const double* o : {(const double*)tracer_bar, (const double*)uFld_bar, (const double*)vFld_bar})
             MJ_REQUIRE(o != wFld_bar, "mj_gad_dst3_adjoint: wFld_bar aliases another output");
     }
+    int rc = check_mask_slots(c, P->ntiles);
+    if (rc) return rc;
     const size_t need = mj_gad_dst3_adjoint_workspace_bytes(c, P->ntiles, wFld != nullptr);
     MJ_REQUIRE(work_bytes >= need, "mj_gad_dst3_adjoint: work %zu < %zu bytes", work_bytes, need);
+    MJ_REQUIRE(((uintptr_t)work & 255) == 0, "mj_gad_dst3_adjoint: work must be 256-byte aligned");
     hipStream_t st = as_stream(stream);
     const int nblk = c->batch * P->ntiles * c->nr;
+    unsigned long long* mbits =
+        (unsigned long long*)((char*)work + adjoint_scratch_bytes(c, P->ntiles, wFld != nullptr));
+    GadDev a = gad_dev(P, g, c, uFld, vFld, nullptr, tracer, hFacW, hFacS, recip_hFacC);
+    a.mbits = mbits;
+    if ((rc = launch_mask_bits(a, c, st))) return rc;
     double* lbar = nullptr;
     if (wFld) {
         // forward replay of the horizontal passes (localTij), then the vertical reverse per column
-        const int64_t ms = std::max<int64_t>(c->member_stride3d, (int64_t)P->ntiles * c->nr * GPLANE);
         double* loc_h = work + (size_t)kAdjChunk * 7 * GPLANE;
-        lbar = loc_h + (size_t)c->batch * ms;
+        lbar = loc_h + batch3_bytes(c, P->ntiles) / sizeof(double);
         GadDev f = gad_dev(P, g, c, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC);
+        f.mbits = mbits;
         f.out = loc_h;
         f.lpb = 1;
         const unsigned nb8 = (unsigned)((nblk + 7) / 8 * 8);
@@ -1092,7 +1160,6 @@ extern "C" int mj_gad_dst3_adjoint(const mj_gad_plan* P, const mj_gad_grid* g, c
                            dim3(256), 0, st, f, (const double*)loc_h, gTracer_bar, lbar, tracer_bar, wFld_bar);
         MJ_HIP(hipGetLastError());
     }
-    GadDev a = gad_dev(P, g, c, uFld, vFld, nullptr, tracer, hFacW, hFacS, recip_hFacC);
     for (int b0 = 0; b0 < nblk; b0 += kAdjChunk) {
         const int n = std::min(kAdjChunk, nblk - b0);
         hipLaunchKernelGGL(gad_hz_adjoint_kernel, dim3((unsigned)n), dim3(1024), 0, st, a, gTracer_bar, tracer_bar,

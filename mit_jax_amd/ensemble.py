@@ -75,6 +75,10 @@ class EnsembleStep:
         plan, grid_s, gcfg, trc, fields, _, keep = gad._prepare(params, None, g2d, vg, u, v, tr, hFacW, hFacS,
                                                                  recip_hFacC, None, nlev)
         self._gad = (plan, grid_s, gcfg, trc, fields, keep)
+        self.gwsb = _lib.lib().mj_gad_dst3_workspace_bytes(ctypes.byref(gcfg), ex.L.nTiles, 0)
+        if self.gwsb == 0:
+            raise _lib.MjError(f"mj_gad_dst3_workspace_bytes: {_lib.last_error()}")
+        self.gws = torch.empty(self.gwsb, dtype=torch.uint8, device=dev)
         # levels at or beyond a member's level count are never written by the DST3 launch: they stay 0
         self.gT = torch.zeros_like(trc)
         self.graph = None
@@ -125,7 +129,8 @@ class EnsembleStep:
         u, v, hW, hS, rhC = fields
         _lib.check(_lib.lib().mj_gad_dst3(plan.handle, ctypes.byref(grid_s), ctypes.byref(gcfg), _lib.ptr(u),
                                           _lib.ptr(v), None, _lib.ptr(trc), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(rhC),
-                                          _lib.ptr(self.gT), None, _lib.stream_ptr()), "mj_gad_dst3")
+                                          _lib.ptr(self.gT), _lib.ptr(self.gws), ctypes.c_size_t(self.gwsb),
+                                          _lib.stream_ptr()), "mj_gad_dst3")
 
     def _side(self):
         if getattr(self, "side", None) is None:

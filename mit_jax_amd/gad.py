@@ -246,6 +246,7 @@ def _prepare(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS, re
         if nl.numel() != B or int(nl.max()) > K or int(nl.min()) < 0:
             raise ValueError("nlev: one value in 0..K per member")
     cfg = _lib.GadConfig(B, K, T * K * ny * nx, mms, _lib.ptr(nl), _lib.ptr(midx))
+    cfg.mask_slots = mshape[0] if len(mshape) == 5 else 1
     plan = tables if isinstance(tables, GadPlan) else gad_plan(params.topo, L, dev)
     keep = (g, vt, nl, midx)
     return plan, grid_s, cfg, tr, fields, w, keep
@@ -318,12 +319,14 @@ def advect_tiles(params, tables, grid2d, vgrid, uFld, vFld, tracer, hFacW, hFacS
         gT = torch.zeros_like(tr) if nlev is not None else torch.empty_like(tr)
     else:
         gT = _check_out(out, tr, "out")
-    work = torch.empty_like(tr) if w is not None else None
+    lib = _lib.lib()
+    wsb = lib.mj_gad_dst3_workspace_bytes(ctypes.byref(cfg), params.layout.nTiles, int(w is not None))
+    work = torch.empty(wsb, dtype=torch.uint8, device=tr.device)
     u, v, hW, hS, rhC = fields
-    _lib.check(_lib.lib().mj_gad_dst3(plan.handle, ctypes.byref(grid_s), ctypes.byref(cfg), _lib.ptr(u), _lib.ptr(v),
-                                      _lib.ptr(w), _lib.ptr(tr), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(rhC),
-                                      _lib.ptr(gT), _lib.ptr(work), _lib.stream_ptr()), "mj_gad_dst3")
-    del keep
+    _lib.check(lib.mj_gad_dst3(plan.handle, ctypes.byref(grid_s), ctypes.byref(cfg), _lib.ptr(u), _lib.ptr(v),
+                               _lib.ptr(w), _lib.ptr(tr), _lib.ptr(hW), _lib.ptr(hS), _lib.ptr(rhC), _lib.ptr(gT),
+                               _lib.ptr(work), ctypes.c_size_t(wsb), _lib.stream_ptr()), "mj_gad_dst3")
+    del keep, work
     return gT
 
 
