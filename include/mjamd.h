@@ -330,8 +330,9 @@ typedef struct {
     int mask_slots;          /* number of mask slots when mask_index is set (else ignored) */
 } mj_gad_config;
 
-/* Workspace of mj_gad_dst3 (with_w: the R flux runs): the maskLocW / maskLocS bit planes of every (mask slot, tile,
- * level), built once per call, and with the R flux the localTij of the batch. */
+/* Workspace of mj_gad_dst3 (with_w: the R flux runs): tables built once per call (the maskLocW / maskLocS bit planes
+ * of every (mask slot, tile, level), the 2-D metrics in the pairs the sweeps load together) and with the R flux the
+ * localTij of the batch. */
 size_t mj_gad_dst3_workspace_bytes(const mj_gad_config* cfg, int ntiles, int with_w);
 
 /* uFld, vFld, tracer, hFacW, hFacS, recip_hFacC, gTracer: [B][T, K, ny, nx]; wFld: same or NULL (no R flux).

@@ -38,6 +38,8 @@ struct GadDev {
     double* out;  // gTracer (no R flux) or localTij after the horizontal passes (R flux follows)
     const int32_t* nlev;
     unsigned long long* mbits;  // maskLocW / maskLocS bit planes per (mask slot, tile, level), gad_maskbits_kernel
+    double2* g2;  // per tile, the 2-D metrics a sweep point reads, in pairs (gad_pack2d_kernel): [3][T][GPLANE]
+                  // {dyG, recip_dxC} (X faces), {dxG, recip_dyC} (Y faces), {recip_rA, maskInC}
 };
 
 // Mask slot of member m: members of one level class share their grid masks.
@@ -220,6 +222,16 @@ __global__ void __launch_bounds__(256) gad_maskbits_kernel(GadDev a, int nslots)
     }
 }
 
+// The 2-D metrics in the pairs the sweeps load together (one 16-byte load instead of two 8-byte ones), per call.
+__global__ void __launch_bounds__(256) gad_pack2d_kernel(GadDev a) {
+    const int64_t n = (int64_t)a.T * GPLANE;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    a.g2[q] = make_double2(a.dyG[q], a.recip_dxC[q]);
+    a.g2[n + q] = make_double2(a.dxG[q], a.recip_dyC[q]);
+    a.g2[2 * n + q] = make_double2(a.recip_rA[q], a.maskInC[q]);
+}
+
 // The tracer plane into LDS (into loc and loc2) and the unit's mask bit planes, all loads in flight together (one
 // memory latency for a 1024-thread block).
 __device__ __forceinline__ void load_tile_and_bits(const double* tr, const unsigned long long* gb, double* loc,
@@ -291,7 +303,7 @@ __device__ __forceinline__ void hz_plane(const GadDev& a, int t, int m, int k, d
     const int64_t b2 = (int64_t)t * GPLANE;
     const double* __restrict__ tr = a.tr + b3;
     const double* __restrict__ rhC = a.recip_hFacC + b3;
-    const double *__restrict__ rrA = a.recip_rA + b2, *__restrict__ mInC = a.maskInC + b2;
+    const double2* __restrict__ am = a.g2 + 2 * (int64_t)a.T * GPLANE + b2;  // {recip_rA, maskInC}
     const double dt = a.dTlev[k];
     const double deepFacC = a.deepFacC[k], rdeepC = a.recip_deepFacC[k], rdeep2C = a.recip_deepFac2C[k];
     const double rhoFacC = a.rhoFacC[k], rrhoFacC = a.recip_rhoFacC[k], drF = a.drF[k], rdrF = a.recip_drF[k];
@@ -305,8 +317,7 @@ __device__ __forceinline__ void hz_plane(const GadDev& a, int t, int m, int k, d
             if (!S.run) continue;
             const double* __restrict__ vel = (dir == 0 ? a.u : a.v) + b3;
             const double* __restrict__ hf = (dir == 0 ? a.hFacW : a.hFacS) + b3;
-            const double* __restrict__ gl = (dir == 0 ? a.dyG : a.dxG) + b2;
-            const double* __restrict__ rdc = (dir == 0 ? a.recip_dxC : a.recip_dyC) + b2;
+            const double2* __restrict__ gr = a.g2 + dir * (int64_t)a.T * GPLANE + b2;  // {dyG|dxG, recip_dxC|dyC}
             const unsigned long long* mb = dir == 0 ? mlw : mls;
             const int d = dir == 0 ? 1 : GNX, cmax = dir == 0 ? GNX - 2 : GNY - 2;
             auto trans = [&](double vv, double g, double h) { return vv * (g * deepFacC * drF * h) * rhoFacC; };
@@ -335,9 +346,10 @@ __device__ __forceinline__ void hz_plane(const GadDev& a, int t, int m, int k, d
                 inc[r] = 0.0;
                 if (idx < n0 + n1) {
                     const int qq = point(idx), qd = qq + d;
-                    const double V0 = vel[qq], G0 = gl[qq], H0 = hf[qq], R0 = rdc[qq];
-                    const double V1 = vel[qd], G1 = gl[qd], H1 = hf[qd], R1 = rdc[qd];
-                    const double TR = trc[qq], RH = rhC[qq], RA = rrA[qq], MI = mInC[qq];
+                    const double2 GR0 = gr[qq], GR1 = gr[qd], AM = am[qq];
+                    const double V0 = vel[qq], G0 = GR0.x, H0 = hf[qq], R0 = GR0.y;
+                    const double V1 = vel[qd], G1 = GR1.x, H1 = hf[qd], R1 = GR1.y;
+                    const double TR = trc[qq], RH = rhC[qq], RA = AM.x, MI = AM.y;
                     const double T0 = trans(V0, G0, H0), T1 = trans(V1, G1, H1);
                     const double f0 = face(qq, V0, T0, R0), f1 = face(qd, V1, T1, R1);
                     // local - coef*(d(af) - tracer*d(trans))*maskInC (gad.py:511-514), coefficient gad.py:494
@@ -975,9 +987,12 @@ size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 int mask_slots_of(const mj_gad_config* c) {
     return c->mask_member_stride == 0 ? 1 : (c->mask_index ? c->mask_slots : c->batch);
 }
-// The bit planes of every (mask slot, tile, level).
+// The bit planes of every (mask slot, tile, level), then the packed 2-D metrics.
 size_t mask_bits_bytes(const mj_gad_config* c, int ntiles) {
     return (size_t)mask_slots_of(c) * ntiles * c->nr * 2 * kMaskWords * sizeof(unsigned long long);
+}
+size_t pre_bytes(const mj_gad_config* c, int ntiles) {
+    return al256(mask_bits_bytes(c, ntiles)) + al256((size_t)3 * ntiles * GPLANE * sizeof(double2));
 }
 // One member-strided FP64 3-D array of the batch (the R flux's localTij, the reverse mode's replay planes).
 size_t batch3_bytes(const mj_gad_config* c, int ntiles) {
@@ -990,10 +1005,15 @@ size_t adjoint_scratch_bytes(const mj_gad_config* c, int ntiles, int with_w) {
     if (with_w) n += 2 * batch3_bytes(c, ntiles);
     return al256(n);
 }
-// The bit planes of maskLocW / maskLocS for every (mask slot, tile, level) into a.mbits.
-int launch_mask_bits(const GadDev& a, const mj_gad_config* c, hipStream_t st) {
+// The per-call tables at `pre` (pre_bytes): the bit planes of maskLocW / maskLocS for every (mask slot, tile,
+// level) and the packed 2-D metrics; sets a.mbits and a.g2.
+int launch_pre(GadDev& a, const mj_gad_config* c, void* pre, hipStream_t st) {
     const int nslots = mask_slots_of(c);
+    a.mbits = (unsigned long long*)pre;
+    a.g2 = (double2*)((char*)pre + al256(mask_bits_bytes(c, a.T)));
     hipLaunchKernelGGL(gad_maskbits_kernel, dim3((unsigned)(nslots * a.T * a.K)), dim3(256), 0, st, a, nslots);
+    MJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gad_pack2d_kernel, dim3((unsigned)((a.T * GPLANE + 255) / 256)), dim3(256), 0, st, a);
     MJ_HIP(hipGetLastError());
     return MJ_OK;
 }
@@ -1055,7 +1075,7 @@ GadDev gad_dev(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* 
 
 extern "C" size_t mj_gad_dst3_workspace_bytes(const mj_gad_config* c, int ntiles, int with_w) {
     if (!c || c->batch < 1 || c->nr < 1 || ntiles < 1) return 0;
-    return (with_w ? al256(batch3_bytes(c, ntiles)) : 0) + al256(mask_bits_bytes(c, ntiles));
+    return (with_w ? al256(batch3_bytes(c, ntiles)) : 0) + pre_bytes(c, ntiles);
 }
 
 extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c, const double* uFld,
@@ -1086,10 +1106,9 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
     }
     double* loc_h = (double*)work;
     GadDev a = gad_dev(P, g, c, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC);
-    a.mbits = (unsigned long long*)((char*)work + (wFld ? al256(batch3_bytes(c, P->ntiles)) : 0));
     a.out = wFld ? loc_h : gTracer;
     hipStream_t st = as_stream(stream);
-    if ((rc = launch_mask_bits(a, c, st))) return rc;
+    if ((rc = launch_pre(a, c, (char*)work + (wFld ? al256(batch3_bytes(c, P->ntiles)) : 0), st))) return rc;
     const int nkb = (c->nr + a.lpb - 1) / a.lpb;
     const unsigned nblk = (unsigned)((c->batch * P->ntiles * nkb + 7) / 8 * 8);
     if (a.lpb > 1)
@@ -1108,7 +1127,7 @@ extern "C" int mj_gad_dst3(const mj_gad_plan* P, const mj_gad_grid* g, const mj_
 
 extern "C" size_t mj_gad_dst3_adjoint_workspace_bytes(const mj_gad_config* c, int ntiles, int with_w) {
     if (!c || c->batch < 1 || c->nr < 1 || ntiles < 1) return 0;
-    return adjoint_scratch_bytes(c, ntiles, with_w) + al256(mask_bits_bytes(c, ntiles));
+    return adjoint_scratch_bytes(c, ntiles, with_w) + pre_bytes(c, ntiles);
 }
 
 extern "C" int mj_gad_dst3_adjoint(const mj_gad_plan* P, const mj_gad_grid* g, const mj_gad_config* c,
@@ -1139,18 +1158,16 @@ extern "C" int mj_gad_dst3_adjoint(const mj_gad_plan* P, const mj_gad_grid* g, c
     MJ_REQUIRE(((uintptr_t)work & 255) == 0, "mj_gad_dst3_adjoint: work must be 256-byte aligned");
     hipStream_t st = as_stream(stream);
     const int nblk = c->batch * P->ntiles * c->nr;
-    unsigned long long* mbits =
-        (unsigned long long*)((char*)work + adjoint_scratch_bytes(c, P->ntiles, wFld != nullptr));
     GadDev a = gad_dev(P, g, c, uFld, vFld, nullptr, tracer, hFacW, hFacS, recip_hFacC);
-    a.mbits = mbits;
-    if ((rc = launch_mask_bits(a, c, st))) return rc;
+    if ((rc = launch_pre(a, c, (char*)work + adjoint_scratch_bytes(c, P->ntiles, wFld != nullptr), st))) return rc;
     double* lbar = nullptr;
     if (wFld) {
         // forward replay of the horizontal passes (localTij), then the vertical reverse per column
         double* loc_h = work + (size_t)kAdjChunk * 7 * GPLANE;
         lbar = loc_h + batch3_bytes(c, P->ntiles) / sizeof(double);
         GadDev f = gad_dev(P, g, c, uFld, vFld, wFld, tracer, hFacW, hFacS, recip_hFacC);
-        f.mbits = mbits;
+        f.mbits = a.mbits;
+        f.g2 = a.g2;
         f.out = loc_h;
         f.lpb = 1;
         const unsigned nb8 = (unsigned)((nblk + 7) / 8 * 8);
