@@ -69,6 +69,31 @@ def test_mixed_workspace_and_config_validation():
         assert word in _lib.last_error(), (word, _lib.last_error())
 
 
+def test_dst3_workspace_query():
+    """mj_gad_dst3's workspace: the per-call tables (corner-filled mask bit planes, 2 x 151 words per (mask slot,
+    tile, level); the 2-D metrics in three double pairs per tile point), with the R flux also the batch's localTij;
+    mask slots as the configuration gives them (one shared set, one per member, or mask_slots with mask_index)."""
+    import ctypes
+    from mit_jax_amd import _lib
+    L = _lib.lib()
+    al = lambda v: (v + 255) // 256 * 256  # noqa: E731
+    T, K, B, plane = 13, 50, 4, 98 * 98
+    metrics = al(3 * T * plane * 16)
+    bits = lambda slots: al(slots * T * K * 2 * 151 * 8)  # noqa: E731
+    cfg = _lib.GadConfig(B, K, T * K * plane, 0, None, None)
+    assert L.mj_gad_dst3_workspace_bytes(ctypes.byref(cfg), T, 0) == bits(1) + metrics
+    assert L.mj_gad_dst3_workspace_bytes(ctypes.byref(cfg), T, 1) == al(B * T * K * plane * 8) + bits(1) + metrics
+    per_member = _lib.GadConfig(B, K, T * K * plane, T * K * plane, None, None)
+    assert L.mj_gad_dst3_workspace_bytes(ctypes.byref(per_member), T, 0) == bits(B) + metrics
+    slots = _lib.GadConfig(B, K, T * K * plane, T * K * plane, None, ctypes.c_void_p(256))
+    slots.mask_slots = 3
+    assert L.mj_gad_dst3_workspace_bytes(ctypes.byref(slots), T, 0) == bits(3) + metrics
+    assert L.mj_gad_dst3_workspace_bytes(ctypes.byref(_lib.GadConfig(0, K, 0, 0, None, None)), T, 0) == 0
+    # the reverse mode's workspace holds the same tables after its replay scratch
+    n = L.mj_gad_dst3_adjoint_workspace_bytes(ctypes.byref(slots), T, 0)
+    assert n == al(1024 * 7 * plane * 8) + bits(3) + metrics
+
+
 def test_operator_and_rhs_producers_validate_before_device_work():
     """mj_update_cg2d / mj_cg2d_rhs refuse null arguments, bad layouts and strides with a message (MJ_ERR_ARG)
     before touching the device."""
