@@ -464,26 +464,26 @@ int ensure_cg_tables(mj_halo_plan* P, int sNx, int sNy, int OL) {
         (rc = up(rsrc.data(), rsrc.size() * 4, (void**)&P->cg_ring_src)) ||
         (rc = up(rsg.data(), rsg.size(), (void**)&P->cg_ring_sign)))
         return rc;
-    // the mixed solve's compact form: LDS offset of each ring point, compact source index (mx_geo)
+    // the mixed solve's form, one 16-byte entry per ring point: compact source index (mx_geo; -1: none), padded
+    // source, LDS offset with the sign in bit 30
     const MxGeo G = mx_geo(sNx, sNy);
-    std::vector<int32_t> clds(rdst.size()), csrc(rdst.size());
+    std::vector<int4> cring(rdst.size());
     for (size_t k = 0; k < rdst.size(); ++k) {
         const int r = rdst[k] % plane, jh = r / nx, ih = r % nx;
-        clds[k] = G.lds(jh - OL, ih - OL);
+        const int lds = G.lds(jh - OL, ih - OL);
+        MJ_REQUIRE(lds >= 0 && lds < (1 << 30), "cg2d: LDS offset out of range");
         const int s = rsrc[k];
-        if (s < 0) {
-            csrc[k] = -1;
-        } else {
+        int csrc = -1;
+        if (s >= 0) {
             const int ts = s / plane, rs = s % plane, js = rs / nx - OL, is = rs % nx - OL;
-            csrc[k] = ts * G.TS + js * G.CP + is;
+            csrc = ts * G.TS + js * G.CP + is;
             // a source in the west or east column: its contiguous copy after the tile's rows
-            if (is == 0) csrc[k] = ts * G.TS + sNy * G.CP + js;
-            if (is == sNx - 1) csrc[k] = ts * G.TS + sNy * G.CP + sNy + js;
+            if (is == 0) csrc = ts * G.TS + sNy * G.CP + js;
+            if (is == sNx - 1) csrc = ts * G.TS + sNy * G.CP + sNy + js;
         }
+        cring[k] = make_int4(csrc, s, lds | (rsg[k] < 0 ? 1 << 30 : 0), 0);
     }
-    if ((rc = up(clds.data(), clds.size() * 4, (void**)&P->cg_cring_lds)) ||
-        (rc = up(csrc.data(), csrc.size() * 4, (void**)&P->cg_cring_src)))
-        return rc;
+    if ((rc = up(cring.data(), cring.size() * sizeof(int4), (void**)&P->cg_cring))) return rc;
     P->cg_sNx = sNx;
     P->cg_sNy = sNy;
     P->cg_OL = OL;

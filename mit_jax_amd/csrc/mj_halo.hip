@@ -179,7 +179,7 @@ mj_halo_plan::~mj_halo_plan() {
     }
     for (void* q : {(void*)cg_full_off, (void*)cg_full_dst, (void*)cg_full_src, (void*)cg_full_sign,
                     (void*)cg_ring_off, (void*)cg_ring_dst, (void*)cg_ring_src, (void*)cg_ring_sign,
-                    (void*)cg_cring_lds, (void*)cg_cring_src})
+                    (void*)cg_cring})
         if (q) (void)hipFree(q);
     for (int c = 0; c < 2; ++c)
         for (void* q : {(void*)tr_off[c], (void*)tr_dst[c], (void*)tr_map[c], (void*)tr_sign[c], (void*)tr_written[c]})

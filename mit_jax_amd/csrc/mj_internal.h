@@ -29,7 +29,7 @@ struct mj_halo_plan {
     int cg_ring_max = 0;  // largest per-tile ring count
     // the same ring entries for the compact work vectors of the mixed solve: the LDS offset of the ring point in
     // the stencil tile and the source's compact index within the member (-1: no source)
-    int32_t *cg_cring_lds = nullptr, *cg_cring_src = nullptr;
+    int4* cg_cring = nullptr;  // ring entries of the mixed solve: {compact source, padded source, LDS offset | neg << 30, 0}
 
     // Transpose tables (built once, on first use): per input component c (0 = u-like, 1 = v-like) and padded
     // point q, the written points that read q (tr_off[c][q] .. tr_off[c][q+1] into tr_dst/tr_map/tr_sign, sorted by

@@ -71,7 +71,7 @@ struct MxDev {
     const int8_t* full_sign;
     const int32_t *ring_off, *ring_dst, *ring_src;
     const int8_t* ring_sign;
-    const int32_t *cring_lds, *cring_src;  // ring entries in compact terms (LDS offset, compact source or -1)
+    const int4* cring;  // ring entries: {compact source or -1, padded source or -1, LDS offset | neg << 30, 0}
     double* r64;
     MxGeo G;     // compact geometry of the work vectors and coefficient copies
     int64_t cs;  // compact member stride = T * G.TS
@@ -123,12 +123,22 @@ __device__ __forceinline__ void lds_put(float* tile, const MxGeo& G, int j, int 
     if (c4 + 1 < sNx) p[1] = v.y;
     if (c4 + 2 < sNx) p[2] = v.z;
 }
-// The west/east column copies of a tile's compact vector (base: the tile's compact start) for one slot.
-__device__ __forceinline__ void edge_put(float* base, const MxGeo& G, int j, int c4, float4 v, int sNx, int sNy) {
-    float* ed = base + sNy * G.CP;
-    if (c4 == 0) ed[j] = v.x;
+// The west (then east) column of a stencil result, gathered in LDS and written as the compact vector's contiguous
+// column copies after the sweep (coalesced, instead of a partial store per slot).
+static __shared__ float g_edge[256];
+__device__ __forceinline__ void edge_stash(int j, int c4, float4 v, int sNx, int sNy) {
+    if (c4 == 0) g_edge[j] = v.x;
     const int k = sNx - 1 - c4;
-    if (k >= 0 && k < 4) ed[sNy + j] = k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+    if (k >= 0 && k < 4) g_edge[sNy + j] = k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+// The west and east column copies of a tile's compact vector (base: the tile's compact start) from the columns 0
+// and sNx - 1 of the LDS tile (after the barrier that completes the tile).
+__device__ __forceinline__ void edges_from_tile(const MxDev& a, const float* tile, float* base) {
+    const int sNy = a.sNy;
+    for (int e = threadIdx.x; e < 2 * sNy; e += blockDim.x) {
+        const bool east = e >= sNy;
+        base[sNy * a.G.CP + e] = tile[a.G.lds(east ? e - sNy : e, east ? a.sNx - 1 : 0)];
+    }
 }
 
 // The FP32 coefficients one stencil slot reads: W (aW or pW at columns c4..c4+3), S0 / S1 (aS or pS of rows j and
@@ -214,7 +224,7 @@ __device__ __forceinline__ void mx_stencil(const MxDev& a, const float* __restri
         const float4 o = mask_pad(stencil4<IS_A>(cur, wr, Xm, Xc, Xp, xl, xr), c4, sNx);
         if (valid) {
             st4(out + j * G.CP + c4, o);
-            edge_put(out, G, j, c4, o, sNx, sNy);
+            edge_stash(j, c4, o, sNx, sNy);
         }
         double pv[NR];
         prods(mask_pad(Xc, c4, sNx), o, pv);
@@ -226,6 +236,7 @@ __device__ __forceinline__ void mx_stencil(const MxDev& a, const float* __restri
         cur = nxt;
     }
     __syncthreads();
+    for (int e = tid; e < 2 * sNy; e += blockDim.x) out[sNy * G.CP + e] = g_edge[e];
     if (tid < NR) {
         double sum = g_rows[tid * 128];
         for (int b = 1; b < nblk; ++b) sum = sum + g_rows[tid * 128 + b];
@@ -240,31 +251,32 @@ struct MxRing {
     int k, src, lds;
     bool neg;
 };
-__device__ __forceinline__ MxRing ring_first(const MxDev& a, int t, const int32_t* src_table) {
+// (compact: the compact source index, else the padded one)
+__device__ __forceinline__ MxRing ring_entry(const int4 e, bool compact) {
     MxRing r;
-    r.k = a.ring_off[t] + (int)threadIdx.x;
-    r.src = -1;
-    r.lds = 0;
-    r.neg = false;
-    if (r.k < a.ring_off[t + 1]) {
-        r.src = src_table[r.k];
-        r.lds = a.cring_lds[r.k];
-        r.neg = a.ring_sign[r.k] < 0;
-    }
+    r.src = compact ? e.x : e.y;
+    r.lds = e.z & ((1 << 30) - 1);
+    r.neg = (e.z >> 30) & 1;
+    return r;
+}
+__device__ __forceinline__ MxRing ring_first(const MxDev& a, int t, bool compact) {
+    const int k = a.ring_off[t] + (int)threadIdx.x;
+    MxRing r = ring_entry(k < a.ring_off[t + 1] ? a.cring[k] : make_int4(-1, -1, 0, 0), compact);
+    r.k = k;
     return r;
 }
 // Fills the ring of the LDS tile: value(s) of each entry's source (0 without one), signed.
 template <class V>
-__device__ __forceinline__ void ring_fill(const MxDev& a, int t, const MxRing& r0, const int32_t* src_table,
-                                          float* tile, V&& value) {
+__device__ __forceinline__ void ring_fill(const MxDev& a, int t, const MxRing& r0, bool compact, float* tile,
+                                          V&& value) {
     if (r0.k < a.ring_off[t + 1]) {
         const float v = r0.src < 0 ? 0.0f : value(r0.src);
         tile[r0.lds] = r0.neg ? -v : v;
     }
     for (int k = r0.k + blockDim.x; k < a.ring_off[t + 1]; k += blockDim.x) {
-        const int s = src_table[k];
-        const float v = s < 0 ? 0.0f : value(s);
-        tile[a.cring_lds[k]] = a.ring_sign[k] < 0 ? -v : v;
+        const MxRing r = ring_entry(a.cring[k], compact);
+        const float v = r.src < 0 ? 0.0f : value(r.src);
+        tile[r.lds] = r.neg ? -v : v;
     }
 }
 
@@ -324,7 +336,7 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
     const int64_t cb = cbase(a, m, t), oc = coef_base(a, m, t);
     const float *cW = a.pW32 + oc, *cS = a.pS32 + oc, *cC = a.pC32 + oc;
     const MxCoef pre = coef_load(G, sNx, cW, cS, cC, wave * 64 + lane, inv);
-    const MxRing rr = ring_first(a, t, a.ring_src);
+    const MxRing rr = ring_first(a, t, false);
     float* r32 = R32(a, n - 1) + cb;
     float* d = a.d + cb;
     for (int f = tid; f < G.NS; f += blockDim.x) {
@@ -334,12 +346,12 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
         const float4 v = mask_pad(make_float4((float)(R0.x / sigma), (float)(R0.y / sigma), (float)(R1.x / sigma),
                                               (float)(R1.y / sigma)), c4, sNx);
         st4(r32 + e, v);
-        edge_put(r32, G, j, c4, v, sNx, sNy);
         lds_put(tile, G, j, c4, v, sNx);
         st4(d + e, make_float4(0.f, 0.f, 0.f, 0.f));
     }
-    ring_fill(a, t, rr, a.ring_src, tile, [&](int s) { return (float)(r[s] / sigma); });
+    ring_fill(a, t, rr, false, tile, [&](int s) { return (float)(r[s] / sigma); });
     __syncthreads();
+    edges_from_tile(a, tile, r32);
     double* const dst[1] = {a.peta + (n & 1) * a.B * a.T + m * a.T + t};  // eta(n)
     mx_stencil<false, 1>(
         a, cW, cS, cC, tile, a.z + cb, pre, [](float4 R, float4 zz, double(&pv)[1]) { pv[0] = dot4(zz, R); }, dst);
@@ -432,7 +444,7 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
     const int64_t cm = m * a.cs, cb = cbase(a, m, t), ot = (int64_t)t * G.TS, oc = coef_base(a, m, t);
     const float *cW = a.aW32 + oc, *cS = a.aS32 + oc, *cC = a.aC32 + oc;
     const MxCoef pre = coef_load(G, sNx, cW, cS, cC, wave * 64 + lane, inv);
-    const MxRing rr = ring_first(a, t, a.cring_src);
+    const MxRing rr = ring_first(a, t, true);
     const float* __restrict__ z = a.z + cm;
     const float* __restrict__ so = S32(a, n - 1) + cm;
     float* __restrict__ sn = S32(a, n) + cb;
@@ -460,11 +472,11 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
         }
         st4(sn + e, v);
         const int j = srow(f, inv), c4 = e - j * G.CP;
-        edge_put(sn, G, j, c4, v, sNx, sNy);
         lds_put(tile, G, j, c4, v, sNx);
     }
-    ring_fill(a, t, rr, a.cring_src, tile, [&](int s) { return first ? z[s] : z[s] + beta * so[s]; });
+    ring_fill(a, t, rr, true, tile, [&](int s) { return first ? z[s] : z[s] + beta * so[s]; });
     __syncthreads();
+    edges_from_tile(a, tile, sn);
     double* const dst[1] = {a.psig + (n & 1) * a.B * a.T + m * a.T + t};
     mx_stencil<true, 1>(
         a, cW, cS, cC, tile, a.q + cb, pre, [](float4 X, float4 q, double(&pv)[1]) { pv[0] = dot4(X, q); }, dst);
@@ -481,7 +493,7 @@ __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, doubl
     const int64_t cm = m * a.cs, cb = cbase(a, m, t), ot = (int64_t)t * G.TS, oc = coef_base(a, m, t);
     const float *cW = a.pW32 + oc, *cS = a.pS32 + oc, *cC = a.pC32 + oc;
     const MxCoef pre = coef_load(G, sNx, cW, cS, cC, wave * 64 + lane, inv);
-    const MxRing rr = ring_first(a, t, a.cring_src);
+    const MxRing rr = ring_first(a, t, true);
     const float* __restrict__ qv = a.q + cm;
     const float* __restrict__ rin = R32(a, n - 1) + cm;
     float* __restrict__ rout = R32(a, n) + cb;
@@ -496,11 +508,11 @@ __device__ __forceinline__ void mx_IC(const MxDev& a, int m, int t, int n, doubl
         rn.w = R.w - alpha * Q.w;
         st4(rout + e, rn);
         const int j = srow(f, inv), c4 = e - j * G.CP;
-        edge_put(rout, G, j, c4, rn, sNx, sNy);
         lds_put(tile, G, j, c4, rn, sNx);
     }
-    ring_fill(a, t, rr, a.cring_src, tile, [&](int s) { return rin[s] - alpha * qv[s]; });
+    ring_fill(a, t, rr, true, tile, [&](int s) { return rin[s] - alpha * qv[s]; });
     __syncthreads();
+    edges_from_tile(a, tile, rout);
     const int BT = a.B * a.T;
     double* const dst[2] = {a.perr + (n & 1) * BT + m * a.T + t, a.peta + ((n + 1) & 1) * BT + m * a.T + t};
     mx_stencil<false, 2>(
@@ -904,8 +916,7 @@ extern "C" int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops
     a.ring_dst = P->cg_ring_dst;
     a.ring_src = P->cg_ring_src;
     a.ring_sign = P->cg_ring_sign;
-    a.cring_lds = P->cg_cring_lds;
-    a.cring_src = P->cg_cring_src;
+    a.cring = P->cg_cring;
     a.cs = (int64_t)a.T * a.G.TS;
     a.r64 = (double*)(w + L.r64);
     a.r0 = (float*)(w + L.r0);
