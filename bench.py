@@ -20,7 +20,7 @@ point (N2 = 105,300 per member): per member-iteration the vectors r, s, x read a
 per operator slot and iteration in which any member of the slot iterates, the six operator and preconditioner arrays
 (6 x s_c); per FP64 residual replacement of the mixed solve 64 B (the FP64 true residual and solution update); s_v = s_c = 4 for
 the mixed inner solve, 8 for the FP64 solve. `implementation_bytes` counts what this build's kernels are laid out to
-stream instead (mixed: 10 FP32 vector streams per inner iteration, 68 B per point per replacement, 48 B per point
+stream instead (mixed: 9.5 FP32 vector streams per inner iteration, 68 B per point per replacement, 48 B per point
 once; FP64: 11 FP64 streams per iteration, 48 B per point once; coefficients served from cache not counted).
 DST3 (`dst3`): per (member, tile, level) the padded 3-D planes the step reads (tracer, u, v, hFacW, hFacS,
 recip_hFacC, maskW, maskS; with the R flux also w) and the tendency it writes, 8 B each, plus the 2-D metrics
@@ -362,7 +362,7 @@ def solve_bytes(precision, d, op_index):
         impl = float(iters.sum()) * N2 * 11 * 8 + B * N2 * 48
     else:
         alg += float(outer.sum()) * N2 * 64
-        impl = float(iters.sum()) * N2 * 10 * 4 + float(outer.sum()) * N2 * 68 + B * N2 * 48
+        impl = float(iters.sum()) * N2 * 9.5 * 4 + float(outer.sum()) * N2 * 68 + B * N2 * 48
     return alg, impl, slot_iters
 
 

@@ -64,7 +64,7 @@ constexpr int kGroup = MJ_GROUP;
 
 // Per-block LDS: row sums of two reductions per group member (slot r + 2k), values broadcast from thread 0.
 static __shared__ double g_rows[2 * kGroup * 128];
-static __shared__ double g_bcast[4];
+static __shared__ double g_bcast[6];
 static __shared__ int g_ibcast;
 
 // Store one row's per-point values of reduction r: tree order -> wave_tree into the row sums; fortran order -> the

@@ -51,11 +51,15 @@ namespace mj {
 struct MxRec {
     int mode, n, iters, outer;  // state, inner iteration, inner iterations done, residuals placed
     int n0, fresh;              // n of the last replacement; 1 until the IB step after a replacement has run
+    int def, pad_;              // 1: the update alpha_{n-2} s_{n-2} is deferred to B(n) (alpha_{n-2} in ap)
     double sigma;               // units of r32: the norm of the first true residual
     double elast;               // err of the last replacement in units of sigma^2
+    double ap;                  // the deferred alpha
 };
 enum { MX_RST = 0, MX_IB = 1, MX_IC = 2, MX_TRUE = 3 };
-enum { AX_FINISH = 1, AX_RESTART, AX_UPD, AX_IB, AX_IBF, AX_IC, AX_TRUE };  // AX_IBF: B(n) with no update pending
+// AX_IBF: B(n) with no update pending (after a replacement); AX_IBD: B(n) deferring alpha_{n-1} s_{n-1}; AX_IB:
+// B(n) applying the deferred and the pending update
+enum { AX_FINISH = 1, AX_RESTART, AX_UPD, AX_IB, AX_IBF, AX_IBD, AX_IC, AX_TRUE };
 
 struct MxDev {
     int B, T, sNx, sNy, OL, nx, ny, plane;
@@ -306,7 +310,7 @@ __device__ __forceinline__ void mx_I1(const MxDev& a, int m, int t) {
     })
     store_tile_sums<MJ_SUM_TREE>(a.perr64, a.psum, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
     if (t == 0 && tid == 0) {
-        a.rec[m] = MxRec{MX_RST, 0, 0, 0, 0, 0, 0.0, 0.0};
+        a.rec[m] = MxRec{MX_RST, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0};
         a.mdone[m] = -1;
     }
 }
@@ -357,9 +361,10 @@ __device__ __forceinline__ void mx_restart(const MxDev& a, int m, int t, double 
         a, cW, cS, cC, tile, a.z + cb, pre, [](float4 R, float4 zz, double(&pv)[1]) { pv[0] = dot4(zz, R); }, dst);
 }
 
-// Solution update of a finished inner solve at B(n): the pending d += alpha_{n-1} s_{n-1} (n > 1), then
-// x = x + sigma * d on the interior (FP64).
-__device__ __forceinline__ void mx_update(const MxDev& a, int m, int t, int n, double sigma, double alpha64) {
+// Solution update at a replacement due in B(n): the deferred d += alpha_{n-2} s_{n-2} (def; s_{n-2} is still in
+// s_n's buffer) and the pending d += alpha_{n-1} s_{n-1} (n > 1), then x = x + sigma * d on the interior (FP64).
+__device__ __forceinline__ void mx_update(const MxDev& a, int m, int t, int n, double sigma, double alpha64, bool def,
+                                          double ap64) {
     const int tid = threadIdx.x;
     const int sNx = a.sNx, nx = a.nx, OL = a.OL, tbase = t * a.plane;
     const MxGeo G = a.G;
@@ -368,10 +373,18 @@ __device__ __forceinline__ void mx_update(const MxDev& a, int m, int t, int n, d
     const int64_t cb = cbase(a, m, t);
     const float* d = a.d + cb;
     const float* so = S32(a, n - 1) + cb;
-    const float alpha = (float)alpha64;
+    const float* so2 = S32(a, n) + cb;
+    const float alpha = (float)alpha64, ap = (float)ap64;
     for (int f = tid; f < G.NS; f += blockDim.x) {
         const int e = 4 * f, j = srow(f, inv), c4 = e - j * G.CP;
         float4 D = ld4(d + e);
+        if (def) {
+            const float4 S2 = ld4(so2 + e);
+            D.x = D.x + ap * S2.x;
+            D.y = D.y + ap * S2.y;
+            D.z = D.z + ap * S2.z;
+            D.w = D.w + ap * S2.w;
+        }
         if (n > 1) {
             const float4 S = ld4(so + e);
             D.x = D.x + alpha * S.x;
@@ -433,9 +446,13 @@ __device__ __forceinline__ void mx_true(const MxDev& a, int m, int t) {
     store_tile_sums<MJ_SUM_TREE>(a.perr64, nullptr, m * a.T + t, lane, wave, nwaves, nullptr, sNx, sNy);
 }
 
-// Inner phase B(n): d += alpha_{n-1} s_{n-1} (pend) and s_n = z + beta s_{n-1} (s_1 = z) on the interior (compact,
-// flat), s_n on the ring (LDS), q = A32 s_n, sigma_n partial (s_n, q).
-__device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64, double alpha64, bool pend) {
+// Inner phase B(n): the solution updates, every other iteration two at once (d is read and written half as
+// often; the same sums in the same order): with both (both, after a deferring B(n-1)) d += alpha_{n-2} s_{n-2}
+// (s_{n-2} read from s_n's buffer before s_n overwrites it), then d += alpha_{n-1} s_{n-1}; otherwise the update of
+// iteration n-1 is deferred to B(n+1), or there is none (after a replacement). Then s_n = z + beta s_{n-1} (s_1 = z)
+// on the interior (compact, flat), s_n on the ring (LDS), q = A32 s_n, sigma_n partial (s_n, q).
+__device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, double beta64, double alpha64, bool both,
+                                      double ap64) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sNx = a.sNx, sNy = a.sNy;
     float* tile = reinterpret_cast<float*>(mj_lds);
@@ -449,20 +466,24 @@ __device__ __forceinline__ void mx_IB(const MxDev& a, int m, int t, int n, doubl
     const float* __restrict__ so = S32(a, n - 1) + cm;
     float* __restrict__ sn = S32(a, n) + cb;
     float* __restrict__ d = a.d + cb;
-    const float beta = (float)beta64, alpha = (float)alpha64;
+    const float beta = (float)beta64, alpha = (float)alpha64, ap = (float)ap64;
     const bool first = n == 1;
     for (int f = tid; f < G.NS; f += blockDim.x) {
         const int e = 4 * f;
         float4 v = ld4(z + ot + e);
         if (!first) {
             const float4 O = ld4(so + ot + e);
-            if (pend) {
-                const float4 D = ld4(d + e);
+            if (both) {
+                const float4 D = ld4(d + e), S2 = ld4(sn + e);
                 float4 dn;
-                dn.x = D.x + alpha * O.x;
-                dn.y = D.y + alpha * O.y;
-                dn.z = D.z + alpha * O.z;
-                dn.w = D.w + alpha * O.w;
+                dn.x = D.x + ap * S2.x;
+                dn.y = D.y + ap * S2.y;
+                dn.z = D.z + ap * S2.z;
+                dn.w = D.w + ap * S2.w;
+                dn.x = dn.x + alpha * O.x;
+                dn.y = dn.y + alpha * O.y;
+                dn.z = dn.z + alpha * O.z;
+                dn.w = dn.w + alpha * O.w;
                 st4(d + e, dn);
             }
             v.x = v.x + beta * O.x;
@@ -542,6 +563,18 @@ __device__ __forceinline__ MxRec mx_next_ib(const MxRec& r) {
     nr.fresh = 0;
     return nr;
 }
+// The record of C(n) after B(n) (stop: a replacement instead): a pending update (pend) alternately deferred
+// (alpha_{n-1} kept in the record) and applied with the deferred one.
+__device__ __forceinline__ MxRec mx_after_ib(const MxRec& r, bool stop, bool pend, double alpha_prev) {
+    MxRec nr = r;
+    nr.mode = stop ? MX_TRUE : MX_IC;
+    nr.fresh = 0;
+    if (!stop && pend) {
+        nr.def = r.def ? 0 : 1;
+        nr.ap = r.def ? 0.0 : alpha_prev;
+    }
+    return nr;
+}
 // Whether B(n) replaces the recursive residual instead (err(n-1) the recursive err in units of sigma^2): a cap
 // reached, the recursive residual fallen by inner_tolerance since the last replacement, or below the target.
 __device__ __forceinline__ bool mx_replace_due(const MxDev& a, const MxRec& r, double errp) {
@@ -598,16 +631,11 @@ mx_step_kernel(MxDev a, int k) {
                 alpha_prev = etap / sigp;
             }
         }
-        if (t == 0 && threadIdx.x == 0) {
-            MxRec nr = r0;
-            nr.mode = stop ? MX_TRUE : MX_IC;
-            nr.fresh = 0;
-            a.rec[(k & 1) * a.B + m] = nr;
-        }
+        if (t == 0 && threadIdx.x == 0) a.rec[(k & 1) * a.B + m] = mx_after_ib(r0, stop, pend, alpha_prev);
         if (stop)
-            mx_update(a, m, t, n, r0.sigma, alpha_prev);
+            mx_update(a, m, t, n, r0.sigma, alpha_prev, r0.def, r0.ap);
         else
-            mx_IB(a, m, t, n, beta, alpha_prev, pend);
+            mx_IB(a, m, t, n, beta, alpha_prev, pend && r0.def, r0.ap);
         return;
     }
     // wave 0 derives the member's action (every lane alike), lane 0 publishes it. The member's record and every
@@ -662,7 +690,7 @@ mx_step_kernel(MxDev a, int k) {
                     act = AX_RESTART;
                     const double sg = r.outer == 0 ? sqrt(e) : r.sigma;
                     const int n1 = r.outer == 0 ? 1 : r.n;
-                    nr = MxRec{MX_IB, n1, r.iters, r.outer + 1, n1, 1, sg, e / (sg * sg)};
+                    nr = MxRec{MX_IB, n1, r.iters, r.outer + 1, n1, 1, 0, 0, sg, e / (sg * sg), 0.0};
                 }
                 break;
             }
@@ -682,14 +710,8 @@ mx_step_kernel(MxDev a, int k) {
                         alpha_prev = etap / sigp;
                     }
                 }
-                if (stop) {
-                    act = AX_UPD;
-                    nr.mode = MX_TRUE;
-                } else {
-                    act = r.fresh ? AX_IBF : AX_IB;
-                    nr.mode = MX_IC;
-                }
-                nr.fresh = 0;
+                act = stop ? AX_UPD : r.fresh ? AX_IBF : r.def ? AX_IB : AX_IBD;
+                nr = mx_after_ib(r, stop, !r.fresh, alpha_prev);
                 break;
             }
             case MX_IC: {
@@ -712,6 +734,8 @@ mx_step_kernel(MxDev a, int k) {
             g_bcast[0] = c;
             g_bcast[1] = nr.sigma;
             g_bcast[2] = (double)(act == AX_RESTART ? nr.n : r.n);  // the step's n
+            g_bcast[4] = r.ap;                                      // a deferred alpha (AX_IB, AX_UPD)
+            g_bcast[5] = (double)r.def;
             if (t == 0) {
                 a.rec[(k & 1) * a.B + m] = nr;
                 if (act == AX_FINISH) {
@@ -727,14 +751,16 @@ mx_step_kernel(MxDev a, int k) {
     }
     __syncthreads();
     const int act = g_ibcast;
-    const double c = g_bcast[0], sigma = g_bcast[1], alpha_prev = g_bcast[3];
+    const double c = g_bcast[0], sigma = g_bcast[1], alpha_prev = g_bcast[3], ap = g_bcast[4];
     const int n = (int)g_bcast[2];
+    const bool def = g_bcast[5] != 0.0;
     switch (act) {
         case AX_FINISH: mx_finish(a, m, t); break;
         case AX_RESTART: mx_restart(a, m, t, sigma, n); break;
-        case AX_UPD: mx_update(a, m, t, n, sigma, alpha_prev); break;
-        case AX_IB: mx_IB(a, m, t, n, c, alpha_prev, true); break;
-        case AX_IBF: mx_IB(a, m, t, n, c, alpha_prev, false); break;
+        case AX_UPD: mx_update(a, m, t, n, sigma, alpha_prev, def, ap); break;
+        case AX_IB: mx_IB(a, m, t, n, c, alpha_prev, true, ap); break;
+        case AX_IBF:
+        case AX_IBD: mx_IB(a, m, t, n, c, alpha_prev, false, 0.0); break;
         case AX_IC: mx_IC(a, m, t, n, c); break;
         default: mx_true(a, m, t); break;
     }
