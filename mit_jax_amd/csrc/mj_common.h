@@ -37,15 +37,20 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // ---------------------------------------------------------------------------------------------------- DST3
 // DST3 interface value (gad_dst3_adv_x.F:81-117 via gad.py:420-428): t_m2, t_m1 upstream side for positive
 // transport, t_0, t_p1 downstream; m_p1, m_0, m_m1 the face masks of Rjp, Rj, Rjm.
+// The same from the masked differences Rjm, Rj, Rjp (the two faces of a point share two of them).
+__device__ __forceinline__ double dst3_flux_r(double tr_, double vel_cfl_abs, double oneSixth, double t_m1, double t_0,
+                                              double Rjm, double Rj, double Rjp) {
+    const double cfl = vel_cfl_abs;
+    const double d0 = (2. - cfl) * (1. - cfl) * oneSixth;
+    const double d1 = (1. - cfl * cfl) * oneSixth;
+    return 0.5 * (tr_ + fabs(tr_)) * (t_m1 + (d0 * Rj + d1 * Rjm)) + 0.5 * (tr_ - fabs(tr_)) * (t_0 - (d0 * Rj + d1 * Rjp));
+}
 __device__ __forceinline__ double dst3_flux(double tr_, double vel_cfl_abs, double oneSixth, double t_m2, double t_m1,
                                             double t_0, double t_p1, double m_m1, double m_0, double m_p1) {
     const double Rjp = (t_p1 - t_0) * m_p1;
     const double Rj = (t_0 - t_m1) * m_0;
     const double Rjm = (t_m1 - t_m2) * m_m1;
-    const double cfl = vel_cfl_abs;
-    const double d0 = (2. - cfl) * (1. - cfl) * oneSixth;
-    const double d1 = (1. - cfl * cfl) * oneSixth;
-    return 0.5 * (tr_ + fabs(tr_)) * (t_m1 + (d0 * Rj + d1 * Rjm)) + 0.5 * (tr_ - fabs(tr_)) * (t_0 - (d0 * Rj + d1 * Rjp));
+    return dst3_flux_r(tr_, vel_cfl_abs, oneSixth, t_m1, t_0, Rjm, Rj, Rjp);
 }
 
 // ---------------------------------------------------------------------------------------------------- reductions

@@ -323,12 +323,28 @@ __device__ __forceinline__ void hz_plane(const GadDev& a, int t, int m, int k, d
             auto trans = [&](double vv, double g, double h) { return vv * (g * deepFacC * drF * h) * rhoFacC; };
             // interface value of face q (gad.py:482-485 transport; 0 outside the Fortran flux loops
             // i = 1-OLx+2 .. sNx+OLx-1, gad_dst3_adv_x.F:80, and j = 1-OLy+2 .. sNy+OLy-1, gad_dst3_adv_y.F:78)
-            auto face = [&](int q, double V, double Tq, double R) {
+            auto in_range = [&](int q) {
                 const int j = q / GNX, c = dir == 0 ? q - j * GNX : j;
-                if (c < 2 || c > cmax) return 0.0;
-                const double cfl = fabs(V * dt * R * rdeepC);
-                return dst3_flux(Tq, cfl, oneSixth, loc[q - 2 * d], loc[q - d], loc[q], loc[q + d], mbit(mb, q - d),
-                                 mbit(mb, q), mbit(mb, q + d));
+                return c >= 2 && c <= cmax;
+            };
+            // the interface values of faces q and q + d of a point: the masked differences Rj, Rjp of face q are
+            // Rjm, Rj of face q + d, so they are formed once (the same operations as one face at a time)
+            auto faces = [&](int q, double V0, double T0, double R0, double V1, double T1, double R1, double& f0,
+                             double& f1) {
+                const bool in0 = in_range(q), in1 = in_range(q + d);
+                f0 = 0.0;
+                f1 = 0.0;
+                if (!in0 && !in1) return;
+                const double t_m1 = loc[q - d], t_0 = loc[q], t_p1 = loc[q + d];
+                const double D0 = (t_0 - t_m1) * mbit(mb, q), Dp1 = (t_p1 - t_0) * mbit(mb, q + d);
+                if (in0) {
+                    const double Dm1 = (t_m1 - loc[q - 2 * d]) * mbit(mb, q - d);
+                    f0 = dst3_flux_r(T0, fabs(V0 * dt * R0 * rdeepC), oneSixth, t_m1, t_0, Dm1, D0, Dp1);
+                }
+                if (in1) {
+                    const double Dp2 = (loc[q + 2 * d] - t_p1) * mbit(mb, q + 2 * d);
+                    f1 = dst3_flux_r(T1, fabs(V1 * dt * R1 * rdeepC), oneSixth, t_0, t_p1, D0, Dp1, Dp2);
+                }
             };
             if (S.fill_before) corner_fill(loc, a, tp, S.fill_before, tid);
             // the boxes' points as one index range (the boxes are disjoint, so at most GPLANE points)
@@ -351,7 +367,8 @@ __device__ __forceinline__ void hz_plane(const GadDev& a, int t, int m, int k, d
                     const double V1 = vel[qd], G1 = GR1.x, H1 = hf[qd], R1 = GR1.y;
                     const double TR = trc[qq], RH = rhC[qq], RA = AM.x, MI = AM.y;
                     const double T0 = trans(V0, G0, H0), T1 = trans(V1, G1, H1);
-                    const double f0 = face(qq, V0, T0, R0), f1 = face(qd, V1, T1, R1);
+                    double f0, f1;
+                    faces(qq, V0, T0, R0, V1, T1, R1, f0, f1);
                     // local - coef*(d(af) - tracer*d(trans))*maskInC (gad.py:511-514), coefficient gad.py:494
                     const double daf = f1 - f0, dtr = T1 - T0;
                     const double coef = dt * rrhoFacC * RH * rdrF * RA * rdeep2C;
