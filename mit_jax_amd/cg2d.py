@@ -7,9 +7,9 @@ rule `err_sq < tolerance**2`; x leaves with the Fortran halos (exchanged normali
 float64 [T, ny, nx], or [B, T, ny, nx] for a batch of members solved in the same launch (per-member convergence;
 ops shared [T, ny, nx] or per member [B, T, ny, nx]; cg2dNorm a scalar or B values).
 
-sum_order "fortran" (the reference default) sums each tile sequentially in j/i order, bitwise the reference's
-order; "tile" uses the library's fixed wavefront tree (the reference's "tile" order is XLA's own tree, so the two
-agree to rounding, not bitwise). sum_unroll only changes the reference's XLA scheduling and is accepted and
+sum_order is required, as in the reference (Cg2dParams defaults it to "fortran"): "fortran" sums each tile
+sequentially in j/i order, bitwise the reference's order; "tile" uses the library's fixed wavefront tree (the
+reference's "tile" order is XLA's own tree, so the two agree to rounding, not bitwise). sum_unroll only changes the reference's XLA scheduling and is accepted and
 ignored.
 
 `cg2d_solve(p, ex, ops, cg2d_b, cg2d_x) -> (x, diag)` (core/cg2d.py:351-375) returns x with 0 halos and is
@@ -134,7 +134,7 @@ def config(L, B, tolerance, max_iters, sum_order, block_threads=0, grid_blocks=0
                            int(poll_iters))
 
 
-def cg2d_fortran(ex, ops, cg2d_b, cg2d_x, *, cg2dNorm, tolerance, max_iters, sum_order="fortran", nIterMin=-1,
+def cg2d_fortran(ex, ops, cg2d_b, cg2d_x, *, cg2dNorm, tolerance, max_iters, sum_order, nIterMin=-1,
                  sum_unroll=1, block_threads=0, grid_blocks=0, out=None, op_index=None, launch="auto", poll_iters=0):
     """CG2D (reference core/cg2d.py:205-269). ops = (aW2d, aS2d, aC2d, pW, pS, pC). Returns (x, diag).
     Batched: cg2d_b, cg2d_x [B, T, ny, nx]; ops shared, per member, or per slot with op_index. launch picks the

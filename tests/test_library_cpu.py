@@ -178,7 +178,8 @@ def test_unported_branches_raise():
     with pytest.raises(NotImplementedError):
         gad.GADParams(topo, useCubedSphereExchange=False)
     with pytest.raises(NotImplementedError):
-        cg2d.cg2d_fortran(None, None, None, None, cg2dNorm=1.0, tolerance=1e-7, max_iters=10, nIterMin=0)
+        cg2d.cg2d_fortran(None, None, None, None, cg2dNorm=1.0, tolerance=1e-7, max_iters=10, sum_order="fortran",
+                          nIterMin=0)
 
 
 def test_ops_refuse_cpu_tensors():
@@ -191,7 +192,7 @@ def test_ops_refuse_cpu_tensors():
     with pytest.raises(TypeError):
         ex.exch_xy(a)
     with pytest.raises(TypeError):
-        cg2d.cg2d_fortran(ex, [a] * 6, a, a, cg2dNorm=1.0, tolerance=1e-7, max_iters=10)
+        cg2d.cg2d_fortran(ex, [a] * 6, a, a, cg2dNorm=1.0, tolerance=1e-7, max_iters=10, sum_order="fortran")
     with pytest.raises(TypeError):
         gad.advect_tiles(gad.GADParams.llc90(), None, {}, {}, a, a, a, a, a, a)
 
