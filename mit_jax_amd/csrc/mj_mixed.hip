@@ -594,7 +594,7 @@ __device__ __forceinline__ double lane_tile_sum(double v, int T) {
 // One step of every member's state machine.
 #ifndef MJ_PH_WAVES_PER_EU
 #define MJ_PH_WAVES_PER_EU 7
-#endif  // 72 VGPRs, no spills; 4 resident blocks of up to 448 threads per CU (the LDS tile allows 4)
+#endif  // 72 VGPRs (one 8-byte spill outside the inner phases); 4 resident blocks of up to 448 threads per CU
 __global__ void __launch_bounds__(MJ_MX_THREADS) __attribute__((amdgpu_waves_per_eu(MJ_PH_WAVES_PER_EU)))
 mx_step_kernel(MxDev a, int k) {
     int m, t;
