@@ -193,7 +193,8 @@ size_t mj_cg2d_mixed_workspace_bytes(const mj_cg2d_config* cfg, const mj_cg2d_mi
                                      int64_t ops_member_stride);
 
 /* Arguments as mj_cg2d_solve. cfg: sum_order must be MJ_SUM_TREE; launch_mode AUTO (= PHASES), PHASES or
- * PHASES_FIXED; max_iters caps the total inner iterations. */
+ * PHASES_FIXED; max_iters caps the total inner iterations; block_threads 0 = 1024 while batch * ntiles blocks fit
+ * one per CU, else 512; poll_iters 0 = 64. */
 int mj_cg2d_solve_mixed(const mj_halo_plan* plan_T, const mj_cg2d_ops* ops, const double* b, const double* x0,
                         double* x, const double* cg2dNorm, const mj_cg2d_config* cfg,
                         const mj_cg2d_mixed_config* mcfg, void* workspace, size_t workspace_bytes,
