@@ -128,10 +128,12 @@ typedef struct {
     double tolerance;    /* cg2dTolerance (normalised target) */
     int max_iters;       /* cg2dMaxIters */
     int sum_order;       /* MJ_SUM_FORTRAN or MJ_SUM_TREE */
-    int block_threads;   /* 0 = launch-mode default (persistent 1024, phases 256); else a multiple of 64 up to it */
+    int block_threads;   /* 0 = launch-mode default (1024; the mixed solve: see mj_cg2d_solve_mixed); else a
+                            multiple of 64 up to it */
     int grid_blocks;     /* persistent launch: 0 = as many co-resident blocks as useful */
     int launch_mode;     /* MJ_CG_LAUNCH_*; results are bitwise the same in every mode */
-    int poll_iters;      /* MJ_CG_LAUNCH_PHASES: iterations enqueued between reads of the active count (0 = 32) */
+    int poll_iters;      /* MJ_CG_LAUNCH_PHASES: iterations enqueued between reads of the active count (0 = 32;
+                            the mixed solve: 64) */
 } mj_cg2d_config;
 
 /* Device workspace the solve needs (bytes), for the given configuration. */

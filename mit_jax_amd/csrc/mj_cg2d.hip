@@ -71,8 +71,8 @@ struct CgDev {
 #define MJ_PH_THREADS 1024
 #endif  // largest block of the phase launches (the group phases: MJ_PH_THREADS * kGroup)
 #ifndef MJ_PH_DEFAULT_THREADS
-#define MJ_PH_DEFAULT_THREADS 256
-#endif  // block of the phase launches when the config leaves it open
+#define MJ_PH_DEFAULT_THREADS 1024
+#endif  // block of the phase launches when the config leaves it open (the fastest of 128-1024 for 128 members)
 #ifndef MJ_PH_WAVES_PER_EU
 #define MJ_PH_WAVES_PER_EU 8
 #endif  // occupancy target of the phase launches (waves per SIMD): bounds their register use (a group's rows)
