@@ -90,11 +90,21 @@ __device__ __forceinline__ double tile_of(int r, int lane, const double* stage, 
         const bool h = lane + 64 < sNy;
         return wave_tree(rb[lane], h ? rb[lane + 64] : 0.0, h);
     }
+    // fortran order: the strict sequence s = ((0 + v[0]) + v[1]) + ... over the staged values in j/i order. The
+    // wave loads them 64 at a time (coalesced, the next 64 in flight while the current ones are added) and every
+    // lane carries the same running sum, taking the values lane by lane: only the chain of additions is serial.
+    const double* pr = stage + (size_t)r * sNy * sNx;
+    const int n = sNy * sNx;
     double s = 0.0;
-    if (lane == 0) {
-        const double* pr = stage + (size_t)r * sNy * sNx;
-        for (int q = 0; q < sNy * sNx; ++q) s = s + pr[q];
+    double cur = lane < n ? pr[lane] : 0.0;
+    int c = 0;
+    for (; c + 64 <= n; c += 64) {
+        const double nxt = c + 64 + lane < n ? pr[c + 64 + lane] : 0.0;
+#pragma unroll
+        for (int i = 0; i < 64; ++i) s = s + read_lane(cur, i);
+        cur = nxt;
     }
+    for (int i = 0; i < n - c; ++i) s = s + read_lane(cur, i);
     return s;
 }
 
