@@ -7,6 +7,7 @@ level counts). Inputs are seeded synthetic fields resident in HBM before timing 
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--members B] [--precision mixed|fp64]
     python bench.py --kernel-profile --block-threads N   (the per-kernel timings alone, the workload for rocprofv3)
+    python bench.py --configs   (configs 0-2 one by one, each beside the FP64 C oracle on the host cores)
 
 The headline (default) is the mixed-precision solve (FP32 PCG + FP64 residual replacement, mj_cg2d_solve_mixed); the
 FP64 solve (mj_cg2d_solve) is timed on the same step for `mixed_over_fp64`, and the two solutions are compared
@@ -80,6 +81,10 @@ def parse():
                     help="only the per-kernel timings (solve alone, DST3 alone, DST3 with R) of the launch "
                          "configuration given by --block-threads / --inner-tolerance, three times each (no autotune): "
                          "the workload of the committed kernel-trace profiles")
+    ap.add_argument("--configs", action="store_true",
+                    help="only the per-configuration table: config 0 (one FP64 solve, diagonal preconditioner), config "
+                         "1 (one and 24 DST3 X/Y/R steps), config 2 (64 rhs, batched FP64 and mixed), each beside the "
+                         "FP64 C oracle on the host cores; prints one JSON line")
     ap.add_argument("--profile-solves", action="store_true",
                     help="only run one FP64 and one mixed solve of --members right-hand sides on the shared 5-point "
                          "operator (the workload of the hardware-counter profiles) and exit")
@@ -495,6 +500,132 @@ def kernel_profile(args):
     mit_jax_amd.release()
 
 
+def _median_ms(fn, reps):
+    """Median wall time (ms) of `reps` calls of fn after one warm-up call, each followed by a device sync."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return float(np.median(ts))
+
+
+def _cpu_s(fn, reps):
+    """Median wall time (s) of `reps` calls of an oracle function on the host cores."""
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def configs_table():
+    """Configs 0-2 of the workload set on the GPU, each beside the FP64 C oracle (OpenMP, all host cores; oracle
+    timings are medians of a few calls, config 2's from 2 of its 64 solves x 32). Config 0: one solve of the 50-level
+    operator with the diagonal preconditioner to 1e-12 (FP64, reference sum order, launch picked for one member; and
+    the mixed solve). Config 1: one DST3 step with the X/Y/R fluxes (one member, 50 levels) and 24 steps, the tracer
+    updated and re-exchanged after each. Config 2: 64 rhs on the config-0 operator in one launch, FP64 (tile order,
+    bitwise the oracle's) and mixed; the mixed solutions' error against the FP64 ones, per rhs (max and mean), with
+    the FP64 solutions of the first 2 rhs checked bitwise against the oracle's. GB/s are algorithmic bytes (the
+    headline's counts) over the wall time."""
+    import torch
+    import oracle
+    from oracle import tables as OT
+    from mit_jax_amd import cg2d, gad, mixed, synthetic as S
+    from mit_jax_amd.exchange import default_exchanger
+    dev = torch.device("cuda:0")
+    ex = default_exchanger()
+    Tm = ex.maps.full("T")
+    Td = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
+    out = {"cores": oracle.threads()}
+
+    print("bench: config 0", file=sys.stderr, flush=True)
+    g, ops, norm = S.cg2d_case(0, "diagonal")
+    b = S.rhs(g, 1)
+    opsd = [Td(o) for o in ops]
+    bd, x0 = Td(b), torch.zeros((13, 98, 98), dtype=torch.float64, device=dev)
+    kw = dict(cg2dNorm=norm, tolerance=TOL, max_iters=4000)
+    res = {}
+    ms = _median_ms(lambda: res.update(r=cg2d.cg2d_fortran(ex, opsd, bd, x0, sum_order="fortran", **kw)), 5)
+    x64, d64 = res["r"]
+    it = int(d64["numIters"])
+    msm = _median_ms(lambda: res.update(m=mixed.cg2d_mixed(ex, opsd, bd, x0, **kw)), 5)
+    xm, dm = res["m"]
+    xr, dr = oracle.cg2d(Tm, ops, b, np.zeros_like(b), sum_order="fortran", **kw)
+    cpu = _cpu_s(lambda: oracle.cg2d(Tm, ops, b, np.zeros_like(b), sum_order="fortran", **kw), 3)
+    inner = (slice(None), slice(4, 94), slice(4, 94))
+    err = float(np.abs(xm.cpu().numpy() - xr)[inner].max() / np.abs(xr[inner]).max())
+    out["config0"] = {"iterations": it, "gpu_fp64_ms": ms, "gpu_fp64_us_per_iteration": ms * 1e3 / it,
+                      "gpu_fp64_GBs": it * N2 * 96 / (ms * 1e-3) / 1e9,
+                      "gpu_fp64_equals_oracle": bool(np.array_equal(x64.cpu().numpy(), xr)),
+                      "gpu_mixed_ms": msm, "mixed_inner_iterations": int(dm["numIters"]),
+                      "mixed_replacements": int(dm["outerIters"]), "mixed_error_vs_oracle": err,
+                      "cpu_fp64_s": cpu, "oracle_iterations": int(dr["numIters"])}
+
+    print("bench: config 1", file=sys.stderr, flush=True)
+    g1 = S.make_grid(0)
+    u, v, w = S.flow(g1, 2)
+    tr = S.tracer(g1, 3)
+    params = gad.GADParams.llc90(implicitAdvection=False)
+    g2, vg = S.gad_inputs(g1)
+    g2d = {k: Td(a) for k, a in g2.items()}
+    g2d["rA"] = Td(g1.rA)
+    vgd = {k: Td(np.asarray(a)) for k, a in vg.items()}
+    dtd = Td(np.asarray(vg["dTlev"])[None, :, None, None])
+    ud, vd, wd, hWd, hSd, rhCd, trd = (Td(a) for a in (u, v, w, g1.hFacW, g1.hFacS, g1.recip_hFacC, tr))
+    plan = gad.gad_tables(params)
+    step = lambda t: gad.advect_tiles(params, plan, g2d, vgd, ud, vd, t, hWd, hSd, rhCd, wFld=wd)  # noqa: E731
+    ms1 = _median_ms(lambda: step(trd), 5)
+
+    def chain():
+        t = trd.clone()
+        for _ in range(24):
+            t = t + step(t) * dtd
+            ex.fill_(t, "T")
+        return t
+
+    ms24 = _median_ms(chain, 3)
+    tab = OT.tables(OT.Topology.from_facets(OT.LLC90_FACETS))
+    g2o = dict(g2, rA=g1.rA)
+    gr = oracle.gad_dst3(tab, g2o, vg, u, v, w, tr, g1.hFacW, g1.hFacS, g1.recip_hFacC)
+    cpu1 = _cpu_s(lambda: oracle.gad_dst3(tab, g2o, vg, u, v, w, tr, g1.hFacW, g1.hFacS, g1.recip_hFacC), 3)
+    gb = dst3_bytes([50], True)
+    out["config1"] = {"gpu_step_ms": ms1, "gpu_step_GBs": gb / (ms1 * 1e-3) / 1e9,
+                      "gpu_step_equals_oracle": bool(np.array_equal(step(trd).cpu().numpy(), gr)),
+                      "gpu_24_steps_ms": ms24, "cpu_step_s": cpu1}
+
+    print("bench: config 2", file=sys.stderr, flush=True)
+    bs = np.stack([S.rhs(g, s) for s in range(100, 164)])
+    bsd, x0s = Td(bs), torch.zeros(bs.shape, dtype=torch.float64, device=dev)
+    ms64 = _median_ms(lambda: res.update(b64=cg2d.cg2d_fortran(ex, opsd, bsd, x0s, sum_order="tile", **kw)), 3)
+    msmx = _median_ms(lambda: res.update(bmx=mixed.cg2d_mixed(ex, opsd, bsd, x0s, **kw)), 3)
+    (xb, db), (xmx, dmx) = res["b64"], res["bmx"]
+    one = torch.zeros(64, dtype=torch.int32, device=dev)
+    alg64 = solve_bytes("fp64", db, one)[0]
+    algmx = solve_bytes("mixed", dmx, one)[0]
+    xbn, xmn = xb.cpu().numpy(), xmx.cpu().numpy()
+    errs = [float(np.abs(xmn[m] - xbn[m])[inner].max() / np.abs(xbn[m][inner]).max()) for m in range(64)]
+    same = all(np.array_equal(xbn[m], oracle.cg2d(Tm, ops, bs[m], np.zeros_like(bs[m]), sum_order="tile", **kw)[0])
+               for m in range(2))
+    cpu2 = _cpu_s(lambda: [oracle.cg2d(Tm, ops, bs[m], np.zeros_like(bs[m]), sum_order="tile", **kw)
+                           for m in range(2)], 1) * 32
+    out["config2"] = {"gpu_fp64_ms": ms64, "gpu_fp64_GBs": alg64 / (ms64 * 1e-3) / 1e9,
+                      "gpu_mixed_ms": msmx, "gpu_mixed_GBs": algmx / (msmx * 1e-3) / 1e9,
+                      "mixed_over_fp64": ms64 / msmx, "fp64_iterations_mean": float(db["numIters"].float().mean()),
+                      "mixed_iterations_mean": float(dmx["numIters"].float().mean()),
+                      "all_converged": bool((dmx["converged"] == 1).all() and (db["converged"] == 1).all()),
+                      "mixed_error_vs_fp64_max": max(errs), "mixed_error_vs_fp64_mean": float(np.mean(errs)),
+                      "fp64_equals_oracle_first_2": bool(same), "cpu_fp64_s": cpu2}
+    print(json.dumps({"configs": out}), flush=True)
+    import mit_jax_amd
+    mit_jax_amd.release()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -505,6 +636,8 @@ def main():
         return profile_solves(args.members)
     if args.kernel_profile:
         return kernel_profile(args)
+    if args.configs:
+        return configs_table()
     import torch
     from mit_jax_amd import cg2d, gad
     from mit_jax_amd.ensemble import autotune
