@@ -91,20 +91,27 @@ __device__ __forceinline__ double tile_of(int r, int lane, const double* stage, 
         return wave_tree(rb[lane], h ? rb[lane + 64] : 0.0, h);
     }
     // fortran order: the strict sequence s = ((0 + v[0]) + v[1]) + ... over the staged values in j/i order. The
-    // wave loads them 64 at a time (coalesced, the next 64 in flight while the current ones are added) and every
-    // lane carries the same running sum, taking the values lane by lane: only the chain of additions is serial.
+    // wave loads them 64 at a time (coalesced, the next 64 in flight) into its own slot of the row-sum array (unused
+    // in this order: reduction r of group member k takes doubles (2k + r) * 64 ..), and lane 0 adds them from there:
+    // only the chain of additions is serial. One wave writes and reads its slot, and a wave's LDS accesses complete
+    // in order, so no barrier is needed.
+    double* slot = g_rows + (2 * k + r) * 64;
     const double* pr = stage + (size_t)r * sNy * sNx;
     const int n = sNy * sNx;
     double s = 0.0;
-    double cur = lane < n ? pr[lane] : 0.0;
-    int c = 0;
-    for (; c + 64 <= n; c += 64) {
-        const double nxt = c + 64 + lane < n ? pr[c + 64 + lane] : 0.0;
+    double nxt = lane < n ? pr[lane] : 0.0;
+    for (int c = 0; c < n; c += 64) {
+        slot[lane] = nxt;
+        nxt = c + 64 + lane < n ? pr[c + 64 + lane] : 0.0;
+        if (lane == 0) {
+            if (c + 64 <= n) {
 #pragma unroll
-        for (int i = 0; i < 64; ++i) s = s + read_lane(cur, i);
-        cur = nxt;
+                for (int i = 0; i < 64; ++i) s = s + slot[i];
+            } else {
+                for (int i = 0; i < n - c; ++i) s = s + slot[i];
+            }
+        }
     }
-    for (int i = 0; i < n - c; ++i) s = s + read_lane(cur, i);
     return s;
 }
 
