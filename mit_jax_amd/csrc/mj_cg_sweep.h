@@ -23,7 +23,8 @@ __device__ __forceinline__ T op_M(const T* pW, const T* pS, const T* pC, const T
 // ------------------------------------------------------------------------------------------------ reductions
 // Reduction of per-point values over a tile interior, MJ_SUM_TREE: each wave reduces whole rows with wave_tree
 // (lane l holds i = l and i = l + 64) into a row-sum array in LDS, then one wave reduces the row sums.
-// MJ_SUM_FORTRAN: per-point values are staged in global memory and one lane adds them in j/i order.
+// MJ_SUM_FORTRAN: per-point values are staged in global memory and added in j/i order by lane 0 of one wave,
+// which brings them into LDS 64 at a time (tile_of).
 
 // Row loop: wave w of the block takes rows jl = w, w + nwaves, ...; p0 = (jl, il = lane), p1 = (jl, lane + 64).
 // Unrolled so that a wave has the loads of several rows in flight at once.
