@@ -7,7 +7,7 @@ level counts). Inputs are seeded synthetic fields resident in HBM before timing 
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--members B] [--precision mixed|fp64]
     python bench.py --kernel-profile --block-threads N   (the per-kernel timings alone, the workload for rocprofv3)
-    python bench.py --configs   (configs 0-2 one by one, each beside the FP64 C oracle on the host cores)
+    python bench.py --configs   (configs 0-3 one by one, 0-2 beside the FP64 C oracle on the host cores)
 
 The headline (default) is the mixed-precision solve (FP32 PCG + FP64 residual replacement, mj_cg2d_solve_mixed); the
 FP64 solve (mj_cg2d_solve) is timed on the same step for `mixed_over_fp64`, and the two solutions are compared
@@ -84,7 +84,8 @@ def parse():
     ap.add_argument("--configs", action="store_true",
                     help="only the per-configuration table: config 0 (one FP64 solve, diagonal preconditioner), config "
                          "1 (one and 24 DST3 X/Y/R steps), config 2 (64 rhs, batched FP64 and mixed), each beside the "
-                         "FP64 C oracle on the host cores; prints one JSON line")
+                         "FP64 C oracle on the host cores, and config 3 (the 24-step rollout and its gradients); "
+                         "prints one JSON line")
     ap.add_argument("--profile-solves", action="store_true",
                     help="only run one FP64 and one mixed solve of --members right-hand sides on the shared 5-point "
                          "operator (the workload of the hardware-counter profiles) and exit")
@@ -531,8 +532,10 @@ def configs_table():
     the mixed solve). Config 1: one DST3 step with the X/Y/R fluxes (one member, 50 levels) and 24 steps, the tracer
     updated and re-exchanged after each. Config 2: 64 rhs on the config-0 operator in one launch, FP64 (tile order,
     bitwise the oracle's) and mixed; the mixed solutions' error against the FP64 ones, per rhs (max and mean), with
-    the FP64 solutions of the first 2 rhs checked bitwise against the oracle's. GB/s are algorithmic bytes (the
-    headline's counts) over the wall time."""
+    the FP64 solutions of the first 2 rhs checked bitwise against the oracle's. Config 3: the 24-step coupled
+    rollout (CG2D + momentum correction + DST3 X/Y, checkpoint every 6 steps) forward alone, and forward with the
+    reverse-mode gradients of its loss w.r.t. T0, u0, v0 and b (no CPU counterpart: the oracle has no reverse
+    mode). GB/s are algorithmic bytes (the headline's counts) over the wall time."""
     import torch
     import oracle
     from oracle import tables as OT
@@ -598,6 +601,23 @@ def configs_table():
     out["config1"] = {"gpu_step_ms": ms1, "gpu_step_GBs": gb / (ms1 * 1e-3) / 1e9,
                       "gpu_step_equals_oracle": bool(np.array_equal(step(trd).cpu().numpy(), gr)),
                       "gpu_24_steps_ms": ms24, "cpu_step_s": cpu1}
+
+    print("bench: config 3", file=sys.stderr, flush=True)
+    from mit_jax_amd import rollout
+    case = rollout.RolloutCase.synthetic(dev)
+    X = [Td(a) for a in (tr, u, v)] + [Td(S.rhs(g1, 1) * 1e-3)]
+
+    def forward():
+        with torch.no_grad():
+            rollout.rollout(case, *X, nsteps=24, checkpoint_every=6, ex=ex)
+
+    def forward_backward():
+        leaves = [x.clone().requires_grad_(True) for x in X]
+        T, _, _ = rollout.rollout(case, *leaves, nsteps=24, checkpoint_every=6, ex=ex)
+        torch.autograd.grad(rollout.loss(case, T), leaves)
+
+    out["config3"] = {"gpu_forward_24_steps_ms": _median_ms(forward, 3),
+                      "gpu_forward_and_gradients_ms": _median_ms(forward_backward, 3)}
 
     print("bench: config 2", file=sys.stderr, flush=True)
     bs = np.stack([S.rhs(g, s) for s in range(100, 164)])
