@@ -515,37 +515,53 @@ def _median_ms(fn, reps):
     return float(np.median(ts))
 
 
-def _cpu_s(fn, reps):
-    """Median wall time (s) of `reps` calls of an oracle function on the host cores."""
-    ts = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        fn()
-        ts.append(time.perf_counter() - t0)
-    return float(np.median(ts))
+def configs_cpu_baseline(c0, c1, c2):
+    """The CPU baseline of the configs table: the FP64 C oracle (OpenMP, all host cores) on the same inputs, timed
+    (medians of a few calls; config 2 from 2 of its 64 solves x 32), with its results for the table's equality
+    flags. c0 = (ops, b, kw), c1 = (grid2d, vgrid, u, v, w, tracer, hFacW, hFacS, recip_hFacC), c2 = (ops, bs, kw)."""
+    import oracle
+    from oracle import tables as OT
+    from mit_jax_amd.exchange import default_exchanger
+    Tm = default_exchanger().maps.full("T")
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), r
+
+    ops, b, kw = c0
+    t0, (x0, d0) = timed(lambda: oracle.cg2d(Tm, ops, b, np.zeros_like(b), sum_order="fortran", **kw), 3)
+    tab = OT.tables(OT.Topology.from_facets(OT.LLC90_FACETS))
+    t1, g1 = timed(lambda: oracle.gad_dst3(tab, *c1), 3)
+    ops2, bs, kw2 = c2
+    t2, x2 = timed(lambda: [oracle.cg2d(Tm, ops2, bs[m], np.zeros_like(bs[m]), sum_order="tile", **kw2)[0]
+                            for m in range(2)], 1)
+    return {"cores": oracle.threads(), "config0": (t0, x0, int(d0["numIters"])), "config1": (t1, g1),
+            "config2": (t2 * 32, x2)}
 
 
 def configs_table():
-    """Configs 0-2 of the workload set on the GPU, each beside the FP64 C oracle (OpenMP, all host cores; oracle
-    timings are medians of a few calls, config 2's from 2 of its 64 solves x 32). Config 0: one solve of the 50-level
-    operator with the diagonal preconditioner to 1e-12 (FP64, reference sum order, launch picked for one member; and
-    the mixed solve). Config 1: one DST3 step with the X/Y/R fluxes (one member, 50 levels) and 24 steps, the tracer
-    updated and re-exchanged after each. Config 2: 64 rhs on the config-0 operator in one launch, FP64 (tile order,
-    bitwise the oracle's) and mixed; the mixed solutions' error against the FP64 ones, per rhs (max and mean), with
-    the FP64 solutions of the first 2 rhs checked bitwise against the oracle's. Config 3: the 24-step coupled
-    rollout (CG2D + momentum correction + DST3 X/Y, checkpoint every 6 steps) forward alone, and forward with the
-    reverse-mode gradients of its loss w.r.t. T0, u0, v0 and b (no CPU counterpart: the oracle has no reverse
-    mode). GB/s are algorithmic bytes (the headline's counts) over the wall time."""
+    """Configs 0-3 of the workload set on the GPU; configs 0-2 beside the FP64 C oracle on the host cores
+    (configs_cpu_baseline). Config 0: one solve of the 50-level operator with the diagonal preconditioner to 1e-12
+    (FP64, reference sum order, launch picked for one member; and the mixed solve). Config 1: one DST3 step with the
+    X/Y/R fluxes (one member, 50 levels) and 24 steps, the tracer updated and re-exchanged after each. Config 2: 64
+    rhs on the config-0 operator in one launch, FP64 (tile order) and mixed; the mixed solutions' error against the
+    FP64 ones, per rhs (max and mean). The FP64 results of configs 0 and 1, and of config 2's first 2 rhs, are
+    compared bitwise with the oracle's. Config 3: the 24-step coupled rollout (CG2D + momentum correction + DST3 X/Y,
+    checkpoint every 6 steps) forward alone, and forward with the reverse-mode gradients of its loss w.r.t. T0, u0,
+    v0 and b (no CPU counterpart: the oracle has no reverse mode). GB/s are algorithmic bytes (the headline's
+    counts) over the wall time."""
     import torch
-    import oracle
-    from oracle import tables as OT
-    from mit_jax_amd import cg2d, gad, mixed, synthetic as S
+    from mit_jax_amd import cg2d, gad, mixed, rollout, synthetic as S
     from mit_jax_amd.exchange import default_exchanger
     dev = torch.device("cuda:0")
     ex = default_exchanger()
-    Tm = ex.maps.full("T")
     Td = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
-    out = {"cores": oracle.threads()}
+    inner = (slice(None), slice(4, 94), slice(4, 94))
+    out, res = {}, {}
 
     print("bench: config 0", file=sys.stderr, flush=True)
     g, ops, norm = S.cg2d_case(0, "diagonal")
@@ -553,22 +569,14 @@ def configs_table():
     opsd = [Td(o) for o in ops]
     bd, x0 = Td(b), torch.zeros((13, 98, 98), dtype=torch.float64, device=dev)
     kw = dict(cg2dNorm=norm, tolerance=TOL, max_iters=4000)
-    res = {}
     ms = _median_ms(lambda: res.update(r=cg2d.cg2d_fortran(ex, opsd, bd, x0, sum_order="fortran", **kw)), 5)
     x64, d64 = res["r"]
     it = int(d64["numIters"])
     msm = _median_ms(lambda: res.update(m=mixed.cg2d_mixed(ex, opsd, bd, x0, **kw)), 5)
     xm, dm = res["m"]
-    xr, dr = oracle.cg2d(Tm, ops, b, np.zeros_like(b), sum_order="fortran", **kw)
-    cpu = _cpu_s(lambda: oracle.cg2d(Tm, ops, b, np.zeros_like(b), sum_order="fortran", **kw), 3)
-    inner = (slice(None), slice(4, 94), slice(4, 94))
-    err = float(np.abs(xm.cpu().numpy() - xr)[inner].max() / np.abs(xr[inner]).max())
     out["config0"] = {"iterations": it, "gpu_fp64_ms": ms, "gpu_fp64_us_per_iteration": ms * 1e3 / it,
-                      "gpu_fp64_GBs": it * N2 * 96 / (ms * 1e-3) / 1e9,
-                      "gpu_fp64_equals_oracle": bool(np.array_equal(x64.cpu().numpy(), xr)),
-                      "gpu_mixed_ms": msm, "mixed_inner_iterations": int(dm["numIters"]),
-                      "mixed_replacements": int(dm["outerIters"]), "mixed_error_vs_oracle": err,
-                      "cpu_fp64_s": cpu, "oracle_iterations": int(dr["numIters"])}
+                      "gpu_fp64_GBs": it * N2 * 96 / (ms * 1e-3) / 1e9, "gpu_mixed_ms": msm,
+                      "mixed_inner_iterations": int(dm["numIters"]), "mixed_replacements": int(dm["outerIters"])}
 
     print("bench: config 1", file=sys.stderr, flush=True)
     g1 = S.make_grid(0)
@@ -593,17 +601,11 @@ def configs_table():
         return t
 
     ms24 = _median_ms(chain, 3)
-    tab = OT.tables(OT.Topology.from_facets(OT.LLC90_FACETS))
-    g2o = dict(g2, rA=g1.rA)
-    gr = oracle.gad_dst3(tab, g2o, vg, u, v, w, tr, g1.hFacW, g1.hFacS, g1.recip_hFacC)
-    cpu1 = _cpu_s(lambda: oracle.gad_dst3(tab, g2o, vg, u, v, w, tr, g1.hFacW, g1.hFacS, g1.recip_hFacC), 3)
-    gb = dst3_bytes([50], True)
-    out["config1"] = {"gpu_step_ms": ms1, "gpu_step_GBs": gb / (ms1 * 1e-3) / 1e9,
-                      "gpu_step_equals_oracle": bool(np.array_equal(step(trd).cpu().numpy(), gr)),
-                      "gpu_24_steps_ms": ms24, "cpu_step_s": cpu1}
+    gT = step(trd).cpu().numpy()
+    out["config1"] = {"gpu_step_ms": ms1, "gpu_step_GBs": dst3_bytes([50], True) / (ms1 * 1e-3) / 1e9,
+                      "gpu_24_steps_ms": ms24}
 
     print("bench: config 3", file=sys.stderr, flush=True)
-    from mit_jax_amd import rollout
     case = rollout.RolloutCase.synthetic(dev)
     X = [Td(a) for a in (tr, u, v)] + [Td(S.rhs(g1, 1) * 1e-3)]
 
@@ -626,21 +628,29 @@ def configs_table():
     msmx = _median_ms(lambda: res.update(bmx=mixed.cg2d_mixed(ex, opsd, bsd, x0s, **kw)), 3)
     (xb, db), (xmx, dmx) = res["b64"], res["bmx"]
     one = torch.zeros(64, dtype=torch.int32, device=dev)
-    alg64 = solve_bytes("fp64", db, one)[0]
-    algmx = solve_bytes("mixed", dmx, one)[0]
     xbn, xmn = xb.cpu().numpy(), xmx.cpu().numpy()
     errs = [float(np.abs(xmn[m] - xbn[m])[inner].max() / np.abs(xbn[m][inner]).max()) for m in range(64)]
-    same = all(np.array_equal(xbn[m], oracle.cg2d(Tm, ops, bs[m], np.zeros_like(bs[m]), sum_order="tile", **kw)[0])
-               for m in range(2))
-    cpu2 = _cpu_s(lambda: [oracle.cg2d(Tm, ops, bs[m], np.zeros_like(bs[m]), sum_order="tile", **kw)
-                           for m in range(2)], 1) * 32
-    out["config2"] = {"gpu_fp64_ms": ms64, "gpu_fp64_GBs": alg64 / (ms64 * 1e-3) / 1e9,
-                      "gpu_mixed_ms": msmx, "gpu_mixed_GBs": algmx / (msmx * 1e-3) / 1e9,
+    out["config2"] = {"gpu_fp64_ms": ms64, "gpu_fp64_GBs": solve_bytes("fp64", db, one)[0] / (ms64 * 1e-3) / 1e9,
+                      "gpu_mixed_ms": msmx, "gpu_mixed_GBs": solve_bytes("mixed", dmx, one)[0] / (msmx * 1e-3) / 1e9,
                       "mixed_over_fp64": ms64 / msmx, "fp64_iterations_mean": float(db["numIters"].float().mean()),
                       "mixed_iterations_mean": float(dmx["numIters"].float().mean()),
                       "all_converged": bool((dmx["converged"] == 1).all() and (db["converged"] == 1).all()),
-                      "mixed_error_vs_fp64_max": max(errs), "mixed_error_vs_fp64_mean": float(np.mean(errs)),
-                      "fp64_equals_oracle_first_2": bool(same), "cpu_fp64_s": cpu2}
+                      "mixed_error_vs_fp64_max": max(errs), "mixed_error_vs_fp64_mean": float(np.mean(errs))}
+
+    print("bench: configs CPU baseline", file=sys.stderr, flush=True)
+    cb = configs_cpu_baseline((ops, b, kw), (dict(g2, rA=g1.rA), vg, u, v, w, tr, g1.hFacW, g1.hFacS,
+                                             g1.recip_hFacC), (ops, bs, kw))
+    out["cores"] = cb["cores"]
+    t0, xr, itr = cb["config0"]
+    out["config0"].update(cpu_fp64_s=t0, oracle_iterations=itr,
+                          gpu_fp64_equals_oracle=bool(np.array_equal(x64.cpu().numpy(), xr)),
+                          mixed_error_vs_oracle=float(np.abs(xm.cpu().numpy() - xr)[inner].max()
+                                                      / np.abs(xr[inner]).max()))
+    t1, gr = cb["config1"]
+    out["config1"].update(cpu_step_s=t1, gpu_step_equals_oracle=bool(np.array_equal(gT, gr)))
+    t2, x2 = cb["config2"]
+    out["config2"].update(cpu_fp64_s=t2, fp64_equals_oracle_first_2=all(np.array_equal(xbn[m], x2[m])
+                                                                        for m in range(2)))
     print(json.dumps({"configs": out}), flush=True)
     import mit_jax_amd
     mit_jax_amd.release()
