@@ -62,10 +62,6 @@ __device__ __forceinline__ double dst3_flux(double tr_, double vel_cfl_abs, doub
 // Lane 0 of the butterfly only ever adds its partner l + m, so the lower half of each step suffices: lane l < m
 // takes the value of lane l + m with the cross-lane moves of the VALU (gfx950 permlane swaps for m = 32, 16, DPP
 // row shifts for m = 8 .. 1) instead of LDS permutes. The other lanes end with partial values.
-__device__ __forceinline__ double read_lane(double v, int i) {  // every lane: the value of lane i (i uniform)
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), i), hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
-    return __hiloint2double(hi, lo);
-}
 __device__ __forceinline__ double lane_up32(double v) {  // lanes 0..31: the value of lane l + 32
     const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
     const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
